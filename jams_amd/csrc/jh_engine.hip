@@ -1,0 +1,1094 @@
+// jh_engine.hip -- host side of the gfx950 Heun-LLG engine and its C ABI
+// (include/jams_hip.h).
+//
+// The context owns one HIP stream and all device buffers.  Terms are added in
+// Hamiltonian config order; the first jh_step / jh_compute_* call after a
+// change "prepares" the device: it converts the exchange input into the
+// sliced-ELL layout, de-duplicates per-spin constants into site classes and
+// uploads everything.  Nothing is allocated inside jh_step.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <new>
+#include <string>
+#include <unordered_map>
+#include <thread>
+#include <vector>
+
+#include "../../include/jams_hip.h"
+#include "jh_dipole.h"
+#include "jh_internal.h"
+#include "jh_launch.h"
+
+using namespace jh;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+constexpr double kBoltzmannIU = 0.0861733326;  // meV/K, helpers/consts.h:35
+constexpr size_t kMaxTableLds = 48 * 1024;     // bytes of LDS for the tensor table
+
+// Static partition of [0, n) over host threads (set-up work only).
+template <class F>
+void parallel_for(int64_t n, F &&f) {
+  unsigned nt = std::thread::hardware_concurrency();
+  nt = std::max(1u, std::min(nt, 16u));
+  if (n < 65536 || nt == 1) {
+    for (int64_t i = 0; i < n; ++i) f(i);
+    return;
+  }
+  const int64_t chunk = (n + nt - 1) / nt;
+  std::vector<std::thread> pool;
+  for (unsigned t = 0; t < nt; ++t) {
+    const int64_t a = (int64_t)t * chunk, b = std::min(n, a + chunk);
+    if (a >= b) break;
+    pool.emplace_back([a, b, &f] {
+      for (int64_t i = a; i < b; ++i) f(i);
+    });
+  }
+  for (auto &th : pool) th.join();
+}
+
+struct JhError {
+  int code;
+  std::string msg;
+};
+
+#define JH_THROW(code, msg) throw JhError{(code), (msg)}
+#define HIPCHK(expr)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      JH_THROW(JH_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));         \
+  } while (0)
+
+template <class T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  void alloc(size_t count) {
+    if (count == n && p) return;
+    release();
+    if (count == 0) return;
+    hipError_t e = hipMalloc(reinterpret_cast<void **>(&p), count * sizeof(T));
+    if (e != hipSuccess) {
+      p = nullptr;
+      JH_THROW(JH_ERR_NOMEM, "hipMalloc of " + std::to_string(count * sizeof(T)) +
+                                 " bytes failed: " + hipGetErrorString(e));
+    }
+    n = count;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DevBuf() { release(); }
+};
+
+// Exchange operator on the host: per-spin neighbour blocks (j ascending) with
+// a tensor id into a table of distinct 3x3 tensors.
+struct XHost {
+  int64_t n = 0;
+  std::vector<int64_t> ptr;   // n + 1
+  std::vector<int32_t> col;   // neighbour spin per block
+  std::vector<int32_t> typ;   // tensor id per block
+  std::vector<double> tab;    // ntypes x 9
+  int64_t csr_nnz = 0;        // non-zeros of the equivalent scalar CSR
+};
+
+struct TermHost {
+  int kind;
+  int xidx = -1;                       // exchange: index into xh
+  std::vector<double> dc, ac, omega;   // zeeman (N x 3, N x 3, N)
+  std::vector<double> K, axis;         // uniaxial
+  int power = 2;
+  int dipole = -1;                     // dipole: index into dipoles
+};
+
+// 72-byte key hashing for tensor de-duplication
+struct Key9 {
+  uint64_t w[9];
+  bool operator==(const Key9 &o) const { return std::memcmp(w, o.w, sizeof w) == 0; }
+};
+struct Key9Hash {
+  size_t operator()(const Key9 &k) const {
+    uint64_t h = 1469598103934665603ull;
+    for (int q = 0; q < 9; ++q) {
+      h ^= k.w[q] + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
+    }
+    return (size_t)h;
+  }
+};
+static Key9 key9(const double *t) {
+  Key9 k;
+  for (int q = 0; q < 9; ++q) {
+    // +0 and -0 are the same tensor entry for the matvec (a zero product adds
+    // nothing), so canonicalise zeros
+    double v = t[q] == 0.0 ? 0.0 : t[q];
+    std::memcpy(&k.w[q], &v, 8);
+  }
+  return k;
+}
+
+}  // namespace
+
+struct jh_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  // sites
+  int64_t n = 0;
+  std::vector<double> mus, gyro, alpha;
+  double dt = 0.0;
+  bool have_dt = false;
+
+  // state
+  DevBuf<double> s, sp, ds, winj, hext, hext2, fout;
+  DevBuf<float> s32, sp32;
+  DevBuf<double> red_partial, red_out;
+  std::vector<double> h_stage;   // host staging for AoS <-> SoA
+  bool spins_set = false;
+  int64_t iteration = 0;
+  double time = 0.0;
+
+  // terms
+  std::vector<TermHost> terms;
+  std::vector<XHost> xh;
+  std::vector<std::unique_ptr<DipoleFFT>> dipoles;
+
+  // thermostat
+  double temperature = 0.0;
+  uint64_t seed = 0;
+  int gilbert = 0;
+  int noise_mode = NM_NONE;
+
+  // options
+  int prec = PR_FP64;
+  int xfmt_req = JH_XFMT_AUTO;
+  int hint_size[3] = {0, 0, 0}, hint_periodic[3] = {0, 0, 0}, hint_nbasis = 0;
+
+  // prepared device state
+  bool dirty = true;
+  int xv = XV_NONE;
+  int xfmt = JH_XFMT_AUTO;
+  int colbits = 0;
+  int ntab = 0;
+  int64_t nent = 0;
+  DevBuf<int64_t> slice_off;
+  DevBuf<uint32_t> ent;
+  DevBuf<double> tab64, val64;
+  DevBuf<float> tab32, val32;
+  DevBuf<uint16_t> cls;
+  DevBuf<double> ctab;
+  int nclass = 0;
+  int ani_power = 2;
+  int has_ac = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace {
+
+void require_sites(jh_ctx *c) {
+  if (c->n <= 0) JH_THROW(JH_ERR_STATE, "jh_set_sites must be called first");
+}
+
+void upload_soa(jh_ctx *c, const double *aos, double *dst) {
+  const int64_t n = c->n;
+  c->h_stage.resize((size_t)(3 * n));
+  double *st = c->h_stage.data();
+  parallel_for(n, [&](int64_t i) {
+    st[i] = aos[3 * i];
+    st[n + i] = aos[3 * i + 1];
+    st[2 * n + i] = aos[3 * i + 2];
+  });
+  HIPCHK(hipMemcpyAsync(dst, st, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+}
+
+void download_aos(jh_ctx *c, const double *src, double *aos) {
+  const int64_t n = c->n;
+  c->h_stage.resize((size_t)(3 * n));
+  double *st = c->h_stage.data();
+  HIPCHK(hipMemcpyAsync(st, src, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  parallel_for(n, [&](int64_t i) {
+    aos[3 * i] = st[i];
+    aos[3 * i + 1] = st[n + i];
+    aos[3 * i + 2] = st[2 * n + i];
+  });
+}
+
+// ---- exchange input conversion -------------------------------------------
+
+// Merge rows 3i, 3i+1, 3i+2 of the scalar CSR into 3x3 blocks keyed by
+// neighbour j = col / 3.  Rows must be strictly column-ascending (the
+// reference's build_csr output is).
+XHost xhost_from_csr(int64_t n, int64_t nrows, int64_t nnz, const int64_t *rowptr,
+                     const int32_t *col, const double *val) {
+  if (nrows != 3 * n) JH_THROW(JH_ERR_INVALID, "exchange CSR must have 3N rows");
+  if (rowptr[0] != 0 || rowptr[nrows] != nnz)
+    JH_THROW(JH_ERR_INVALID, "exchange CSR row pointers do not span [0, nnz]");
+  for (int64_t r = 0; r < nrows; ++r) {
+    if (rowptr[r + 1] < rowptr[r]) JH_THROW(JH_ERR_INVALID, "exchange CSR row pointers decrease");
+    for (int64_t q = rowptr[r]; q < rowptr[r + 1]; ++q) {
+      if (col[q] < 0 || col[q] >= 3 * n) JH_THROW(JH_ERR_INVALID, "exchange CSR column out of range");
+      if (q > rowptr[r] && col[q] <= col[q - 1])
+        JH_THROW(JH_ERR_INVALID, "exchange CSR columns must be strictly ascending in each row");
+    }
+  }
+  XHost x;
+  x.n = n;
+  x.csr_nnz = nnz;
+  x.ptr.assign((size_t)n + 1, 0);
+  // pass 1: block counts
+  parallel_for(n, [&](int64_t i) {
+    int64_t q[3], e[3];
+    for (int m = 0; m < 3; ++m) { q[m] = rowptr[3 * i + m]; e[m] = rowptr[3 * i + m + 1]; }
+    int64_t cnt = 0;
+    for (;;) {
+      int64_t jmin = INT64_MAX;
+      for (int m = 0; m < 3; ++m) if (q[m] < e[m]) jmin = std::min<int64_t>(jmin, col[q[m]] / 3);
+      if (jmin == INT64_MAX) break;
+      for (int m = 0; m < 3; ++m) while (q[m] < e[m] && col[q[m]] / 3 == jmin) ++q[m];
+      ++cnt;
+    }
+    x.ptr[i + 1] = cnt;
+  });
+  for (int64_t i = 0; i < n; ++i) x.ptr[i + 1] += x.ptr[i];
+  const int64_t nb = x.ptr[n];
+  x.col.resize((size_t)nb);
+  std::vector<double> blk((size_t)nb * 9, 0.0);
+  parallel_for(n, [&](int64_t i) {
+    int64_t q[3], e[3];
+    for (int m = 0; m < 3; ++m) { q[m] = rowptr[3 * i + m]; e[m] = rowptr[3 * i + m + 1]; }
+    int64_t b = x.ptr[i];
+    for (;;) {
+      int64_t jmin = INT64_MAX;
+      for (int m = 0; m < 3; ++m) if (q[m] < e[m]) jmin = std::min<int64_t>(jmin, col[q[m]] / 3);
+      if (jmin == INT64_MAX) break;
+      x.col[b] = (int32_t)jmin;
+      for (int m = 0; m < 3; ++m)
+        while (q[m] < e[m] && col[q[m]] / 3 == jmin) {
+          blk[(size_t)b * 9 + 3 * m + (col[q[m]] % 3)] = val[q[m]];
+          ++q[m];
+        }
+      ++b;
+    }
+  });
+  // de-duplicate tensors
+  x.typ.resize((size_t)nb);
+  std::unordered_map<Key9, int32_t, Key9Hash> map;
+  for (int64_t b = 0; b < nb; ++b) {
+    const Key9 k = key9(&blk[(size_t)b * 9]);
+    auto it = map.find(k);
+    if (it == map.end()) {
+      const int32_t id = (int32_t)(x.tab.size() / 9);
+      map.emplace(k, id);
+      for (int q2 = 0; q2 < 9; ++q2) x.tab.push_back(blk[(size_t)b * 9 + q2]);
+      x.typ[b] = id;
+    } else {
+      x.typ[b] = it->second;
+    }
+  }
+  return x;
+}
+
+XHost xhost_from_list(int64_t n, int64_t np, const int32_t *pi, const int32_t *pj,
+                      const int32_t *vi, int64_t nv, const double *v9) {
+  XHost x;
+  x.n = n;
+  x.ptr.assign((size_t)n + 1, 0);
+  for (int64_t q = 0; q < np; ++q) {
+    if (pi[q] < 0 || pi[q] >= n || pj[q] < 0 || pj[q] >= n)
+      JH_THROW(JH_ERR_INVALID, "neighbour list index out of range");
+    if (vi[q] < 0 || vi[q] >= nv) JH_THROW(JH_ERR_INVALID, "neighbour list value index out of range");
+    if (q > 0 && (pi[q] < pi[q - 1] || (pi[q] == pi[q - 1] && pj[q] <= pj[q - 1])))
+      JH_THROW(JH_ERR_INVALID, "neighbour list must be strictly sorted by (i, j)");
+    x.ptr[(size_t)pi[q] + 1]++;
+  }
+  for (int64_t i = 0; i < n; ++i) x.ptr[i + 1] += x.ptr[i];
+  x.col.assign(pj, pj + np);
+  // de-duplicate the value table (identical tensors share an id)
+  std::vector<int32_t> remap((size_t)nv);
+  std::unordered_map<Key9, int32_t, Key9Hash> map;
+  for (int64_t v = 0; v < nv; ++v) {
+    const Key9 k = key9(v9 + 9 * v);
+    auto it = map.find(k);
+    if (it == map.end()) {
+      const int32_t id = (int32_t)(x.tab.size() / 9);
+      map.emplace(k, id);
+      for (int q2 = 0; q2 < 9; ++q2) x.tab.push_back(v9[9 * v + q2]);
+      remap[v] = id;
+    } else {
+      remap[v] = it->second;
+    }
+  }
+  x.typ.resize((size_t)np);
+  int64_t nnz = 0;
+  std::vector<int64_t> nz_per_val((size_t)nv, 0);
+  for (int64_t v = 0; v < nv; ++v)
+    for (int q2 = 0; q2 < 9; ++q2) nz_per_val[v] += (v9[9 * v + q2] != 0.0);
+  for (int64_t q = 0; q < np; ++q) {
+    x.typ[q] = remap[vi[q]];
+    nnz += nz_per_val[vi[q]];
+  }
+  x.csr_nnz = nnz;
+  return x;
+}
+
+// ---- prepare: device formats and site classes ----------------------------
+
+void prepare_exchange(jh_ctx *c) {
+  c->xv = XV_NONE;
+  c->ntab = 0;
+  c->nent = 0;
+  if (c->xh.empty()) return;
+  const XHost &x = c->xh[0];
+  const int64_t n = c->n;
+  const int ntypes = (int)(x.tab.size() / 9);
+  bool diag = true;
+  for (int t = 0; t < ntypes && diag; ++t)
+    for (int m = 0; m < 3; ++m)
+      for (int q = 0; q < 3; ++q)
+        if (m != q && x.tab[9 * t + 3 * m + q] != 0.0) diag = false;
+  const int vpt = diag ? 3 : 9;
+  const int ntab = ntypes + 1;  // + trailing zero tensor used by padding
+  int colbits = 1;
+  while (colbits < 32 && ((int64_t)1 << colbits) < n) ++colbits;
+  int tbits = 0;
+  while ((1 << tbits) < ntab) ++tbits;
+  const size_t lds_need = (size_t)ntab * vpt * (c->prec == PR_FP64 ? 8 : 4);
+  bool typed_ok = (colbits + tbits <= 32) && lds_need <= kMaxTableLds;
+  int fmt = c->xfmt_req;
+  if (fmt == JH_XFMT_AUTO) fmt = typed_ok ? JH_XFMT_TYPED : JH_XFMT_BLOCK;
+  if (fmt == JH_XFMT_TYPED && !typed_ok)
+    JH_THROW(JH_ERR_UNSUPPORTED, "TYPED exchange format does not fit (too many distinct tensors)");
+  c->xfmt = fmt;
+  const bool typed = fmt == JH_XFMT_TYPED;
+  if (!typed) colbits = 32;
+  const uint32_t colmask = colbits >= 32 ? 0xFFFFFFFFu : ((1u << colbits) - 1u);
+
+  const int64_t nsl = (n + kSlice - 1) / kSlice;
+  std::vector<int64_t> off((size_t)nsl + 1, 0);
+  for (int64_t s = 0; s < nsl; ++s) {
+    int64_t w = 0;
+    for (int64_t i = s * kSlice; i < std::min(n, (s + 1) * kSlice); ++i)
+      w = std::max(w, x.ptr[i + 1] - x.ptr[i]);
+    off[s + 1] = off[s] + w * kSlice;
+  }
+  const int64_t E = off[nsl];
+  std::vector<uint32_t> ent((size_t)E);
+  std::vector<double> val64;
+  std::vector<float> val32;
+  if (!typed) {
+    if (c->prec == PR_FP64) val64.assign((size_t)E * vpt, 0.0);
+    else val32.assign((size_t)E * vpt, 0.f);
+  }
+  const uint32_t zero_t = (uint32_t)ntypes;
+  parallel_for(nsl, [&](int64_t s) {
+    const int64_t w = (off[s + 1] - off[s]) / kSlice;
+    for (int64_t k = 0; k < w; ++k) {
+      for (int l = 0; l < kSlice; ++l) {
+        const int64_t i = s * kSlice + l;
+        const int64_t e = off[s] + k * kSlice + l;
+        const int64_t len = i < n ? x.ptr[i + 1] - x.ptr[i] : 0;
+        uint32_t j, t;
+        if (k < len) {
+          j = (uint32_t)x.col[x.ptr[i] + k];
+          t = (uint32_t)x.typ[x.ptr[i] + k];
+        } else {
+          j = (uint32_t)(i < n ? i : 0);
+          t = zero_t;
+        }
+        ent[e] = typed ? (j | (t << colbits)) : j;
+        if (!typed && t != zero_t) {
+          const double *T = &x.tab[(size_t)t * 9];
+          for (int q = 0; q < vpt; ++q) {
+            const double v = diag ? T[4 * q] : T[q];
+            const size_t vi = (size_t)(off[s] + k * kSlice) * vpt + (size_t)q * kSlice + l;
+            if (c->prec == PR_FP64) val64[vi] = v; else val32[vi] = (float)v;
+          }
+        }
+      }
+    }
+  });
+  c->slice_off.alloc((size_t)nsl + 1);
+  HIPCHK(hipMemcpy(c->slice_off.p, off.data(), sizeof(int64_t) * off.size(), hipMemcpyHostToDevice));
+  c->ent.alloc((size_t)std::max<int64_t>(E, 1));
+  if (E > 0) HIPCHK(hipMemcpy(c->ent.p, ent.data(), sizeof(uint32_t) * E, hipMemcpyHostToDevice));
+  if (typed) {
+    std::vector<double> t64((size_t)ntab * vpt, 0.0);
+    std::vector<float> t32((size_t)ntab * vpt, 0.f);
+    for (int t = 0; t < ntypes; ++t)
+      for (int q = 0; q < vpt; ++q) {
+        const double v = diag ? x.tab[9 * t + 4 * q] : x.tab[9 * t + q];
+        t64[(size_t)t * vpt + q] = v;
+        t32[(size_t)t * vpt + q] = (float)v;
+      }
+    c->tab64.alloc(t64.size());
+    c->tab32.alloc(t32.size());
+    HIPCHK(hipMemcpy(c->tab64.p, t64.data(), sizeof(double) * t64.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->tab32.p, t32.data(), sizeof(float) * t32.size(), hipMemcpyHostToDevice));
+    c->val64.release();
+    c->val32.release();
+    c->xv = diag ? XV_TDIAG : XV_TFULL;
+  } else {
+    if (c->prec == PR_FP64) {
+      c->val64.alloc(std::max<size_t>(val64.size(), 1));
+      if (!val64.empty())
+        HIPCHK(hipMemcpy(c->val64.p, val64.data(), sizeof(double) * val64.size(), hipMemcpyHostToDevice));
+      c->val32.release();
+    } else {
+      c->val32.alloc(std::max<size_t>(val32.size(), 1));
+      if (!val32.empty())
+        HIPCHK(hipMemcpy(c->val32.p, val32.data(), sizeof(float) * val32.size(), hipMemcpyHostToDevice));
+      c->val64.release();
+    }
+    c->xv = diag ? XV_BDIAG : XV_BFULL;
+  }
+  c->colbits = colbits;
+  c->ntab = typed ? ntab : 0;
+  c->nent = E;
+}
+
+void prepare_classes(jh_ctx *c) {
+  const int64_t n = c->n;
+  const TermHost *zee = nullptr, *ani = nullptr;
+  for (auto &t : c->terms) {
+    if (t.kind == TK_ZEEMAN) zee = &t;
+    if (t.kind == TK_UNIAXIAL) ani = &t;
+  }
+  c->has_ac = (zee && !zee->ac.empty()) ? 1 : 0;
+  c->ani_power = ani ? ani->power : 2;
+  std::vector<double> row(CF_STRIDE);
+  std::map<std::vector<uint64_t>, int> map;
+  std::vector<double> table;
+  std::vector<uint16_t> ids((size_t)n);
+  for (int64_t i = 0; i < n; ++i) {
+    std::fill(row.begin(), row.end(), 0.0);
+    row[CF_MUS] = c->mus[i];
+    row[CF_GYRO] = c->gyro[i];
+    row[CF_ALPHA] = c->alpha[i];
+    if (c->have_dt) {
+      // solvers/cpu_llg_heun.cc:36-43 and cuda_thermostat_classical.cc:33-45
+      double denominator = 1.0;
+      if (c->gilbert) denominator = 1.0 + c->alpha[i] * c->alpha[i];
+      row[CF_SIGMA] = std::sqrt((2.0 * kBoltzmannIU * c->alpha[i]) /
+                                (c->mus[i] * c->gyro[i] * c->dt * denominator));
+    }
+    if (zee) {
+      for (int q = 0; q < 3; ++q) row[CF_ZDC + q] = zee->dc[3 * i + q];
+      if (!zee->ac.empty()) {
+        for (int q = 0; q < 3; ++q) row[CF_ZAC + q] = zee->ac[3 * i + q];
+        row[CF_ZOMEGA] = zee->omega[i];
+      }
+    }
+    if (ani) {
+      row[CF_K] = ani->K[i];
+      for (int q = 0; q < 3; ++q) row[CF_AXIS + q] = ani->axis[3 * i + q];
+    }
+    std::vector<uint64_t> key(CF_STRIDE);
+    std::memcpy(key.data(), row.data(), sizeof(double) * CF_STRIDE);
+    auto it = map.find(key);
+    int id;
+    if (it == map.end()) {
+      id = (int)map.size();
+      if (id > 65535) JH_THROW(JH_ERR_UNSUPPORTED, "more than 65536 distinct site classes");
+      map.emplace(key, id);
+      table.insert(table.end(), row.begin(), row.end());
+    } else {
+      id = it->second;
+    }
+    ids[i] = (uint16_t)id;
+  }
+  c->nclass = (int)map.size();
+  c->ctab.alloc(table.size());
+  HIPCHK(hipMemcpy(c->ctab.p, table.data(), sizeof(double) * table.size(), hipMemcpyHostToDevice));
+  if (c->nclass > 1) {
+    c->cls.alloc((size_t)n);
+    HIPCHK(hipMemcpy(c->cls.p, ids.data(), sizeof(uint16_t) * n, hipMemcpyHostToDevice));
+  } else {
+    c->cls.release();
+  }
+}
+
+void sync_f32(jh_ctx *c) {
+  if (c->prec != PR_FP32) return;
+  c->s32.alloc((size_t)(3 * c->n));
+  c->sp32.alloc((size_t)(3 * c->n));
+  HIPCHK(launch_to_f32(c->s.p, c->s32.p, 3 * c->n, c->stream));
+}
+
+void prepare(jh_ctx *c) {
+  require_sites(c);
+  if (!c->spins_set) JH_THROW(JH_ERR_STATE, "spins have not been set (jh_set_spins)");
+  if (!c->dirty) return;
+  if (c->xh.size() > 1)
+    JH_THROW(JH_ERR_UNSUPPORTED, "more than one exchange term; merge them into one list");
+  int nzee = 0, nani = 0;
+  for (auto &t : c->terms) {
+    nzee += t.kind == TK_ZEEMAN;
+    nani += t.kind == TK_UNIAXIAL;
+  }
+  if (nzee > 1 || nani > 1)
+    JH_THROW(JH_ERR_UNSUPPORTED, "at most one zeeman and one uniaxial term per context");
+  if (c->terms.size() > 4) JH_THROW(JH_ERR_UNSUPPORTED, "at most 4 Hamiltonian terms");
+  prepare_exchange(c);
+  prepare_classes(c);
+  sync_f32(c);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->dirty = false;
+}
+
+KParams make_params(jh_ctx *c) {
+  KParams p;
+  std::memset(&p, 0, sizeof p);
+  p.n = c->n;
+  p.s = c->s.p;
+  p.sp = c->sp.p;
+  p.ds = c->ds.p;
+  p.snew = c->s.p;
+  p.cls = c->nclass > 1 ? c->cls.p : nullptr;
+  p.ctab = c->ctab.p;
+  p.x.slice_off = c->slice_off.p;
+  p.x.ent = c->ent.p;
+  p.x.colbits = c->colbits;
+  p.x.colmask = c->colbits >= 32 ? 0xFFFFFFFFu : ((1u << c->colbits) - 1u);
+  p.x.ntab = c->ntab;
+  p.x.tab64 = c->tab64.p;
+  p.x.tab32 = c->tab32.p;
+  p.x.val64 = c->val64.p;
+  p.x.val32 = c->val32.p;
+  p.nterms = (int)c->terms.size();
+  for (int q = 0; q < p.nterms; ++q) p.kind[q] = c->terms[q].kind;
+  p.hext = c->hext.p;
+  p.has_ac = c->has_ac;
+  p.ani_power = c->ani_power;
+  p.noise = c->noise_mode;
+  p.seed = c->seed;
+  p.sqrtT = std::sqrt(c->temperature);
+  p.winj = c->winj.p;
+  p.dt = c->dt;
+  return p;
+}
+
+int dipole_term(const jh_ctx *c) {
+  for (auto &t : c->terms) if (t.kind == TK_DIPOLE) return t.dipole;
+  return -1;
+}
+
+void do_steps(jh_ctx *c, int64_t nsteps) {
+  KParams p = make_params(c);
+  const int dip = dipole_term(c);
+  for (int64_t k = 0; k < nsteps; ++k) {
+    p.step = (uint64_t)c->iteration;
+    p.t = c->time;
+    // predictor
+    if (dip >= 0) HIPCHK(c->dipoles[dip]->field(c->s.p, c->hext.p, c->stream));
+    p.g32 = c->s32.p;
+    p.w32 = c->sp32.p;
+    HIPCHK(launch_heun(p, c->prec, c->xv, 0, c->stream));
+    // corrector at t + dt
+    if (dip >= 0) HIPCHK(c->dipoles[dip]->field(c->sp.p, c->hext.p, c->stream));
+    p.g32 = c->sp32.p;
+    p.w32 = c->s32.p;
+    HIPCHK(launch_heun(p, c->prec, c->xv, 1, c->stream));
+    c->iteration += 1;
+    c->time = (double)c->iteration * c->dt;
+  }
+}
+
+template <class F>
+int guarded(jh_ctx *c, F &&f) {
+  if (!c) return JH_ERR_INVALID;
+  try {
+    f();
+    return JH_OK;
+  } catch (const JhError &e) {
+    c->err = e.msg;
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    c->err = "host allocation failed";
+    return JH_ERR_NOMEM;
+  } catch (const std::exception &e) {
+    c->err = e.what();
+    return JH_ERR_INVALID;
+  }
+}
+
+void field_into(jh_ctx *c, int term, double t, double *dst) {
+  KParams p = make_params(c);
+  p.t = t;
+  p.g32 = c->s32.p;
+  const int dip = dipole_term(c);
+  if (dip >= 0 && (term < 0 || c->terms[term].kind == TK_DIPOLE))
+    HIPCHK(c->dipoles[dip]->field(c->s.p, c->hext.p, c->stream));
+  HIPCHK(launch_field(p, c->prec, c->xv, term, dst, c->stream));
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+
+extern "C" {
+
+int jh_version(void) { return 10000; }
+
+int jh_create(int device, jh_ctx **out) {
+  if (!out) return JH_ERR_INVALID;
+  *out = nullptr;
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count <= 0) {
+    g_create_error = std::string("no HIP device available: ") + hipGetErrorString(e);
+    return JH_ERR_HIP;
+  }
+  if (device < 0 || device >= count) {
+    g_create_error = "device index out of range";
+    return JH_ERR_INVALID;
+  }
+  e = hipSetDevice(device);
+  if (e != hipSuccess) {
+    g_create_error = std::string("hipSetDevice: ") + hipGetErrorString(e);
+    return JH_ERR_HIP;
+  }
+  jh_ctx *c = new (std::nothrow) jh_ctx();
+  if (!c) return JH_ERR_NOMEM;
+  c->device = device;
+  if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess ||
+      (e = hipEventCreate(&c->ev0)) != hipSuccess || (e = hipEventCreate(&c->ev1)) != hipSuccess) {
+    g_create_error = std::string("stream/event creation: ") + hipGetErrorString(e);
+    delete c;
+    return JH_ERR_HIP;
+  }
+  *out = c;
+  return JH_OK;
+}
+
+int jh_destroy(jh_ctx *c) {
+  if (!c) return JH_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  c->dipoles.clear();
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return JH_OK;
+}
+
+const char *jh_last_error(const jh_ctx *c) {
+  return c ? c->err.c_str() : g_create_error.c_str();
+}
+
+int jh_set_sites(jh_ctx *c, int64_t n, const double *mus, const double *gyro, const double *alpha) {
+  return guarded(c, [&] {
+    if (n <= 0 || !mus || !gyro || !alpha) JH_THROW(JH_ERR_INVALID, "jh_set_sites: bad arguments");
+    if (n > (int64_t)1 << 31) JH_THROW(JH_ERR_INVALID, "jh_set_sites: at most 2^31 spins");
+    HIPCHK(hipSetDevice(c->device));
+    c->n = n;
+    c->mus.assign(mus, mus + n);
+    c->gyro.assign(gyro, gyro + n);
+    c->alpha.assign(alpha, alpha + n);
+    for (int64_t i = 0; i < n; ++i)
+      if (!(c->mus[i] != 0.0)) JH_THROW(JH_ERR_INVALID, "jh_set_sites: mus must be non-zero");
+    c->terms.clear();
+    c->xh.clear();
+    c->dipoles.clear();
+    c->s.alloc((size_t)(3 * n));
+    c->sp.alloc((size_t)(3 * n));
+    c->ds.alloc((size_t)(3 * n));
+    c->fout.alloc((size_t)(3 * n));
+    const int64_t nb = (n + kBlock - 1) / kBlock;
+    c->red_partial.alloc((size_t)(4 * nb));
+    c->red_out.alloc(4);
+    c->spins_set = false;
+    c->noise_mode = NM_NONE;
+    c->iteration = 0;
+    c->time = 0.0;
+    c->dirty = true;
+  });
+}
+
+int jh_set_spins(jh_ctx *c, const double *s) {
+  return guarded(c, [&] {
+    require_sites(c);
+    if (!s) JH_THROW(JH_ERR_INVALID, "jh_set_spins: null");
+    upload_soa(c, s, c->s.p);
+    c->spins_set = true;
+    if (!c->dirty) sync_f32(c);
+  });
+}
+
+int jh_get_spins(jh_ctx *c, double *s) {
+  return guarded(c, [&] {
+    require_sites(c);
+    if (!s) JH_THROW(JH_ERR_INVALID, "jh_get_spins: null");
+    download_aos(c, c->s.p, s);
+  });
+}
+
+int jh_set_time_step(jh_ctx *c, double dt) {
+  return guarded(c, [&] {
+    if (!(dt > 0.0)) JH_THROW(JH_ERR_INVALID, "time step must be positive");
+    c->dt = dt;
+    c->have_dt = true;
+    c->time = (double)c->iteration * dt;
+    c->dirty = true;
+  });
+}
+
+int jh_add_exchange_csr(jh_ctx *c, int64_t nrows, int64_t nnz, const int64_t *rowptr,
+                        const int32_t *col, const double *val, int *term_id) {
+  return guarded(c, [&] {
+    require_sites(c);
+    if (!rowptr || (nnz > 0 && (!col || !val)) || nnz < 0)
+      JH_THROW(JH_ERR_INVALID, "jh_add_exchange_csr: bad arguments");
+    c->xh.push_back(xhost_from_csr(c->n, nrows, nnz, rowptr, col, val));
+    TermHost t;
+    t.kind = TK_EXCHANGE;
+    t.xidx = (int)c->xh.size() - 1;
+    c->terms.push_back(std::move(t));
+    if (term_id) *term_id = (int)c->terms.size() - 1;
+    c->dirty = true;
+  });
+}
+
+int jh_add_exchange_list(jh_ctx *c, int64_t np, const int32_t *pi, const int32_t *pj,
+                         const int32_t *vi, int64_t nv, const double *v9, int *term_id) {
+  return guarded(c, [&] {
+    require_sites(c);
+    if (np < 0 || nv < 0 || (np > 0 && (!pi || !pj || !vi || !v9)))
+      JH_THROW(JH_ERR_INVALID, "jh_add_exchange_list: bad arguments");
+    c->xh.push_back(xhost_from_list(c->n, np, pi, pj, vi, nv, v9));
+    TermHost t;
+    t.kind = TK_EXCHANGE;
+    t.xidx = (int)c->xh.size() - 1;
+    c->terms.push_back(std::move(t));
+    if (term_id) *term_id = (int)c->terms.size() - 1;
+    c->dirty = true;
+  });
+}
+
+int jh_add_zeeman(jh_ctx *c, const double *dc, const double *ac, const double *omega, int *term_id) {
+  return guarded(c, [&] {
+    require_sites(c);
+    if (!dc || ((ac == nullptr) != (omega == nullptr)))
+      JH_THROW(JH_ERR_INVALID, "jh_add_zeeman: dc required; ac and omega together");
+    TermHost t;
+    t.kind = TK_ZEEMAN;
+    t.dc.assign(dc, dc + 3 * c->n);
+    if (ac) {
+      t.ac.assign(ac, ac + 3 * c->n);
+      t.omega.assign(omega, omega + c->n);
+    }
+    c->terms.push_back(std::move(t));
+    if (term_id) *term_id = (int)c->terms.size() - 1;
+    c->dirty = true;
+  });
+}
+
+int jh_add_uniaxial(jh_ctx *c, const double *K, const double *axis, int power, int *term_id) {
+  return guarded(c, [&] {
+    require_sites(c);
+    if (!K || !axis) JH_THROW(JH_ERR_INVALID, "jh_add_uniaxial: null");
+    if (power != 2 && power != 4 && power != 6)
+      JH_THROW(JH_ERR_INVALID, "jh_add_uniaxial: power must be 2, 4 or 6");
+    TermHost t;
+    t.kind = TK_UNIAXIAL;
+    t.K.assign(K, K + c->n);
+    t.axis.assign(axis, axis + 3 * c->n);
+    t.power = power;
+    c->terms.push_back(std::move(t));
+    if (term_id) *term_id = (int)c->terms.size() - 1;
+    c->dirty = true;
+  });
+}
+
+int jh_add_dipole_fft(jh_ctx *c, const int32_t size[3], const int32_t periodic[3], int nbasis,
+                      const double cell[9], const double *basis_frac, double a_m, double r_cutoff,
+                      int *term_id) {
+  return guarded(c, [&] {
+    require_sites(c);
+    if (!size || !periodic || !cell || !basis_frac || nbasis <= 0)
+      JH_THROW(JH_ERR_INVALID, "jh_add_dipole_fft: bad arguments");
+    for (auto &t : c->terms)
+      if (t.kind == TK_DIPOLE) JH_THROW(JH_ERR_UNSUPPORTED, "at most one dipole term");
+    const int64_t cells = (int64_t)size[0] * size[1] * size[2];
+    if (cells * nbasis != c->n) JH_THROW(JH_ERR_INVALID, "dipole lattice does not match N");
+    std::string msg;
+    std::unique_ptr<DipoleFFT> d(DipoleFFT::create(size, periodic, nbasis, cell, basis_frac, a_m,
+                                                   r_cutoff, c->mus.data(), c->stream, &msg));
+    if (!d) JH_THROW(JH_ERR_INVALID, "dipole-fft: " + msg);
+    c->dipoles.push_back(std::move(d));
+    c->hext.alloc((size_t)(3 * c->n));
+    TermHost t;
+    t.kind = TK_DIPOLE;
+    t.dipole = (int)c->dipoles.size() - 1;
+    c->terms.push_back(std::move(t));
+    if (term_id) *term_id = (int)c->terms.size() - 1;
+    c->dirty = true;
+  });
+}
+
+int jh_num_terms(const jh_ctx *c) { return c ? (int)c->terms.size() : 0; }
+
+int jh_term_kind(const jh_ctx *c, int term_id) {
+  if (!c || term_id < 0 || term_id >= (int)c->terms.size()) return -1;
+  return c->terms[term_id].kind;
+}
+
+int jh_set_thermostat(jh_ctx *c, double temperature, uint64_t seed, int gilbert) {
+  return guarded(c, [&] {
+    require_sites(c);
+    if (!(temperature >= 0.0)) JH_THROW(JH_ERR_INVALID, "temperature must be >= 0");
+    c->temperature = temperature;
+    c->seed = seed;
+    c->gilbert = gilbert ? 1 : 0;
+    c->noise_mode = temperature > 0.0 ? NM_PHILOX : NM_NONE;
+    c->dirty = true;
+  });
+}
+
+int jh_set_noise(jh_ctx *c, const double *w) {
+  return guarded(c, [&] {
+    require_sites(c);
+    if (!w) JH_THROW(JH_ERR_INVALID, "jh_set_noise: null");
+    c->winj.alloc((size_t)(3 * c->n));
+    upload_soa(c, w, c->winj.p);
+    c->noise_mode = NM_INJECTED;
+  });
+}
+
+int jh_get_noise(jh_ctx *c, int64_t step, double *w) {
+  return guarded(c, [&] {
+    prepare(c);
+    if (!c->have_dt) JH_THROW(JH_ERR_STATE, "time step not set");
+    KParams p = make_params(c);
+    p.step = (uint64_t)step;
+    HIPCHK(launch_noise(p, c->fout.p, c->stream));
+    download_aos(c, c->fout.p, w);
+  });
+}
+
+int jh_set_precision(jh_ctx *c, int mode) {
+  return guarded(c, [&] {
+    if (mode != JH_PREC_FP64 && mode != JH_PREC_FP32_FIELD)
+      JH_THROW(JH_ERR_INVALID, "unknown precision mode");
+    c->prec = mode == JH_PREC_FP64 ? PR_FP64 : PR_FP32;
+    c->dirty = true;
+  });
+}
+
+int jh_set_exchange_format(jh_ctx *c, int fmt) {
+  return guarded(c, [&] {
+    if (fmt != JH_XFMT_AUTO && fmt != JH_XFMT_TYPED && fmt != JH_XFMT_BLOCK)
+      JH_THROW(JH_ERR_INVALID, "unknown exchange format");
+    c->xfmt_req = fmt;
+    c->dirty = true;
+  });
+}
+
+int jh_set_lattice_hint(jh_ctx *c, const int32_t size[3], const int32_t periodic[3], int nbasis) {
+  return guarded(c, [&] {
+    if (!size || !periodic || nbasis <= 0) JH_THROW(JH_ERR_INVALID, "jh_set_lattice_hint: bad arguments");
+    for (int q = 0; q < 3; ++q) {
+      c->hint_size[q] = size[q];
+      c->hint_periodic[q] = periodic[q];
+    }
+    c->hint_nbasis = nbasis;
+  });
+}
+
+int jh_step(jh_ctx *c, int64_t nsteps) {
+  return guarded(c, [&] {
+    if (nsteps < 0) JH_THROW(JH_ERR_INVALID, "nsteps must be >= 0");
+    if (!c->have_dt) JH_THROW(JH_ERR_STATE, "time step not set");
+    HIPCHK(hipSetDevice(c->device));
+    prepare(c);
+    do_steps(c, nsteps);
+  });
+}
+
+int jh_get_iteration(const jh_ctx *c, int64_t *it, double *t) {
+  if (!c) return JH_ERR_INVALID;
+  if (it) *it = c->iteration;
+  if (t) *t = c->time;
+  return JH_OK;
+}
+
+int jh_set_iteration(jh_ctx *c, int64_t it) {
+  return guarded(c, [&] {
+    if (it < 0) JH_THROW(JH_ERR_INVALID, "iteration must be >= 0");
+    c->iteration = it;
+    c->time = (double)it * c->dt;
+  });
+}
+
+int jh_compute_term_field(jh_ctx *c, int term, double t, double *out) {
+  return guarded(c, [&] {
+    prepare(c);
+    if (term < 0 || term >= (int)c->terms.size()) JH_THROW(JH_ERR_INVALID, "term id out of range");
+    field_into(c, term, t, c->fout.p);
+    download_aos(c, c->fout.p, out);
+  });
+}
+
+int jh_compute_fields(jh_ctx *c, double t, double *out) {
+  return guarded(c, [&] {
+    prepare(c);
+    field_into(c, -1, t, c->fout.p);
+    download_aos(c, c->fout.p, out);
+  });
+}
+
+int jh_term_energy(jh_ctx *c, int term, double t, double *energy) {
+  return guarded(c, [&] {
+    prepare(c);
+    if (term < 0 || term >= (int)c->terms.size()) JH_THROW(JH_ERR_INVALID, "term id out of range");
+    KParams p = make_params(c);
+    p.t = t;
+    const int kind = c->terms[term].kind;
+    if (kind == TK_UNIAXIAL) {
+      HIPCHK(launch_reduce(p, 2, nullptr, c->red_partial.p, c->red_out.p, c->stream));
+    } else {
+      field_into(c, term, t, c->fout.p);
+      HIPCHK(launch_reduce(p, 1, c->fout.p, c->red_partial.p, c->red_out.p, c->stream));
+    }
+    double r[4];
+    HIPCHK(hipMemcpyAsync(r, c->red_out.p, sizeof r, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    // exchange and dipole: -1/2 sum s.h (sparse_interaction.cc:86-100,
+    // dipole_bruteforce.cc:55-58); zeeman: -sum s.h; uniaxial: -sum K (s.e)^p
+    *energy = (kind == TK_EXCHANGE || kind == TK_DIPOLE) ? 0.5 * r[0] : r[0];
+  });
+}
+
+int jh_reduce_magnetisation(jh_ctx *c, double out[4]) {
+  return guarded(c, [&] {
+    prepare(c);
+    KParams p = make_params(c);
+    HIPCHK(launch_reduce(p, 0, nullptr, c->red_partial.p, c->red_out.p, c->stream));
+    HIPCHK(hipMemcpyAsync(out, c->red_out.p, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int jh_time_steps(jh_ctx *c, int64_t nsteps, double *ms) {
+  return guarded(c, [&] {
+    if (!c->have_dt) JH_THROW(JH_ERR_STATE, "time step not set");
+    prepare(c);
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    do_steps(c, nsteps);
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    HIPCHK(hipEventSynchronize(c->ev1));
+    float f = 0.f;
+    HIPCHK(hipEventElapsedTime(&f, c->ev0, c->ev1));
+    *ms = f;
+  });
+}
+
+int jh_time_phase(jh_ctx *c, int phase, int reps, double *ms) {
+  return guarded(c, [&] {
+    if (!c->have_dt) JH_THROW(JH_ERR_STATE, "time step not set");
+    if (reps <= 0 || (phase != 0 && phase != 1)) JH_THROW(JH_ERR_INVALID, "bad phase or reps");
+    prepare(c);
+    // The corrector overwrites s; keep a copy and restore it afterwards so the
+    // timing leaves the simulation state unchanged.
+    DevBuf<double> save;
+    DevBuf<float> save32;
+    save.alloc((size_t)(3 * c->n));
+    HIPCHK(hipMemcpyAsync(save.p, c->s.p, sizeof(double) * 3 * c->n, hipMemcpyDeviceToDevice, c->stream));
+    if (c->prec == PR_FP32) {
+      save32.alloc((size_t)(3 * c->n));
+      HIPCHK(hipMemcpyAsync(save32.p, c->s32.p, sizeof(float) * 3 * c->n, hipMemcpyDeviceToDevice, c->stream));
+    }
+    KParams p = make_params(c);
+    p.step = (uint64_t)c->iteration;
+    p.t = c->time;
+    p.g32 = phase == 0 ? c->s32.p : c->sp32.p;
+    p.w32 = phase == 0 ? c->sp32.p : c->s32.p;
+    // one untimed launch of each phase so the corrector reads a valid s*
+    {
+      KParams q = p;
+      q.g32 = c->s32.p;
+      q.w32 = c->sp32.p;
+      HIPCHK(launch_heun(q, c->prec, c->xv, 0, c->stream));
+    }
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    for (int r = 0; r < reps; ++r) HIPCHK(launch_heun(p, c->prec, c->xv, phase, c->stream));
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    HIPCHK(hipEventSynchronize(c->ev1));
+    float f = 0.f;
+    HIPCHK(hipEventElapsedTime(&f, c->ev0, c->ev1));
+    *ms = (double)f / reps;
+    HIPCHK(hipMemcpyAsync(c->s.p, save.p, sizeof(double) * 3 * c->n, hipMemcpyDeviceToDevice, c->stream));
+    if (c->prec == PR_FP32)
+      HIPCHK(hipMemcpyAsync(c->s32.p, save32.p, sizeof(float) * 3 * c->n, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int jh_phase_bytes(jh_ctx *c, int phase, double *fmt_bytes, double *csr_bytes) {
+  return guarded(c, [&] {
+    prepare(c);
+    const double n = (double)c->n;
+    const double sv = c->prec == PR_FP64 ? 8.0 : 4.0;
+    const int vpb = (c->xv == XV_BDIAG || c->xv == XV_TDIAG) ? 3 : 9;
+    double mat = 0.0, gather = 0.0;
+    if (c->xv != XV_NONE) {
+      mat = (double)c->nent * 4.0 + (double)(c->slice_off.n) * 8.0;
+      if (c->xv == XV_BDIAG || c->xv == XV_BFULL) mat += (double)c->nent * vpb * sv;
+      // every neighbour vector read once (ideal reuse of the gathers)
+      gather = 3.0 * n * sv;
+    }
+    // per-spin state traffic: the FP64 vector the field is evaluated at (its
+    // gathers are counted above: in FP64 they are the same array, so only
+    // the FP32 form adds its own FP64 read), plus the Heun state
+    double state = c->prec == PR_FP32 || c->xv == XV_NONE ? 3 * n * 8 : 0.0;
+    if (phase == 0) state += 3 * n * 8 /*s* w*/ + 3 * n * 8 /*ds w*/;
+    else state += 3 * n * 8 /*s_old r*/ + 3 * n * 8 /*ds r*/ + 3 * n * 8 /*s w*/;
+    if (c->prec == PR_FP32) state += 3 * n * 4;  // FP32 copy written
+    if (c->nclass > 1) state += 2 * n;
+    if (c->noise_mode == NM_INJECTED) state += 3 * n * 8;
+    if (dipole_term(c) >= 0) state += 3 * n * 8;
+    *fmt_bytes = mat + gather + state;
+    // reference CSR SpMV (FP64 values, int32 indices): nnz (8+4) + (3N+1) 4 + 3N 8 + 3N 8
+    const double nnz = c->xh.empty() ? 0.0 : (double)c->xh[0].csr_nnz;
+    *csr_bytes = nnz * 12.0 + (3 * n + 1) * 4.0 + 3 * n * 8.0 + 3 * n * 8.0;
+  });
+}
+
+int jh_describe(jh_ctx *c, char *buf, int64_t buflen) {
+  return guarded(c, [&] {
+    prepare(c);
+    static const char *xvn[] = {"none", "typed-diag", "typed-full", "block-diag", "block-full"};
+    char tmp[512];
+    std::snprintf(tmp, sizeof tmp,
+                  "n=%lld prec=%s exchange=%s entries=%lld tensors=%d colbits=%d classes=%d "
+                  "terms=%d noise=%d",
+                  (long long)c->n, c->prec == PR_FP64 ? "fp64" : "fp32-field", xvn[c->xv],
+                  (long long)c->nent, c->ntab > 0 ? c->ntab - 1 : 0, c->colbits, c->nclass,
+                  (int)c->terms.size(), c->noise_mode);
+    if (buf && buflen > 0) {
+      std::strncpy(buf, tmp, (size_t)buflen - 1);
+      buf[buflen - 1] = 0;
+    }
+  });
+}
+
+int jh_synchronize(jh_ctx *c) {
+  return guarded(c, [&] { HIPCHK(hipStreamSynchronize(c->stream)); });
+}
+
+}  // extern "C"

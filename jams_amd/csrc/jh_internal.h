@@ -1,0 +1,79 @@
+// jh_internal.h -- data structures shared by the engine's host code and its
+// kernels.
+//
+// Device layout (HBM):
+//  - spin-like vectors (s, predictor s*, ds, noise, fields) are SoA FP64:
+//    one allocation of 3N doubles, component c of spin i at [c*N + i];
+//  - the exchange operator is a sliced-ELL over spins: slices of 64
+//    consecutive spins (one wavefront), each padded to the longest row in the
+//    slice and stored column-major, so lane l of slot k reads entry
+//    slice_off[s] + 64 k + l (coalesced).  TYPED entries pack the neighbour
+//    column and the index of its 3x3 tensor in a table held in LDS; BLOCK
+//    entries hold the column and the explicit tensor (FP64 or FP32);
+//  - per-spin constants (mus, gyro, alpha, sigma, Zeeman and anisotropy
+//    parameters) are de-duplicated into a small table of site classes with
+//    a 16-bit class index per spin (no index at all when there is one class).
+#pragma once
+
+#include <stdint.h>
+
+namespace jh {
+
+constexpr int kBlock = 256;          // threads per workgroup (4 wave64)
+constexpr int kSlice = 64;           // spins per ELL slice (one wavefront)
+constexpr uint32_t kPad = 0xFFFFFFFFu;
+
+enum TermKind { TK_EXCHANGE = 0, TK_ZEEMAN = 1, TK_UNIAXIAL = 2, TK_DIPOLE = 3 };
+enum NoiseMode { NM_NONE = 0, NM_PHILOX = 1, NM_INJECTED = 2 };
+enum Prec { PR_FP64 = 0, PR_FP32 = 1 };
+// exchange kernel variants
+enum XVariant { XV_NONE = 0, XV_TDIAG = 1, XV_TFULL = 2, XV_BDIAG = 3, XV_BFULL = 4 };
+
+// site-class table: 16 doubles per class
+enum ClassField {
+  CF_MUS = 0, CF_GYRO = 1, CF_ALPHA = 2, CF_SIGMA = 3,
+  CF_ZDC = 4,   // 4..6
+  CF_ZAC = 7,   // 7..9
+  CF_ZOMEGA = 10, CF_K = 11,
+  CF_AXIS = 12,  // 12..14
+  CF_STRIDE = 16
+};
+
+struct XView {
+  const int64_t *slice_off;   // nslices + 1 entry offsets (multiples of 64)
+  const uint32_t *ent;        // packed entries
+  uint32_t colmask;
+  int colbits;
+  int ntab;                   // table rows incl. the trailing zero tensor
+  const double *tab64;        // TYPED: ntab x (3|9)
+  const float *tab32;
+  const double *val64;        // BLOCK: per entry (3|9) values, slice-major
+  const float *val32;
+};
+
+struct KParams {
+  int64_t n;
+  const double *s;     // current spins (phase 0 source, phase 1 s_old)
+  double *sp;          // predictor spins
+  double *ds;          // ds/dt accumulator
+  double *snew;        // phase 1 output (== s)
+  const float *g32;    // FP32 gather source (s32 for phase 0, sp32 for phase 1)
+  float *w32;          // FP32 copy written by the phase (sp32 / s32)
+  const uint16_t *cls; // per-spin class or null
+  const double *ctab;  // class table
+  XView x;
+  int nterms;
+  int kind[4];
+  const double *hext;  // precomputed field term (dipole), SoA, meV
+  int has_ac;
+  int ani_power;
+  int noise;
+  uint64_t seed;
+  uint64_t step;
+  double sqrtT;
+  const double *winj;  // injected noise SoA
+  double dt;
+  double t;
+};
+
+}  // namespace jh

@@ -1,0 +1,431 @@
+// jh_kernels.hip -- gfx950 kernels of the Heun-LLG step.
+//
+// One Heun step is two launches (solvers/cuda_llg_heun.cu:64-122 does ~10-12):
+//   phase 0 (predictor): h(s) = sum of terms in config order, w from the
+//     in-register Philox stream, rhs, ds = rhs/2, s* = unit(s + dt rhs);
+//   phase 1 (corrector): h(s*) at t+dt, same w, ds += rhs/2,
+//     s = unit(s_old + dt ds).
+// Each thread owns one spin.  The exchange field is a row-sequential sum in
+// the reference CSR order (neighbour column ascending, tensor column
+// ascending), which is what makes the FP64 path bit-identical to the CPU
+// solver; the FP32 form keeps the order and adds Dot2 compensation.
+#include "jh_device.h"
+#include "jh_internal.h"
+#include "jh_launch.h"
+
+namespace jh {
+
+// ---------------------------------------------------------------------------
+// exchange field of spin i, FP64, reference order
+template <int XVk>
+__device__ __forceinline__ void xfield64(const XView &x, const double *__restrict__ src, int64_t n,
+                                         int64_t i, const double *tab, double &h0, double &h1,
+                                         double &h2) {
+  const int64_t sl = i >> 6;
+  const int lane = (int)(i & 63);
+  const int64_t b0 = x.slice_off[sl], b1 = x.slice_off[sl + 1];
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+#pragma unroll 4
+  for (int64_t e = b0 + lane; e < b1; e += kSlice) {
+    const uint32_t w = __builtin_nontemporal_load(x.ent + e);
+    const uint32_t j = w & x.colmask;
+    const double sx = src[j], sy = src[n + j], sz = src[2 * n + j];
+    if constexpr (XVk == XV_TDIAG) {
+      const double *T = tab + 3 * (w >> x.colbits);
+      a0 = a0 + sx * T[0];
+      a1 = a1 + sy * T[1];
+      a2 = a2 + sz * T[2];
+    } else if constexpr (XVk == XV_TFULL) {
+      const double *T = tab + 9 * (w >> x.colbits);
+      a0 = a0 + sx * T[0]; a0 = a0 + sy * T[1]; a0 = a0 + sz * T[2];
+      a1 = a1 + sx * T[3]; a1 = a1 + sy * T[4]; a1 = a1 + sz * T[5];
+      a2 = a2 + sx * T[6]; a2 = a2 + sy * T[7]; a2 = a2 + sz * T[8];
+    } else if constexpr (XVk == XV_BDIAG) {
+      const double *V = x.val64 + (e - lane) * 3 + lane;
+      a0 = a0 + sx * __builtin_nontemporal_load(V);
+      a1 = a1 + sy * __builtin_nontemporal_load(V + 64);
+      a2 = a2 + sz * __builtin_nontemporal_load(V + 128);
+    } else if constexpr (XVk == XV_BFULL) {
+      const double *V = x.val64 + (e - lane) * 9 + lane;
+      double v[9];
+#pragma unroll
+      for (int c = 0; c < 9; ++c) v[c] = __builtin_nontemporal_load(V + 64 * c);
+      a0 = a0 + sx * v[0]; a0 = a0 + sy * v[1]; a0 = a0 + sz * v[2];
+      a1 = a1 + sx * v[3]; a1 = a1 + sy * v[4]; a1 = a1 + sz * v[5];
+      a2 = a2 + sx * v[6]; a2 = a2 + sy * v[7]; a2 = a2 + sz * v[8];
+    }
+  }
+  h0 = a0; h1 = a1; h2 = a2;
+}
+
+// exchange field of spin i, FP32 inputs, Dot2-compensated row sums
+template <int XVk>
+__device__ __forceinline__ void xfield32(const XView &x, const float *__restrict__ src, int64_t n,
+                                         int64_t i, const float *tab, double &h0, double &h1,
+                                         double &h2) {
+  const int64_t sl = i >> 6;
+  const int lane = (int)(i & 63);
+  const int64_t b0 = x.slice_off[sl], b1 = x.slice_off[sl + 1];
+  float a0 = 0.f, a1 = 0.f, a2 = 0.f, c0 = 0.f, c1 = 0.f, c2 = 0.f;
+#pragma unroll 4
+  for (int64_t e = b0 + lane; e < b1; e += kSlice) {
+    const uint32_t w = __builtin_nontemporal_load(x.ent + e);
+    const uint32_t j = w & x.colmask;
+    const float sx = src[j], sy = src[n + j], sz = src[2 * n + j];
+    if constexpr (XVk == XV_TDIAG) {
+      const float *T = tab + 3 * (w >> x.colbits);
+      dot2_acc(a0, c0, sx, T[0]);
+      dot2_acc(a1, c1, sy, T[1]);
+      dot2_acc(a2, c2, sz, T[2]);
+    } else if constexpr (XVk == XV_TFULL) {
+      const float *T = tab + 9 * (w >> x.colbits);
+      dot2_acc(a0, c0, sx, T[0]); dot2_acc(a0, c0, sy, T[1]); dot2_acc(a0, c0, sz, T[2]);
+      dot2_acc(a1, c1, sx, T[3]); dot2_acc(a1, c1, sy, T[4]); dot2_acc(a1, c1, sz, T[5]);
+      dot2_acc(a2, c2, sx, T[6]); dot2_acc(a2, c2, sy, T[7]); dot2_acc(a2, c2, sz, T[8]);
+    } else if constexpr (XVk == XV_BDIAG) {
+      const float *V = x.val32 + (e - lane) * 3 + lane;
+      dot2_acc(a0, c0, sx, __builtin_nontemporal_load(V));
+      dot2_acc(a1, c1, sy, __builtin_nontemporal_load(V + 64));
+      dot2_acc(a2, c2, sz, __builtin_nontemporal_load(V + 128));
+    } else if constexpr (XVk == XV_BFULL) {
+      const float *V = x.val32 + (e - lane) * 9 + lane;
+      float v[9];
+#pragma unroll
+      for (int c = 0; c < 9; ++c) v[c] = __builtin_nontemporal_load(V + 64 * c);
+      dot2_acc(a0, c0, sx, v[0]); dot2_acc(a0, c0, sy, v[1]); dot2_acc(a0, c0, sz, v[2]);
+      dot2_acc(a1, c1, sx, v[3]); dot2_acc(a1, c1, sy, v[4]); dot2_acc(a1, c1, sz, v[5]);
+      dot2_acc(a2, c2, sx, v[6]); dot2_acc(a2, c2, sy, v[7]); dot2_acc(a2, c2, sz, v[8]);
+    }
+  }
+  h0 = (double)a0 + (double)c0;
+  h1 = (double)a1 + (double)c1;
+  h2 = (double)a2 + (double)c2;
+}
+
+template <int PREC, int XVk>
+__device__ __forceinline__ void load_table(const XView &x, unsigned char *lds) {
+  if constexpr (XVk == XV_TDIAG || XVk == XV_TFULL) {
+    const int cnt = x.ntab * (XVk == XV_TDIAG ? 3 : 9);
+    if constexpr (PREC == PR_FP64) {
+      double *t = reinterpret_cast<double *>(lds);
+      for (int q = threadIdx.x; q < cnt; q += blockDim.x) t[q] = x.tab64[q];
+    } else {
+      float *t = reinterpret_cast<float *>(lds);
+      for (int q = threadIdx.x; q < cnt; q += blockDim.x) t[q] = x.tab32[q];
+    }
+  }
+}
+
+template <int PREC, int XVk>
+__device__ __forceinline__ void exchange_at(const KParams &p, const double *src64,
+                                            const float *src32, int64_t i,
+                                            const unsigned char *lds, double &f0, double &f1,
+                                            double &f2) {
+  if constexpr (XVk == XV_NONE) {
+    f0 = f1 = f2 = 0.0;
+  } else if constexpr (PREC == PR_FP64) {
+    xfield64<XVk>(p.x, src64, p.n, i, reinterpret_cast<const double *>(lds), f0, f1, f2);
+  } else {
+    xfield32<XVk>(p.x, src32, p.n, i, reinterpret_cast<const float *>(lds), f0, f1, f2);
+  }
+}
+
+__device__ __forceinline__ double cparam(const KParams &p, uint32_t c, int f) {
+  return p.ctab[c * CF_STRIDE + f];
+}
+
+// Local (non-exchange) term q of spin i with spin vector (s0, s1, s2).
+__device__ __forceinline__ void local_term(const KParams &p, int kind, uint32_t c, int64_t i,
+                                           double s0, double s1, double s2, double t, double &f0,
+                                           double &f1, double &f2) {
+  if (kind == TK_ZEEMAN) {
+    // zeeman.cc:121-132: field = dc; field += ac * cos(omega t)
+    f0 = cparam(p, c, CF_ZDC + 0);
+    f1 = cparam(p, c, CF_ZDC + 1);
+    f2 = cparam(p, c, CF_ZDC + 2);
+    if (p.has_ac) {
+      const double cw = cos(cparam(p, c, CF_ZOMEGA) * t);
+      f0 = f0 + cparam(p, c, CF_ZAC + 0) * cw;
+      f1 = f1 + cparam(p, c, CF_ZAC + 1) * cw;
+      f2 = f2 + cparam(p, c, CF_ZAC + 2) * cw;
+    }
+  } else if (kind == TK_UNIAXIAL) {
+    // uniaxial_anisotropy.cc:155-163: K p (s.e)^(p-1) e
+    const double e0 = cparam(p, c, CF_AXIS + 0), e1 = cparam(p, c, CF_AXIS + 1),
+                 e2 = cparam(p, c, CF_AXIS + 2);
+    const double d = (e0 * s0 + e1 * s1 + e2 * s2);
+    const double pw = p.ani_power == 2 ? d : pow(d, (double)(p.ani_power - 1));
+    const double a = cparam(p, c, CF_K) * (double)p.ani_power * pw;
+    f0 = a * e0;
+    f1 = a * e1;
+    f2 = a * e2;
+  } else {  // TK_DIPOLE: precomputed field array
+    f0 = p.hext[i];
+    f1 = p.hext[p.n + i];
+    f2 = p.hext[2 * p.n + i];
+  }
+}
+
+// Total field (meV) of spin i, terms summed in config order
+// (core/solver.cc:43-57: h = f0; h = h + 1.0 * f_k).
+template <int PREC, int XVk>
+__device__ __forceinline__ void total_field(const KParams &p, const double *src64,
+                                            const float *src32, int64_t i, uint32_t c,
+                                            const unsigned char *lds, double s0, double s1,
+                                            double s2, double t, double &h0, double &h1,
+                                            double &h2) {
+  double x0, x1, x2;
+  exchange_at<PREC, XVk>(p, src64, src32, i, lds, x0, x1, x2);
+  for (int q = 0; q < p.nterms; ++q) {
+    double f0, f1, f2;
+    const int k = p.kind[q];
+    if (k == TK_EXCHANGE) {
+      f0 = x0; f1 = x1; f2 = x2;
+    } else {
+      local_term(p, k, c, i, s0, s1, s2, t, f0, f1, f2);
+    }
+    if (q == 0) {
+      h0 = f0; h1 = f1; h2 = f2;
+    } else {
+      h0 = h0 + f0; h1 = h1 + f1; h2 = h2 + f2;
+    }
+  }
+  if (p.nterms == 0) { h0 = h1 = h2 = 0.0; }
+}
+
+template <int PREC, int XVk, int PHASE>
+__global__ void __launch_bounds__(kBlock) k_heun(const KParams p) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  load_table<PREC, XVk>(p.x, lds);
+  if constexpr (XVk == XV_TDIAG || XVk == XV_TFULL) __syncthreads();
+  const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t i = blk * kBlock + threadIdx.x;
+  if (i >= p.n) return;
+  const int64_t n = p.n;
+  const uint32_t c = p.cls ? (uint32_t)p.cls[i] : 0u;
+
+  // spin that the field is evaluated at
+  const double *src = PHASE == 0 ? p.s : p.sp;
+  const double s0 = src[i], s1 = src[n + i], s2 = src[2 * n + i];
+  const double t = PHASE == 0 ? p.t : p.t + p.dt;
+
+  double h0, h1, h2;
+  total_field<PREC, XVk>(p, src, p.g32, i, c, lds, s0, s1, s2, t, h0, h1, h2);
+
+  // h = w + h / mus  (solvers/cpu_llg_heun.cc:68-81)
+  const double mus = cparam(p, c, CF_MUS);
+  if (p.noise == NM_NONE) {
+    h0 = h0 / mus; h1 = h1 / mus; h2 = h2 / mus;
+  } else {
+    double w0, w1, w2;
+    if (p.noise == NM_PHILOX) {
+      double z0, z1, z2;
+      normals3(p.seed, p.step, (uint64_t)i, z0, z1, z2);
+      const double sg = cparam(p, c, CF_SIGMA);
+      w0 = z0 * sg * p.sqrtT; w1 = z1 * sg * p.sqrtT; w2 = z2 * sg * p.sqrtT;
+    } else {
+      w0 = p.winj[i]; w1 = p.winj[n + i]; w2 = p.winj[2 * n + i];
+    }
+    h0 = w0 + h0 / mus; h1 = w1 + h1 / mus; h2 = w2 + h2 / mus;
+  }
+
+  double r0, r1, r2;
+  llg_rhs(s0, s1, s2, h0, h1, h2, cparam(p, c, CF_GYRO), cparam(p, c, CF_ALPHA), r0, r1, r2);
+
+  if constexpr (PHASE == 0) {
+    p.ds[i] = 0.5 * r0; p.ds[n + i] = 0.5 * r1; p.ds[2 * n + i] = 0.5 * r2;
+    double v0 = s0 + p.dt * r0, v1 = s1 + p.dt * r1, v2 = s2 + p.dt * r2;
+    unit_vector(v0, v1, v2);
+    p.sp[i] = v0; p.sp[n + i] = v1; p.sp[2 * n + i] = v2;
+    if constexpr (PREC == PR_FP32) {
+      p.w32[i] = (float)v0; p.w32[n + i] = (float)v1; p.w32[2 * n + i] = (float)v2;
+    }
+  } else {
+    const double d0 = p.ds[i] + 0.5 * r0;
+    const double d1 = p.ds[n + i] + 0.5 * r1;
+    const double d2 = p.ds[2 * n + i] + 0.5 * r2;
+    double v0 = p.s[i] + p.dt * d0, v1 = p.s[n + i] + p.dt * d1, v2 = p.s[2 * n + i] + p.dt * d2;
+    unit_vector(v0, v1, v2);
+    p.snew[i] = v0; p.snew[n + i] = v1; p.snew[2 * n + i] = v2;
+    if constexpr (PREC == PR_FP32) {
+      p.w32[i] = (float)v0; p.w32[n + i] = (float)v1; p.w32[2 * n + i] = (float)v2;
+    }
+  }
+}
+
+// Field of one term (term >= 0) or of the ordered sum (term < 0) at spins p.s,
+// time p.t, into out (SoA).
+template <int PREC, int XVk>
+__global__ void __launch_bounds__(kBlock) k_field(const KParams p, int term, double *out) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  load_table<PREC, XVk>(p.x, lds);
+  if constexpr (XVk == XV_TDIAG || XVk == XV_TFULL) __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= p.n) return;
+  const int64_t n = p.n;
+  const uint32_t c = p.cls ? (uint32_t)p.cls[i] : 0u;
+  const double s0 = p.s[i], s1 = p.s[n + i], s2 = p.s[2 * n + i];
+  double h0, h1, h2;
+  if (term < 0) {
+    total_field<PREC, XVk>(p, p.s, p.g32, i, c, lds, s0, s1, s2, p.t, h0, h1, h2);
+  } else if (p.kind[term] == TK_EXCHANGE) {
+    exchange_at<PREC, XVk>(p, p.s, p.g32, i, lds, h0, h1, h2);
+  } else {
+    local_term(p, p.kind[term], c, i, s0, s1, s2, p.t, h0, h1, h2);
+  }
+  out[i] = h0; out[n + i] = h1; out[2 * n + i] = h2;
+}
+
+// Generated, scaled noise of one step (for inspection and parity tests).
+__global__ void __launch_bounds__(kBlock) k_noise(const KParams p, double *out) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= p.n) return;
+  const uint32_t c = p.cls ? (uint32_t)p.cls[i] : 0u;
+  double z0, z1, z2;
+  normals3(p.seed, p.step, (uint64_t)i, z0, z1, z2);
+  const double sg = cparam(p, c, CF_SIGMA);
+  out[i] = z0 * sg * p.sqrtT;
+  out[p.n + i] = z1 * sg * p.sqrtT;
+  out[2 * p.n + i] = z2 * sg * p.sqrtT;
+}
+
+__global__ void __launch_bounds__(kBlock) k_to_f32(const double *a, float *b, int64_t m) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < m) b[i] = (float)a[i];
+}
+
+// Deterministic pairwise block reduction of 4 FP64 quantities per spin:
+//   mode 0: (mus s_x, mus s_y, mus s_z, mus)
+//   mode 1: (-s.h, 0, 0, 0) with h = field array (energy density)
+//   mode 2: (-K (s.e)^p, 0, 0, 0) (uniaxial energy)
+__global__ void __launch_bounds__(kBlock) k_reduce(const KParams p, int mode, const double *h,
+                                                   double *partial) {
+  __shared__ double red[4][kBlock];
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  if (i < p.n) {
+    const int64_t n = p.n;
+    const uint32_t c = p.cls ? (uint32_t)p.cls[i] : 0u;
+    const double s0 = p.s[i], s1 = p.s[n + i], s2 = p.s[2 * n + i];
+    if (mode == 0) {
+      const double mu = cparam(p, c, CF_MUS);
+      v[0] = mu * s0; v[1] = mu * s1; v[2] = mu * s2; v[3] = mu;
+    } else if (mode == 1) {
+      v[0] = -(s0 * h[i] + s1 * h[n + i] + s2 * h[2 * n + i]);
+    } else {
+      const double d = (cparam(p, c, CF_AXIS) * s0 + cparam(p, c, CF_AXIS + 1) * s1 +
+                        cparam(p, c, CF_AXIS + 2) * s2);
+      v[0] = (-cparam(p, c, CF_K) * pow(d, (double)p.ani_power));
+    }
+  }
+  for (int q = 0; q < 4; ++q) red[q][threadIdx.x] = v[q];
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int q = 0; q < 4; ++q) red[q][threadIdx.x] = red[q][threadIdx.x] + red[q][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) partial[(int64_t)blockIdx.x * 4 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// Final pass: pairwise sum of nb partial 4-vectors in one workgroup.
+__global__ void __launch_bounds__(kBlock) k_reduce_final(const double *partial, int64_t nb,
+                                                         double *out) {
+  __shared__ double red[4][kBlock];
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int64_t b = threadIdx.x; b < nb; b += kBlock)
+    for (int q = 0; q < 4; ++q) v[q] = v[q] + partial[b * 4 + q];
+  for (int q = 0; q < 4; ++q) red[q][threadIdx.x] = v[q];
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int q = 0; q < 4; ++q) red[q][threadIdx.x] = red[q][threadIdx.x] + red[q][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) out[threadIdx.x] = red[threadIdx.x][0];
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+
+static size_t table_lds(int prec, int xv, int ntab) {
+  if (xv != XV_TDIAG && xv != XV_TFULL) return 0;
+  const size_t w = (xv == XV_TDIAG ? 3 : 9);
+  return (size_t)ntab * w * (prec == PR_FP64 ? 8 : 4);
+}
+
+template <int PREC, int XVk>
+static hipError_t launch_heun_t(const KParams &p, int phase, hipStream_t st) {
+  const int64_t nb = (p.n + kBlock - 1) / kBlock;
+  const size_t lds = table_lds(PREC, XVk, p.x.ntab);
+  if (phase == 0)
+    hipLaunchKernelGGL((k_heun<PREC, XVk, 0>), dim3((unsigned)nb), dim3(kBlock), lds, st, p);
+  else
+    hipLaunchKernelGGL((k_heun<PREC, XVk, 1>), dim3((unsigned)nb), dim3(kBlock), lds, st, p);
+  return hipGetLastError();
+}
+
+template <int PREC>
+static hipError_t launch_heun_p(const KParams &p, int xv, int phase, hipStream_t st) {
+  switch (xv) {
+    case XV_NONE: return launch_heun_t<PREC, XV_NONE>(p, phase, st);
+    case XV_TDIAG: return launch_heun_t<PREC, XV_TDIAG>(p, phase, st);
+    case XV_TFULL: return launch_heun_t<PREC, XV_TFULL>(p, phase, st);
+    case XV_BDIAG: return launch_heun_t<PREC, XV_BDIAG>(p, phase, st);
+    case XV_BFULL: return launch_heun_t<PREC, XV_BFULL>(p, phase, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_heun(const KParams &p, int prec, int xv, int phase, hipStream_t st) {
+  return prec == PR_FP64 ? launch_heun_p<PR_FP64>(p, xv, phase, st)
+                         : launch_heun_p<PR_FP32>(p, xv, phase, st);
+}
+
+template <int PREC, int XVk>
+static hipError_t launch_field_t(const KParams &p, int term, double *out, hipStream_t st) {
+  const int64_t nb = (p.n + kBlock - 1) / kBlock;
+  const size_t lds = table_lds(PREC, XVk, p.x.ntab);
+  hipLaunchKernelGGL((k_field<PREC, XVk>), dim3((unsigned)nb), dim3(kBlock), lds, st, p, term, out);
+  return hipGetLastError();
+}
+
+template <int PREC>
+static hipError_t launch_field_p(const KParams &p, int xv, int term, double *out, hipStream_t st) {
+  switch (xv) {
+    case XV_NONE: return launch_field_t<PREC, XV_NONE>(p, term, out, st);
+    case XV_TDIAG: return launch_field_t<PREC, XV_TDIAG>(p, term, out, st);
+    case XV_TFULL: return launch_field_t<PREC, XV_TFULL>(p, term, out, st);
+    case XV_BDIAG: return launch_field_t<PREC, XV_BDIAG>(p, term, out, st);
+    case XV_BFULL: return launch_field_t<PREC, XV_BFULL>(p, term, out, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_field(const KParams &p, int prec, int xv, int term, double *out,
+                        hipStream_t st) {
+  return prec == PR_FP64 ? launch_field_p<PR_FP64>(p, xv, term, out, st)
+                         : launch_field_p<PR_FP32>(p, xv, term, out, st);
+}
+
+hipError_t launch_noise(const KParams &p, double *out, hipStream_t st) {
+  const int64_t nb = (p.n + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(k_noise, dim3((unsigned)nb), dim3(kBlock), 0, st, p, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_to_f32(const double *a, float *b, int64_t m, hipStream_t st) {
+  const int64_t nb = (m + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(k_to_f32, dim3((unsigned)nb), dim3(kBlock), 0, st, a, b, m);
+  return hipGetLastError();
+}
+
+hipError_t launch_reduce(const KParams &p, int mode, const double *h, double *partial,
+                         double *out, hipStream_t st) {
+  const int64_t nb = (p.n + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(k_reduce, dim3((unsigned)nb), dim3(kBlock), 0, st, p, mode, h, partial);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kBlock), 0, st, (const double *)partial, nb, out);
+  return hipGetLastError();
+}
+
+}  // namespace jh

@@ -1,0 +1,18 @@
+// jh_launch.h -- host-callable launchers of the step kernels (jh_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "jh_internal.h"
+
+namespace jh {
+
+hipError_t launch_heun(const KParams &p, int prec, int xv, int phase, hipStream_t st);
+hipError_t launch_field(const KParams &p, int prec, int xv, int term, double *out,
+                        hipStream_t st);
+hipError_t launch_noise(const KParams &p, double *out, hipStream_t st);
+hipError_t launch_to_f32(const double *a, float *b, int64_t m, hipStream_t st);
+hipError_t launch_reduce(const KParams &p, int mode, const double *h, double *partial,
+                         double *out, hipStream_t st);
+
+}  // namespace jh

@@ -1,0 +1,204 @@
+"""Seeded synthetic systems of the benchmark configurations, built through the
+oracle's reference-faithful lattice/neighbour/CSR restatement.
+
+Unit conventions follow the reference (helpers/consts.h:18-38): mus in meV/T
+(moment x kBohrMagnetonIU), gyro in rad/(ps T) (gyro x kGyromagneticRatioIU),
+energies in meV (energy_units = "meV"), time step t_step / 1e-12 ps.
+"""
+from __future__ import annotations
+
+import itertools
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from oracle import pyoracle as po
+
+BOHR = 0.0578838181
+KB = 0.0861733326
+SC_CELL = np.eye(3)
+
+
+def gyro_e() -> float:
+    return 2.0023193043625 * 0.0578838181 / 0.6582119569
+
+
+def dt_ps(t_step_s: float) -> float:
+    return t_step_s / 1e-12
+
+
+@dataclass
+class Built:
+    lat: po.LatticeSpec
+    n: int
+    mus: np.ndarray
+    gyro: np.ndarray
+    alpha: np.ndarray
+    pairs: tuple = None            # (pi, pj, pJ) neighbour list
+    csr: tuple = None              # (rowptr, col, val)
+    zeeman_dc: np.ndarray = None
+    ani_K: np.ndarray = None
+    ani_axis: np.ndarray = None
+    order: list = field(default_factory=list)   # term kinds in config order
+    dt: float = 1e-3
+    temperature: float = 0.0
+    s0: np.ndarray = None
+
+    def oracle(self, nthreads=1) -> po.OracleSystem:
+        o = po.OracleSystem(self.mus, self.gyro, self.alpha, nthreads=nthreads)
+        for k in self.order:
+            if k == "exchange":
+                o.add_exchange(*self.csr)
+            elif k == "zeeman":
+                o.add_zeeman(self.zeeman_dc)
+            elif k == "uniaxial":
+                o.add_uniaxial(self.ani_K, self.ani_axis, 2)
+        return o
+
+    def engine(self, make, prec=0, fmt=0, use_list=False):
+        e = make()
+        e.set_sites(self.mus, self.gyro, self.alpha)
+        for k in self.order:
+            if k == "exchange":
+                if use_list:
+                    pi, pj, pJ = self.pairs
+                    e.add_exchange_list(pi, pj, np.arange(pi.size, dtype=np.int32), pJ)
+                else:
+                    e.add_exchange_csr(*self.csr)
+            elif k == "zeeman":
+                e.add_zeeman(self.zeeman_dc)
+            elif k == "uniaxial":
+                e.add_uniaxial(self.ani_K, self.ani_axis, 2)
+        e.set_time_step(self.dt)
+        e.set_precision(prec)
+        e.set_exchange_format(fmt)
+        e.set_spins(self.s0)
+        return e
+
+
+def sc_nn_rows(J):
+    return [(0, 0, v, J) for v in ([1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1])]
+
+
+def bcc_lattice(L, periodic=(1, 1, 1)):
+    return po.LatticeSpec(SC_CELL, [[0, 0, 0], [0.5, 0.5, 0.5]], [0, 0], (L, L, L), periodic)
+
+
+def bcc_two_shell_rows(J1, J2):
+    rows = [(0, 0, [sx * 0.5, sy * 0.5, sz * 0.5], J1)
+            for sx, sy, sz in itertools.product((1, -1), repeat=3)]
+    rows += sc_nn_rows(J2)
+    return rows
+
+
+def bcc_shell_vectors():
+    """The first five bcc shells (8, 6, 12, 24, 8 = 58 vectors), lattice
+    constants, at r/a = sqrt(3)/2, 1, sqrt(2), sqrt(11)/2, sqrt(3)."""
+    vecs = []
+    for sx, sy, sz in itertools.product((1, -1), repeat=3):
+        vecs.append((0.5 * sx, 0.5 * sy, 0.5 * sz))
+    for ax in range(3):
+        for sg in (1, -1):
+            v = [0, 0, 0]; v[ax] = sg; vecs.append(tuple(float(x) for x in v))
+    for a, b in ((0, 1), (0, 2), (1, 2)):
+        for sa, sb in itertools.product((1, -1), repeat=2):
+            v = [0, 0, 0]; v[a] = sa; v[b] = sb; vecs.append(tuple(float(x) for x in v))
+    for ax in range(3):
+        for s0, s1, s2 in itertools.product((1, -1), repeat=3):
+            v = [0.5 * s1, 0.5 * s2, 0.5 * s2]
+            v = [0.0, 0.0, 0.0]
+            others = [q for q in range(3) if q != ax]
+            v[ax] = 1.5 * s0
+            v[others[0]] = 0.5 * s1
+            v[others[1]] = 0.5 * s2
+            vecs.append(tuple(v))
+    for sx, sy, sz in itertools.product((1, -1), repeat=3):
+        vecs.append((float(sx), float(sy), float(sz)))
+    assert len(vecs) == 58 and len(set(vecs)) == 58
+    return vecs
+
+
+def dmi_matrix(D):
+    """[D]_x with s_i . [D]_x s_j = D . (s_i x s_j)."""
+    Dx, Dy, Dz = D
+    return np.array([[0.0, Dz, -Dy], [-Dz, 0.0, Dx], [Dy, -Dx, 0.0]])
+
+
+def bcc_five_shell_dmi_rows(seed=7, J0=3.0, dmi=0.1):
+    """J_n = J0 exp(-r/a) meV on each shell plus D = dmi J_n d with a seeded
+    random unit d per bond pair, d(-r) = -d(r) so J(-r) = J(r)^T exactly."""
+    rows = []
+    vecs = bcc_shell_vectors()
+    idx = 0
+    chosen = {}
+    for v in vecs:
+        key = tuple(-x for x in v)
+        if key in chosen:
+            d = -chosen[key]
+        else:
+            u1 = po.lib().or_setup_uniform(seed, 7, 2 * idx)
+            u2 = po.lib().or_setup_uniform(seed, 7, 2 * idx + 1)
+            idx += 1
+            z = 2.0 * u1 - 1.0
+            phi = 2.0 * math.pi * u2
+            rr = math.sqrt(max(0.0, 1.0 - z * z))
+            d = np.array([rr * math.cos(phi), rr * math.sin(phi), z])
+            chosen[v] = d
+        r = math.sqrt(sum(x * x for x in v))
+        Jn = J0 * math.exp(-r)
+        T = Jn * np.eye(3) + dmi_matrix(dmi * Jn * d)
+        rows.append((0, 0, list(v), T.reshape(9)))
+    return rows
+
+
+def build(lat, rows, moment, alpha, order, zeeman_B=None, K=None, dt=1e-3, spins="random",
+          seed=1, temperature=0.0, with_csr=True):
+    n = lat.n
+    mus = np.full(n, moment * BOHR)
+    gyro = np.full(n, gyro_e())
+    alph = np.full(n, alpha)
+    b = Built(lat=lat, n=n, mus=mus, gyro=gyro, alpha=alph, order=list(order), dt=dt,
+              temperature=temperature)
+    if rows:
+        b.pairs = po.neighbour_list(lat, rows)
+        if with_csr:
+            b.csr = po.exchange_csr(n, *b.pairs)
+    if zeeman_B is not None:
+        b.zeeman_dc = mus[:, None] * np.asarray(zeeman_B, np.float64)[None, :]
+        # applied_field.cc:146-148 computes mus(i) * field[j]
+        b.zeeman_dc = np.ascontiguousarray(b.zeeman_dc)
+    if K is not None:
+        b.ani_K = np.full(n, K)
+        b.ani_axis = np.tile(np.array([0.0, 0.0, 1.0]), (n, 1))
+    if isinstance(spins, str) and spins == "random":
+        b.s0 = po.random_spins(seed, n)
+    elif isinstance(spins, str) and spins == "+z":
+        b.s0 = np.tile(np.array([0.0, 0.0, 1.0]), (n, 1))
+    else:
+        b.s0 = np.ascontiguousarray(spins, np.float64)
+    return b
+
+
+def c1(L=16):
+    """SC L^3 periodic, mu_s = 2 muB, J = 1 meV (6 NN), B = (0,0,1) T,
+    alpha = 0.1, dt = 1 fs, T = 0, random spins from seed 1."""
+    lat = po.LatticeSpec(SC_CELL, [[0, 0, 0]], [0], (L, L, L), (1, 1, 1))
+    return build(lat, sc_nn_rows(1.0), 2.0, 0.1, ["exchange", "zeeman"], zeeman_B=[0, 0, 1],
+                 dt=dt_ps(1.0e-15), spins="random", seed=1)
+
+
+def c2(L=64, with_csr=True):
+    """bcc L^3 x 2, J1 = 3 meV (8), J2 = 1.5 meV (6), K1 = 0.05 meV along z,
+    alpha = 0.05, dt = 0.5 fs, T = 300 K, spins along +z (mu_s = 2 muB)."""
+    return build(bcc_lattice(L), bcc_two_shell_rows(3.0, 1.5), 2.0, 0.05,
+                 ["exchange", "uniaxial"], K=0.05, dt=dt_ps(0.5e-15), spins="+z",
+                 temperature=300.0, with_csr=with_csr)
+
+
+def c3(L=128, with_csr=False):
+    """bcc L^3 x 2, five shells (58 neighbours), J_n = 3 exp(-r/a) meV plus
+    DMI 0.1 J_n d; alpha = 0.05 (unstated, recorded), T = 100 K, dt = 0.5 fs."""
+    return build(bcc_lattice(L), bcc_five_shell_dmi_rows(), 2.0, 0.05, ["exchange"],
+                 dt=dt_ps(0.5e-15), spins="random", seed=3, temperature=100.0,
+                 with_csr=with_csr)
