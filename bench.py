@@ -1,0 +1,166 @@
+"""Benchmark of the gfx950 Heun-LLG step (one JSON line on stdout).
+
+Workload (default): bcc 64^3 x 2 = 524,288 spins, exchange to the second
+shell (8 x 3.0 meV + 6 x 1.5 meV), uniaxial K1 = 0.05 meV, alpha = 0.05,
+dt = 0.5 fs, Langevin thermostat at 300 K from the Philox stream, FP64.
+It is set up through the JAMS-compatible host layer from config text, the
+way a JAMS user selects the solver, and runs on one GPU.
+
+A "step" is one full Heun step (predictor + corrector, both field
+evaluations, in-kernel noise).  value = spins x steps / event time of the
+step loop, inputs resident in HBM.
+
+  python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-fp32]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "LLG spin-updates/s, FP64 vs FP32-field mixed at stated error; % HBM peak"
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def measure(sim_text: str, steps: int, warmup: int, reps: int = 200):
+    from jams_amd.host import Simulation
+
+    t0 = time.perf_counter()
+    sim = Simulation(sim_text, device=0, output_dir="/tmp")
+    eng = sim.engine()
+    setup_s = time.perf_counter() - t0
+    log(f"  set-up {setup_s:.1f} s: {eng.describe()}")
+    eng.step(warmup)
+    eng.synchronize()
+    w0 = time.perf_counter()
+    ms = eng.time_steps(steps)
+    wall = time.perf_counter() - w0
+    phases = []
+    for ph in (0, 1):
+        t_ms = eng.time_phase(ph, reps)
+        fmt_b, csr_b = eng.phase_bytes(ph)
+        phases.append({"phase": ("predictor", "corrector")[ph], "ms": t_ms, "bytes": fmt_b,
+                       "gbs": fmt_b / (t_ms * 1e-3) / 1e9,
+                       "csr_equiv_gbs": csr_b / (t_ms * 1e-3) / 1e9})
+    n = sim.n
+    out = {"n": n, "ms": ms, "wall_s": wall, "setup_s": setup_s, "phases": phases,
+           "value": n * steps / (ms * 1e-3), "describe": eng.describe()}
+    sim.close()
+    return out
+
+
+def cpu_baseline(L: int, budget_s: float = 15.0):
+    """The oracle (C restatement of llg-heun-cpu) on the same C2 system, timed
+    on this host's cores over a bounded number of steps."""
+    import numpy as np
+
+    from oracle import pyoracle as po
+    from tests import systems as S
+
+    nthreads = max(1, min(16, os.cpu_count() or 1))
+    b = S.c2(L)
+    o = b.oracle(nthreads=nthreads)
+    sigma = po.thermostat_sigma(b.mus, b.gyro, b.alpha, b.dt)
+    s = b.s0.copy()
+    t0 = time.perf_counter()
+    k = 0
+    while True:
+        w = po.scale_noise(po.normals(1, k, b.n), sigma, b.temperature)
+        o.heun_step(s, b.dt, k * b.dt, w)
+        k += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or k >= 10000:
+            break
+    return {"value": b.n * k / el, "unit": "spin-updates/s", "cores": nthreads, "kind": "port",
+            "sample": f"bcc {L}^3 x 2 ({b.n} spins), {k} Heun steps at 300 K with serial "
+                      f"noise generation (as solvers/cpu_llg_heun.cc:55), {el:.1f} s"}
+
+
+def read_traffic(name: str):
+    p = ROOT / "profiles" / name
+    if not p.exists():
+        return None
+    try:
+        return json.loads(p.read_text()).get("hbm_bytes_per_launch")
+    except Exception:
+        return None
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--L", type=int, default=64)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(the reference is single-GPU; sharding is a design note only).", file=sys.stderr)
+        return 2
+
+    from jams_amd import configs
+
+    log(f"C2 FP64: bcc {args.L}^3 x 2, {args.steps} timed steps after {args.warmup} warm-up")
+    fp64 = measure(configs.c2(args.L), args.steps, args.warmup)
+    log(f"  {fp64['value']:.4e} spin-updates/s, {fp64['ms'] / args.steps * 1e3:.2f} us/step")
+    fp32 = None
+    if not args.no_fp32:
+        log("C2 FP32-field")
+        fp32 = measure(configs.c2(args.L, precision="fp32-field"), args.steps, args.warmup)
+        log(f"  {fp32['value']:.4e} spin-updates/s")
+    cpu = None
+    if not args.no_cpu:
+        log("CPU baseline (oracle, same system)")
+        cpu = cpu_baseline(args.L)
+        log(f"  {cpu['value']:.4e} spin-updates/s on {cpu['cores']} threads")
+
+    dom = max(fp64["phases"], key=lambda p: p["ms"])
+    traffic = read_traffic("traffic_c2_fp64.json")
+    line = {
+        "metric": METRIC,
+        "value": fp64["value"],
+        "unit": "spin-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": fp64["ms"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"C2: bcc {args.L}^3x2 ({fp64['n']} spins), J1=3.0 meV x8 + "
+                               f"J2=1.5 meV x6, K1=0.05 meV, alpha=0.05, dt=0.5 fs, T=300 K "
+                               f"(Philox noise), FP64 Heun",
+                   "spins": fp64["n"], "engine": fp64["describe"]},
+        "roofline": {"bound": "hbm", "kernel": f"k_heun {dom['phase']}",
+                     "achieved": dom["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": dom["gbs"] / HBM_PEAK_GBS, "traffic": traffic,
+                     "algorithmic_bytes_per_launch": dom["bytes"],
+                     "launch_ms": dom["ms"],
+                     "csr_equivalent_gbs": dom["csr_equiv_gbs"]},
+        "phases": fp64["phases"],
+        "cpu_baseline": cpu,
+    }
+    if fp32 is not None:
+        line["fp32_field"] = {"value": fp32["value"], "ms_per_step": fp32["ms"] / args.steps,
+                              "speedup_vs_fp64": fp32["value"] / fp64["value"],
+                              "phases": fp32["phases"]}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

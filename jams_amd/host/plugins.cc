@@ -1,0 +1,427 @@
+// plugins.cc -- see plugins.h.
+#include "plugins.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <iomanip>
+#include <iostream>
+#include <map>
+
+namespace jams {
+
+void engine_check(jh_ctx *ctx, int rc) {
+  if (rc != JH_OK) throw RuntimeError(std::string("engine: ") + jh_last_error(ctx));
+}
+
+namespace {
+
+// core/units.h:15-26
+double energy_unit(const std::string &name) {
+  static const std::map<std::string, double> units = {
+      {"joules", kJoule2meV}, {"J", kJoule2meV}, {"milli_electron_volts", 1.0}, {"meV", 1.0},
+      {"milli_rydbergs", kmRyd2meV}, {"mRyd", kmRyd2meV}, {"rydbergs", kmRyd2meV * 1e3},
+      {"Ryd", kmRyd2meV * 1e3}, {"Kelvin", kBoltzmannIU}, {"K", kBoltzmannIU}};
+  auto it = units.find(name);
+  if (it == units.end()) throw RuntimeError("energy units: " + name + " is not known");
+  return it->second;
+}
+
+double max_abs(const Mat3 &a) {
+  double m = 0.0;
+  for (auto &r : a)
+    for (double v : r) m = std::max(m, std::abs(v));
+  return m;
+}
+
+Vec3 vec3(const Setting &s) {
+  if (!s.is_aggregate() || s.length() != 3) throw ConfigError("expected a 3-vector at " + s.path());
+  return {s[0].as_double(), s[1].as_double(), s[2].as_double()};
+}
+
+}  // namespace
+
+// ---- Hamiltonian --------------------------------------------------------------
+
+Hamiltonian::Hamiltonian(const Setting &s, const Lattice &lat) : lat_(lat) {
+  // core/hamiltonian.cc:116-157
+  std::string unit = s.get<std::string>("energy_units", "joules");
+  if (s.exists("unit_name")) unit = s["unit_name"].as_string();
+  energy_unit_conversion_ = energy_unit(unit);
+  name_ = s["module"].as_string();
+  field_.assign((size_t)(3 * lat.num_sites()), 0.0);
+}
+
+void Hamiltonian::calculate_fields(jh_ctx *ctx, double t) {
+  engine_check(ctx, jh_compute_term_field(ctx, term_id_, t, field_.data()));
+}
+
+double Hamiltonian::calculate_total_energy(jh_ctx *ctx, double t) {
+  double e = 0.0;
+  engine_check(ctx, jh_term_energy(ctx, term_id_, t, &e));
+  return e;
+}
+
+std::unique_ptr<Hamiltonian> Hamiltonian::create(const Setting &s, const Lattice &lat) {
+  // core/hamiltonian.cc:80-113 (names of the terms on the hot path)
+  if (s.exists("strategy"))
+    throw RuntimeError("dipole hamiltonians now have specific names and 'strategy' has been removed");
+  const std::string m = lowercase(s["module"].as_string());
+  if (m == "exchange") return std::make_unique<ExchangeHamiltonian>(s, lat);
+  if (m == "zeeman") return std::make_unique<ZeemanHamiltonian>(s, lat, false);
+  if (m == "applied-field") return std::make_unique<ZeemanHamiltonian>(s, lat, true);
+  if (m == "uniaxial" || m == "uniaxial-anisotropy") return std::make_unique<UniaxialHamiltonian>(s, lat);
+  if (m == "dipole-fft") return std::make_unique<DipoleFFTHamiltonian>(s, lat);
+  throw RuntimeError("unknown hamiltonian " + s["module"].as_string() +
+                     " (this build provides exchange, zeeman, applied-field, uniaxial, dipole-fft)");
+  return nullptr;
+}
+
+// ---- exchange -------------------------------------------------------------------
+
+ExchangeHamiltonian::ExchangeHamiltonian(const Setting &s, const Lattice &lat) : Hamiltonian(s, lat) {
+  // hamiltonian/exchange.cc:12-172
+  NeighbourOptions opt;
+  opt.use_symops = s.get<bool>("symops", true);
+  opt.energy_cutoff = s.get<double>("energy_cutoff", 0.0);
+  opt.radius_cutoff = s.get<double>("radius_cutoff", 100.0);
+  opt.distance_tolerance = s.get<double>("distance_tolerance", kLatticeTolerance);
+  const double prefactor = s.get<double>("interaction_prefactor", 1.0);
+  opt.check_no_zero = s.get<bool>("check_no_zero_motif_neighbour_count", true);
+  opt.check_identical_count = s.get<bool>("check_identical_motif_neighbour_count", true);
+  opt.check_identical_total = s.get<bool>("check_identical_motif_total_exchange", true);
+  const bool check_symmetry = s.get<bool>("check_sparse_matrix_symmetry", true);
+  const std::string fmt = capitalize(s.get<std::string>("coordinate_format", "CARTESIAN"));
+  if (fmt == "CART" || fmt == "CARTESIAN") opt.coords = CoordinateFormat::kCartesian;
+  else if (fmt == "FRAC" || fmt == "FRACTIONAL") opt.coords = CoordinateFormat::kFractional;
+  else throw RuntimeError("Unknown coordinate format");
+  // safety_check_distance_tolerance (core/interactions.cc:512-526)
+  for (int i = 0; i < lat.num_basis_sites(); ++i)
+    for (int j = i + 1; j < lat.num_basis_sites(); ++j)
+      if (norm(lat.basis_site(i).frac - lat.basis_site(j).frac) < opt.distance_tolerance)
+        throw RuntimeError("Atoms " + std::to_string(i) + " and " + std::to_string(j) +
+                           " in the unit cell are closer together than the distance_tolerance");
+
+  bool kkr = false;
+  std::vector<InteractionData> rows;
+  if (s.exists("exc_file")) rows = interactions_from_file(s["exc_file"].as_string(), kkr);
+  else if (s.exists("interactions")) rows = interactions_from_setting(s["interactions"], kkr);
+  else throw ConfigError("'exc_file' or 'interactions' settings are required");
+
+  NeighbourList nl = generate_neighbour_list(lat, rows, kkr, opt);
+  computed_ = nl.size();
+  std::cout << "    computed interactions: " << computed_ << "\n";
+
+  // insertion (exchange.cc:162-169): Jij = (prefactor * conversion) * J,
+  // kept if max|Jij| > energy_cutoff * conversion
+  const double scale = prefactor * energy_unit_conversion_;
+  const double cutoff = opt.energy_cutoff * energy_unit_conversion_;
+  std::vector<Mat3> vals(nl.values.size());
+  std::vector<char> keep(nl.values.size());
+  for (size_t v = 0; v < nl.values.size(); ++v) {
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) vals[v][a][b] = scale * nl.values[v][a][b];
+    keep[v] = max_abs(vals[v]) > cutoff;
+  }
+  scaled_.values = vals;
+  scaled_.i.reserve(nl.i.size());
+  scaled_.j.reserve(nl.j.size());
+  scaled_.v.reserve(nl.v.size());
+  for (int64_t p = 0; p < nl.size(); ++p) {
+    if (!keep[nl.v[p]]) continue;
+    scaled_.i.push_back(nl.i[p]);
+    scaled_.j.push_back(nl.j[p]);
+    scaled_.v.push_back(nl.v[p]);
+  }
+  if (check_symmetry && !is_symmetric())
+    throw RuntimeError("sparse matrix for " + name_ + " is not symmetric");
+}
+
+bool ExchangeHamiltonian::is_symmetric() const {
+  // Builder::is_symmetric (sparse_matrix_builder.h:320-361) at block level:
+  // A[3i+m][3j+n] == A[3j+n][3i+m] exactly, absent entries being zero
+  const int64_t n = lat_.num_sites();
+  std::vector<int64_t> ptr((size_t)n + 1, 0);
+  for (auto i : scaled_.i) ptr[(size_t)i + 1]++;
+  for (int64_t s = 0; s < n; ++s) ptr[s + 1] += ptr[s];
+  for (int64_t p = 0; p < scaled_.size(); ++p) {
+    const int32_t i = scaled_.i[p], j = scaled_.j[p];
+    auto b = scaled_.j.begin() + ptr[j], e = scaled_.j.begin() + ptr[j + 1];
+    auto it = std::lower_bound(b, e, i);
+    const Mat3 &T = scaled_.values[scaled_.v[p]];
+    if (it == e || *it != i) {
+      if (max_abs(T) != 0.0) return false;
+      continue;
+    }
+    const Mat3 &U = scaled_.values[scaled_.v[it - scaled_.j.begin()]];
+    for (int a = 0; a < 3; ++a)
+      for (int c = 0; c < 3; ++c)
+        if (T[a][c] != U[c][a]) return false;
+  }
+  return true;
+}
+
+void ExchangeHamiltonian::build_csr(std::vector<int64_t> &rowptr, std::vector<int32_t> &col,
+                                    std::vector<double> &val) const {
+  // insert_interaction_tensor (sparse_interaction.cc:25-33) then the sorted,
+  // merged CSR of build_csr (sparse_matrix_builder.h:187-248); the neighbour
+  // list is (i, j)-sorted and duplicate free, so rows come out in order.
+  const int64_t n = lat_.num_sites();
+  rowptr.assign((size_t)(3 * n + 1), 0);
+  for (int64_t p = 0; p < scaled_.size(); ++p) {
+    const Mat3 &T = scaled_.values[scaled_.v[p]];
+    for (int m = 0; m < 3; ++m)
+      for (int c = 0; c < 3; ++c)
+        if (T[m][c] != 0.0) rowptr[3 * (int64_t)scaled_.i[p] + m + 1]++;
+  }
+  for (int64_t r = 0; r < 3 * n; ++r) rowptr[r + 1] += rowptr[r];
+  col.resize((size_t)rowptr[3 * n]);
+  val.resize((size_t)rowptr[3 * n]);
+  std::vector<int64_t> fill(rowptr.begin(), rowptr.end() - 1);
+  for (int64_t p = 0; p < scaled_.size(); ++p) {
+    const Mat3 &T = scaled_.values[scaled_.v[p]];
+    for (int m = 0; m < 3; ++m)
+      for (int c = 0; c < 3; ++c)
+        if (T[m][c] != 0.0) {
+          const int64_t r = 3 * (int64_t)scaled_.i[p] + m;
+          col[fill[r]] = 3 * scaled_.j[p] + c;
+          val[fill[r]] = T[m][c];
+          fill[r]++;
+        }
+  }
+}
+
+void ExchangeHamiltonian::register_term(jh_ctx *ctx) {
+  if (!ctx) return;
+  std::vector<double> v9(scaled_.values.size() * 9);
+  for (size_t v = 0; v < scaled_.values.size(); ++v)
+    for (int q = 0; q < 9; ++q) v9[9 * v + q] = scaled_.values[v][q / 3][q % 3];
+  engine_check(ctx, jh_add_exchange_list(ctx, scaled_.size(), scaled_.i.data(), scaled_.j.data(),
+                                         scaled_.v.data(), (int64_t)scaled_.values.size(),
+                                         v9.data(), &term_id_));
+}
+
+// ---- zeeman / applied field ------------------------------------------------
+
+ZeemanHamiltonian::ZeemanHamiltonian(const Setting &s, const Lattice &lat, bool applied_field)
+    : Hamiltonian(s, lat) {
+  const int64_t n = lat.num_sites();
+  dc.assign((size_t)(3 * n), 0.0);
+  if (applied_field) {
+    // applied_field.cc:84-148, static field only: h_i = mus_i B
+    if (s.exists("type") && lowercase(s["type"].as_string()) != "static")
+      throw RuntimeError("applied-field type " + s["type"].as_string() + " is not supported by this build");
+    const Vec3 B = vec3(s["field"]);
+    for (int64_t i = 0; i < n; ++i)
+      for (int j = 0; j < 3; ++j) dc[3 * i + j] = lat.mus[i] * B[j];
+    name_ += "-static";
+    return;
+  }
+  // zeeman.cc:12-73
+  const int nmat = lat.num_materials();
+  if (s.exists("dc_local_field")) {
+    const Setting &f = s["dc_local_field"];
+    if (f.length() != nmat) throw ConfigError("dc_local_field must be specified for every material");
+    for (int64_t i = 0; i < n; ++i) {
+      const Vec3 B = vec3(f[lat.site_material(i)]);
+      for (int j = 0; j < 3; ++j) {
+        dc[3 * i + j] = B[j];
+        dc[3 * i + j] *= lat.mus[i];
+      }
+    }
+  }
+  if (s.exists("ac_local_field") || s.exists("ac_local_frequency")) {
+    if (!(s.exists("ac_local_field") && s.exists("ac_local_frequency")))
+      throw ConfigError("ac_local_field must have a field and a frequency");
+    const Setting &f = s["ac_local_field"], &w = s["ac_local_frequency"];
+    if (f.length() != nmat || w.length() != nmat)
+      throw ConfigError("ac_local_field and ac_local_frequency must be specified for every material");
+    ac.assign((size_t)(3 * n), 0.0);
+    omega.assign((size_t)n, 0.0);
+    for (int64_t i = 0; i < n; ++i) {
+      const Vec3 B = vec3(f[lat.site_material(i)]);
+      for (int j = 0; j < 3; ++j) {
+        ac[3 * i + j] = B[j];
+        ac[3 * i + j] *= lat.mus[i];
+      }
+      omega[i] = 2.0 * kPi * w[lat.site_material(i)].as_double();
+    }
+  }
+}
+
+void ZeemanHamiltonian::register_term(jh_ctx *ctx) {
+  if (!ctx) return;
+  engine_check(ctx, jh_add_zeeman(ctx, dc.data(), ac.empty() ? nullptr : ac.data(),
+                                  omega.empty() ? nullptr : omega.data(), &term_id_));
+}
+
+// ---- uniaxial ------------------------------------------------------------------
+
+UniaxialHamiltonian::UniaxialHamiltonian(const Setting &s, const Lattice &lat) : Hamiltonian(s, lat) {
+  // uniaxial_anisotropy.cc:79-113
+  if (s.exists("d2z") || s.exists("d4z") || s.exists("d6z") || s.exists("K1") || s.exists("K2") ||
+      s.exists("K3"))
+    throw ConfigError("UniaxialHamiltonian: anisotropy should only be specified for a single K1, K2 or K3.");
+  const std::string order = s["order"].as_string();
+  if (order == "K1") power = 2;
+  else if (order == "K2") power = 4;
+  else if (order == "K3") power = 6;
+  else throw RuntimeError("Unsupported anisotropy: " + order);
+  const int64_t n = lat.num_sites();
+  K.assign((size_t)n, 0.0);
+  axis.assign((size_t)(3 * n), 0.0);
+  const Setting &list = s["anisotropies"];
+  for (int q = 0; q < list.length(); ++q) {
+    const Setting &a = list[q];
+    if (!a.is_list()) throw RuntimeError("Incorrectly formatted anisotropy setting");
+    int motif = -1;
+    int material = -1;
+    if (a[0].is_number()) {
+      motif = a[0].as_int() - 1;
+      if (motif < 0 || motif >= lat.num_basis_sites()) throw RuntimeError("uniaxial anisotropy motif position is invalid");
+    } else {
+      material = lat.material_index(a[0].as_string());
+      if (material < 0) throw RuntimeError("uniaxial anisotropy material is invalid");
+    }
+    Vec3 ax = vec3(a[1]);
+    const double len = norm(ax);
+    ax = {ax[0] / len, ax[1] / len, ax[2] / len};
+    const double energy = a[2].as_double();
+    for (int64_t i = 0; i < n; ++i) {
+      if (lat.site_basis(i) == motif || (material >= 0 && lat.site_material(i) == material)) {
+        K[i] = energy * energy_unit_conversion_;
+        for (int j = 0; j < 3; ++j) axis[3 * i + j] = ax[j];
+      }
+    }
+  }
+}
+
+void UniaxialHamiltonian::register_term(jh_ctx *ctx) {
+  if (!ctx) return;
+  engine_check(ctx, jh_add_uniaxial(ctx, K.data(), axis.data(), power, &term_id_));
+}
+
+// ---- dipole-fft ----------------------------------------------------------------
+
+DipoleFFTHamiltonian::DipoleFFTHamiltonian(const Setting &s, const Lattice &lat) : Hamiltonian(s, lat) {
+  r_cutoff = s["r_cutoff"].as_double();
+}
+
+void DipoleFFTHamiltonian::register_term(jh_ctx *ctx) {
+  if (!ctx) return;
+  int32_t size[3], per[3];
+  for (int d = 0; d < 3; ++d) {
+    size[d] = lat_.size(d);
+    per[d] = lat_.periodic(d) ? 1 : 0;
+  }
+  double cell[9];
+  for (int q = 0; q < 9; ++q) cell[q] = lat_.cell()[q / 3][q % 3];
+  std::vector<double> frac;
+  for (int m = 0; m < lat_.num_basis_sites(); ++m)
+    for (int q = 0; q < 3; ++q) frac.push_back(lat_.basis_site(m).frac[q]);
+  engine_check(ctx, jh_add_dipole_fft(ctx, size, per, lat_.num_basis_sites(), cell, frac.data(),
+                                      lat_.parameter(), r_cutoff, &term_id_));
+}
+
+// ---- solver ----------------------------------------------------------------------
+
+std::unique_ptr<Solver> Solver::create(const Setting &s) {
+  // core/solver.cc:60-76: the GPU Heun solver under its reference name and
+  // an explicit alias
+  const std::string m = lowercase(s["module"].as_string());
+  if (m == "llg-heun-gpu" || m == "llg-heun-hip") return std::make_unique<HipHeunLLGSolver>();
+  throw RuntimeError("unknown solver " + s["module"].as_string() +
+                     " (this build provides llg-heun-gpu / llg-heun-hip)");
+  return nullptr;
+}
+
+void HipHeunLLGSolver::initialize(const Setting &s) {
+  // solvers/cuda_llg_heun.cu:21-37
+  step_size_ = s["t_step"].as_double() / 1e-12;
+  const double t_max = s["t_max"].as_double() / 1e-12;
+  const double t_min = s.get<double>("t_min", 0.0) / 1e-12;
+  max_steps_ = static_cast<int>(t_max / step_size_);
+  min_steps_ = static_cast<int>(t_min / step_size_);
+  gilbert_ = s.get<bool>("gilbert_prefactor", false);
+  thermostat_ = s.get<std::string>("thermostat", "langevin-white-gpu");
+  // core/thermostat.cc:25-27 aliases of the classical white-noise thermostat
+  const std::string t = capitalize(thermostat_);
+  if (t != "CLASSICAL-GPU" && t != "LANGEVIN-WHITE-GPU" && t != "CUDA_LANGEVIN_WHITE")
+    throw RuntimeError("unknown thermostat " + thermostat_ +
+                       " (this build provides langevin-white-gpu / classical-gpu)");
+  std::cout << "  timestep (ps) " << step_size_ << "\n  t_max (ps) " << t_max << " steps "
+            << max_steps_ << "\n  thermostat " << thermostat_ << "\n";
+}
+
+void Solver::run_steps(int64_t k) {
+  if (k <= 0) return;
+  engine_check(ctx_, jh_step(ctx_, k));
+  iteration_ += k;
+  time_ = (double)iteration_ * step_size_;
+}
+
+void Solver::register_monitor(std::unique_ptr<Monitor> m) { monitors_.push_back(std::move(m)); }
+
+void Solver::notify_monitors() {
+  for (auto &m : monitors_)
+    if (m->is_updating(iteration_)) m->update(*this);
+}
+
+void Solver::compute_fields(std::vector<double> &h) {
+  h.resize(hamiltonians_.empty() ? 0 : hamiltonians_[0]->field().size());
+  if (hamiltonians_.empty()) return;
+  engine_check(ctx_, jh_compute_fields(ctx_, time_, h.data()));
+}
+
+// ---- monitors ------------------------------------------------------------------
+
+std::unique_ptr<Monitor> Monitor::create(const Setting &s, const std::string &prefix, Solver &solver,
+                                         const Lattice &lat) {
+  const std::string m = lowercase(s["module"].as_string());
+  if (m == "magnetisation") return std::make_unique<MagnetisationMonitor>(s, prefix + "mag.tsv");
+  if (m == "energy") return std::make_unique<EnergyMonitor>(s, prefix + "eng.tsv", solver);
+  throw RuntimeError("unknown monitor " + s["module"].as_string() +
+                     " (this build provides magnetisation, energy)");
+  return nullptr;
+}
+
+MagnetisationMonitor::MagnetisationMonitor(const Setting &s, const std::string &path)
+    : Monitor(s), tsv_(path) {
+  normalize_ = s.get<bool>("normalize", true);
+  tsv_ << std::setw(12) << "time" << "\tT\tHx\tHy\tHz\tmx\tmy\tmz\tm\n";
+}
+
+void MagnetisationMonitor::update(Solver &solver) {
+  // m = sum mus s / (Kahan) sum mus, or / muB without normalisation
+  double r[4];
+  engine_check(solver.engine(), jh_reduce_magnetisation(solver.engine(), r));
+  const double f = normalize_ ? 1.0 / r[3] : 1.0 / kBohrMagnetonIU;
+  const double mag = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+  char line[512];
+  std::snprintf(line, sizeof line, "%.8e\t%.8e\t%.8e\t%.8e\t%.8e\t%.8e\t%.8e\t%.8e\t%.8e\n",
+                solver.time(), solver.temperature(), solver.applied_field()[0],
+                solver.applied_field()[1], solver.applied_field()[2], r[0] * f, r[1] * f, r[2] * f,
+                mag * f);
+  tsv_ << line;
+  tsv_.flush();
+}
+
+EnergyMonitor::EnergyMonitor(const Setting &s, const std::string &path, Solver &solver)
+    : Monitor(s), tsv_(path) {
+  tsv_ << "time\t";
+  for (auto &h : solver.hamiltonians()) tsv_ << h->name() << "_E_meV\t";
+  tsv_ << "\n";
+}
+
+void EnergyMonitor::update(Solver &solver) {
+  char buf[64];
+  std::snprintf(buf, sizeof buf, "%e\t", solver.time());
+  tsv_ << buf;
+  for (auto &h : solver.hamiltonians()) {
+    std::snprintf(buf, sizeof buf, "%.15e\t", h->calculate_total_energy(solver.engine(), solver.time()));
+    tsv_ << buf;
+  }
+  tsv_ << "\n";
+  tsv_.flush();
+}
+
+}  // namespace jams

@@ -1,0 +1,185 @@
+// plugins.h -- the reference's plugin API (Solver / Hamiltonian / Thermostat /
+// Physics / Monitor, core/solver.h:15-88, core/hamiltonian.h:15-77,
+// core/thermostat.h:10-39) restated on top of the gfx950 engine's C ABI.
+//
+// Hamiltonians are descriptors: each one registers its term with the engine
+// in config order; calculate_fields and calculate_total_energy are served by
+// the engine (jh_compute_term_field, jh_term_energy).  Module names are the
+// reference's, so unmodified configs select these implementations.
+#pragma once
+
+#include <cstdint>
+#include <fstream>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/jams_hip.h"
+#include "common.h"
+#include "config.h"
+#include "interactions.h"
+#include "lattice.h"
+
+namespace jams {
+
+class Hamiltonian {
+ public:
+  Hamiltonian(const Setting &s, const Lattice &lat);
+  virtual ~Hamiltonian() = default;
+  const std::string &name() const { return name_; }
+  // add this term to the engine (ctx may be null in set-up-only mode)
+  virtual void register_term(jh_ctx *ctx) = 0;
+  int term_id() const { return term_id_; }
+  // Hamiltonian::calculate_fields / calculate_total_energy through the engine
+  void calculate_fields(jh_ctx *ctx, double t);
+  double calculate_total_energy(jh_ctx *ctx, double t);
+  const std::vector<double> &field() const { return field_; }
+
+  static std::unique_ptr<Hamiltonian> create(const Setting &s, const Lattice &lat);
+
+ protected:
+  std::string name_;
+  double energy_unit_conversion_ = 1.0;
+  int term_id_ = -1;
+  std::vector<double> field_;
+  const Lattice &lat_;
+};
+
+// hamiltonian/exchange.cc:12-172 + sparse_interaction.cc
+class ExchangeHamiltonian : public Hamiltonian {
+ public:
+  ExchangeHamiltonian(const Setting &s, const Lattice &lat);
+  void register_term(jh_ctx *ctx) override;
+  // neighbour list after insertion scaling and energy cutoff
+  const NeighbourList &neighbours() const { return scaled_; }
+  int64_t computed_interactions() const { return computed_; }
+  // the reference's scalar CSR (insert_interaction_tensor zero skip, rows
+  // ascending by column); int64 row pointers
+  void build_csr(std::vector<int64_t> &rowptr, std::vector<int32_t> &col,
+                 std::vector<double> &val) const;
+  bool is_symmetric() const;
+
+ private:
+  NeighbourList scaled_;
+  int64_t computed_ = 0;
+};
+
+// hamiltonian/zeeman.cc and applied_field.cc: h = mus B (+ ac cos(omega t))
+class ZeemanHamiltonian : public Hamiltonian {
+ public:
+  ZeemanHamiltonian(const Setting &s, const Lattice &lat, bool applied_field);
+  void register_term(jh_ctx *ctx) override;
+  std::vector<double> dc, ac, omega;
+};
+
+// hamiltonian/uniaxial_anisotropy.cc
+class UniaxialHamiltonian : public Hamiltonian {
+ public:
+  UniaxialHamiltonian(const Setting &s, const Lattice &lat);
+  void register_term(jh_ctx *ctx) override;
+  std::vector<double> K, axis;
+  int power = 2;
+};
+
+// hamiltonian/dipole_fft.cc
+class DipoleFFTHamiltonian : public Hamiltonian {
+ public:
+  DipoleFFTHamiltonian(const Setting &s, const Lattice &lat);
+  void register_term(jh_ctx *ctx) override;
+  double r_cutoff = 0.0;
+};
+
+class Monitor;
+
+class Solver {
+ public:
+  virtual ~Solver() = default;
+  virtual void initialize(const Setting &s) = 0;
+  virtual void run() = 0;                 // exactly one step
+  virtual void run_steps(int64_t k);      // k steps with no monitor in between
+  virtual std::string name() const = 0;
+  virtual bool is_cuda_solver() const { return true; }
+  bool is_running() const { return iteration_ < max_steps_; }
+  int64_t iteration() const { return iteration_; }
+  double time() const { return time_; }
+  double time_step() const { return step_size_; }
+  int64_t max_steps() const { return max_steps_; }
+  double temperature() const { return temperature_; }
+  const Vec3 &applied_field() const { return applied_field_; }
+  bool gilbert_prefactor() const { return gilbert_; }
+  const std::string &thermostat_name() const { return thermostat_; }
+
+  void attach(jh_ctx *ctx) { ctx_ = ctx; }
+  jh_ctx *engine() const { return ctx_; }
+  void register_hamiltonian(std::unique_ptr<Hamiltonian> h) { hamiltonians_.push_back(std::move(h)); }
+  void register_monitor(std::unique_ptr<Monitor> m);
+  std::vector<std::unique_ptr<Hamiltonian>> &hamiltonians() { return hamiltonians_; }
+  std::vector<std::unique_ptr<Monitor>> &monitors() { return monitors_; }
+  void set_physics(double temperature, const Vec3 &field) {
+    temperature_ = temperature;
+    applied_field_ = field;
+  }
+  void notify_monitors();
+  // Solver::compute_fields (core/solver.cc:43-57) into h (N x 3)
+  void compute_fields(std::vector<double> &h);
+
+  static std::unique_ptr<Solver> create(const Setting &s);
+
+ protected:
+  jh_ctx *ctx_ = nullptr;
+  int64_t iteration_ = 0, max_steps_ = 0, min_steps_ = 0;
+  double time_ = 0.0, step_size_ = 1.0;
+  double temperature_ = 0.0;
+  Vec3 applied_field_ = {0, 0, 0};
+  bool gilbert_ = false;
+  std::string thermostat_ = "langevin-white-gpu";
+  std::vector<std::unique_ptr<Hamiltonian>> hamiltonians_;
+  std::vector<std::unique_ptr<Monitor>> monitors_;
+};
+
+// CUDAHeunLLGSolver (solvers/cuda_llg_heun.cu) on the gfx950 engine
+class HipHeunLLGSolver : public Solver {
+ public:
+  void initialize(const Setting &s) override;
+  void run() override { run_steps(1); }
+  std::string name() const override { return "llg-heun-gpu"; }
+};
+
+class Monitor {
+ public:
+  Monitor(const Setting &s) : output_steps_(s.get<int>("output_steps", 100)) {}
+  virtual ~Monitor() = default;
+  bool is_updating(int64_t iteration) const { return iteration % output_steps_ == 0; }
+  int64_t output_steps() const { return output_steps_; }
+  virtual void update(Solver &solver) = 0;
+  static std::unique_ptr<Monitor> create(const Setting &s, const std::string &prefix,
+                                         Solver &solver, const Lattice &lat);
+
+ protected:
+  int64_t output_steps_;
+};
+
+// monitors/magnetisation.cc:77-104 (one group: all spins)
+class MagnetisationMonitor : public Monitor {
+ public:
+  MagnetisationMonitor(const Setting &s, const std::string &path);
+  void update(Solver &solver) override;
+
+ private:
+  std::ofstream tsv_;
+  bool normalize_ = true;
+};
+
+// monitors/energy.cc:22-33
+class EnergyMonitor : public Monitor {
+ public:
+  EnergyMonitor(const Setting &s, const std::string &path, Solver &solver);
+  void update(Solver &solver) override;
+
+ private:
+  std::ofstream tsv_;
+};
+
+void engine_check(jh_ctx *ctx, int rc);
+
+}  // namespace jams

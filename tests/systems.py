@@ -92,64 +92,12 @@ def bcc_two_shell_rows(J1, J2):
     return rows
 
 
-def bcc_shell_vectors():
-    """The first five bcc shells (8, 6, 12, 24, 8 = 58 vectors), lattice
-    constants, at r/a = sqrt(3)/2, 1, sqrt(2), sqrt(11)/2, sqrt(3)."""
-    vecs = []
-    for sx, sy, sz in itertools.product((1, -1), repeat=3):
-        vecs.append((0.5 * sx, 0.5 * sy, 0.5 * sz))
-    for ax in range(3):
-        for sg in (1, -1):
-            v = [0, 0, 0]; v[ax] = sg; vecs.append(tuple(float(x) for x in v))
-    for a, b in ((0, 1), (0, 2), (1, 2)):
-        for sa, sb in itertools.product((1, -1), repeat=2):
-            v = [0, 0, 0]; v[a] = sa; v[b] = sb; vecs.append(tuple(float(x) for x in v))
-    for ax in range(3):
-        for s0, s1, s2 in itertools.product((1, -1), repeat=3):
-            v = [0.5 * s1, 0.5 * s2, 0.5 * s2]
-            v = [0.0, 0.0, 0.0]
-            others = [q for q in range(3) if q != ax]
-            v[ax] = 1.5 * s0
-            v[others[0]] = 0.5 * s1
-            v[others[1]] = 0.5 * s2
-            vecs.append(tuple(v))
-    for sx, sy, sz in itertools.product((1, -1), repeat=3):
-        vecs.append((float(sx), float(sy), float(sz)))
-    assert len(vecs) == 58 and len(set(vecs)) == 58
-    return vecs
-
-
-def dmi_matrix(D):
-    """[D]_x with s_i . [D]_x s_j = D . (s_i x s_j)."""
-    Dx, Dy, Dz = D
-    return np.array([[0.0, Dz, -Dy], [-Dz, 0.0, Dx], [Dy, -Dx, 0.0]])
-
-
-def bcc_five_shell_dmi_rows(seed=7, J0=3.0, dmi=0.1):
-    """J_n = J0 exp(-r/a) meV on each shell plus D = dmi J_n d with a seeded
-    random unit d per bond pair, d(-r) = -d(r) so J(-r) = J(r)^T exactly."""
-    rows = []
-    vecs = bcc_shell_vectors()
-    idx = 0
-    chosen = {}
-    for v in vecs:
-        key = tuple(-x for x in v)
-        if key in chosen:
-            d = -chosen[key]
-        else:
-            u1 = po.lib().or_setup_uniform(seed, 7, 2 * idx)
-            u2 = po.lib().or_setup_uniform(seed, 7, 2 * idx + 1)
-            idx += 1
-            z = 2.0 * u1 - 1.0
-            phi = 2.0 * math.pi * u2
-            rr = math.sqrt(max(0.0, 1.0 - z * z))
-            d = np.array([rr * math.cos(phi), rr * math.sin(phi), z])
-            chosen[v] = d
-        r = math.sqrt(sum(x * x for x in v))
-        Jn = J0 * math.exp(-r)
-        T = Jn * np.eye(3) + dmi_matrix(dmi * Jn * d)
-        rows.append((0, 0, list(v), T.reshape(9)))
-    return rows
+def bcc_five_shell_dmi_rows(seed=7):
+    """The C3 template (five bcc shells with DMI) in the oracle's row format,
+    taken from the workload generator so both sides use the same tensors."""
+    from jams_amd import configs
+    return [(0, 0, list(r), np.asarray(J, np.float64)) for (_, _, r, J) in
+            configs.bcc_five_shell_dmi_rows(seed)]
 
 
 def build(lat, rows, moment, alpha, order, zeeman_B=None, K=None, dt=1e-3, spins="random",

@@ -1,0 +1,173 @@
+"""Host layer (include/jams_host.h) on the CPU: config parsing, lattice,
+neighbour list and CSR structure against the oracle (byte-identical), the
+reference's CrPS4 symmetry-completion count, and the C ABI exports."""
+import re
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from jams_amd import configs
+from jams_amd.host import Simulation
+from oracle import pyoracle as po
+from tests import systems as S
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def oracle_csr(lat, rows):
+    pi, pj, pJ = po.neighbour_list(lat, rows)
+    return (pi, pj, pJ), po.exchange_csr(lat.n, pi, pj, pJ)
+
+
+def assert_same_exchange(sim, lat, rows):
+    (pi, pj, pJ), (rp, col, val) = oracle_csr(lat, rows)
+    hi, hj, hJ = sim.exchange_pairs()
+    assert np.array_equal(hi, pi) and np.array_equal(hj, pj)
+    assert hJ.tobytes() == pJ.tobytes()
+    hrp, hcol, hval = sim.exchange_csr()
+    assert hrp.tobytes() == rp.tobytes()
+    assert hcol.tobytes() == col.tobytes()
+    assert hval.tobytes() == val.tobytes()
+    return rp[-1]
+
+
+def test_c1_config_matches_oracle_bytewise():
+    sim = Simulation(configs.c1(16))
+    assert sim.n == 4096 and sim.max_steps == 1000
+    dt, T, seed = sim.params()
+    assert dt == 1.0e-15 / 1e-12 and T == 0.0 and seed == 1
+    b = S.c1(16)
+    nnz = assert_same_exchange(sim, b.lat, S.sc_nn_rows(1.0))
+    assert nnz == 73728
+    mus, gyro, alpha, s = sim.sites()
+    assert np.array_equal(mus, b.mus) and np.array_equal(gyro, b.gyro)
+    assert np.array_equal(alpha, b.alpha)
+    # seeded "random" spins: the oracle's Marsaglia stream, then unit_vector
+    # (normalise_spins defaults to true, core/lattice.cc:724-729)
+    r = po.random_spins(1, 4096)
+    length = np.sqrt((r[:, 0] * r[:, 0] + r[:, 1] * r[:, 1]) + r[:, 2] * r[:, 2])
+    assert np.array_equal(s, r / length[:, None])
+
+
+def test_c2_small_config_matches_oracle_bytewise():
+    sim = Simulation(configs.c2(8))
+    assert sim.max_steps == 10000
+    b = S.c2(8)
+    nnz = assert_same_exchange(sim, b.lat, S.bcc_two_shell_rows(3.0, 1.5))
+    assert nnz == 1024 * 14 * 3
+    computed, npairs, _ = sim.exchange_counts()
+    assert computed == npairs == 1024 * 14
+
+
+def test_c3_small_config_matches_oracle_bytewise():
+    sim = Simulation(configs.c3(6))
+    b = S.bcc_lattice(6)
+    nnz = assert_same_exchange(sim, b, S.bcc_five_shell_dmi_rows())
+    assert nnz == 432 * 58 * 9
+
+
+@pytest.mark.parametrize("L,periodic", [((32, 32, 4), (False, False, False)),
+                                        ((6, 5, 3), (True, False, True))])
+def test_open_and_mixed_boundaries_match_oracle(L, periodic):
+    txt = configs.sc_config(L=L, periodic=periodic, J=2.0)
+    # open boundaries give different neighbour counts per site
+    txt = txt.replace('symops = false; interactions',
+                      'symops = false; check_identical_motif_neighbour_count = false; interactions')
+    sim = Simulation(txt)
+    lat = po.LatticeSpec(np.eye(3), [[0, 0, 0]], [0], L, periodic)
+    assert_same_exchange(sim, lat, S.sc_nn_rows(2.0))
+
+
+def test_c5_film_pair_count():
+    sim = Simulation(configs.sc_config(L=(256, 256, 8), periodic=(False, False, False), J=2.0,
+                                       field=None))
+    computed, npairs, nnz = sim.exchange_counts()
+    assert computed == npairs == 3006464
+    assert nnz == 3 * 3006464
+
+
+CRPS4 = """
+lattice : { size = [4, 4, 4]; periodic = [true, true, true]; };
+unitcell : {
+  parameter = 1.08510e-9;
+  basis = ( [ 0.50000, 0.50000, -0.0400], [-0.33435, 0.33435, 0.0000], [ 0.0000, 0.0000, 1.1246] );
+  positions = ( ("CrA", [0.752600, 0.247400, 0.0]), ("CrA", [0.247400, 0.752600, 0.0]),
+                ("CrB", [0.752600, 0.247400, 0.5]), ("CrB", [0.247400, 0.752600, 0.5]) );
+};
+physics : { temperature = 10.0; };
+materials = (
+  { name = "CrA"; moment = 2.8; gyro = 1.0; alpha = 0.1; spin = [0.0, 0.0, 1.0]; },
+  { name = "CrB"; moment = 2.8; gyro = 1.0; alpha = 0.1; spin = [0.0, 0.0, -1.0]; } );
+hamiltonians = (
+  { module = "uniaxial"; order = "K1"; energy_units = "meV";
+    anisotropies = ( ("CrA", [0.0, 0.0, 1.0], 0.0058), ("CrB", [0.0, 0.0, 1.0], 0.0058) ); },
+  { module = "exchange"; energy_units = "meV"; symops = true; coordinate_format = "fractional";
+    interactions = (
+      (1, 2, [ 0.494800, -0.494800, 0.000000], 2.96), (1, 2, [-0.505200, 0.505200, 0.000000], 2.09),
+      (1, 2, [-0.505200, -0.494800, 0.000000], 0.51), (1, 3, [-0.000000, -0.000000, -0.500000], -0.16),
+      (2, 1, [-0.494800, 0.494800, 0.000000], 2.96), (2, 1, [ 0.505200, -0.505200, 0.000000], 2.09),
+      (2, 1, [-0.494800, -0.505200, 0.000000], 0.51), (2, 4, [-0.000000, -0.000000, -0.500000], -0.16),
+      (3, 4, [ 0.494800, -0.494800, 0.000000], 2.96), (3, 4, [-0.505200, 0.505200, 0.000000], 2.09),
+      (3, 4, [ 0.494800, 0.505200, 0.000000], 0.51), (3, 1, [-0.000000, -0.000000, -0.500000], -0.16),
+      (4, 3, [-0.494800, 0.494800, 0.000000], 2.96), (4, 3, [ 0.505200, -0.505200, 0.000000], 2.09),
+      (4, 3, [ 0.505200,  0.494800, 0.000000], 0.51), (4, 2, [-0.000000, -0.000000, -0.500000], -0.16) ); } );
+solver : { module = "llg-heun-gpu"; t_step = 1.0e-16; t_max = 1.0e-14; };
+"""
+
+
+def test_crps4_symmetry_completion_count():
+    """The reference's integration test (test/test_exchange_symops.py:15):
+    CrPS4 4x4x4 with the site point groups completing half-listed shells
+    gives 'computed interactions: 1536'."""
+    sim = Simulation(CRPS4)
+    computed, npairs, _ = sim.exchange_counts(0)
+    assert computed == 1536
+
+
+def test_config_parser_grammar_and_merge():
+    base = configs.c1(4)
+    # patch strings merge into groups and override scalars (interface/config.cc:145-)
+    patch = 'solver : { t_max = 2.0e-14; }; sim = { seed = 99L; }; /* comment */ # comment'
+    sim = Simulation([base, patch])
+    dt, T, seed = sim.params()
+    assert sim.max_steps == 20 and seed == 99
+    with pytest.raises(RuntimeError, match="parse error"):
+        Simulation("lattice : { size = [4, 4, 4] ")
+    with pytest.raises(RuntimeError, match="unknown solver"):
+        Simulation([base, 'solver : { module = "llg-rk4-cpu"; };'])
+    with pytest.raises(RuntimeError, match="not symmetric"):
+        Simulation(base.replace('("A", "A", [1.0, 0.0, 0.0], 1.0)', '("A", "A", [1.0, 0.0, 0.0], 1.5)'))
+
+
+def test_spins_tsv_loader(tmp_path):
+    s = po.random_spins(4, 64)
+    f = tmp_path / "spins.tsv"
+    f.write_text("\n".join("\t".join(repr(float(x)) for x in row) for row in s))
+    sim = Simulation(configs.c1(4).replace('size = [4, 4, 4];', f'size = [4, 4, 4]; spins = "{f}";'))
+    assert np.array_equal(sim.sites()[3], s)
+
+
+def declared(header: Path):
+    text = header.read_text()
+    return sorted(set(re.findall(r"^\w[\w\s\*]*?\b(jh_\w+|jams_sim_\w+)\s*\(", text, re.M)))
+
+
+def test_libraries_export_every_declared_symbol():
+    libdir = ROOT / "jams_amd" / "lib"
+    for header, lib in ((ROOT / "include" / "jams_hip.h", libdir / "libjamship.so"),
+                        (ROOT / "include" / "jams_host.h", libdir / "libjamshost.so")):
+        names = declared(header)
+        assert len(names) > 5
+        out = subprocess.run(["nm", "-D", "--defined-only", str(lib)], capture_output=True,
+                             text=True, check=True).stdout
+        exported = set(line.split()[-1] for line in out.splitlines() if line.strip())
+        missing = [n for n in names if n not in exported]
+        assert not missing, missing
+
+
+def test_python_binding_covers_the_engine_abi():
+    from jams_amd import engine
+    names = declared(ROOT / "include" / "jams_hip.h")
+    assert sorted(engine.ABI) == names

@@ -1,0 +1,301 @@
+"""Pins of the oracle (CPU restatement of the reference path) against
+published known answers, closed forms, brute force and the reference's own
+golden values.  Each test names the mistake it would catch."""
+import math
+
+import numpy as np
+import pytest
+from scipy import stats
+
+from oracle import pyoracle as po
+from tests import systems as S
+
+
+# ---- RNG ---------------------------------------------------------------------
+# Random123 known-answer vectors for philox4x32_10 (counter, key) -> output.
+KAT = [
+    ([0, 0, 0, 0], [0, 0], [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]),
+    ([0xffffffff] * 4, [0xffffffff] * 2, [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]),
+    ([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0],
+     [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]),
+]
+
+
+@pytest.mark.parametrize("ctr,key,out", KAT)
+def test_philox_known_answers(ctr, key, out):
+    assert po.philox(ctr, key) == out
+
+
+def test_noise_log_and_sincos_match_libm():
+    rng = np.random.default_rng(3)
+    for x in np.concatenate([rng.random(2000), [2.0**-53, 0.5, 1.0 - 2**-53, 0.7071067811865476]]):
+        ref = math.log(x)
+        assert abs(po.noise_log(x) - ref) <= 4 * np.spacing(abs(ref)) + 1e-300
+    # reference values in 80-bit extended precision (2 pi u rounded in double
+    # would itself be off by ~u ulp(2 pi))
+    two_pi = np.longdouble("6.28318530717958647692528676655900577")
+    for u in np.concatenate([rng.random(2000), [0.0, 0.125, 0.25, 0.5, 0.75, 0.999999]]):
+        c, s = po.noise_sincos2pi(u)
+        rc = float(np.cos(two_pi * np.longdouble(u)))
+        rs = float(np.sin(two_pi * np.longdouble(u)))
+        assert abs(c - rc) <= 3e-16 and abs(s - rs) <= 3e-16, u
+
+
+def test_normal_stream_statistics():
+    z = po.normals(2024, 17, 100000).reshape(-1)
+    assert abs(z.mean()) < 5 * (1 / math.sqrt(z.size))
+    assert abs(z.var() - 1.0) < 5 * math.sqrt(2 / z.size)
+    assert stats.kstest(z, "norm").pvalue > 1e-3
+    # independent streams for different steps / seeds
+    z2 = po.normals(2024, 18, 100000).reshape(-1)
+    assert abs(np.corrcoef(z, z2)[0, 1]) < 0.02
+    assert not np.array_equal(po.normals(1, 0, 8), po.normals(2, 0, 8))
+
+
+def test_random_spins_uniform_on_sphere():
+    s = po.random_spins(1, 50000)
+    assert np.allclose(np.linalg.norm(s, axis=1), 1.0, atol=4e-16)
+    assert stats.kstest(s[:, 2], "uniform", args=(-1, 2)).pvalue > 1e-3
+    phi = np.arctan2(s[:, 1], s[:, 0])
+    assert stats.kstest(phi, "uniform", args=(-math.pi, 2 * math.pi)).pvalue > 1e-3
+
+
+# ---- lattice, neighbour list, CSR ----------------------------------------------
+def test_site_index_closed_form():
+    lat = po.LatticeSpec(np.eye(3), [[0, 0, 0], [0.5, 0.5, 0.5]], [0, 0], (3, 4, 5), (1, 1, 1))
+    L = lat.c()
+    for (i, j, k, m) in [(0, 0, 0, 0), (2, 3, 4, 1), (1, 2, 3, 0)]:
+        assert po.lib().or_site_index(po.C.byref(L), i, j, k, m) == ((i * 4 + j) * 5 + k) * 2 + m
+
+
+CASES = [
+    ("sc periodic", po.LatticeSpec(np.eye(3), [[0, 0, 0]], [0], (5, 6, 7), (1, 1, 1)), S.sc_nn_rows(1.0), 6),
+    ("sc open", po.LatticeSpec(np.eye(3), [[0, 0, 0]], [0], (5, 4, 3), (0, 0, 0)), S.sc_nn_rows(1.0), None),
+    ("sc mixed", po.LatticeSpec(np.eye(3), [[0, 0, 0]], [0], (6, 5, 3), (1, 0, 1)), S.sc_nn_rows(1.0), None),
+    ("bcc two shells", S.bcc_lattice(5), S.bcc_two_shell_rows(3.0, 1.5), 14),
+    ("bcc five shells dmi", S.bcc_lattice(6), S.bcc_five_shell_dmi_rows(), 58),
+    ("bcc open", S.bcc_lattice(4, (0, 1, 0)), S.bcc_two_shell_rows(3.0, 1.5), None),
+]
+
+
+@pytest.mark.parametrize("name,lat,rows,per_site", CASES, ids=[c[0] for c in CASES])
+def test_neighbour_list_matches_bruteforce(name, lat, rows, per_site):
+    pi, pj, pJ = po.neighbour_list(lat, rows)
+    bi, bj, bJ = po.neighbour_bruteforce(lat, rows)
+    assert np.array_equal(pi, bi) and np.array_equal(pj, bj) and np.array_equal(pJ, bJ)
+    if per_site is not None:
+        assert pi.size == lat.n * per_site
+        assert np.all(np.bincount(pi, minlength=lat.n) == per_site)
+
+
+def test_open_film_pair_count_closed_form():
+    """C5 film 256x256x8 open: 2 (255*256*8 + 256*255*8 + 256*256*7) pairs."""
+    lat = po.LatticeSpec(np.eye(3), [[0, 0, 0]], [0], (256, 256, 8), (0, 0, 0))
+    pi, pj, _ = po.neighbour_list(lat, S.sc_nn_rows(2.0))
+    assert pi.size == 2 * (255 * 256 * 8 + 256 * 255 * 8 + 256 * 256 * 7) == 3006464
+
+
+def test_duplicate_pairs_are_rejected():
+    # a 2-cell periodic chain maps +x and -x onto the same neighbour
+    lat = po.LatticeSpec(np.eye(3), [[0, 0, 0]], [0], (2, 3, 3), (1, 1, 1))
+    with pytest.raises(ValueError):
+        po.neighbour_list(lat, S.sc_nn_rows(1.0))
+
+
+def dense_from_pairs(n, pi, pj, pJ):
+    A = np.zeros((3 * n, 3 * n))
+    for i, j, J in zip(pi, pj, pJ.reshape(-1, 3, 3)):
+        A[3 * i:3 * i + 3, 3 * j:3 * j + 3] += J
+    return A
+
+
+@pytest.mark.parametrize("name,lat,rows,per_site", CASES[:5], ids=[c[0] for c in CASES[:5]])
+def test_exchange_csr_structure_and_values(name, lat, rows, per_site):
+    pi, pj, pJ = po.neighbour_list(lat, rows)
+    rowptr, col, val = po.exchange_csr(lat.n, pi, pj, pJ)
+    # nnz closed form: isotropic -> 3 per pair, full tensor -> 9 per pair
+    nz_per_pair = (pJ != 0).sum(axis=1)
+    assert rowptr[-1] == nz_per_pair.sum()
+    # rows sorted, strictly ascending columns, exact symmetry
+    for r in range(3 * lat.n):
+        c = col[rowptr[r]:rowptr[r + 1]]
+        assert np.all(np.diff(c) > 0)
+    assert po.csr_is_symmetric(rowptr, col, val)
+    # structure rule: row 3i+m holds 3j+n for neighbours j with J[m][n] != 0
+    A = dense_from_pairs(lat.n, pi, pj, pJ)
+    D = np.zeros_like(A)
+    for r in range(3 * lat.n):
+        D[r, col[rowptr[r]:rowptr[r + 1]]] = val[rowptr[r]:rowptr[r + 1]]
+    assert np.array_equal(A, D)
+    assert np.array_equal(np.nonzero(A), np.nonzero(D))
+
+
+def test_builder_quirks_as_written():
+    """merge() skips the element after an erase (triple duplicates survive as
+    two entries); build_csr() starts from row_[1] so an empty row 0 followed
+    by non-empty rows gives row 0 a phantom entry."""
+    rowptr, col, val = po.coo_to_csr(3, [1, 1, 1, 2], [0, 0, 0, 1], [1.0, 2.0, 4.0, 8.0])
+    assert list(col) == [0, 0, 1] and list(val) == [3.0, 4.0, 8.0]
+    assert list(rowptr) == [0, 1, 2, 3]  # row 0 claims entry 0, which belongs to row 1
+    rowptr, col, val = po.coo_to_csr(3, [0, 1, 1, 2], [1, 0, 0, 2], [1.0, 2.0, 4.0, 8.0])
+    assert list(rowptr) == [0, 1, 2, 3] and list(val) == [1.0, 6.0, 8.0]
+
+
+def test_csrmv_dense_brute_force_and_uniform_state():
+    lat = S.bcc_lattice(6)
+    rows = S.bcc_five_shell_dmi_rows()
+    pi, pj, pJ = po.neighbour_list(lat, rows)
+    rowptr, col, val = po.exchange_csr(lat.n, pi, pj, pJ)
+    A = dense_from_pairs(lat.n, pi, pj, pJ)
+    x = po.random_spins(11, lat.n).reshape(-1)
+    y = po.csrmv(rowptr, col, val, x)
+    assert np.allclose(y, A @ x, rtol=0, atol=1e-13)
+    # uniform state: h_i = (sum_j J_ij) s (DMI cancels in the sum over +-r)
+    s = np.tile([0.3, -0.4, math.sqrt(1 - 0.25)], lat.n)
+    J0 = sum(np.asarray(J, float).reshape(3, 3) for (_, _, _, J) in rows)
+    y = po.csrmv(rowptr, col, val, s).reshape(-1, 3)
+    assert np.allclose(y, (J0 @ s[:3])[None, :], rtol=0, atol=1e-12)
+
+
+def test_csrmv_plane_wave_eigenvector():
+    L = 8
+    lat = po.LatticeSpec(np.eye(3), [[0, 0, 0]], [0], (L, L, L), (1, 1, 1))
+    J = 1.7
+    pi, pj, pJ = po.neighbour_list(lat, S.sc_nn_rows(J))
+    rowptr, col, val = po.exchange_csr(lat.n, pi, pj, pJ)
+    r = lat.positions()
+    k = 2 * math.pi * np.array([1, 2, 3]) / L
+    eps = 0.1
+    ph = r @ k
+    s = np.stack([eps * np.cos(ph), eps * np.sin(ph), np.full(lat.n, math.sqrt(1 - eps * eps))], 1)
+    h = po.csrmv(rowptr, col, val, s.reshape(-1)).reshape(-1, 3)
+    Jk = J * 2 * np.cos(k).sum()
+    assert np.allclose(h[:, 0], Jk * s[:, 0], atol=1e-13)
+    assert np.allclose(h[:, 1], Jk * s[:, 1], atol=1e-13)
+    assert np.allclose(h[:, 2], 6 * J * s[:, 2], atol=1e-13)
+
+
+# ---- Heun LLG ----------------------------------------------------------------
+def test_macrospin_precession_and_damping_closed_form():
+    """Single spin in B z: tan(theta/2) = tan(theta0/2) exp(-alpha gamma B t),
+    phi = phi0 + gamma B t (Landau-Lifshitz form)."""
+    g = S.gyro_e()
+    alpha, B, dt, nsteps = 0.1, 2.0, 1e-3, 2000
+    mus = np.array([2.0 * S.BOHR])
+    o = po.OracleSystem(mus, np.array([g]), np.array([alpha]))
+    o.add_zeeman(mus[:, None] * np.array([[0.0, 0.0, B]]))
+    th0, ph0 = 1.1, 0.3
+    s = np.array([[math.sin(th0) * math.cos(ph0), math.sin(th0) * math.sin(ph0), math.cos(th0)]])
+    for k in range(nsteps):
+        o.heun_step(s, dt, k * dt)
+    t = nsteps * dt
+    th = 2 * math.atan(math.tan(th0 / 2) * math.exp(-alpha * g * B * t))
+    ph = ph0 + g * B * t
+    ref = np.array([math.sin(th) * math.cos(ph), math.sin(th) * math.sin(ph), math.cos(th)])
+    assert np.abs(s[0] - ref).max() < 1e-8
+    assert abs(np.linalg.norm(s[0]) - 1) < 4e-16
+
+
+def test_unit_length_and_energy_descent_at_zero_temperature():
+    b = S.c1(8)
+    o = b.oracle()
+    s = b.s0.copy()
+    e_prev = sum(o.term_energy(t, s) for t in range(len(b.order)))
+    for k in range(200):
+        o.heun_step(s, b.dt, k * b.dt)
+        if k % 20 == 19:
+            e = sum(o.term_energy(t, s) for t in range(len(b.order)))
+            assert e <= e_prev + 1e-9
+            e_prev = e
+    assert np.abs(np.linalg.norm(s, axis=1) - 1).max() < 1e-15
+
+
+def test_thermostat_sigma_closed_form():
+    mus = np.array([2.0 * S.BOHR, 1.5 * S.BOHR])
+    g = np.array([S.gyro_e()] * 2)
+    a = np.array([0.05, 0.2])
+    dt = 5e-4
+    sig = po.thermostat_sigma(mus, g, a, dt)
+    assert np.allclose(sig, np.sqrt(2 * S.KB * a / (mus * g * dt)), rtol=1e-15)
+    sig_g = po.thermostat_sigma(mus, g, a, dt, gilbert=True)
+    assert np.allclose(sig_g, np.sqrt(2 * S.KB * a / (mus * g * dt * (1 + a * a))), rtol=1e-15)
+
+
+def paramagnet_mean_sz(gilbert, n=2048, B=10.0, T=10.0, alpha=1.0, dt=1e-3, steps=10000):
+    mu = 2.0 * S.BOHR
+    mus = np.full(n, mu)
+    g = np.full(n, S.gyro_e() / (1 + alpha * alpha) if gilbert else S.gyro_e())
+    al = np.full(n, alpha)
+    o = po.OracleSystem(mus, g, al)
+    o.add_zeeman(mus[:, None] * np.array([[0.0, 0.0, B]]))
+    sigma = po.thermostat_sigma(mus, g, al, dt, gilbert=gilbert)
+    s = np.tile([0.0, 0.0, 1.0], (n, 1))
+    acc = []
+    for k in range(steps):
+        w = po.scale_noise(po.normals(77, k, n), sigma, T)
+        o.heun_step(s, dt, k * dt, w)
+        if k >= 5000 and k % 10 == 0:
+            acc.append(s[:, 2].mean())
+    return np.mean(acc), mu * B / (S.KB * T)
+
+
+def langevin(x):
+    return 1 / math.tanh(x) - 1 / x
+
+
+def test_langevin_function_paramagnet_gilbert_form():
+    """Non-interacting spins in B with solver.gilbert_prefactor = true
+    (gyro / (1+alpha^2), sigma with the same denominator):
+    <s_z> = coth(x) - 1/x, x = mus B / (kB T)."""
+    m, x = paramagnet_mean_sz(True)
+    assert abs(m - langevin(x)) < 0.01, (m, langevin(x))
+
+
+def test_langevin_function_paramagnet_ll_form_as_coded():
+    """Without gilbert_prefactor the reference pairs the LL equation with
+    sigma^2 = 2 kB alpha / (mus gyro dt): equilibrium is at (1 + alpha^2) T."""
+    m, x = paramagnet_mean_sz(False)
+    assert abs(m - langevin(x / 2.0)) < 0.01, (m, langevin(x / 2.0))
+
+
+def test_kahan_sums_exact_on_integers():
+    rng = np.random.default_rng(5)
+    x = rng.integers(-1000, 1000, size=(10000, 3)).astype(np.float64)
+    assert np.array_equal(po.kahan_sum3(x), x.sum(axis=0))
+    assert po.kahan_sum(x[:, 0]) == x[:, 0].sum()
+    mus = np.full(10000, 2.0)
+    assert np.array_equal(po.sum_moments(x, mus), (2.0 * x).sum(axis=0))
+
+
+# ---- dipole (reference known answers, test/hamiltonian/test_dipole.h) --------
+DIP_PREFACTOR = None
+
+
+def dipole_prefactor():
+    mu0 = po.vacuum_permeability()
+    return -((0.5 * mu0) / (4 * math.pi)) * (2 * S.BOHR) ** 2 / (0.3e-9) ** 3
+
+
+@pytest.mark.parametrize("spin,expect", [((1, 0, 0), 4.808228), ((0, 1, 0), -2.404114),
+                                         ((0, 0, 1), -2.404114)])
+def test_dipole_bruteforce_1d_chain_known_answers(spin, expect):
+    """1D FM chain of 2000 cells periodic along x (test_dipole.h:256-284)."""
+    lat = po.LatticeSpec(np.eye(3), [[0, 0, 0]], [0], (2000, 1, 1), (1, 0, 0))
+    n = lat.n
+    mus = np.full(n, 2.0 * S.BOHR)
+    s = np.tile(np.asarray(spin, float), (n, 1))
+    h = po.dipole_bruteforce(lat, mus, s, 1000.0, 0.3e-9, nthreads=8)
+    e = -0.5 * np.sum(s * h) / n
+    assert abs(e - dipole_prefactor() * expect) < 1e-5
+
+
+def test_dipole_bruteforce_1d_afm_known_answer():
+    """1D AFM chain along x (test_dipole.h:371-404): x = -3.60617."""
+    lat = po.LatticeSpec(np.eye(3), [[0, 0, 0]], [0], (2000, 1, 1), (1, 0, 0))
+    n = lat.n
+    mus = np.full(n, 2.0 * S.BOHR)
+    s = np.zeros((n, 3))
+    s[:, 0] = np.where(np.arange(n) % 2 == 0, 1.0, -1.0)
+    h = po.dipole_bruteforce(lat, mus, s, 1000.0, 0.3e-9, nthreads=8)
+    e = -0.5 * np.sum(s * h) / n
+    assert abs(e - dipole_prefactor() * -3.60617) < 1e-5
