@@ -48,9 +48,11 @@ extern "C" {
                                  renormalisation, noise and reductions         */
 
 /* exchange storage formats on the device */
-#define JH_XFMT_AUTO 0        /* TYPED when the distinct 3x3 tensors fit, else BLOCK */
+#define JH_XFMT_AUTO 0        /* PATTERN, else TYPED, else BLOCK, whichever fits  */
 #define JH_XFMT_TYPED 1       /* per neighbour: column | tensor id in 32 bits    */
 #define JH_XFMT_BLOCK 2       /* per neighbour: column + explicit 3x3 tensor     */
+#define JH_XFMT_PATTERN 3     /* per spin: offset of its row pattern, a shared
+                                 sequence of (column - row, tensor id) pairs     */
 
 /* term kinds reported by jh_term_kind */
 #define JH_TERM_EXCHANGE 0
@@ -145,10 +147,6 @@ int jh_get_noise(jh_ctx *ctx, int64_t step, double *w);
 /* ---- engine options ----------------------------------------------------- */
 int jh_set_precision(jh_ctx *ctx, int mode);
 int jh_set_exchange_format(jh_ctx *ctx, int fmt);
-/* Lattice hint: lets the engine replace stored neighbour columns by offsets
- * when the exchange is translation invariant.  Optional. */
-int jh_set_lattice_hint(jh_ctx *ctx, const int32_t size[3], const int32_t periodic[3],
-                        int nbasis);
 
 /* ---- stepping (Solver::run, one call = nsteps Heun steps) --------------- */
 int jh_step(jh_ctx *ctx, int64_t nsteps);

@@ -38,9 +38,10 @@ __device__ __forceinline__ double u01(uint32_t hi, uint32_t lo) {
   return (double)(2 * k + 1) * 0x1.0p-53;
 }
 
-// log(x) for a positive normal double with plain IEEE operations in a fixed
-// order: x = m 2^e, m in [sqrt(1/2), sqrt(2)), log m = 2 atanh(s) with
-// s = (m-1)/(m+1) summed as the odd series to s^21.  About 1 ulp.
+// log(x) for a positive normal double: x = m 2^e, m in [sqrt(1/2), sqrt(2)),
+// log m = 2 atanh(s) with s = (m-1)/(m+1) summed as the odd series to s^21 by
+// fused multiply-add Horner steps.  Every operation is one correctly rounded
+// IEEE-754 operation (fma included), so the CPU oracle reproduces the bits.
 __device__ __forceinline__ double noise_log(double x) {
   uint64_t b = (uint64_t)__double_as_longlong(x);
   int e = (int)((b >> 52) & 0x7ff) - 1023;
@@ -51,15 +52,18 @@ __device__ __forceinline__ double noise_log(double x) {
   const double s = f / (2.0 + f);
   const double z = s * s;
   double p = 1.0 / 21.0;
-  p = p * z + 1.0 / 19.0; p = p * z + 1.0 / 17.0; p = p * z + 1.0 / 15.0;
-  p = p * z + 1.0 / 13.0; p = p * z + 1.0 / 11.0; p = p * z + 1.0 / 9.0;
-  p = p * z + 1.0 / 7.0;  p = p * z + 1.0 / 5.0;  p = p * z + 1.0 / 3.0;
-  const double lm = 2.0 * s + (2.0 * s) * (z * p);
-  return (double)e * 0.6931471805599453 + lm;
+  p = __builtin_fma(p, z, 1.0 / 19.0); p = __builtin_fma(p, z, 1.0 / 17.0);
+  p = __builtin_fma(p, z, 1.0 / 15.0); p = __builtin_fma(p, z, 1.0 / 13.0);
+  p = __builtin_fma(p, z, 1.0 / 11.0); p = __builtin_fma(p, z, 1.0 / 9.0);
+  p = __builtin_fma(p, z, 1.0 / 7.0);  p = __builtin_fma(p, z, 1.0 / 5.0);
+  p = __builtin_fma(p, z, 1.0 / 3.0);
+  const double s2 = 2.0 * s;
+  const double lm = __builtin_fma(s2, z * p, s2);
+  return __builtin_fma((double)e, 0.6931471805599453, lm);
 }
 
 // cos(2 pi u), sin(2 pi u) for u in (0, 1): exact quadrant reduction, Taylor
-// series on [-pi/4, pi/4], exact quadrant rotation.
+// series on [-pi/4, pi/4] by fma Horner steps, exact quadrant rotation.
 __device__ __forceinline__ void noise_sincos2pi(double u, double &c, double &s) {
   const double v = u * 4.0;
   const double q = rint(v);
@@ -67,25 +71,25 @@ __device__ __forceinline__ void noise_sincos2pi(double u, double &c, double &s) 
   const double x = r * 1.5707963267948966;
   const double z = x * x;
   double ps = -1.0 / 121645100408832000.0;
-  ps = ps * z + 1.0 / 355687428096000.0;
-  ps = ps * z - 1.0 / 1307674368000.0;
-  ps = ps * z + 1.0 / 6227020800.0;
-  ps = ps * z - 1.0 / 39916800.0;
-  ps = ps * z + 1.0 / 362880.0;
-  ps = ps * z - 1.0 / 5040.0;
-  ps = ps * z + 1.0 / 120.0;
-  ps = ps * z - 1.0 / 6.0;
-  const double sn = x + x * (z * ps);
+  ps = __builtin_fma(ps, z, 1.0 / 355687428096000.0);
+  ps = __builtin_fma(ps, z, -1.0 / 1307674368000.0);
+  ps = __builtin_fma(ps, z, 1.0 / 6227020800.0);
+  ps = __builtin_fma(ps, z, -1.0 / 39916800.0);
+  ps = __builtin_fma(ps, z, 1.0 / 362880.0);
+  ps = __builtin_fma(ps, z, -1.0 / 5040.0);
+  ps = __builtin_fma(ps, z, 1.0 / 120.0);
+  ps = __builtin_fma(ps, z, -1.0 / 6.0);
+  const double sn = __builtin_fma(x, z * ps, x);
   double pc = 1.0 / 6402373705728000.0;
-  pc = pc * z - 1.0 / 20922789888000.0;
-  pc = pc * z + 1.0 / 87178291200.0;
-  pc = pc * z - 1.0 / 479001600.0;
-  pc = pc * z + 1.0 / 3628800.0;
-  pc = pc * z - 1.0 / 40320.0;
-  pc = pc * z + 1.0 / 720.0;
-  pc = pc * z - 1.0 / 24.0;
-  pc = pc * z + 0.5;
-  const double cs = 1.0 - z * pc;
+  pc = __builtin_fma(pc, z, -1.0 / 20922789888000.0);
+  pc = __builtin_fma(pc, z, 1.0 / 87178291200.0);
+  pc = __builtin_fma(pc, z, -1.0 / 479001600.0);
+  pc = __builtin_fma(pc, z, 1.0 / 3628800.0);
+  pc = __builtin_fma(pc, z, -1.0 / 40320.0);
+  pc = __builtin_fma(pc, z, 1.0 / 720.0);
+  pc = __builtin_fma(pc, z, -1.0 / 24.0);
+  pc = __builtin_fma(pc, z, 0.5);
+  const double cs = __builtin_fma(-z, pc, 1.0);
   switch (((int)q) & 3) {
     case 0: c = cs;  s = sn;  break;
     case 1: c = -sn; s = cs;  break;
@@ -101,16 +105,22 @@ __device__ __forceinline__ void normals3(uint64_t seed, uint64_t step, uint64_t 
                                          double &z0, double &z1, double &z2) {
   const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
   const uint32_t shi = (uint32_t)(spin >> 32) << 1;
-  u32x4 a = philox4x32_10(u32x4{(uint32_t)spin, 0u | shi, (uint32_t)step, (uint32_t)(step >> 32)}, k0, k1);
-  u32x4 b = philox4x32_10(u32x4{(uint32_t)spin, 1u | shi, (uint32_t)step, (uint32_t)(step >> 32)}, k0, k1);
   double c, s;
-  const double r0 = sqrt(-2.0 * noise_log(u01(a.x, a.y)));
-  noise_sincos2pi(u01(a.z, a.w), c, s);
-  z0 = r0 * c;
-  z1 = r0 * s;
-  const double r1 = sqrt(-2.0 * noise_log(u01(b.x, b.y)));
-  noise_sincos2pi(u01(b.z, b.w), c, s);
-  z2 = r1 * c;
+  {
+    const u32x4 a = philox4x32_10(
+        u32x4{(uint32_t)spin, 0u | shi, (uint32_t)step, (uint32_t)(step >> 32)}, k0, k1);
+    const double r0 = sqrt(-2.0 * noise_log(u01(a.x, a.y)));
+    noise_sincos2pi(u01(a.z, a.w), c, s);
+    z0 = r0 * c;
+    z1 = r0 * s;
+  }
+  {
+    const u32x4 b = philox4x32_10(
+        u32x4{(uint32_t)spin, 1u | shi, (uint32_t)step, (uint32_t)(step >> 32)}, k0, k1);
+    const double r1 = sqrt(-2.0 * noise_log(u01(b.x, b.y)));
+    noise_sincos2pi(u01(b.z, b.w), c, s);
+    z2 = r1 * c;
+  }
 }
 
 // rhs = -gyro (s x h + alpha s x (s x h)) with the reference's grouping

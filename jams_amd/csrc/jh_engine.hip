@@ -152,11 +152,13 @@ struct jh_ctx {
   bool have_dt = false;
 
   // state
-  DevBuf<double> s, sp, ds, winj, hext, hext2, fout;
+  DevBuf<double> s, sp, ds, winj, wbuf, hext, fout;
   DevBuf<float> s32, sp32;
   DevBuf<double> red_partial, red_out;
   std::vector<double> h_stage;   // host staging for AoS <-> SoA
   bool spins_set = false;
+  int order_nb = 1;              // device order: basis-major with this period
+  int64_t order_cells = 0;
   int64_t iteration = 0;
   double time = 0.0;
 
@@ -174,7 +176,6 @@ struct jh_ctx {
   // options
   int prec = PR_FP64;
   int xfmt_req = JH_XFMT_AUTO;
-  int hint_size[3] = {0, 0, 0}, hint_periodic[3] = {0, 0, 0}, hint_nbasis = 0;
 
   // prepared device state
   bool dirty = true;
@@ -187,6 +188,8 @@ struct jh_ctx {
   DevBuf<uint32_t> ent;
   DevBuf<double> tab64, val64;
   DevBuf<float> tab32, val32;
+  DevBuf<uint32_t> pofs, pat;
+  int pat_words = 0, pat_lds = 0, tbits = 0, npatterns = 0;
   DevBuf<uint16_t> cls;
   DevBuf<double> ctab;
   int nclass = 0;
@@ -201,14 +204,24 @@ void require_sites(jh_ctx *c) {
   if (c->n <= 0) JH_THROW(JH_ERR_STATE, "jh_set_sites must be called first");
 }
 
+// device index <-> reference index under the basis-major order
+inline int64_t ref_of(const jh_ctx *c, int64_t d) {
+  return c->order_nb == 1 ? d : (d % c->order_cells) * c->order_nb + d / c->order_cells;
+}
+inline int64_t dev_of(const jh_ctx *c, int64_t i) {
+  return c->order_nb == 1 ? i : (i % c->order_nb) * c->order_cells + i / c->order_nb;
+}
+
+// reference AoS (N x 3) -> device SoA in device order
 void upload_soa(jh_ctx *c, const double *aos, double *dst) {
   const int64_t n = c->n;
   c->h_stage.resize((size_t)(3 * n));
   double *st = c->h_stage.data();
-  parallel_for(n, [&](int64_t i) {
-    st[i] = aos[3 * i];
-    st[n + i] = aos[3 * i + 1];
-    st[2 * n + i] = aos[3 * i + 2];
+  parallel_for(n, [&](int64_t d) {
+    const int64_t i = ref_of(c, d);
+    st[d] = aos[3 * i];
+    st[n + d] = aos[3 * i + 1];
+    st[2 * n + d] = aos[3 * i + 2];
   });
   HIPCHK(hipMemcpyAsync(dst, st, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -220,10 +233,11 @@ void download_aos(jh_ctx *c, const double *src, double *aos) {
   double *st = c->h_stage.data();
   HIPCHK(hipMemcpyAsync(st, src, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  parallel_for(n, [&](int64_t i) {
-    aos[3 * i] = st[i];
-    aos[3 * i + 1] = st[n + i];
-    aos[3 * i + 2] = st[2 * n + i];
+  parallel_for(n, [&](int64_t d) {
+    const int64_t i = ref_of(c, d);
+    aos[3 * i] = st[d];
+    aos[3 * i + 1] = st[n + d];
+    aos[3 * i + 2] = st[2 * n + d];
   });
 }
 
@@ -347,6 +361,74 @@ XHost xhost_from_list(int64_t n, int64_t np, const int32_t *pi, const int32_t *p
 
 // ---- prepare: device formats and site classes ----------------------------
 
+// Smallest period nb (N % nb == 0, nb <= 16) with which most rows repeat
+// their (column - row, tensor id) sequence: the number of basis sites of a
+// crystal.  1 when there is no exchange or no such period.
+int choose_order(const XHost &x) {
+  const int64_t n = x.n;
+  std::vector<uint64_t> h((size_t)n);
+  parallel_for(n, [&](int64_t i) {
+    uint64_t v = 1469598103934665603ull ^ (uint64_t)(x.ptr[i + 1] - x.ptr[i]);
+    for (int64_t q = x.ptr[i]; q < x.ptr[i + 1]; ++q) {
+      v = (v ^ (uint64_t)((int64_t)x.col[q] - i)) * 1099511628211ull;
+      v = (v ^ (uint64_t)x.typ[q]) * 1099511628211ull;
+    }
+    h[i] = v;
+  });
+  for (int nb = 1; nb <= 16; ++nb) {
+    if (n % nb != 0 || n < 4 * nb) continue;
+    int64_t same = 0;
+    for (int64_t i = 0; i + nb < n; ++i) same += h[i] == h[i + nb];
+    if (same * 2 > n) return nb;
+  }
+  return 1;
+}
+
+// Group rows by their sequence of (column - row, tensor id).  Entries pack
+// the signed offset above `tbits` bits of tensor id.  Fails (returns false)
+// when an offset does not fit or the rows are too diverse to share patterns.
+bool build_patterns(const jh_ctx *c, const XHost &x, int tbits, std::vector<uint32_t> &pofs,
+                    std::vector<uint32_t> &pat, int &npatterns) {
+  const int64_t n = x.n;
+  const int64_t lim = (int64_t)1 << (31 - tbits);
+  pofs.assign((size_t)n, 0);
+  pat.clear();
+  npatterns = 0;
+  std::unordered_map<uint64_t, std::vector<uint32_t>> seen;  // hash -> pattern offsets
+  std::vector<uint32_t> row;
+  for (int64_t d = 0; d < n; ++d) {
+    const int64_t i = ref_of(c, d);
+    row.clear();
+    uint64_t h = 1469598103934665603ull ^ (uint64_t)(x.ptr[i + 1] - x.ptr[i]);
+    for (int64_t q = x.ptr[i]; q < x.ptr[i + 1]; ++q) {
+      const int64_t off = dev_of(c, x.col[q]) - d;
+      if (off >= lim || off < -lim) return false;
+      const uint32_t w = ((uint32_t)(int32_t)off << tbits) | (uint32_t)x.typ[q];
+      row.push_back(w);
+      h = (h ^ w) * 1099511628211ull;
+    }
+    auto &bucket = seen[h];
+    uint32_t found = UINT32_MAX;
+    for (uint32_t o : bucket) {
+      if (pat[o] == row.size() && std::equal(row.begin(), row.end(), pat.begin() + o + 1)) {
+        found = o;
+        break;
+      }
+    }
+    if (found == UINT32_MAX) {
+      if (pat.size() + row.size() + 1 > ((size_t)1 << 24)) return false;
+      found = (uint32_t)pat.size();
+      pat.push_back((uint32_t)row.size());
+      pat.insert(pat.end(), row.begin(), row.end());
+      bucket.push_back(found);
+      ++npatterns;
+    }
+    pofs[d] = found;
+  }
+  // worth it only if the table is much smaller than per-neighbour storage
+  return (int64_t)pat.size() * 8 < x.ptr[n] + 1024;
+}
+
 void prepare_exchange(jh_ctx *c) {
   c->xv = XV_NONE;
   c->ntab = 0;
@@ -368,11 +450,50 @@ void prepare_exchange(jh_ctx *c) {
   while ((1 << tbits) < ntab) ++tbits;
   const size_t lds_need = (size_t)ntab * vpt * (c->prec == PR_FP64 ? 8 : 4);
   bool typed_ok = (colbits + tbits <= 32) && lds_need <= kMaxTableLds;
+  // row patterns: identical sequences of (column - row, tensor id)
+  std::vector<uint32_t> pofs, pat;
+  bool pattern_ok = typed_ok && (c->xfmt_req == JH_XFMT_AUTO || c->xfmt_req == JH_XFMT_PATTERN) &&
+                    build_patterns(c, x, tbits, pofs, pat, c->npatterns);
   int fmt = c->xfmt_req;
-  if (fmt == JH_XFMT_AUTO) fmt = typed_ok ? JH_XFMT_TYPED : JH_XFMT_BLOCK;
-  if (fmt == JH_XFMT_TYPED && !typed_ok)
+  if (fmt == JH_XFMT_AUTO) fmt = pattern_ok ? JH_XFMT_PATTERN : typed_ok ? JH_XFMT_TYPED : JH_XFMT_BLOCK;
+  if ((fmt == JH_XFMT_TYPED || fmt == JH_XFMT_PATTERN) && !typed_ok)
     JH_THROW(JH_ERR_UNSUPPORTED, "TYPED exchange format does not fit (too many distinct tensors)");
+  if (fmt == JH_XFMT_PATTERN && !pattern_ok)
+    JH_THROW(JH_ERR_UNSUPPORTED, "PATTERN exchange format does not fit (too many distinct rows)");
   c->xfmt = fmt;
+  if (fmt == JH_XFMT_PATTERN) {
+    std::vector<double> t64((size_t)ntab * vpt, 0.0);
+    std::vector<float> t32((size_t)ntab * vpt, 0.f);
+    for (int t = 0; t < ntypes; ++t)
+      for (int q = 0; q < vpt; ++q) {
+        const double v = diag ? x.tab[9 * t + 4 * q] : x.tab[9 * t + q];
+        t64[(size_t)t * vpt + q] = v;
+        t32[(size_t)t * vpt + q] = (float)v;
+      }
+    c->tab64.alloc(t64.size());
+    c->tab32.alloc(t32.size());
+    HIPCHK(hipMemcpy(c->tab64.p, t64.data(), sizeof(double) * t64.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->tab32.p, t32.data(), sizeof(float) * t32.size(), hipMemcpyHostToDevice));
+    c->pofs.alloc(pofs.size());
+    c->pat.alloc(pat.size());
+    HIPCHK(hipMemcpy(c->pofs.p, pofs.data(), sizeof(uint32_t) * pofs.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->pat.p, pat.data(), sizeof(uint32_t) * pat.size(), hipMemcpyHostToDevice));
+    c->pat_words = (int)pat.size();
+    c->pat_lds = (((lds_need + 15) & ~(size_t)15) + pat.size() * 4 <= kMaxTableLds) ? 1 : 0;
+    c->tbits = tbits;
+    c->ntab = ntab;
+    c->nent = x.ptr[n];
+    c->colbits = 0;
+    c->xv = c->pat_lds ? (diag ? XV_PDIAG : XV_PFULL) : (diag ? XV_GDIAG : XV_GFULL);
+    c->slice_off.release();
+    c->ent.release();
+    c->val64.release();
+    c->val32.release();
+    return;
+  }
+  c->pofs.release();
+  c->pat.release();
+  c->pat_words = 0;
   const bool typed = fmt == JH_XFMT_TYPED;
   if (!typed) colbits = 32;
   const uint32_t colmask = colbits >= 32 ? 0xFFFFFFFFu : ((1u << colbits) - 1u);
@@ -381,8 +502,10 @@ void prepare_exchange(jh_ctx *c) {
   std::vector<int64_t> off((size_t)nsl + 1, 0);
   for (int64_t s = 0; s < nsl; ++s) {
     int64_t w = 0;
-    for (int64_t i = s * kSlice; i < std::min(n, (s + 1) * kSlice); ++i)
+    for (int64_t d = s * kSlice; d < std::min(n, (s + 1) * kSlice); ++d) {
+      const int64_t i = ref_of(c, d);
       w = std::max(w, x.ptr[i + 1] - x.ptr[i]);
+    }
     off[s + 1] = off[s] + w * kSlice;
   }
   const int64_t E = off[nsl];
@@ -398,15 +521,16 @@ void prepare_exchange(jh_ctx *c) {
     const int64_t w = (off[s + 1] - off[s]) / kSlice;
     for (int64_t k = 0; k < w; ++k) {
       for (int l = 0; l < kSlice; ++l) {
-        const int64_t i = s * kSlice + l;
+        const int64_t d = s * kSlice + l;
+        const int64_t i = d < n ? ref_of(c, d) : 0;
         const int64_t e = off[s] + k * kSlice + l;
-        const int64_t len = i < n ? x.ptr[i + 1] - x.ptr[i] : 0;
+        const int64_t len = d < n ? x.ptr[i + 1] - x.ptr[i] : 0;
         uint32_t j, t;
         if (k < len) {
-          j = (uint32_t)x.col[x.ptr[i] + k];
+          j = (uint32_t)dev_of(c, x.col[x.ptr[i] + k]);
           t = (uint32_t)x.typ[x.ptr[i] + k];
         } else {
-          j = (uint32_t)(i < n ? i : 0);
+          j = (uint32_t)(d < n ? d : 0);
           t = zero_t;
         }
         ent[e] = typed ? (j | (t << colbits)) : j;
@@ -473,7 +597,8 @@ void prepare_classes(jh_ctx *c) {
   std::map<std::vector<uint64_t>, int> map;
   std::vector<double> table;
   std::vector<uint16_t> ids((size_t)n);
-  for (int64_t i = 0; i < n; ++i) {
+  for (int64_t d = 0; d < n; ++d) {
+    const int64_t i = ref_of(c, d);
     std::fill(row.begin(), row.end(), 0.0);
     row[CF_MUS] = c->mus[i];
     row[CF_GYRO] = c->gyro[i];
@@ -508,7 +633,7 @@ void prepare_classes(jh_ctx *c) {
     } else {
       id = it->second;
     }
-    ids[i] = (uint16_t)id;
+    ids[d] = (uint16_t)id;
   }
   c->nclass = (int)map.size();
   c->ctab.alloc(table.size());
@@ -528,6 +653,11 @@ void sync_f32(jh_ctx *c) {
   HIPCHK(launch_to_f32(c->s.p, c->s32.p, 3 * c->n, c->stream));
 }
 
+int dipole_term(const jh_ctx *c) {
+  for (auto &t : c->terms) if (t.kind == TK_DIPOLE) return t.dipole;
+  return -1;
+}
+
 void prepare(jh_ctx *c) {
   require_sites(c);
   if (!c->spins_set) JH_THROW(JH_ERR_STATE, "spins have not been set (jh_set_spins)");
@@ -542,8 +672,25 @@ void prepare(jh_ctx *c) {
   if (nzee > 1 || nani > 1)
     JH_THROW(JH_ERR_UNSUPPORTED, "at most one zeeman and one uniaxial term per context");
   if (c->terms.size() > 4) JH_THROW(JH_ERR_UNSUPPORTED, "at most 4 Hamiltonian terms");
+  // device order: basis-major when rows repeat with the basis period (the
+  // dipole term works on the reference order)
+  const int nb = (!c->xh.empty() && dipole_term(c) < 0) ? choose_order(c->xh[0]) : 1;
+  if (nb != c->order_nb) {
+    std::vector<double> tmp_s((size_t)(3 * c->n)), tmp_w;
+    download_aos(c, c->s.p, tmp_s.data());
+    const bool inj = c->noise_mode == NM_INJECTED && c->winj.p;
+    if (inj) {
+      tmp_w.resize((size_t)(3 * c->n));
+      download_aos(c, c->winj.p, tmp_w.data());
+    }
+    c->order_nb = nb;
+    c->order_cells = c->n / nb;
+    upload_soa(c, tmp_s.data(), c->s.p);
+    if (inj) upload_soa(c, tmp_w.data(), c->winj.p);
+  }
   prepare_exchange(c);
   prepare_classes(c);
+  if (c->noise_mode == NM_PHILOX) c->wbuf.alloc((size_t)(3 * c->n));
   sync_f32(c);
   HIPCHK(hipStreamSynchronize(c->stream));
   c->dirty = false;
@@ -568,6 +715,10 @@ KParams make_params(jh_ctx *c) {
   p.x.tab32 = c->tab32.p;
   p.x.val64 = c->val64.p;
   p.x.val32 = c->val32.p;
+  p.x.pofs = c->pofs.p;
+  p.x.pat = c->pat.p;
+  p.x.pat_words = c->pat_words;
+  p.x.tbits = c->tbits;
   p.nterms = (int)c->terms.size();
   for (int q = 0; q < p.nterms; ++q) p.kind[q] = c->terms[q].kind;
   p.hext = c->hext.p;
@@ -577,14 +728,13 @@ KParams make_params(jh_ctx *c) {
   p.seed = c->seed;
   p.sqrtT = std::sqrt(c->temperature);
   p.winj = c->winj.p;
+  p.wbuf = c->wbuf.p;
   p.dt = c->dt;
+  p.order_nb = c->order_nb;
+  p.order_cells = c->order_cells;
   return p;
 }
 
-int dipole_term(const jh_ctx *c) {
-  for (auto &t : c->terms) if (t.kind == TK_DIPOLE) return t.dipole;
-  return -1;
-}
 
 void do_steps(jh_ctx *c, int64_t nsteps) {
   KParams p = make_params(c);
@@ -713,6 +863,8 @@ int jh_set_sites(jh_ctx *c, int64_t n, const double *mus, const double *gyro, co
     c->red_partial.alloc((size_t)(4 * nb));
     c->red_out.alloc(4);
     c->spins_set = false;
+    c->order_nb = 1;
+    c->order_cells = n;
     c->noise_mode = NM_NONE;
     c->iteration = 0;
     c->time = 0.0;
@@ -892,21 +1044,10 @@ int jh_set_precision(jh_ctx *c, int mode) {
 
 int jh_set_exchange_format(jh_ctx *c, int fmt) {
   return guarded(c, [&] {
-    if (fmt != JH_XFMT_AUTO && fmt != JH_XFMT_TYPED && fmt != JH_XFMT_BLOCK)
+    if (fmt != JH_XFMT_AUTO && fmt != JH_XFMT_TYPED && fmt != JH_XFMT_BLOCK && fmt != JH_XFMT_PATTERN)
       JH_THROW(JH_ERR_INVALID, "unknown exchange format");
     c->xfmt_req = fmt;
     c->dirty = true;
-  });
-}
-
-int jh_set_lattice_hint(jh_ctx *c, const int32_t size[3], const int32_t periodic[3], int nbasis) {
-  return guarded(c, [&] {
-    if (!size || !periodic || nbasis <= 0) JH_THROW(JH_ERR_INVALID, "jh_set_lattice_hint: bad arguments");
-    for (int q = 0; q < 3; ++q) {
-      c->hint_size[q] = size[q];
-      c->hint_periodic[q] = periodic[q];
-    }
-    c->hint_nbasis = nbasis;
   });
 }
 
@@ -1046,7 +1187,10 @@ int jh_phase_bytes(jh_ctx *c, int phase, double *fmt_bytes, double *csr_bytes) {
     const double sv = c->prec == PR_FP64 ? 8.0 : 4.0;
     const int vpb = (c->xv == XV_BDIAG || c->xv == XV_TDIAG) ? 3 : 9;
     double mat = 0.0, gather = 0.0;
-    if (c->xv != XV_NONE) {
+    if (xv_pattern(c->xv)) {
+      mat = n * 4.0 + (c->pat_lds ? 0.0 : (double)c->pat_words * 4.0);
+      gather = 3.0 * n * sv;
+    } else if (c->xv != XV_NONE) {
       mat = (double)c->nent * 4.0 + (double)(c->slice_off.n) * 8.0;
       if (c->xv == XV_BDIAG || c->xv == XV_BFULL) mat += (double)c->nent * vpb * sv;
       // every neighbour vector read once (ideal reuse of the gathers)
@@ -1061,6 +1205,7 @@ int jh_phase_bytes(jh_ctx *c, int phase, double *fmt_bytes, double *csr_bytes) {
     if (c->prec == PR_FP32) state += 3 * n * 4;  // FP32 copy written
     if (c->nclass > 1) state += 2 * n;
     if (c->noise_mode == NM_INJECTED) state += 3 * n * 8;
+    if (c->noise_mode == NM_PHILOX) state += 3 * n * 8;  // noise written, then read back
     if (dipole_term(c) >= 0) state += 3 * n * 8;
     *fmt_bytes = mat + gather + state;
     // reference CSR SpMV (FP64 values, int32 indices): nnz (8+4) + (3N+1) 4 + 3N 8 + 3N 8
@@ -1072,13 +1217,16 @@ int jh_phase_bytes(jh_ctx *c, int phase, double *fmt_bytes, double *csr_bytes) {
 int jh_describe(jh_ctx *c, char *buf, int64_t buflen) {
   return guarded(c, [&] {
     prepare(c);
-    static const char *xvn[] = {"none", "typed-diag", "typed-full", "block-diag", "block-full"};
+    static const char *xvn[] = {"none", "typed-diag", "typed-full", "block-diag", "block-full",
+                                "pattern-diag", "pattern-full", "pattern-diag-global",
+                                "pattern-full-global"};
     char tmp[512];
     std::snprintf(tmp, sizeof tmp,
-                  "n=%lld prec=%s exchange=%s entries=%lld tensors=%d colbits=%d classes=%d "
-                  "terms=%d noise=%d",
+                  "n=%lld prec=%s exchange=%s entries=%lld tensors=%d colbits=%d patterns=%d "
+                  "pattern_words=%d lds=%d order=%d classes=%d terms=%d noise=%d",
                   (long long)c->n, c->prec == PR_FP64 ? "fp64" : "fp32-field", xvn[c->xv],
-                  (long long)c->nent, c->ntab > 0 ? c->ntab - 1 : 0, c->colbits, c->nclass,
+                  (long long)c->nent, c->ntab > 0 ? c->ntab - 1 : 0, c->colbits,
+                  c->npatterns, c->pat_words, c->pat_lds, c->order_nb, c->nclass,
                   (int)c->terms.size(), c->noise_mode);
     if (buf && buflen > 0) {
       std::strncpy(buf, tmp, (size_t)buflen - 1);

@@ -10,6 +10,17 @@
 //    slice_off[s] + 64 k + l (coalesced).  TYPED entries pack the neighbour
 //    column and the index of its 3x3 tensor in a table held in LDS; BLOCK
 //    entries hold the column and the explicit tensor (FP64 or FP32);
+//  - PATTERN entries: rows with the same sequence of (column - row, tensor)
+//    pairs (every interior site of a crystal, and each kind of boundary site)
+//    share one pattern; a spin stores only the word offset of its pattern in
+//    a small table (LDS-resident when it fits), so a translation-invariant
+//    lattice costs 4 bytes per spin and no dependent load precedes the
+//    neighbour gathers;
+//  - device spin order is basis-major when the rows repeat with a period nb
+//    (one pattern per basis site of a crystal): device index d holds
+//    reference spin ((d mod C) nb + d / C), C = N / nb, so the lanes of a
+//    wave read contiguous neighbours; the reference (AoS) order is restored
+//    at the C ABI;
 //  - per-spin constants (mus, gyro, alpha, sigma, Zeeman and anisotropy
 //    parameters) are de-duplicated into a small table of site classes with
 //    a 16-bit class index per spin (no index at all when there is one class).
@@ -27,7 +38,17 @@ enum TermKind { TK_EXCHANGE = 0, TK_ZEEMAN = 1, TK_UNIAXIAL = 2, TK_DIPOLE = 3 }
 enum NoiseMode { NM_NONE = 0, NM_PHILOX = 1, NM_INJECTED = 2 };
 enum Prec { PR_FP64 = 0, PR_FP32 = 1 };
 // exchange kernel variants
-enum XVariant { XV_NONE = 0, XV_TDIAG = 1, XV_TFULL = 2, XV_BDIAG = 3, XV_BFULL = 4 };
+// (PDIAG/PFULL keep the pattern table in LDS, GDIAG/GFULL in global memory)
+enum XVariant {
+  XV_NONE = 0, XV_TDIAG = 1, XV_TFULL = 2, XV_BDIAG = 3, XV_BFULL = 4,
+  XV_PDIAG = 5, XV_PFULL = 6, XV_GDIAG = 7, XV_GFULL = 8
+};
+constexpr bool xv_pattern(int v) { return v >= XV_PDIAG && v <= XV_GFULL; }
+constexpr bool xv_pattern_lds(int v) { return v == XV_PDIAG || v == XV_PFULL; }
+constexpr bool xv_typed(int v) { return v == XV_TDIAG || v == XV_TFULL || xv_pattern(v); }
+constexpr bool xv_diag(int v) {
+  return v == XV_TDIAG || v == XV_BDIAG || v == XV_PDIAG || v == XV_GDIAG;
+}
 
 // site-class table: 16 doubles per class
 enum ClassField {
@@ -49,6 +70,12 @@ struct XView {
   const float *tab32;
   const double *val64;        // BLOCK: per entry (3|9) values, slice-major
   const float *val32;
+  // PATTERN: pofs[i] = word offset of spin i's pattern in pat; pat[o] = length,
+  // pat[o+1..o+len] = (offset << tbits) | tensor id, offset = column - row
+  const uint32_t *pofs;
+  const uint32_t *pat;
+  int pat_words;              // table size in 32-bit words
+  int tbits;
 };
 
 struct KParams {
@@ -72,8 +99,18 @@ struct KParams {
   uint64_t step;
   double sqrtT;
   const double *winj;  // injected noise SoA
+  double *wbuf;        // noise of the current step: written by the predictor,
+                       // read by the corrector (SoA)
   double dt;
   double t;
+  int order_nb;         // basis-major period (1: reference order)
+  int64_t order_cells;  // N / order_nb
 };
+
+// reference spin index of device index d (the noise counter uses it, so the
+// stream does not depend on the device layout)
+__host__ __device__ inline int64_t ref_index(const KParams &p, int64_t d) {
+  return p.order_nb == 1 ? d : (d % p.order_cells) * p.order_nb + d / p.order_cells;
+}
 
 }  // namespace jh

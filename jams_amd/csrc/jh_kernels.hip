@@ -13,7 +13,16 @@
 #include "jh_internal.h"
 #include "jh_launch.h"
 
+#ifndef JH_GATHER_UNROLL
+#define JH_GATHER_UNROLL 4
+#endif
+#ifndef JH_WAVES_PER_EU
+#define JH_WAVES_PER_EU 1
+#endif
+
 namespace jh {
+
+constexpr int kGatherUnroll = JH_GATHER_UNROLL;
 
 // ---------------------------------------------------------------------------
 // exchange field of spin i, FP64, reference order
@@ -102,10 +111,80 @@ __device__ __forceinline__ void xfield32(const XView &x, const float *__restrict
   h2 = (double)a2 + (double)c2;
 }
 
+
+// exchange field of spin i from its row pattern, FP64, reference order
+template <int XVk>
+__device__ __forceinline__ void pfield64(const XView &x, const double *__restrict__ src, int64_t n,
+                                         int64_t i, const double *tab, const uint32_t *pat,
+                                         double &h0, double &h1, double &h2) {
+  const uint32_t o = x.pofs[i];
+  const int len = (int)pat[o];
+  const uint32_t tmask = (1u << x.tbits) - 1u;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+#pragma unroll kGatherUnroll
+  for (int k = 1; k <= len; ++k) {
+    const uint32_t w = pat[o + k];
+    const int64_t j = i + (int64_t)(((int32_t)w) >> x.tbits);
+    const uint32_t t = w & tmask;
+    const double sx = src[j], sy = src[n + j], sz = src[2 * n + j];
+    if constexpr (xv_diag(XVk)) {
+      const double *T = tab + 3 * t;
+      a0 = a0 + sx * T[0];
+      a1 = a1 + sy * T[1];
+      a2 = a2 + sz * T[2];
+    } else {
+      const double *T = tab + 9 * t;
+      a0 = a0 + sx * T[0]; a0 = a0 + sy * T[1]; a0 = a0 + sz * T[2];
+      a1 = a1 + sx * T[3]; a1 = a1 + sy * T[4]; a1 = a1 + sz * T[5];
+      a2 = a2 + sx * T[6]; a2 = a2 + sy * T[7]; a2 = a2 + sz * T[8];
+    }
+  }
+  h0 = a0; h1 = a1; h2 = a2;
+}
+
+// the same from FP32 spins and tensors with Dot2-compensated row sums
+template <int XVk>
+__device__ __forceinline__ void pfield32(const XView &x, const float *__restrict__ src, int64_t n,
+                                         int64_t i, const float *tab, const uint32_t *pat,
+                                         double &h0, double &h1, double &h2) {
+  const uint32_t o = x.pofs[i];
+  const int len = (int)pat[o];
+  const uint32_t tmask = (1u << x.tbits) - 1u;
+  float a0 = 0.f, a1 = 0.f, a2 = 0.f, c0 = 0.f, c1 = 0.f, c2 = 0.f;
+#pragma unroll kGatherUnroll
+  for (int k = 1; k <= len; ++k) {
+    const uint32_t w = pat[o + k];
+    const int64_t j = i + (int64_t)(((int32_t)w) >> x.tbits);
+    const uint32_t t = w & tmask;
+    const float sx = src[j], sy = src[n + j], sz = src[2 * n + j];
+    if constexpr (xv_diag(XVk)) {
+      const float *T = tab + 3 * t;
+      dot2_acc(a0, c0, sx, T[0]);
+      dot2_acc(a1, c1, sy, T[1]);
+      dot2_acc(a2, c2, sz, T[2]);
+    } else {
+      const float *T = tab + 9 * t;
+      dot2_acc(a0, c0, sx, T[0]); dot2_acc(a0, c0, sy, T[1]); dot2_acc(a0, c0, sz, T[2]);
+      dot2_acc(a1, c1, sx, T[3]); dot2_acc(a1, c1, sy, T[4]); dot2_acc(a1, c1, sz, T[5]);
+      dot2_acc(a2, c2, sx, T[6]); dot2_acc(a2, c2, sy, T[7]); dot2_acc(a2, c2, sz, T[8]);
+    }
+  }
+  h0 = (double)a0 + (double)c0;
+  h1 = (double)a1 + (double)c1;
+  h2 = (double)a2 + (double)c2;
+}
+
+// bytes of LDS used by the tensor table (the pattern table follows it)
+__host__ __device__ inline size_t table_bytes(int prec, int xv, int ntab) {
+  if (!xv_typed(xv)) return 0;
+  const size_t b = (size_t)ntab * (xv_diag(xv) ? 3 : 9) * (prec == PR_FP64 ? 8 : 4);
+  return (b + 15) & ~(size_t)15;
+}
+
 template <int PREC, int XVk>
 __device__ __forceinline__ void load_table(const XView &x, unsigned char *lds) {
-  if constexpr (XVk == XV_TDIAG || XVk == XV_TFULL) {
-    const int cnt = x.ntab * (XVk == XV_TDIAG ? 3 : 9);
+  if constexpr (xv_typed(XVk)) {
+    const int cnt = x.ntab * (xv_diag(XVk) ? 3 : 9);
     if constexpr (PREC == PR_FP64) {
       double *t = reinterpret_cast<double *>(lds);
       for (int q = threadIdx.x; q < cnt; q += blockDim.x) t[q] = x.tab64[q];
@@ -113,6 +192,10 @@ __device__ __forceinline__ void load_table(const XView &x, unsigned char *lds) {
       float *t = reinterpret_cast<float *>(lds);
       for (int q = threadIdx.x; q < cnt; q += blockDim.x) t[q] = x.tab32[q];
     }
+  }
+  if constexpr (xv_pattern_lds(XVk)) {
+    uint32_t *p = reinterpret_cast<uint32_t *>(lds + table_bytes(PREC, XVk, x.ntab));
+    for (int q = threadIdx.x; q < x.pat_words; q += blockDim.x) p[q] = x.pat[q];
   }
 }
 
@@ -123,6 +206,16 @@ __device__ __forceinline__ void exchange_at(const KParams &p, const double *src6
                                             double &f2) {
   if constexpr (XVk == XV_NONE) {
     f0 = f1 = f2 = 0.0;
+  } else if constexpr (xv_pattern(XVk)) {
+    const uint32_t *pat;
+    if constexpr (xv_pattern_lds(XVk))
+      pat = reinterpret_cast<const uint32_t *>(lds + table_bytes(PREC, XVk, p.x.ntab));
+    else
+      pat = p.x.pat;
+    if constexpr (PREC == PR_FP64)
+      pfield64<XVk>(p.x, src64, p.n, i, reinterpret_cast<const double *>(lds), pat, f0, f1, f2);
+    else
+      pfield32<XVk>(p.x, src32, p.n, i, reinterpret_cast<const float *>(lds), pat, f0, f1, f2);
   } else if constexpr (PREC == PR_FP64) {
     xfield64<XVk>(p.x, src64, p.n, i, reinterpret_cast<const double *>(lds), f0, f1, f2);
   } else {
@@ -135,6 +228,9 @@ __device__ __forceinline__ double cparam(const KParams &p, uint32_t c, int f) {
 }
 
 // Local (non-exchange) term q of spin i with spin vector (s0, s1, s2).
+// GEN = false compiles only the common forms (static Zeeman, K1 anisotropy);
+// GEN = true adds the ac Zeeman field and the K2/K3 powers.
+template <bool GEN>
 __device__ __forceinline__ void local_term(const KParams &p, int kind, uint32_t c, int64_t i,
                                            double s0, double s1, double s2, double t, double &f0,
                                            double &f1, double &f2) {
@@ -143,7 +239,7 @@ __device__ __forceinline__ void local_term(const KParams &p, int kind, uint32_t 
     f0 = cparam(p, c, CF_ZDC + 0);
     f1 = cparam(p, c, CF_ZDC + 1);
     f2 = cparam(p, c, CF_ZDC + 2);
-    if (p.has_ac) {
+    if (GEN && p.has_ac) {
       const double cw = cos(cparam(p, c, CF_ZOMEGA) * t);
       f0 = f0 + cparam(p, c, CF_ZAC + 0) * cw;
       f1 = f1 + cparam(p, c, CF_ZAC + 1) * cw;
@@ -154,7 +250,8 @@ __device__ __forceinline__ void local_term(const KParams &p, int kind, uint32_t 
     const double e0 = cparam(p, c, CF_AXIS + 0), e1 = cparam(p, c, CF_AXIS + 1),
                  e2 = cparam(p, c, CF_AXIS + 2);
     const double d = (e0 * s0 + e1 * s1 + e2 * s2);
-    const double pw = p.ani_power == 2 ? d : pow(d, (double)(p.ani_power - 1));
+    // pow(d, 1) == d exactly, so K1 needs no library call
+    const double pw = (!GEN || p.ani_power == 2) ? d : pow(d, (double)(p.ani_power - 1));
     const double a = cparam(p, c, CF_K) * (double)p.ani_power * pw;
     f0 = a * e0;
     f1 = a * e1;
@@ -168,7 +265,7 @@ __device__ __forceinline__ void local_term(const KParams &p, int kind, uint32_t 
 
 // Total field (meV) of spin i, terms summed in config order
 // (core/solver.cc:43-57: h = f0; h = h + 1.0 * f_k).
-template <int PREC, int XVk>
+template <int PREC, int XVk, bool GEN>
 __device__ __forceinline__ void total_field(const KParams &p, const double *src64,
                                             const float *src32, int64_t i, uint32_t c,
                                             const unsigned char *lds, double s0, double s1,
@@ -182,7 +279,7 @@ __device__ __forceinline__ void total_field(const KParams &p, const double *src6
     if (k == TK_EXCHANGE) {
       f0 = x0; f1 = x1; f2 = x2;
     } else {
-      local_term(p, k, c, i, s0, s1, s2, t, f0, f1, f2);
+      local_term<GEN>(p, k, c, i, s0, s1, s2, t, f0, f1, f2);
     }
     if (q == 0) {
       h0 = f0; h1 = f1; h2 = f2;
@@ -193,11 +290,11 @@ __device__ __forceinline__ void total_field(const KParams &p, const double *src6
   if (p.nterms == 0) { h0 = h1 = h2 = 0.0; }
 }
 
-template <int PREC, int XVk, int PHASE>
-__global__ void __launch_bounds__(kBlock) k_heun(const KParams p) {
+template <int PREC, int XVk, int PHASE, bool GEN>
+__global__ void __launch_bounds__(kBlock, JH_WAVES_PER_EU) k_heun(const KParams p) {
   extern __shared__ __align__(16) unsigned char lds[];
   load_table<PREC, XVk>(p.x, lds);
-  if constexpr (XVk == XV_TDIAG || XVk == XV_TFULL) __syncthreads();
+  if constexpr (xv_typed(XVk)) __syncthreads();
   const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
   const int64_t i = blk * kBlock + threadIdx.x;
   if (i >= p.n) return;
@@ -210,19 +307,24 @@ __global__ void __launch_bounds__(kBlock) k_heun(const KParams p) {
   const double t = PHASE == 0 ? p.t : p.t + p.dt;
 
   double h0, h1, h2;
-  total_field<PREC, XVk>(p, src, p.g32, i, c, lds, s0, s1, s2, t, h0, h1, h2);
+  total_field<PREC, XVk, GEN>(p, src, p.g32, i, c, lds, s0, s1, s2, t, h0, h1, h2);
 
-  // h = w + h / mus  (solvers/cpu_llg_heun.cc:68-81)
+  // h = w + h / mus  (solvers/cpu_llg_heun.cc:68-81); one draw per step is
+  // used by both phases (cpu_llg_heun.cc:53-62): the predictor generates it
+  // and leaves it for the corrector
   const double mus = cparam(p, c, CF_MUS);
   if (p.noise == NM_NONE) {
     h0 = h0 / mus; h1 = h1 / mus; h2 = h2 / mus;
   } else {
     double w0, w1, w2;
-    if (p.noise == NM_PHILOX) {
+    if (p.noise == NM_PHILOX && PHASE == 0) {
       double z0, z1, z2;
-      normals3(p.seed, p.step, (uint64_t)i, z0, z1, z2);
+      normals3(p.seed, p.step, (uint64_t)ref_index(p, i), z0, z1, z2);
       const double sg = cparam(p, c, CF_SIGMA);
       w0 = z0 * sg * p.sqrtT; w1 = z1 * sg * p.sqrtT; w2 = z2 * sg * p.sqrtT;
+      p.wbuf[i] = w0; p.wbuf[n + i] = w1; p.wbuf[2 * n + i] = w2;
+    } else if (p.noise == NM_PHILOX) {
+      w0 = p.wbuf[i]; w1 = p.wbuf[n + i]; w2 = p.wbuf[2 * n + i];
     } else {
       w0 = p.winj[i]; w1 = p.winj[n + i]; w2 = p.winj[2 * n + i];
     }
@@ -259,7 +361,7 @@ template <int PREC, int XVk>
 __global__ void __launch_bounds__(kBlock) k_field(const KParams p, int term, double *out) {
   extern __shared__ __align__(16) unsigned char lds[];
   load_table<PREC, XVk>(p.x, lds);
-  if constexpr (XVk == XV_TDIAG || XVk == XV_TFULL) __syncthreads();
+  if constexpr (xv_typed(XVk)) __syncthreads();
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= p.n) return;
   const int64_t n = p.n;
@@ -267,11 +369,11 @@ __global__ void __launch_bounds__(kBlock) k_field(const KParams p, int term, dou
   const double s0 = p.s[i], s1 = p.s[n + i], s2 = p.s[2 * n + i];
   double h0, h1, h2;
   if (term < 0) {
-    total_field<PREC, XVk>(p, p.s, p.g32, i, c, lds, s0, s1, s2, p.t, h0, h1, h2);
+    total_field<PREC, XVk, true>(p, p.s, p.g32, i, c, lds, s0, s1, s2, p.t, h0, h1, h2);
   } else if (p.kind[term] == TK_EXCHANGE) {
     exchange_at<PREC, XVk>(p, p.s, p.g32, i, lds, h0, h1, h2);
   } else {
-    local_term(p, p.kind[term], c, i, s0, s1, s2, p.t, h0, h1, h2);
+    local_term<true>(p, p.kind[term], c, i, s0, s1, s2, p.t, h0, h1, h2);
   }
   out[i] = h0; out[n + i] = h1; out[2 * n + i] = h2;
 }
@@ -282,7 +384,7 @@ __global__ void __launch_bounds__(kBlock) k_noise(const KParams p, double *out) 
   if (i >= p.n) return;
   const uint32_t c = p.cls ? (uint32_t)p.cls[i] : 0u;
   double z0, z1, z2;
-  normals3(p.seed, p.step, (uint64_t)i, z0, z1, z2);
+  normals3(p.seed, p.step, (uint64_t)ref_index(p, i), z0, z1, z2);
   const double sg = cparam(p, c, CF_SIGMA);
   out[i] = z0 * sg * p.sqrtT;
   out[p.n + i] = z1 * sg * p.sqrtT;
@@ -348,20 +450,25 @@ __global__ void __launch_bounds__(kBlock) k_reduce_final(const double *partial, 
 // ---------------------------------------------------------------------------
 // launchers
 
-static size_t table_lds(int prec, int xv, int ntab) {
-  if (xv != XV_TDIAG && xv != XV_TFULL) return 0;
-  const size_t w = (xv == XV_TDIAG ? 3 : 9);
-  return (size_t)ntab * w * (prec == PR_FP64 ? 8 : 4);
+static size_t table_lds(const KParams &p, int prec, int xv) {
+  size_t b = table_bytes(prec, xv, p.x.ntab);
+  if (xv_pattern_lds(xv)) b += (size_t)p.x.pat_words * 4;
+  return b;
 }
 
 template <int PREC, int XVk>
 static hipError_t launch_heun_t(const KParams &p, int phase, hipStream_t st) {
   const int64_t nb = (p.n + kBlock - 1) / kBlock;
-  const size_t lds = table_lds(PREC, XVk, p.x.ntab);
-  if (phase == 0)
-    hipLaunchKernelGGL((k_heun<PREC, XVk, 0>), dim3((unsigned)nb), dim3(kBlock), lds, st, p);
+  const size_t lds = table_lds(p, PREC, XVk);
+  const bool gen = p.has_ac || p.ani_power != 2;
+  if (phase == 0 && !gen)
+    hipLaunchKernelGGL((k_heun<PREC, XVk, 0, false>), dim3((unsigned)nb), dim3(kBlock), lds, st, p);
+  else if (phase == 0)
+    hipLaunchKernelGGL((k_heun<PREC, XVk, 0, true>), dim3((unsigned)nb), dim3(kBlock), lds, st, p);
+  else if (!gen)
+    hipLaunchKernelGGL((k_heun<PREC, XVk, 1, false>), dim3((unsigned)nb), dim3(kBlock), lds, st, p);
   else
-    hipLaunchKernelGGL((k_heun<PREC, XVk, 1>), dim3((unsigned)nb), dim3(kBlock), lds, st, p);
+    hipLaunchKernelGGL((k_heun<PREC, XVk, 1, true>), dim3((unsigned)nb), dim3(kBlock), lds, st, p);
   return hipGetLastError();
 }
 
@@ -373,6 +480,10 @@ static hipError_t launch_heun_p(const KParams &p, int xv, int phase, hipStream_t
     case XV_TFULL: return launch_heun_t<PREC, XV_TFULL>(p, phase, st);
     case XV_BDIAG: return launch_heun_t<PREC, XV_BDIAG>(p, phase, st);
     case XV_BFULL: return launch_heun_t<PREC, XV_BFULL>(p, phase, st);
+    case XV_PDIAG: return launch_heun_t<PREC, XV_PDIAG>(p, phase, st);
+    case XV_PFULL: return launch_heun_t<PREC, XV_PFULL>(p, phase, st);
+    case XV_GDIAG: return launch_heun_t<PREC, XV_GDIAG>(p, phase, st);
+    case XV_GFULL: return launch_heun_t<PREC, XV_GFULL>(p, phase, st);
   }
   return hipErrorInvalidValue;
 }
@@ -385,7 +496,7 @@ hipError_t launch_heun(const KParams &p, int prec, int xv, int phase, hipStream_
 template <int PREC, int XVk>
 static hipError_t launch_field_t(const KParams &p, int term, double *out, hipStream_t st) {
   const int64_t nb = (p.n + kBlock - 1) / kBlock;
-  const size_t lds = table_lds(PREC, XVk, p.x.ntab);
+  const size_t lds = table_lds(p, PREC, XVk);
   hipLaunchKernelGGL((k_field<PREC, XVk>), dim3((unsigned)nb), dim3(kBlock), lds, st, p, term, out);
   return hipGetLastError();
 }
@@ -398,6 +509,10 @@ static hipError_t launch_field_p(const KParams &p, int xv, int term, double *out
     case XV_TFULL: return launch_field_t<PREC, XV_TFULL>(p, term, out, st);
     case XV_BDIAG: return launch_field_t<PREC, XV_BDIAG>(p, term, out, st);
     case XV_BFULL: return launch_field_t<PREC, XV_BFULL>(p, term, out, st);
+    case XV_PDIAG: return launch_field_t<PREC, XV_PDIAG>(p, term, out, st);
+    case XV_PFULL: return launch_field_t<PREC, XV_PFULL>(p, term, out, st);
+    case XV_GDIAG: return launch_field_t<PREC, XV_GDIAG>(p, term, out, st);
+    case XV_GFULL: return launch_field_t<PREC, XV_GFULL>(p, term, out, st);
   }
   return hipErrorInvalidValue;
 }

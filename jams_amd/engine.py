@@ -17,7 +17,7 @@ ENGINE_LIB = LIB_DIR / "libjamship.so"
 JH_OK = 0
 PREC_FP64 = 0
 PREC_FP32_FIELD = 1
-XFMT_AUTO, XFMT_TYPED, XFMT_BLOCK = 0, 1, 2
+XFMT_AUTO, XFMT_TYPED, XFMT_BLOCK, XFMT_PATTERN = 0, 1, 2, 3
 TERM_EXCHANGE, TERM_ZEEMAN, TERM_UNIAXIAL, TERM_DIPOLE = 0, 1, 2, 3
 
 # every entry point declared in include/jams_hip.h
@@ -47,7 +47,6 @@ ABI = {
     "jh_get_noise": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p]),
     "jh_set_precision": (C.c_int, [C.c_void_p, C.c_int]),
     "jh_set_exchange_format": (C.c_int, [C.c_void_p, C.c_int]),
-    "jh_set_lattice_hint": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
     "jh_step": (C.c_int, [C.c_void_p, C.c_int64]),
     "jh_get_iteration": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
     "jh_set_iteration": (C.c_int, [C.c_void_p, C.c_int64]),
@@ -216,11 +215,6 @@ class Engine:
 
     def set_exchange_format(self, fmt: int):
         self._chk(self.lib.jh_set_exchange_format(self.h, int(fmt)))
-
-    def set_lattice_hint(self, size, periodic, nbasis):
-        size = np.ascontiguousarray(size, np.int32)
-        periodic = np.ascontiguousarray(periodic, np.int32)
-        self._chk(self.lib.jh_set_lattice_hint(self.h, _ptr(size), _ptr(periodic), int(nbasis)))
 
     # ---- running ------------------------------------------------------
     def step(self, nsteps: int = 1):

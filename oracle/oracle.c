@@ -74,11 +74,11 @@ double or_u01(uint32_t hi, uint32_t lo) {
   return (double)(2 * k + 1) * 0x1.0p-53;                           /* exact */
 }
 
-/* Natural log of a positive normal double, written with plain IEEE
- * operations in a fixed order so the device engine can produce the same bits
- * (no library call, no fused multiply-add).  x = m 2^e with m in [sqrt(1/2),
+/* Natural log of a positive normal double with IEEE operations in a fixed
+ * order (fma() is the correctly rounded fused multiply-add of C99), so the
+ * device engine produces the same bits.  x = m 2^e with m in [sqrt(1/2),
  * sqrt(2)); log m = 2 atanh(s), s = (m-1)/(m+1), |s| < 0.1716, summed as the
- * odd series 2 s (1 + s^2/3 + s^4/5 + ... + s^20/21). */
+ * odd series 2 s (1 + s^2/3 + ... + s^20/21) by Horner steps. */
 double or_noise_log(double x) {
   uint64_t b;
   memcpy(&b, &x, 8);
@@ -91,16 +91,17 @@ double or_noise_log(double x) {
   const double s = f / (2.0 + f);
   const double z = s * s;
   double p = 1.0 / 21.0;
-  p = p * z + 1.0 / 19.0; p = p * z + 1.0 / 17.0; p = p * z + 1.0 / 15.0;
-  p = p * z + 1.0 / 13.0; p = p * z + 1.0 / 11.0; p = p * z + 1.0 / 9.0;
-  p = p * z + 1.0 / 7.0;  p = p * z + 1.0 / 5.0;  p = p * z + 1.0 / 3.0;
-  const double lm = 2.0 * s + (2.0 * s) * (z * p);
-  return (double)e * 0.6931471805599453 + lm;
+  p = fma(p, z, 1.0 / 19.0); p = fma(p, z, 1.0 / 17.0); p = fma(p, z, 1.0 / 15.0);
+  p = fma(p, z, 1.0 / 13.0); p = fma(p, z, 1.0 / 11.0); p = fma(p, z, 1.0 / 9.0);
+  p = fma(p, z, 1.0 / 7.0);  p = fma(p, z, 1.0 / 5.0);  p = fma(p, z, 1.0 / 3.0);
+  const double s2 = 2.0 * s;
+  const double lm = fma(s2, z * p, s2);
+  return fma((double)e, 0.6931471805599453, lm);
 }
 
 /* cos(2 pi u) and sin(2 pi u) for u in [0, 1): quadrant reduction u*4 = q + r
- * (exact), x = r pi/2 in [-pi/4, pi/4], Taylor series to x^18 / x^19, then
- * an exact quadrant rotation.  Plain IEEE operations in a fixed order. */
+ * (exact), x = r pi/2 in [-pi/4, pi/4], Taylor series to x^18 / x^19 by fma
+ * Horner steps, then an exact quadrant rotation. */
 void or_noise_sincos2pi(double u, double *c, double *s) {
   const double v = u * 4.0;
   const double q = nearbyint(v);
@@ -108,25 +109,25 @@ void or_noise_sincos2pi(double u, double *c, double *s) {
   const double x = r * 1.5707963267948966;
   const double z = x * x;
   double ps = -1.0 / 121645100408832000.0;               /* -1/19! */
-  ps = ps * z + 1.0 / 355687428096000.0;                  /*  1/17! */
-  ps = ps * z - 1.0 / 1307674368000.0;                    /* -1/15! */
-  ps = ps * z + 1.0 / 6227020800.0;                       /*  1/13! */
-  ps = ps * z - 1.0 / 39916800.0;                         /* -1/11! */
-  ps = ps * z + 1.0 / 362880.0;                           /*  1/9!  */
-  ps = ps * z - 1.0 / 5040.0;                             /* -1/7!  */
-  ps = ps * z + 1.0 / 120.0;                              /*  1/5!  */
-  ps = ps * z - 1.0 / 6.0;                                /* -1/3!  */
-  const double sn = x + x * (z * ps);
+  ps = fma(ps, z, 1.0 / 355687428096000.0);               /*  1/17! */
+  ps = fma(ps, z, -1.0 / 1307674368000.0);                /* -1/15! */
+  ps = fma(ps, z, 1.0 / 6227020800.0);                    /*  1/13! */
+  ps = fma(ps, z, -1.0 / 39916800.0);                     /* -1/11! */
+  ps = fma(ps, z, 1.0 / 362880.0);                        /*  1/9!  */
+  ps = fma(ps, z, -1.0 / 5040.0);                         /* -1/7!  */
+  ps = fma(ps, z, 1.0 / 120.0);                           /*  1/5!  */
+  ps = fma(ps, z, -1.0 / 6.0);                            /* -1/3!  */
+  const double sn = fma(x, z * ps, x);
   double pc = 1.0 / 6402373705728000.0;                   /*  1/18! */
-  pc = pc * z - 1.0 / 20922789888000.0;                   /* -1/16! */
-  pc = pc * z + 1.0 / 87178291200.0;                      /*  1/14! */
-  pc = pc * z - 1.0 / 479001600.0;                        /* -1/12! */
-  pc = pc * z + 1.0 / 3628800.0;                          /*  1/10! */
-  pc = pc * z - 1.0 / 40320.0;                            /* -1/8!  */
-  pc = pc * z + 1.0 / 720.0;                              /*  1/6!  */
-  pc = pc * z - 1.0 / 24.0;                               /* -1/4!  */
-  pc = pc * z + 0.5;                                      /*  1/2!  */
-  const double cs = 1.0 - z * pc;
+  pc = fma(pc, z, -1.0 / 20922789888000.0);               /* -1/16! */
+  pc = fma(pc, z, 1.0 / 87178291200.0);                   /*  1/14! */
+  pc = fma(pc, z, -1.0 / 479001600.0);                    /* -1/12! */
+  pc = fma(pc, z, 1.0 / 3628800.0);                       /*  1/10! */
+  pc = fma(pc, z, -1.0 / 40320.0);                        /* -1/8!  */
+  pc = fma(pc, z, 1.0 / 720.0);                           /*  1/6!  */
+  pc = fma(pc, z, -1.0 / 24.0);                           /* -1/4!  */
+  pc = fma(pc, z, 0.5);                                   /*  1/2!  */
+  const double cs = fma(-z, pc, 1.0);
   const int quad = ((int)q) & 3;
   switch (quad) {
     case 0: *c = cs;  *s = sn;  break;

@@ -41,8 +41,9 @@ double or_vacuum_permeability(void);  /* 4*pi*1e-7 / kJoule2meV */
 void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 /* uniform in (0,1): (2k+1) * 2^-53 with k the top 52 bits of (hi:lo) */
 double or_u01(uint32_t hi, uint32_t lo);
-/* log(x) and (cos, sin)(2 pi u) as used by the noise stream: plain IEEE
- * operations in a fixed order, reproduced bit for bit by the device code. */
+/* log(x) and (cos, sin)(2 pi u) as used by the noise stream: correctly
+ * rounded IEEE operations (fma included) in a fixed order, reproduced bit for
+ * bit by the device code. */
 double or_noise_log(double x);
 void or_noise_sincos2pi(double u, double *c, double *s);
 /* Three standard normals for (seed, step, spin): the thermostat stream. */

@@ -40,7 +40,7 @@ def test_c1_trajectory_1000_steps_bitwise(engine_factory):
     assert np.array_equal(got, ref)
 
 
-@pytest.mark.parametrize("fmt", [1, 2])
+@pytest.mark.parametrize("fmt", [1, 2, 3])
 def test_formats_are_bit_identical(engine_factory, fmt):
     """TYPED and BLOCK storage give the same bits (C1 lattice, 50 steps)."""
     b = S.c1(8)
