@@ -43,8 +43,9 @@ extern "C" {
 
 /* precision forms */
 #define JH_PREC_FP64 0        /* FP64 everywhere; exchange sums in CSR order   */
-#define JH_PREC_FP32_FIELD 1  /* FP32 spins + tensors in the exchange matvec with
-                                 a compensated (Dot2) row sum; FP64 Heun state,
+#define JH_PREC_FP32_FIELD 1  /* FP32 spins + tensors in the exchange matvec,
+                                 their exact products accumulated in FP64 with
+                                 one rounding each; FP64 Heun state,
                                  renormalisation, noise and reductions         */
 
 /* exchange storage formats on the device */

@@ -149,18 +149,6 @@ __device__ __forceinline__ void unit_vector(double &v0, double &v1, double &v2) 
   v2 = v2 / len;
 }
 
-// Error-free transformations for the FP32 compensated row sum (Ogita, Rump,
-// Oishi, "Accurate sum and dot product", SIAM J. Sci. Comput. 2005: Dot2).
-__device__ __forceinline__ void dot2_acc(float &sum, float &comp, float a, float b) {
-  const float p = a * b;
-  const float ep = __builtin_fmaf(a, b, -p);
-  const float t = sum + p;
-  const float z = t - sum;
-  const float es = (sum - (t - z)) + (p - z);
-  sum = t;
-  comp = comp + (es + ep);
-}
-
 // XCD-aware block order: hardware hands consecutive workgroups to the 8
 // XCDs round robin; remap so each XCD walks one contiguous range of spins and
 // neighbour gathers hit its own L2.

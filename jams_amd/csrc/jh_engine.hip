@@ -187,7 +187,7 @@ struct jh_ctx {
   DevBuf<int64_t> slice_off;
   DevBuf<uint32_t> ent;
   DevBuf<double> tab64, val64;
-  DevBuf<float> tab32, val32;
+  DevBuf<float> val32;
   DevBuf<uint32_t> pofs, pat;
   int pat_words = 0, pat_lds = 0, tbits = 0, npatterns = 0;
   DevBuf<uint16_t> cls;
@@ -361,6 +361,19 @@ XHost xhost_from_list(int64_t n, int64_t np, const int32_t *pi, const int32_t *p
 
 // ---- prepare: device formats and site classes ----------------------------
 
+// Distinct tensors for the LDS table (diagonal or full rows); the FP32 form
+// stores them rounded to FP32, held as doubles so products stay exact.
+void upload_table(jh_ctx *c, const XHost &x, int ntypes, int ntab, int vpt, bool diag) {
+  std::vector<double> t64((size_t)ntab * vpt, 0.0);
+  for (int t = 0; t < ntypes; ++t)
+    for (int q = 0; q < vpt; ++q) {
+      const double v = diag ? x.tab[9 * t + 4 * q] : x.tab[9 * t + q];
+      t64[(size_t)t * vpt + q] = c->prec == PR_FP64 ? v : (double)(float)v;
+    }
+  c->tab64.alloc(t64.size());
+  HIPCHK(hipMemcpy(c->tab64.p, t64.data(), sizeof(double) * t64.size(), hipMemcpyHostToDevice));
+}
+
 // Smallest period nb (N % nb == 0, nb <= 16) with which most rows repeat
 // their (column - row, tensor id) sequence: the number of basis sites of a
 // crystal.  1 when there is no exchange or no such period.
@@ -448,7 +461,7 @@ void prepare_exchange(jh_ctx *c) {
   while (colbits < 32 && ((int64_t)1 << colbits) < n) ++colbits;
   int tbits = 0;
   while ((1 << tbits) < ntab) ++tbits;
-  const size_t lds_need = (size_t)ntab * vpt * (c->prec == PR_FP64 ? 8 : 4);
+  const size_t lds_need = (size_t)ntab * vpt * 8;
   bool typed_ok = (colbits + tbits <= 32) && lds_need <= kMaxTableLds;
   // row patterns: identical sequences of (column - row, tensor id)
   std::vector<uint32_t> pofs, pat;
@@ -462,18 +475,7 @@ void prepare_exchange(jh_ctx *c) {
     JH_THROW(JH_ERR_UNSUPPORTED, "PATTERN exchange format does not fit (too many distinct rows)");
   c->xfmt = fmt;
   if (fmt == JH_XFMT_PATTERN) {
-    std::vector<double> t64((size_t)ntab * vpt, 0.0);
-    std::vector<float> t32((size_t)ntab * vpt, 0.f);
-    for (int t = 0; t < ntypes; ++t)
-      for (int q = 0; q < vpt; ++q) {
-        const double v = diag ? x.tab[9 * t + 4 * q] : x.tab[9 * t + q];
-        t64[(size_t)t * vpt + q] = v;
-        t32[(size_t)t * vpt + q] = (float)v;
-      }
-    c->tab64.alloc(t64.size());
-    c->tab32.alloc(t32.size());
-    HIPCHK(hipMemcpy(c->tab64.p, t64.data(), sizeof(double) * t64.size(), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->tab32.p, t32.data(), sizeof(float) * t32.size(), hipMemcpyHostToDevice));
+    upload_table(c, x, ntypes, ntab, vpt, diag);
     c->pofs.alloc(pofs.size());
     c->pat.alloc(pat.size());
     HIPCHK(hipMemcpy(c->pofs.p, pofs.data(), sizeof(uint32_t) * pofs.size(), hipMemcpyHostToDevice));
@@ -550,18 +552,7 @@ void prepare_exchange(jh_ctx *c) {
   c->ent.alloc((size_t)std::max<int64_t>(E, 1));
   if (E > 0) HIPCHK(hipMemcpy(c->ent.p, ent.data(), sizeof(uint32_t) * E, hipMemcpyHostToDevice));
   if (typed) {
-    std::vector<double> t64((size_t)ntab * vpt, 0.0);
-    std::vector<float> t32((size_t)ntab * vpt, 0.f);
-    for (int t = 0; t < ntypes; ++t)
-      for (int q = 0; q < vpt; ++q) {
-        const double v = diag ? x.tab[9 * t + 4 * q] : x.tab[9 * t + q];
-        t64[(size_t)t * vpt + q] = v;
-        t32[(size_t)t * vpt + q] = (float)v;
-      }
-    c->tab64.alloc(t64.size());
-    c->tab32.alloc(t32.size());
-    HIPCHK(hipMemcpy(c->tab64.p, t64.data(), sizeof(double) * t64.size(), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->tab32.p, t32.data(), sizeof(float) * t32.size(), hipMemcpyHostToDevice));
+    upload_table(c, x, ntypes, ntab, vpt, diag);
     c->val64.release();
     c->val32.release();
     c->xv = diag ? XV_TDIAG : XV_TFULL;
@@ -648,9 +639,9 @@ void prepare_classes(jh_ctx *c) {
 
 void sync_f32(jh_ctx *c) {
   if (c->prec != PR_FP32) return;
-  c->s32.alloc((size_t)(3 * c->n));
-  c->sp32.alloc((size_t)(3 * c->n));
-  HIPCHK(launch_to_f32(c->s.p, c->s32.p, 3 * c->n, c->stream));
+  c->s32.alloc((size_t)(4 * c->n));
+  c->sp32.alloc((size_t)(4 * c->n));
+  HIPCHK(launch_to_f32(c->s.p, c->s32.p, c->n, c->stream));
 }
 
 int dipole_term(const jh_ctx *c) {
@@ -712,7 +703,6 @@ KParams make_params(jh_ctx *c) {
   p.x.colmask = c->colbits >= 32 ? 0xFFFFFFFFu : ((1u << c->colbits) - 1u);
   p.x.ntab = c->ntab;
   p.x.tab64 = c->tab64.p;
-  p.x.tab32 = c->tab32.p;
   p.x.val64 = c->val64.p;
   p.x.val32 = c->val32.p;
   p.x.pofs = c->pofs.p;
@@ -1151,8 +1141,8 @@ int jh_time_phase(jh_ctx *c, int phase, int reps, double *ms) {
     save.alloc((size_t)(3 * c->n));
     HIPCHK(hipMemcpyAsync(save.p, c->s.p, sizeof(double) * 3 * c->n, hipMemcpyDeviceToDevice, c->stream));
     if (c->prec == PR_FP32) {
-      save32.alloc((size_t)(3 * c->n));
-      HIPCHK(hipMemcpyAsync(save32.p, c->s32.p, sizeof(float) * 3 * c->n, hipMemcpyDeviceToDevice, c->stream));
+      save32.alloc((size_t)(4 * c->n));
+      HIPCHK(hipMemcpyAsync(save32.p, c->s32.p, sizeof(float) * 4 * c->n, hipMemcpyDeviceToDevice, c->stream));
     }
     KParams p = make_params(c);
     p.step = (uint64_t)c->iteration;
@@ -1175,7 +1165,7 @@ int jh_time_phase(jh_ctx *c, int phase, int reps, double *ms) {
     *ms = (double)f / reps;
     HIPCHK(hipMemcpyAsync(c->s.p, save.p, sizeof(double) * 3 * c->n, hipMemcpyDeviceToDevice, c->stream));
     if (c->prec == PR_FP32)
-      HIPCHK(hipMemcpyAsync(c->s32.p, save32.p, sizeof(float) * 3 * c->n, hipMemcpyDeviceToDevice, c->stream));
+      HIPCHK(hipMemcpyAsync(c->s32.p, save32.p, sizeof(float) * 4 * c->n, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
   });
 }
@@ -1189,12 +1179,12 @@ int jh_phase_bytes(jh_ctx *c, int phase, double *fmt_bytes, double *csr_bytes) {
     double mat = 0.0, gather = 0.0;
     if (xv_pattern(c->xv)) {
       mat = n * 4.0 + (c->pat_lds ? 0.0 : (double)c->pat_words * 4.0);
-      gather = 3.0 * n * sv;
+      gather = c->prec == PR_FP64 ? 3.0 * n * 8 : 4.0 * n * 4;
     } else if (c->xv != XV_NONE) {
       mat = (double)c->nent * 4.0 + (double)(c->slice_off.n) * 8.0;
       if (c->xv == XV_BDIAG || c->xv == XV_BFULL) mat += (double)c->nent * vpb * sv;
       // every neighbour vector read once (ideal reuse of the gathers)
-      gather = 3.0 * n * sv;
+      gather = c->prec == PR_FP64 ? 3.0 * n * 8 : 4.0 * n * 4;
     }
     // per-spin state traffic: the FP64 vector the field is evaluated at (its
     // gathers are counted above: in FP64 they are the same array, so only
@@ -1202,7 +1192,7 @@ int jh_phase_bytes(jh_ctx *c, int phase, double *fmt_bytes, double *csr_bytes) {
     double state = c->prec == PR_FP32 || c->xv == XV_NONE ? 3 * n * 8 : 0.0;
     if (phase == 0) state += 3 * n * 8 /*s* w*/ + 3 * n * 8 /*ds w*/;
     else state += 3 * n * 8 /*s_old r*/ + 3 * n * 8 /*ds r*/ + 3 * n * 8 /*s w*/;
-    if (c->prec == PR_FP32) state += 3 * n * 4;  // FP32 copy written
+    if (c->prec == PR_FP32) state += 4 * n * 4;  // FP32 gather copy written
     if (c->nclass > 1) state += 2 * n;
     if (c->noise_mode == NM_INJECTED) state += 3 * n * 8;
     if (c->noise_mode == NM_PHILOX) state += 3 * n * 8;  // noise written, then read back

@@ -66,8 +66,7 @@ struct XView {
   uint32_t colmask;
   int colbits;
   int ntab;                   // table rows incl. the trailing zero tensor
-  const double *tab64;        // TYPED: ntab x (3|9)
-  const float *tab32;
+  const double *tab64;        // TYPED: ntab x (3|9); FP32 form: values rounded to FP32
   const double *val64;        // BLOCK: per entry (3|9) values, slice-major
   const float *val32;
   // PATTERN: pofs[i] = word offset of spin i's pattern in pat; pat[o] = length,
@@ -84,7 +83,7 @@ struct KParams {
   double *sp;          // predictor spins
   double *ds;          // ds/dt accumulator
   double *snew;        // phase 1 output (== s)
-  const float *g32;    // FP32 gather source (s32 for phase 0, sp32 for phase 1)
+  const float *g32;    // FP32 gather source, (x, y, z, 0) per spin (s32 / sp32)
   float *w32;          // FP32 copy written by the phase (sp32 / s32)
   const uint16_t *cls; // per-spin class or null
   const double *ctab;  // class table

@@ -67,50 +67,51 @@ __device__ __forceinline__ void xfield64(const XView &x, const double *__restric
   h0 = a0; h1 = a1; h2 = a2;
 }
 
-// exchange field of spin i, FP32 inputs, Dot2-compensated row sums
+// exchange field of spin i from FP32 storage: spins gathered as FP32 and
+// tensors rounded to FP32 (held as doubles), so every product is exact in
+// FP64 and one fused multiply-add per product accumulates it with a single
+// rounding; the row order is the reference's.
 template <int XVk>
 __device__ __forceinline__ void xfield32(const XView &x, const float *__restrict__ src, int64_t n,
-                                         int64_t i, const float *tab, double &h0, double &h1,
+                                         int64_t i, const double *tab, double &h0, double &h1,
                                          double &h2) {
   const int64_t sl = i >> 6;
   const int lane = (int)(i & 63);
   const int64_t b0 = x.slice_off[sl], b1 = x.slice_off[sl + 1];
-  float a0 = 0.f, a1 = 0.f, a2 = 0.f, c0 = 0.f, c1 = 0.f, c2 = 0.f;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
 #pragma unroll 4
   for (int64_t e = b0 + lane; e < b1; e += kSlice) {
     const uint32_t w = __builtin_nontemporal_load(x.ent + e);
     const uint32_t j = w & x.colmask;
-    const float sx = src[j], sy = src[n + j], sz = src[2 * n + j];
+    const float4 v4 = reinterpret_cast<const float4 *>(src)[j];
+    const double sx = v4.x, sy = v4.y, sz = v4.z;
     if constexpr (XVk == XV_TDIAG) {
-      const float *T = tab + 3 * (w >> x.colbits);
-      dot2_acc(a0, c0, sx, T[0]);
-      dot2_acc(a1, c1, sy, T[1]);
-      dot2_acc(a2, c2, sz, T[2]);
+      const double *T = tab + 3 * (w >> x.colbits);
+      a0 = __builtin_fma(sx, T[0], a0);
+      a1 = __builtin_fma(sy, T[1], a1);
+      a2 = __builtin_fma(sz, T[2], a2);
     } else if constexpr (XVk == XV_TFULL) {
-      const float *T = tab + 9 * (w >> x.colbits);
-      dot2_acc(a0, c0, sx, T[0]); dot2_acc(a0, c0, sy, T[1]); dot2_acc(a0, c0, sz, T[2]);
-      dot2_acc(a1, c1, sx, T[3]); dot2_acc(a1, c1, sy, T[4]); dot2_acc(a1, c1, sz, T[5]);
-      dot2_acc(a2, c2, sx, T[6]); dot2_acc(a2, c2, sy, T[7]); dot2_acc(a2, c2, sz, T[8]);
+      const double *T = tab + 9 * (w >> x.colbits);
+      a0 = __builtin_fma(sx, T[0], a0); a0 = __builtin_fma(sy, T[1], a0); a0 = __builtin_fma(sz, T[2], a0);
+      a1 = __builtin_fma(sx, T[3], a1); a1 = __builtin_fma(sy, T[4], a1); a1 = __builtin_fma(sz, T[5], a1);
+      a2 = __builtin_fma(sx, T[6], a2); a2 = __builtin_fma(sy, T[7], a2); a2 = __builtin_fma(sz, T[8], a2);
     } else if constexpr (XVk == XV_BDIAG) {
       const float *V = x.val32 + (e - lane) * 3 + lane;
-      dot2_acc(a0, c0, sx, __builtin_nontemporal_load(V));
-      dot2_acc(a1, c1, sy, __builtin_nontemporal_load(V + 64));
-      dot2_acc(a2, c2, sz, __builtin_nontemporal_load(V + 128));
+      a0 = __builtin_fma(sx, (double)__builtin_nontemporal_load(V), a0);
+      a1 = __builtin_fma(sy, (double)__builtin_nontemporal_load(V + 64), a1);
+      a2 = __builtin_fma(sz, (double)__builtin_nontemporal_load(V + 128), a2);
     } else if constexpr (XVk == XV_BFULL) {
       const float *V = x.val32 + (e - lane) * 9 + lane;
-      float v[9];
+      double v[9];
 #pragma unroll
-      for (int c = 0; c < 9; ++c) v[c] = __builtin_nontemporal_load(V + 64 * c);
-      dot2_acc(a0, c0, sx, v[0]); dot2_acc(a0, c0, sy, v[1]); dot2_acc(a0, c0, sz, v[2]);
-      dot2_acc(a1, c1, sx, v[3]); dot2_acc(a1, c1, sy, v[4]); dot2_acc(a1, c1, sz, v[5]);
-      dot2_acc(a2, c2, sx, v[6]); dot2_acc(a2, c2, sy, v[7]); dot2_acc(a2, c2, sz, v[8]);
+      for (int c = 0; c < 9; ++c) v[c] = (double)__builtin_nontemporal_load(V + 64 * c);
+      a0 = __builtin_fma(sx, v[0], a0); a0 = __builtin_fma(sy, v[1], a0); a0 = __builtin_fma(sz, v[2], a0);
+      a1 = __builtin_fma(sx, v[3], a1); a1 = __builtin_fma(sy, v[4], a1); a1 = __builtin_fma(sz, v[5], a1);
+      a2 = __builtin_fma(sx, v[6], a2); a2 = __builtin_fma(sy, v[7], a2); a2 = __builtin_fma(sz, v[8], a2);
     }
   }
-  h0 = (double)a0 + (double)c0;
-  h1 = (double)a1 + (double)c1;
-  h2 = (double)a2 + (double)c2;
+  h0 = a0; h1 = a1; h2 = a2;
 }
-
 
 // exchange field of spin i from its row pattern, FP64, reference order
 template <int XVk>
@@ -142,42 +143,41 @@ __device__ __forceinline__ void pfield64(const XView &x, const double *__restric
   h0 = a0; h1 = a1; h2 = a2;
 }
 
-// the same from FP32 spins and tensors with Dot2-compensated row sums
+// the same from FP32 storage with exact FP64 products (see xfield32)
 template <int XVk>
 __device__ __forceinline__ void pfield32(const XView &x, const float *__restrict__ src, int64_t n,
-                                         int64_t i, const float *tab, const uint32_t *pat,
+                                         int64_t i, const double *tab, const uint32_t *pat,
                                          double &h0, double &h1, double &h2) {
   const uint32_t o = x.pofs[i];
   const int len = (int)pat[o];
   const uint32_t tmask = (1u << x.tbits) - 1u;
-  float a0 = 0.f, a1 = 0.f, a2 = 0.f, c0 = 0.f, c1 = 0.f, c2 = 0.f;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
 #pragma unroll kGatherUnroll
   for (int k = 1; k <= len; ++k) {
     const uint32_t w = pat[o + k];
     const int64_t j = i + (int64_t)(((int32_t)w) >> x.tbits);
     const uint32_t t = w & tmask;
-    const float sx = src[j], sy = src[n + j], sz = src[2 * n + j];
+    const float4 v4 = reinterpret_cast<const float4 *>(src)[j];
+    const double sx = v4.x, sy = v4.y, sz = v4.z;
     if constexpr (xv_diag(XVk)) {
-      const float *T = tab + 3 * t;
-      dot2_acc(a0, c0, sx, T[0]);
-      dot2_acc(a1, c1, sy, T[1]);
-      dot2_acc(a2, c2, sz, T[2]);
+      const double *T = tab + 3 * t;
+      a0 = __builtin_fma(sx, T[0], a0);
+      a1 = __builtin_fma(sy, T[1], a1);
+      a2 = __builtin_fma(sz, T[2], a2);
     } else {
-      const float *T = tab + 9 * t;
-      dot2_acc(a0, c0, sx, T[0]); dot2_acc(a0, c0, sy, T[1]); dot2_acc(a0, c0, sz, T[2]);
-      dot2_acc(a1, c1, sx, T[3]); dot2_acc(a1, c1, sy, T[4]); dot2_acc(a1, c1, sz, T[5]);
-      dot2_acc(a2, c2, sx, T[6]); dot2_acc(a2, c2, sy, T[7]); dot2_acc(a2, c2, sz, T[8]);
+      const double *T = tab + 9 * t;
+      a0 = __builtin_fma(sx, T[0], a0); a0 = __builtin_fma(sy, T[1], a0); a0 = __builtin_fma(sz, T[2], a0);
+      a1 = __builtin_fma(sx, T[3], a1); a1 = __builtin_fma(sy, T[4], a1); a1 = __builtin_fma(sz, T[5], a1);
+      a2 = __builtin_fma(sx, T[6], a2); a2 = __builtin_fma(sy, T[7], a2); a2 = __builtin_fma(sz, T[8], a2);
     }
   }
-  h0 = (double)a0 + (double)c0;
-  h1 = (double)a1 + (double)c1;
-  h2 = (double)a2 + (double)c2;
+  h0 = a0; h1 = a1; h2 = a2;
 }
 
 // bytes of LDS used by the tensor table (the pattern table follows it)
 __host__ __device__ inline size_t table_bytes(int prec, int xv, int ntab) {
   if (!xv_typed(xv)) return 0;
-  const size_t b = (size_t)ntab * (xv_diag(xv) ? 3 : 9) * (prec == PR_FP64 ? 8 : 4);
+  const size_t b = (size_t)ntab * (xv_diag(xv) ? 3 : 9) * 8;
   return (b + 15) & ~(size_t)15;
 }
 
@@ -185,13 +185,8 @@ template <int PREC, int XVk>
 __device__ __forceinline__ void load_table(const XView &x, unsigned char *lds) {
   if constexpr (xv_typed(XVk)) {
     const int cnt = x.ntab * (xv_diag(XVk) ? 3 : 9);
-    if constexpr (PREC == PR_FP64) {
-      double *t = reinterpret_cast<double *>(lds);
-      for (int q = threadIdx.x; q < cnt; q += blockDim.x) t[q] = x.tab64[q];
-    } else {
-      float *t = reinterpret_cast<float *>(lds);
-      for (int q = threadIdx.x; q < cnt; q += blockDim.x) t[q] = x.tab32[q];
-    }
+    double *t = reinterpret_cast<double *>(lds);
+    for (int q = threadIdx.x; q < cnt; q += blockDim.x) t[q] = x.tab64[q];
   }
   if constexpr (xv_pattern_lds(XVk)) {
     uint32_t *p = reinterpret_cast<uint32_t *>(lds + table_bytes(PREC, XVk, x.ntab));
@@ -215,11 +210,11 @@ __device__ __forceinline__ void exchange_at(const KParams &p, const double *src6
     if constexpr (PREC == PR_FP64)
       pfield64<XVk>(p.x, src64, p.n, i, reinterpret_cast<const double *>(lds), pat, f0, f1, f2);
     else
-      pfield32<XVk>(p.x, src32, p.n, i, reinterpret_cast<const float *>(lds), pat, f0, f1, f2);
+      pfield32<XVk>(p.x, src32, p.n, i, reinterpret_cast<const double *>(lds), pat, f0, f1, f2);
   } else if constexpr (PREC == PR_FP64) {
     xfield64<XVk>(p.x, src64, p.n, i, reinterpret_cast<const double *>(lds), f0, f1, f2);
   } else {
-    xfield32<XVk>(p.x, src32, p.n, i, reinterpret_cast<const float *>(lds), f0, f1, f2);
+    xfield32<XVk>(p.x, src32, p.n, i, reinterpret_cast<const double *>(lds), f0, f1, f2);
   }
 }
 
@@ -340,7 +335,7 @@ __global__ void __launch_bounds__(kBlock, JH_WAVES_PER_EU) k_heun(const KParams 
     unit_vector(v0, v1, v2);
     p.sp[i] = v0; p.sp[n + i] = v1; p.sp[2 * n + i] = v2;
     if constexpr (PREC == PR_FP32) {
-      p.w32[i] = (float)v0; p.w32[n + i] = (float)v1; p.w32[2 * n + i] = (float)v2;
+      reinterpret_cast<float4 *>(p.w32)[i] = make_float4((float)v0, (float)v1, (float)v2, 0.f);
     }
   } else {
     const double d0 = p.ds[i] + 0.5 * r0;
@@ -350,7 +345,7 @@ __global__ void __launch_bounds__(kBlock, JH_WAVES_PER_EU) k_heun(const KParams 
     unit_vector(v0, v1, v2);
     p.snew[i] = v0; p.snew[n + i] = v1; p.snew[2 * n + i] = v2;
     if constexpr (PREC == PR_FP32) {
-      p.w32[i] = (float)v0; p.w32[n + i] = (float)v1; p.w32[2 * n + i] = (float)v2;
+      reinterpret_cast<float4 *>(p.w32)[i] = make_float4((float)v0, (float)v1, (float)v2, 0.f);
     }
   }
 }
@@ -391,9 +386,11 @@ __global__ void __launch_bounds__(kBlock) k_noise(const KParams p, double *out) 
   out[2 * p.n + i] = z2 * sg * p.sqrtT;
 }
 
-__global__ void __launch_bounds__(kBlock) k_to_f32(const double *a, float *b, int64_t m) {
+// FP32 gather copy of SoA FP64 spins: (x, y, z, 0) per spin, 16 bytes
+__global__ void __launch_bounds__(kBlock) k_to_f32(const double *a, float *b, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (i < m) b[i] = (float)a[i];
+  if (i < n)
+    reinterpret_cast<float4 *>(b)[i] = make_float4((float)a[i], (float)a[n + i], (float)a[2 * n + i], 0.f);
 }
 
 // Deterministic pairwise block reduction of 4 FP64 quantities per spin:
@@ -529,9 +526,9 @@ hipError_t launch_noise(const KParams &p, double *out, hipStream_t st) {
   return hipGetLastError();
 }
 
-hipError_t launch_to_f32(const double *a, float *b, int64_t m, hipStream_t st) {
-  const int64_t nb = (m + kBlock - 1) / kBlock;
-  hipLaunchKernelGGL(k_to_f32, dim3((unsigned)nb), dim3(kBlock), 0, st, a, b, m);
+hipError_t launch_to_f32(const double *a, float *b, int64_t n, hipStream_t st) {
+  const int64_t nb = (n + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(k_to_f32, dim3((unsigned)nb), dim3(kBlock), 0, st, a, b, n);
   return hipGetLastError();
 }
 
