@@ -10,7 +10,16 @@ A "step" is one full Heun step (predictor + corrector, both field
 evaluations, in-kernel noise).  value = spins x steps / event time of the
 step loop, inputs resident in HBM.
 
+The precision section runs the 4,194,304-spin system (bcc 128^3 x 2, full
+3x3 tensors to the fifth shell with DMI, T = 100 K) for --c3-steps steps in
+FP64 and again, from the same initial state and noise stream, in the
+FP32-field form, and states the speed-up and the per-spin angle and |dm|
+differences between the two final states.
+
   python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-fp32]
+                  [--no-c3] [--c3-steps S] [--reps R]
+  python bench.py --copy-calibration   # a plain 1 GiB copy kernel, for
+                                       # scaling rocprofv3 FETCH/WRITE_SIZE
 """
 from __future__ import annotations
 
@@ -86,6 +95,74 @@ def cpu_baseline(L: int, budget_s: float = 15.0):
                       f"noise generation (as solvers/cpu_llg_heun.cc:55), {el:.1f} s"}
 
 
+def measure_precision(L: int, steps: int, reps: int):
+    """The same C3 system in FP64 and in the FP32-field form from one initial
+    state: throughput of each and the difference of the final states."""
+    import numpy as np
+
+    from jams_amd import configs
+    from jams_amd.engine import PREC_FP32_FIELD, PREC_FP64
+    from jams_amd.host import Simulation
+
+    t0 = time.perf_counter()
+    sim = Simulation(configs.c3(L), device=0, output_dir="/tmp")
+    eng = sim.engine()
+    n = sim.n
+    setup_s = time.perf_counter() - t0
+    s0 = eng.get_spins()
+    mus = sim.sites()[0]
+    out = {"n": n, "steps": steps, "setup_s": setup_s}
+    finals = {}
+    for name, mode in (("fp64", PREC_FP64), ("fp32_field", PREC_FP32_FIELD)):
+        eng.set_precision(mode)
+        eng.set_spins(s0)
+        eng.set_iteration(0)
+        eng.step(0)  # lays out the device formats outside the timed region
+        ms = eng.time_steps(steps)
+        finals[name] = eng.get_spins()
+        phases = []
+        for ph in (0, 1):
+            t_ms = eng.time_phase(ph, reps)
+            fmt_b, csr_b = eng.phase_bytes(ph)
+            phases.append({"phase": ("predictor", "corrector")[ph], "ms": t_ms, "bytes": fmt_b,
+                           "gbs": fmt_b / (t_ms * 1e-3) / 1e9,
+                           "csr_equiv_gbs": csr_b / (t_ms * 1e-3) / 1e9})
+        out[name] = {"value": n * steps / (ms * 1e-3), "ms_per_step": ms / steps,
+                     "phases": phases, "engine": eng.describe()}
+        log(f"  C3 {name}: {ms / steps:.4f} ms/step, {out[name]['value']:.4e} spin-updates/s")
+    a, b = finals["fp64"], finals["fp32_field"]
+    cosang = np.clip(np.einsum("ij,ij->i", a, b) /
+                     (np.linalg.norm(a, axis=1) * np.linalg.norm(b, axis=1)), -1.0, 1.0)
+    # the angle from the cross product keeps its precision near zero
+    ang = np.arctan2(np.linalg.norm(np.cross(a, b), axis=1), cosang)
+    ma = (mus[:, None] * a).sum(0) / mus.sum()
+    mb = (mus[:, None] * b).sum(0) / mus.sum()
+    out["speedup_fp32_vs_fp64"] = out["fp32_field"]["value"] / out["fp64"]["value"]
+    out["angle_rad_max"] = float(ang.max())
+    out["angle_rad_rms"] = float(np.sqrt((ang ** 2).mean()))
+    out["dm_abs"] = float(np.linalg.norm(ma - mb))
+    out["m_fp64"] = float(np.linalg.norm(ma))
+    log(f"  C3 after {steps} steps: max angle {out['angle_rad_max']:.3e} rad, rms "
+        f"{out['angle_rad_rms']:.3e}, |dm| {out['dm_abs']:.3e}, speed-up "
+        f"{out['speedup_fp32_vs_fp64']:.3f}")
+    sim.close()
+    return out
+
+
+def copy_calibration(nbytes: int = 1 << 30, reps: int = 5) -> int:
+    """A plain elementwise copy of nbytes (read nbytes, write nbytes), so that
+    rocprofv3's FETCH_SIZE / WRITE_SIZE per dispatch can be scaled to bytes."""
+    import torch
+
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    log(f"copy calibration: {reps} x elementwise copy of {nbytes} bytes")
+    return 0
+
+
 def read_traffic(name: str):
     p = ROOT / "profiles" / name
     if not p.exists():
@@ -104,22 +181,35 @@ def main() -> int:
     ap.add_argument("--L", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-c3", action="store_true")
+    ap.add_argument("--c3-steps", type=int, default=10000)
+    ap.add_argument("--c3-L", type=int, default=128)
+    ap.add_argument("--reps", type=int, default=200)
+    ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(the reference is single-GPU; sharding is a design note only).", file=sys.stderr)
         return 2
 
+    if args.copy_calibration:
+        return copy_calibration()
+
     from jams_amd import configs
 
     log(f"C2 FP64: bcc {args.L}^3 x 2, {args.steps} timed steps after {args.warmup} warm-up")
-    fp64 = measure(configs.c2(args.L), args.steps, args.warmup)
+    fp64 = measure(configs.c2(args.L), args.steps, args.warmup, args.reps)
     log(f"  {fp64['value']:.4e} spin-updates/s, {fp64['ms'] / args.steps * 1e3:.2f} us/step")
     fp32 = None
     if not args.no_fp32:
         log("C2 FP32-field")
-        fp32 = measure(configs.c2(args.L, precision="fp32-field"), args.steps, args.warmup)
+        fp32 = measure(configs.c2(args.L, precision="fp32-field"), args.steps, args.warmup,
+                       args.reps)
         log(f"  {fp32['value']:.4e} spin-updates/s")
+    c3 = None
+    if not args.no_c3:
+        log(f"C3 precision form: bcc {args.c3_L}^3 x 2, {args.c3_steps} steps FP64 and FP32-field")
+        c3 = measure_precision(args.c3_L, args.c3_steps, min(args.reps, 50))
     cpu = None
     if not args.no_cpu:
         log("CPU baseline (oracle, same system)")
@@ -158,6 +248,8 @@ def main() -> int:
         line["fp32_field"] = {"value": fp32["value"], "ms_per_step": fp32["ms"] / args.steps,
                               "speedup_vs_fp64": fp32["value"] / fp64["value"],
                               "phases": fp32["phases"]}
+    if c3 is not None:
+        line["precision_c3"] = c3
     print(json.dumps(line), flush=True)
     return 0
 
