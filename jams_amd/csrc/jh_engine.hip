@@ -190,6 +190,9 @@ struct jh_ctx {
   DevBuf<float> val32;
   DevBuf<uint32_t> pofs, pat;
   int pat_words = 0, pat_lds = 0, tbits = 0, npatterns = 0;
+  DevBuf<int32_t> sseq;          // UNIFORM: (offset, length) per slice
+  DevBuf<double> seqtab;
+  int64_t nuniform = 0, nslices = 0;
   DevBuf<uint16_t> cls;
   DevBuf<double> ctab;
   int nclass = 0;
@@ -400,10 +403,13 @@ int choose_order(const XHost &x) {
 // Group rows by their sequence of (column - row, tensor id).  Entries pack
 // the signed offset above `tbits` bits of tensor id.  Fails (returns false)
 // when an offset does not fit or the rows are too diverse to share patterns.
-bool build_patterns(const jh_ctx *c, const XHost &x, int tbits, std::vector<uint32_t> &pofs,
-                    std::vector<uint32_t> &pat, int &npatterns) {
+// by_tensor: order each row by (tensor id, column) instead of by column (the
+// FP32 form's order, which makes tensor sequences shareable across lanes).
+bool build_patterns(const jh_ctx *c, const XHost &x, int tbits, bool by_tensor,
+                    std::vector<uint32_t> &pofs, std::vector<uint32_t> &pat, int &npatterns) {
   const int64_t n = x.n;
   const int64_t lim = (int64_t)1 << (31 - tbits);
+  const uint32_t tmask = (1u << tbits) - 1u;
   pofs.assign((size_t)n, 0);
   pat.clear();
   npatterns = 0;
@@ -412,14 +418,18 @@ bool build_patterns(const jh_ctx *c, const XHost &x, int tbits, std::vector<uint
   for (int64_t d = 0; d < n; ++d) {
     const int64_t i = ref_of(c, d);
     row.clear();
-    uint64_t h = 1469598103934665603ull ^ (uint64_t)(x.ptr[i + 1] - x.ptr[i]);
     for (int64_t q = x.ptr[i]; q < x.ptr[i + 1]; ++q) {
       const int64_t off = dev_of(c, x.col[q]) - d;
       if (off >= lim || off < -lim) return false;
-      const uint32_t w = ((uint32_t)(int32_t)off << tbits) | (uint32_t)x.typ[q];
-      row.push_back(w);
-      h = (h ^ w) * 1099511628211ull;
+      row.push_back(((uint32_t)(int32_t)off << tbits) | (uint32_t)x.typ[q]);
     }
+    if (by_tensor)
+      std::sort(row.begin(), row.end(), [&](uint32_t a, uint32_t b) {
+        if ((a & tmask) != (b & tmask)) return (a & tmask) < (b & tmask);
+        return ((int32_t)a >> tbits) < ((int32_t)b >> tbits);
+      });
+    uint64_t h = 1469598103934665603ull ^ (uint64_t)row.size();
+    for (uint32_t w : row) h = (h ^ w) * 1099511628211ull;
     auto &bucket = seen[h];
     uint32_t found = UINT32_MAX;
     for (uint32_t o : bucket) {
@@ -440,6 +450,64 @@ bool build_patterns(const jh_ctx *c, const XHost &x, int tbits, std::vector<uint
   }
   // worth it only if the table is much smaller than per-neighbour storage
   return (int64_t)pat.size() * 8 < x.ptr[n] + 1024;
+}
+
+// Slices (64 device rows) whose rows all carry the same tensor-id sequence
+// (rows ordered by tensor id): for each, the (offset, length) of that
+// sequence's tensors, rounded to FP32, in seqtab; (-1, 0) for the others.
+// Returns the number of uniform slices, or -1 when the sequences would not
+// fit the table.
+int64_t build_uniform(const XHost &x, int64_t n, int tbits, bool diag,
+                      const std::vector<uint32_t> &pofs, const std::vector<uint32_t> &pat,
+                      std::vector<int32_t> &sseq, std::vector<double> &seqtab) {
+  const uint32_t tmask = (1u << tbits) - 1u;
+  const int vpt = diag ? 3 : 9;
+  // tensor sequence of each pattern, de-duplicated by its id sequence
+  std::unordered_map<uint32_t, int32_t> pat_seq;              // pattern offset -> sequence
+  std::unordered_map<uint64_t, std::vector<int32_t>> byhash;  // hash -> sequences
+  std::vector<std::pair<int32_t, int32_t>> seqs;              // (seqtab offset, length)
+  std::vector<std::vector<uint32_t>> seq_ids;
+  std::vector<uint32_t> ids;
+  const size_t kMaxSeqTab = (size_t)1 << 20;  // doubles (8 MB)
+  for (int64_t d = 0; d < n; ++d) {
+    const uint32_t o = pofs[d];
+    if (pat_seq.count(o)) continue;
+    const uint32_t len = pat[o];
+    ids.assign(len, 0);
+    uint64_t h = 1469598103934665603ull ^ len;
+    for (uint32_t k = 0; k < len; ++k) {
+      ids[k] = pat[o + 1 + k] & tmask;
+      h = (h ^ ids[k]) * 1099511628211ull;
+    }
+    int32_t found = -1;
+    for (int32_t sq : byhash[h])
+      if (seq_ids[sq] == ids) { found = sq; break; }
+    if (found < 0) {
+      if (seqtab.size() + (size_t)len * vpt > kMaxSeqTab) return -1;
+      found = (int32_t)seqs.size();
+      seqs.push_back({(int32_t)seqtab.size(), (int32_t)len});
+      seq_ids.push_back(ids);
+      for (uint32_t k = 0; k < len; ++k) {
+        const double *T = &x.tab[(size_t)ids[k] * 9];
+        for (int q = 0; q < vpt; ++q) seqtab.push_back((double)(float)(diag ? T[4 * q] : T[q]));
+      }
+      byhash[h].push_back(found);
+    }
+    pat_seq[o] = found;
+  }
+  const int64_t nsl = (n + kSlice - 1) / kSlice;
+  sseq.assign((size_t)nsl * 2, 0);
+  int64_t nu = 0;
+  for (int64_t sl = 0; sl < nsl; ++sl) {
+    const int32_t first = pat_seq[pofs[(size_t)sl * kSlice]];
+    bool uni = true;
+    for (int64_t d = sl * kSlice; d < std::min(n, (sl + 1) * kSlice) && uni; ++d)
+      uni = pat_seq[pofs[d]] == first;
+    sseq[2 * sl] = uni ? seqs[first].first : -1;
+    sseq[2 * sl + 1] = uni ? seqs[first].second : 0;
+    nu += uni;
+  }
+  return nu;
 }
 
 void prepare_exchange(jh_ctx *c) {
@@ -465,8 +533,9 @@ void prepare_exchange(jh_ctx *c) {
   bool typed_ok = (colbits + tbits <= 32) && lds_need <= kMaxTableLds;
   // row patterns: identical sequences of (column - row, tensor id)
   std::vector<uint32_t> pofs, pat;
+  const bool by_tensor = c->prec == PR_FP32;
   bool pattern_ok = typed_ok && (c->xfmt_req == JH_XFMT_AUTO || c->xfmt_req == JH_XFMT_PATTERN) &&
-                    build_patterns(c, x, tbits, pofs, pat, c->npatterns);
+                    build_patterns(c, x, tbits, by_tensor, pofs, pat, c->npatterns);
   int fmt = c->xfmt_req;
   if (fmt == JH_XFMT_AUTO) fmt = pattern_ok ? JH_XFMT_PATTERN : typed_ok ? JH_XFMT_TYPED : JH_XFMT_BLOCK;
   if ((fmt == JH_XFMT_TYPED || fmt == JH_XFMT_PATTERN) && !typed_ok)
@@ -487,6 +556,27 @@ void prepare_exchange(jh_ctx *c) {
     c->nent = x.ptr[n];
     c->colbits = 0;
     c->xv = c->pat_lds ? (diag ? XV_PDIAG : XV_PFULL) : (diag ? XV_GDIAG : XV_GFULL);
+    c->nslices = (n + kSlice - 1) / kSlice;
+    c->nuniform = 0;
+    if (by_tensor && c->pat_lds) {
+      std::vector<int32_t> sseq;
+      std::vector<double> seqtab;
+      const int64_t nu = build_uniform(x, n, tbits, diag, pofs, pat, sseq, seqtab);
+      // worth its own variant when at least half of the slices are uniform
+      if (nu >= 0 && nu * 2 >= c->nslices) {
+        c->nuniform = nu;
+        c->sseq.alloc(sseq.size());
+        HIPCHK(hipMemcpy(c->sseq.p, sseq.data(), sizeof(int32_t) * sseq.size(), hipMemcpyHostToDevice));
+        c->seqtab.alloc(std::max<size_t>(seqtab.size(), 1));
+        if (!seqtab.empty())
+          HIPCHK(hipMemcpy(c->seqtab.p, seqtab.data(), sizeof(double) * seqtab.size(), hipMemcpyHostToDevice));
+        c->xv = diag ? XV_UDIAG : XV_UFULL;
+      }
+    }
+    if (!xv_uniform(c->xv)) {
+      c->sseq.release();
+      c->seqtab.release();
+    }
     c->slice_off.release();
     c->ent.release();
     c->val64.release();
@@ -495,6 +585,9 @@ void prepare_exchange(jh_ctx *c) {
   }
   c->pofs.release();
   c->pat.release();
+  c->sseq.release();
+  c->seqtab.release();
+  c->nuniform = 0;
   c->pat_words = 0;
   const bool typed = fmt == JH_XFMT_TYPED;
   if (!typed) colbits = 32;
@@ -709,6 +802,8 @@ KParams make_params(jh_ctx *c) {
   p.x.pat = c->pat.p;
   p.x.pat_words = c->pat_words;
   p.x.tbits = c->tbits;
+  p.x.sseq = reinterpret_cast<const int2 *>(c->sseq.p);
+  p.x.seqtab = c->seqtab.p;
   p.nterms = (int)c->terms.size();
   for (int q = 0; q < p.nterms; ++q) p.kind[q] = c->terms[q].kind;
   p.hext = c->hext.p;
@@ -1209,14 +1304,17 @@ int jh_describe(jh_ctx *c, char *buf, int64_t buflen) {
     prepare(c);
     static const char *xvn[] = {"none", "typed-diag", "typed-full", "block-diag", "block-full",
                                 "pattern-diag", "pattern-full", "pattern-diag-global",
-                                "pattern-full-global"};
+                                "pattern-full-global", "pattern-uniform-diag",
+                                "pattern-uniform-full"};
     char tmp[512];
     std::snprintf(tmp, sizeof tmp,
                   "n=%lld prec=%s exchange=%s entries=%lld tensors=%d colbits=%d patterns=%d "
-                  "pattern_words=%d lds=%d order=%d classes=%d terms=%d noise=%d",
+                  "pattern_words=%d lds=%d uniform_slices=%lld/%lld order=%d classes=%d "
+                  "terms=%d noise=%d",
                   (long long)c->n, c->prec == PR_FP64 ? "fp64" : "fp32-field", xvn[c->xv],
                   (long long)c->nent, c->ntab > 0 ? c->ntab - 1 : 0, c->colbits,
-                  c->npatterns, c->pat_words, c->pat_lds, c->order_nb, c->nclass,
+                  c->npatterns, c->pat_words, c->pat_lds, (long long)c->nuniform,
+                  (long long)c->nslices, c->order_nb, c->nclass,
                   (int)c->terms.size(), c->noise_mode);
     if (buf && buflen > 0) {
       std::strncpy(buf, tmp, (size_t)buflen - 1);

@@ -26,6 +26,7 @@
 //    a 16-bit class index per spin (no index at all when there is one class).
 #pragma once
 
+#include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace jh {
@@ -38,16 +39,20 @@ enum TermKind { TK_EXCHANGE = 0, TK_ZEEMAN = 1, TK_UNIAXIAL = 2, TK_DIPOLE = 3 }
 enum NoiseMode { NM_NONE = 0, NM_PHILOX = 1, NM_INJECTED = 2 };
 enum Prec { PR_FP64 = 0, PR_FP32 = 1 };
 // exchange kernel variants
-// (PDIAG/PFULL keep the pattern table in LDS, GDIAG/GFULL in global memory)
+// (PDIAG/PFULL keep the pattern table in LDS, GDIAG/GFULL in global memory;
+// UDIAG/UFULL are PDIAG/PFULL plus wave-uniform tensor sequences, FP32 form)
 enum XVariant {
   XV_NONE = 0, XV_TDIAG = 1, XV_TFULL = 2, XV_BDIAG = 3, XV_BFULL = 4,
-  XV_PDIAG = 5, XV_PFULL = 6, XV_GDIAG = 7, XV_GFULL = 8
+  XV_PDIAG = 5, XV_PFULL = 6, XV_GDIAG = 7, XV_GFULL = 8, XV_UDIAG = 9, XV_UFULL = 10
 };
-constexpr bool xv_pattern(int v) { return v >= XV_PDIAG && v <= XV_GFULL; }
-constexpr bool xv_pattern_lds(int v) { return v == XV_PDIAG || v == XV_PFULL; }
+constexpr bool xv_uniform(int v) { return v == XV_UDIAG || v == XV_UFULL; }
+constexpr bool xv_pattern(int v) { return (v >= XV_PDIAG && v <= XV_GFULL) || xv_uniform(v); }
+constexpr bool xv_pattern_lds(int v) {
+  return v == XV_PDIAG || v == XV_PFULL || xv_uniform(v);
+}
 constexpr bool xv_typed(int v) { return v == XV_TDIAG || v == XV_TFULL || xv_pattern(v); }
 constexpr bool xv_diag(int v) {
-  return v == XV_TDIAG || v == XV_BDIAG || v == XV_PDIAG || v == XV_GDIAG;
+  return v == XV_TDIAG || v == XV_BDIAG || v == XV_PDIAG || v == XV_GDIAG || v == XV_UDIAG;
 }
 
 // site-class table: 16 doubles per class
@@ -75,6 +80,12 @@ struct XView {
   const uint32_t *pat;
   int pat_words;              // table size in 32-bit words
   int tbits;
+  // UNIFORM (FP32 form): rows are ordered by tensor id, so in a slice whose
+  // rows all carry the same tensor sequence the tensor of slot k is the same
+  // for every lane.  sseq[slice] = (offset of that sequence's tensors in
+  // seqtab, length), or (-1, 0) when the slice's rows differ.
+  const int2 *sseq;
+  const double *seqtab;       // per sequence: len x (3|9) values rounded to FP32
 };
 
 struct KParams {

@@ -8,7 +8,9 @@
 // Each thread owns one spin.  The exchange field is a row-sequential sum in
 // the reference CSR order (neighbour column ascending, tensor column
 // ascending), which is what makes the FP64 path bit-identical to the CPU
-// solver; the FP32 form keeps the order and adds Dot2 compensation.
+// solver.  The FP32 form gathers FP32 spins, multiplies them by tensors
+// rounded to FP32 (each product exact in FP64) and accumulates in FP64; it
+// may order a row by tensor id so that tensors become wave-uniform.
 #include "jh_device.h"
 #include "jh_internal.h"
 #include "jh_launch.h"
@@ -174,6 +176,49 @@ __device__ __forceinline__ void pfield32(const XView &x, const float *__restrict
   h0 = a0; h1 = a1; h2 = a2;
 }
 
+// FP32 form with rows ordered by tensor id: in a slice whose rows share one
+// tensor sequence the tensor of slot k is wave-uniform and is read through
+// scalar loads into SGPRs (no LDS traffic for it); only the neighbour
+// offsets come from each lane's pattern.  Other slices take pfield32.
+template <int XVk>
+__device__ __forceinline__ void ufield32(const XView &x, const float *__restrict__ src, int64_t n,
+                                         int64_t i, const double *tab, const uint32_t *pat,
+                                         double &h0, double &h1, double &h2) {
+  const int sl = __builtin_amdgcn_readfirstlane((int)(i >> 6));
+  const int2 info = x.sseq[sl];
+  const int soff = __builtin_amdgcn_readfirstlane(info.x);
+  if (soff < 0) {
+    pfield32<XVk>(x, src, n, i, tab, pat, h0, h1, h2);
+    return;
+  }
+  const int len = __builtin_amdgcn_readfirstlane(info.y);
+  const double *__restrict__ T = x.seqtab + soff;
+  const uint32_t o = x.pofs[i];
+  const int tb = x.tbits;
+  const uint32_t ib = (uint32_t)i;
+  const float4 *__restrict__ s4 = reinterpret_cast<const float4 *>(src);
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+#pragma unroll kGatherUnroll
+  for (int k = 0; k < len; ++k) {
+    const uint32_t w = pat[o + 1 + k];
+    const uint32_t j = ib + (uint32_t)(((int32_t)w) >> tb);
+    const float4 v4 = s4[j];
+    const double sx = v4.x, sy = v4.y, sz = v4.z;
+    if constexpr (xv_diag(XVk)) {
+      const double *Tk = T + 3 * k;
+      a0 = __builtin_fma(sx, Tk[0], a0);
+      a1 = __builtin_fma(sy, Tk[1], a1);
+      a2 = __builtin_fma(sz, Tk[2], a2);
+    } else {
+      const double *Tk = T + 9 * k;
+      a0 = __builtin_fma(sx, Tk[0], a0); a0 = __builtin_fma(sy, Tk[1], a0); a0 = __builtin_fma(sz, Tk[2], a0);
+      a1 = __builtin_fma(sx, Tk[3], a1); a1 = __builtin_fma(sy, Tk[4], a1); a1 = __builtin_fma(sz, Tk[5], a1);
+      a2 = __builtin_fma(sx, Tk[6], a2); a2 = __builtin_fma(sy, Tk[7], a2); a2 = __builtin_fma(sz, Tk[8], a2);
+    }
+  }
+  h0 = a0; h1 = a1; h2 = a2;
+}
+
 // bytes of LDS used by the tensor table (the pattern table follows it)
 __host__ __device__ inline size_t table_bytes(int prec, int xv, int ntab) {
   if (!xv_typed(xv)) return 0;
@@ -209,6 +254,8 @@ __device__ __forceinline__ void exchange_at(const KParams &p, const double *src6
       pat = p.x.pat;
     if constexpr (PREC == PR_FP64)
       pfield64<XVk>(p.x, src64, p.n, i, reinterpret_cast<const double *>(lds), pat, f0, f1, f2);
+    else if constexpr (xv_uniform(XVk))
+      ufield32<XVk>(p.x, src32, p.n, i, reinterpret_cast<const double *>(lds), pat, f0, f1, f2);
     else
       pfield32<XVk>(p.x, src32, p.n, i, reinterpret_cast<const double *>(lds), pat, f0, f1, f2);
   } else if constexpr (PREC == PR_FP64) {
@@ -481,6 +528,12 @@ static hipError_t launch_heun_p(const KParams &p, int xv, int phase, hipStream_t
     case XV_PFULL: return launch_heun_t<PREC, XV_PFULL>(p, phase, st);
     case XV_GDIAG: return launch_heun_t<PREC, XV_GDIAG>(p, phase, st);
     case XV_GFULL: return launch_heun_t<PREC, XV_GFULL>(p, phase, st);
+    case XV_UDIAG:
+      if constexpr (PREC == PR_FP32) return launch_heun_t<PREC, XV_UDIAG>(p, phase, st);
+      break;
+    case XV_UFULL:
+      if constexpr (PREC == PR_FP32) return launch_heun_t<PREC, XV_UFULL>(p, phase, st);
+      break;
   }
   return hipErrorInvalidValue;
 }
@@ -510,6 +563,12 @@ static hipError_t launch_field_p(const KParams &p, int xv, int term, double *out
     case XV_PFULL: return launch_field_t<PREC, XV_PFULL>(p, term, out, st);
     case XV_GDIAG: return launch_field_t<PREC, XV_GDIAG>(p, term, out, st);
     case XV_GFULL: return launch_field_t<PREC, XV_GFULL>(p, term, out, st);
+    case XV_UDIAG:
+      if constexpr (PREC == PR_FP32) return launch_field_t<PREC, XV_UDIAG>(p, term, out, st);
+      break;
+    case XV_UFULL:
+      if constexpr (PREC == PR_FP32) return launch_field_t<PREC, XV_UFULL>(p, term, out, st);
+      break;
   }
   return hipErrorInvalidValue;
 }

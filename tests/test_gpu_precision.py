@@ -15,11 +15,34 @@ def angle(a, b):
     return np.arccos(c)
 
 
+def open_film():
+    """sc 12 x 10 x 4 film, all boundaries open: rows of 3 to 6 neighbours, so
+    most slices mix tensor sequences (the non-uniform path)."""
+    lat = po.LatticeSpec(np.eye(3), [[0, 0, 0]], [0], (12, 10, 4), (0, 0, 0))
+    return S.build(lat, S.sc_nn_rows(2.0), 2.2, 0.02, ["exchange"], dt=1e-3, spins="random", seed=4)
+
+
+def open_bcc_dmi():
+    """bcc 5^3 x 2 with the five-shell DMI template, open in z only."""
+    return S.build(S.bcc_lattice(5, (1, 1, 0)), S.bcc_five_shell_dmi_rows(), 2.0, 0.05,
+                   ["exchange"], dt=S.dt_ps(0.5e-15), spins="random", seed=3)
+
+
+SYSTEMS = {
+    "bcc_dmi_periodic": lambda: S.c3(6, with_csr=True),
+    "bcc_two_shell_diag": lambda: S.c2(6),
+    "sc_open_film": open_film,
+    "bcc_dmi_open_z": open_bcc_dmi,
+}
+
+
+@pytest.mark.parametrize("name", list(SYSTEMS))
 @pytest.mark.parametrize("fmt", [0, 1, 2])
-def test_fp32_field_error_bound(engine_factory, fmt):
+def test_fp32_field_error_bound(engine_factory, fmt, name):
     """|h32 - h64| <= 2^-22 sum_j |J_ij| |s_j| per component: the inputs are
-    rounded to FP32 (2^-24 each), products and sums are exact to FP64."""
-    b = S.c3(6, with_csr=True)
+    rounded to FP32 (2^-24 each), products and sums are exact to FP64 (in
+    whatever order the form sums a row)."""
+    b = SYSTEMS[name]()
     s = po.random_spins(21, b.n)
     e64 = b.engine(engine_factory, prec=0, fmt=fmt)
     e32 = b.engine(engine_factory, prec=1, fmt=fmt)
@@ -48,3 +71,21 @@ def test_fp32_field_trajectory_angle_c3_small(engine_factory):
     print(f"FP32-field vs FP64 after 2000 steps: max angle {ang.max():.3e} rad, "
           f"rms {np.sqrt((ang ** 2).mean()):.3e}, |dm| {dm:.3e}")
     assert ang.max() < 1e-5
+
+
+def test_fp32_uniform_slices_are_used(engine_factory):
+    """Periodic crystals get the wave-uniform tensor variant in every slice
+    (both bcc sites see the same tensors, so even a slice that straddles the
+    two sites is uniform); an open film, whose every slice mixes surface and
+    interior rows, keeps the per-lane variant; FP64 keeps the reference order."""
+    e = S.c3(8).engine(engine_factory, prec=1, use_list=True)
+    d = e.describe()
+    assert "exchange=pattern-uniform-full" in d and "uniform_slices=16/16" in d, d
+    e = S.c2(6).engine(engine_factory, prec=1)
+    d = e.describe()
+    assert "exchange=pattern-uniform-diag" in d and "uniform_slices=7/7" in d, d
+    e = open_film().engine(engine_factory, prec=1)
+    d = e.describe()
+    assert "exchange=pattern-diag " in d, d
+    e = S.c3(8).engine(engine_factory, prec=0, use_list=True)
+    assert "exchange=pattern-full " in e.describe()
