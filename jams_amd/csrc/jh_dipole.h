@@ -11,14 +11,19 @@ namespace jh {
 
 class DipoleFFT {
  public:
-  // Builds the k-space interaction tensors on the host and uploads them.
-  // Returns nullptr (with *err set) on invalid input.
+  // Builds the k-space interaction tensors (real-space tensor on the host,
+  // transformed on the device).  Returns nullptr (with *err set) on invalid
+  // input.  mus: per-spin moments (meV/T), reference order.
   static DipoleFFT *create(const int32_t size[3], const int32_t periodic[3], int nbasis,
                            const double cell[9], const double *basis_frac, double a_m,
-                           double r_cutoff, const double *mus, hipStream_t st, std::string *err);
+                           double r_cutoff, const double *mus, int64_t n, hipStream_t st,
+                           std::string *err);
   virtual ~DipoleFFT() = default;
-  // h (SoA, meV) = mus_i sum_j mus_j W(r_ij) s_j for spins s (SoA).
-  virtual hipError_t field(const double *s, double *h, hipStream_t st) = 0;
+  // h (SoA, meV) = mus_i sum_j mus_j W(r_ij) s_j for spins s (SoA, reference
+  // order); fp32: transforms and tensors in FP32 (the precision form).
+  virtual hipError_t field(const double *s, double *h, bool fp32, hipStream_t st) = 0;
+  // algorithmic HBM bytes of one field evaluation
+  virtual double bytes(bool fp32) const = 0;
 };
 
 }  // namespace jh

@@ -1,13 +1,509 @@
-// jh_dipole.hip -- zero-padded FFT dipole field (see jh_dipole.h).
+// jh_dipole.hip -- zero-padded FFT dipole field (hamiltonian/dipole_fft.cc:
+// 35-350, hamiltonian/cuda_dipole_fft.cu:62-342), written for gfx950.
+//
+// h_i = mus_i sum_j mus_j W(r_ij) s_j with
+//   W_ab(r) = mu0 / (4 pi a^3) (3 r_a r_b - r^2 delta_ab) / |r|^5
+// (r in lattice constants, |r| <= r_cutoff + 1e-4, no self term;
+// dipole_fft.cc:225-251), evaluated as a convolution on the padded cell grid:
+// a periodic dimension keeps its L cells and takes the minimum image; an open
+// dimension is padded to 2L and its kernel is two-sided (offsets -L < n < L),
+// which is the brute-force sum (dipole_bruteforce.cc:75-120).  The reference's
+// FFT forms are one-sided (CPU) or unpadded (GPU) on open dimensions; this
+// build does not copy that.
+//
+// Per evaluation (spins s, SoA, reference order):
+//   z pass   r2c along the contiguous axis, fused with the scatter of mus s
+//            into the zero-padded grid (only the L0 x L1 occupied lines);
+//   y pass   forward, inputs pruned to y < L1, lines x < L0 only;
+//   x pass   forward, inputs pruned to x < L0;
+//   multiply H_i(k) = sum_j W_ij(k) S_j(k), in place;
+//   x pass   inverse, outputs x < L0 only;
+//   y pass   inverse, lines x < L0, outputs y < L1;
+//   z pass   c2r, fused with the gather h_i = mus_i Re(...), outputs z < L2.
+// The 1/P normalisation is folded into the tensor.  For one basis site the
+// tensor is real (W is even in n), 6 values per k-point.
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <memory>
+#include <thread>
+#include <vector>
+
 #include "jh_dipole.h"
+#include "jh_fft.h"
 
 namespace jh {
 
-DipoleFFT *DipoleFFT::create(const int32_t *, const int32_t *, int, const double *,
-                             const double *, double, double, const double *, hipStream_t,
+using fft::LineMap;
+using fft::Pass;
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kMu0IU = 4.0 * kPi * 1e-7 / 6.24150907e21;  // helpers/consts.h:38
+constexpr double kDistTol = 1e-4;                              // helpers/defaults.h:44
+constexpr size_t kMaxPassLds = 64 * 1024;  // one line of the longest axis per workgroup
+constexpr int kMaxBasis = 8;
+
+#define DCHK(x)                              \
+  do {                                       \
+    hipError_t e_ = (x);                     \
+    if (e_ != hipSuccess) return e_;         \
+  } while (0)
+
+template <class T>
+struct DBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  hipError_t alloc(size_t cnt) {
+    release();
+    n = cnt;
+    return hipMalloc((void **)&p, sizeof(T) * std::max<size_t>(cnt, 1));
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DBuf() { release(); }
+};
+
+// radices of n: 8s first, then 4, 2, 5, 3, then any other prime factor
+std::vector<int> factorise(int n) {
+  std::vector<int> r;
+  while (n % 8 == 0) { r.push_back(8); n /= 8; }
+  if (n % 4 == 0) { r.push_back(4); n /= 4; }
+  if (n % 2 == 0) { r.push_back(2); n /= 2; }
+  for (int f : {5, 3}) while (n % f == 0) { r.push_back(f); n /= f; }
+  for (int f = 7; n > 1; f += 2) while (n % f == 0) { r.push_back(f); n /= f; }
+  return r;
+}
+
+template <class T2>
+std::vector<T2> twiddles(int n) {
+  std::vector<T2> w((size_t)n);
+  for (int t = 0; t < n; ++t) {
+    // exp(-2 pi i t / n), angle reduced to the first octant in long double
+    const long double a = -2.0L * 3.141592653589793238462643383279502884L * (long double)t / n;
+    w[t].x = (decltype(w[t].x))cosl(a);
+    w[t].y = (decltype(w[t].y))sinl(a);
+  }
+  return w;
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_dip_mult_real(typename fft::C2<T>::type *K,
+                                                       const T *W, int64_t M) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= M) return;
+  using T2 = typename fft::C2<T>::type;
+  const T2 sx = K[p], sy = K[M + p], sz = K[2 * M + p];
+  const T xx = W[p], xy = W[M + p], xz = W[2 * M + p], yy = W[3 * M + p], yz = W[4 * M + p],
+          zz = W[5 * M + p];
+  T2 hx, hy, hz;
+  hx.x = xx * sx.x + xy * sy.x + xz * sz.x; hx.y = xx * sx.y + xy * sy.y + xz * sz.y;
+  hy.x = xy * sx.x + yy * sy.x + yz * sz.x; hy.y = xy * sx.y + yy * sy.y + yz * sz.y;
+  hz.x = xz * sx.x + yz * sy.x + zz * sz.x; hz.y = xz * sx.y + yz * sy.y + zz * sz.y;
+  K[p] = hx;
+  K[M + p] = hy;
+  K[2 * M + p] = hz;
+}
+
+// several basis sites: complex tensors per ordered pair (i, j), 6 per k
+template <class T>
+__global__ void __launch_bounds__(256) k_dip_mult_cplx(typename fft::C2<T>::type *K,
+                                                       const typename fft::C2<T>::type *W,
+                                                       int64_t M, int nb) {
+  using T2 = typename fft::C2<T>::type;
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= M) return;
+  T2 S[3 * kMaxBasis];
+  for (int g = 0; g < 3 * nb; ++g) S[g] = K[(int64_t)g * M + p];
+  for (int i = 0; i < nb; ++i) {
+    T2 h[3];
+    for (int a = 0; a < 3; ++a) { h[a].x = 0; h[a].y = 0; }
+    for (int j = 0; j < nb; ++j) {
+      const T2 *Wij = W + (int64_t)(i * nb + j) * 6 * M + p;
+      // symmetric tensor: xx xy xz yy yz zz
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+          const int c = a <= b ? (a == 0 ? b : a + b + 1) : (b == 0 ? a : a + b + 1);
+          h[a] = fft::cadd(h[a], fft::cmul(Wij[c * M], S[3 * j + b]));
+        }
+    }
+    for (int a = 0; a < 3; ++a) K[(int64_t)(3 * i + a) * M + p] = h[a];
+  }
+}
+
+// tensor set-up: the q2 < Q2 half of the transformed grid into the tensor
+// arrays (real part for one basis site)
+template <class T>
+__global__ void __launch_bounds__(256) k_dip_extract(const double2 *F, int64_t P2, int64_t Q2,
+                                                     int64_t rows, int ncomp, bool real_only,
+                                                     T *Wr, typename fft::C2<T>::type *Wc) {
+  const int64_t M = rows * Q2;
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= M * ncomp) return;
+  const int64_t g = q / M, r = q % M;
+  const double2 v = F[g * rows * P2 + (r / Q2) * P2 + r % Q2];
+  if (real_only) {
+    Wr[q] = (T)v.x;
+  } else {
+    Wc[q].x = (T)v.x;
+    Wc[q].y = (T)v.y;
+  }
+}
+
+template <class T>
+size_t pass_lds(int n, int B) {
+  return 2 * (size_t)n * B * sizeof(typename fft::C2<T>::type);
+}
+
+template <class T>
+int pick_lines(int n, int64_t nlines) {
+  int B = (int)std::min<size_t>(64, (64 * 1024) / pass_lds<T>(n, 1));
+  B = std::max(B, 1);
+  return (int)std::min<int64_t>(B, std::max<int64_t>(nlines, 1));
+}
+
+template <class T, int LD, int ST>
+hipError_t run_pass(Pass<T> p, hipStream_t st) {
+  if (p.nlines <= 0) return hipSuccess;
+  p.B = pick_lines<T>(p.n, p.nlines);
+  const size_t lds = pass_lds<T>(p.n, p.B);
+  const int64_t blocks = (p.nlines + p.B - 1) / p.B;
+  hipLaunchKernelGGL((fft::k_fft_pass<T, LD, ST>), dim3((unsigned)blocks), dim3(fft::kThreads),
+                     lds, st, p);
+  return hipGetLastError();
+}
+
+// per-precision device state
+template <class T>
+struct Plane {
+  using T2 = typename fft::C2<T>::type;
+  DBuf<T2> K;            // [3 nb][P0][P1][Q2]
+  DBuf<T> Wr;            // nb == 1: [6][P0 P1 Q2]
+  DBuf<T2> Wc;           // nb > 1: [nb nb][6][P0 P1 Q2]
+  DBuf<T2> tw[3];
+  bool ready = false;
+};
+
+class DipoleFFTImpl : public DipoleFFT {
+ public:
+  int L[3], P[3], per[3], nb;
+  int64_t n, Q2, M;  // M = P0 P1 Q2
+  std::vector<int> rad[3];
+  DBuf<double> mus;
+  Plane<double> f64;
+  Plane<float> f32;
+  DBuf<double2> Fsetup;  // transformed tensor grid (FP64), kept until both planes are built
+
+  template <class T>
+  Pass<T> base_pass(Plane<T> &pl, int axis, bool inverse) {
+    Pass<T> p;
+    std::memset(&p, 0, sizeof p);
+    p.n = P[axis];
+    p.nst = (int)rad[axis].size();
+    for (int s = 0; s < p.nst; ++s) p.radix[s] = rad[axis][s];
+    p.inverse = inverse ? 1 : 0;
+    p.tw = pl.tw[axis].p;
+    p.grid.n = n;
+    p.grid.L[0] = L[0];
+    p.grid.L[1] = L[1];
+    p.grid.L[2] = L[2];
+    p.grid.nb = nb;
+    p.grid.mus = mus.p;
+    return p;
+  }
+
+  template <class T>
+  hipError_t ensure(Plane<T> &pl, hipStream_t st) {
+    if (pl.ready) return hipSuccess;
+    using T2 = typename fft::C2<T>::type;
+    for (int a = 0; a < 3; ++a) {
+      const auto w = twiddles<T2>(P[a]);
+      DCHK(pl.tw[a].alloc(w.size()));
+      DCHK(hipMemcpyAsync(pl.tw[a].p, w.data(), sizeof(T2) * w.size(), hipMemcpyHostToDevice, st));
+      DCHK(hipStreamSynchronize(st));
+    }
+    DCHK(pl.K.alloc((size_t)(3 * nb) * M));
+    const int ncomp = 6 * nb * nb;
+    if (nb == 1) {
+      DCHK(pl.Wr.alloc((size_t)ncomp * M));
+    } else {
+      DCHK(pl.Wc.alloc((size_t)ncomp * M));
+    }
+    const int64_t blocks = ((int64_t)ncomp * M + 255) / 256;
+    hipLaunchKernelGGL((k_dip_extract<T>), dim3((unsigned)blocks), dim3(256), 0, st,
+                       (const double2 *)Fsetup.p, (int64_t)P[2], Q2, (int64_t)P[0] * P[1], ncomp,
+                       nb == 1, pl.Wr.p, pl.Wc.p);
+    DCHK(hipGetLastError());
+    DCHK(hipStreamSynchronize(st));
+    pl.ready = true;
+    return hipSuccess;
+  }
+
+  // transform the real-space tensor grid (complex, [6 nb nb][P0][P1][P2]) in place
+  hipError_t transform_tensor(hipStream_t st) {
+    const int ncomp = 6 * nb * nb;
+    const int64_t P01 = (int64_t)P[0] * P[1], Pall = P01 * P[2];
+    Plane<double> tmp;
+    for (int a = 0; a < 3; ++a) {
+      const auto w = twiddles<double2>(P[a]);
+      DCHK(tmp.tw[a].alloc(w.size()));
+      DCHK(hipMemcpyAsync(tmp.tw[a].p, w.data(), sizeof(double2) * w.size(), hipMemcpyHostToDevice, st));
+      DCHK(hipStreamSynchronize(st));
+    }
+    // z: lines (g, x, y), contiguous
+    {
+      auto p = base_pass<double>(tmp, 2, false);
+      p.n_in = p.n_out = P[2];
+      p.nlines = ncomp * P01;
+      p.src = p.dst = LineMap{P[0], P[1], Pall, (int64_t)P[1] * P[2], P[2], 1};
+      p.in = Fsetup.p;
+      p.out = Fsetup.p;
+      DCHK((run_pass<double, fft::LD_CPLX, fft::ST_CPLX>(p, st)));
+    }
+    // y: lines (g, x, z)
+    {
+      auto p = base_pass<double>(tmp, 1, false);
+      p.n_in = p.n_out = P[1];
+      p.nlines = ncomp * (int64_t)P[0] * P[2];
+      p.src = p.dst = LineMap{P[0], P[2], Pall, (int64_t)P[1] * P[2], 1, P[2]};
+      p.in = Fsetup.p;
+      p.out = Fsetup.p;
+      DCHK((run_pass<double, fft::LD_CPLX, fft::ST_CPLX>(p, st)));
+    }
+    // x: lines (g, y z)
+    {
+      auto p = base_pass<double>(tmp, 0, false);
+      p.n_in = p.n_out = P[0];
+      p.nlines = ncomp * (int64_t)P[1] * P[2];
+      p.src = p.dst = LineMap{1, (int64_t)P[1] * P[2], Pall, 0, 1, (int64_t)P[1] * P[2]};
+      p.in = Fsetup.p;
+      p.out = Fsetup.p;
+      DCHK((run_pass<double, fft::LD_CPLX, fft::ST_CPLX>(p, st)));
+    }
+    return hipStreamSynchronize(st);
+  }
+
+  template <class T>
+  hipError_t eval(Plane<T> &pl, const double *s, double *h, hipStream_t st) {
+    using T2 = typename fft::C2<T>::type;
+    DCHK(ensure(pl, st));
+    const int64_t G = 3 * nb;
+    const int64_t sa = (int64_t)P[0] * P[1] * Q2;
+    // z forward, scatter
+    {
+      auto p = base_pass<T>(pl, 2, false);
+      p.n_in = L[2];
+      p.n_out = (int)Q2;
+      p.nlines = G * L[0] * L[1];
+      p.dst = LineMap{L[0], L[1], sa, (int64_t)P[1] * Q2, Q2, 1};
+      p.out = pl.K.p;
+      p.spins = s;
+      DCHK((run_pass<T, fft::LD_SPIN, fft::ST_CPLX>(p, st)));
+    }
+    // y forward: lines (g, x < L0, q2)
+    {
+      auto p = base_pass<T>(pl, 1, false);
+      p.n_in = L[1];
+      p.n_out = P[1];
+      p.nlines = G * L[0] * Q2;
+      p.src = p.dst = LineMap{L[0], Q2, sa, (int64_t)P[1] * Q2, 1, Q2};
+      p.in = pl.K.p;
+      p.out = pl.K.p;
+      DCHK((run_pass<T, fft::LD_CPLX, fft::ST_CPLX>(p, st)));
+    }
+    // x forward: lines (g, y q2)
+    {
+      auto p = base_pass<T>(pl, 0, false);
+      p.n_in = L[0];
+      p.n_out = P[0];
+      p.nlines = G * P[1] * Q2;
+      p.src = p.dst = LineMap{1, (int64_t)P[1] * Q2, sa, 0, 1, (int64_t)P[1] * Q2};
+      p.in = pl.K.p;
+      p.out = pl.K.p;
+      DCHK((run_pass<T, fft::LD_CPLX, fft::ST_CPLX>(p, st)));
+    }
+    // multiply
+    {
+      const int64_t blocks = (M + 255) / 256;
+      if (nb == 1)
+        hipLaunchKernelGGL((k_dip_mult_real<T>), dim3((unsigned)blocks), dim3(256), 0, st, pl.K.p,
+                           (const T *)pl.Wr.p, M);
+      else
+        hipLaunchKernelGGL((k_dip_mult_cplx<T>), dim3((unsigned)blocks), dim3(256), 0, st, pl.K.p,
+                           (const T2 *)pl.Wc.p, M, nb);
+      DCHK(hipGetLastError());
+    }
+    // x inverse, outputs x < L0
+    {
+      auto p = base_pass<T>(pl, 0, true);
+      p.n_in = P[0];
+      p.n_out = L[0];
+      p.nlines = G * P[1] * Q2;
+      p.src = p.dst = LineMap{1, (int64_t)P[1] * Q2, sa, 0, 1, (int64_t)P[1] * Q2};
+      p.in = pl.K.p;
+      p.out = pl.K.p;
+      DCHK((run_pass<T, fft::LD_CPLX, fft::ST_CPLX>(p, st)));
+    }
+    // y inverse: lines (g, x < L0, q2), outputs y < L1
+    {
+      auto p = base_pass<T>(pl, 1, true);
+      p.n_in = P[1];
+      p.n_out = L[1];
+      p.nlines = G * L[0] * Q2;
+      p.src = p.dst = LineMap{L[0], Q2, sa, (int64_t)P[1] * Q2, 1, Q2};
+      p.in = pl.K.p;
+      p.out = pl.K.p;
+      DCHK((run_pass<T, fft::LD_CPLX, fft::ST_CPLX>(p, st)));
+    }
+    // z inverse (c2r), gather
+    {
+      auto p = base_pass<T>(pl, 2, true);
+      p.n_in = (int)Q2;
+      p.n_out = L[2];
+      p.nlines = G * L[0] * L[1];
+      p.src = LineMap{L[0], L[1], sa, (int64_t)P[1] * Q2, Q2, 1};
+      p.in = pl.K.p;
+      p.field = h;
+      DCHK((run_pass<T, fft::LD_HERM, fft::ST_FIELD>(p, st)));
+    }
+    return hipSuccess;
+  }
+
+  hipError_t field(const double *s, double *h, bool fp32, hipStream_t st) override {
+    return fp32 ? eval(f32, s, h, st) : eval(f64, s, h, st);
+  }
+
+  double bytes(bool fp32) const override {
+    const double c = fp32 ? 8.0 : 16.0, G = 3.0 * nb;
+    const double occ = (double)L[0] * L[1] * Q2, half = (double)L[0] * P[1] * Q2, full = (double)M;
+    const double spin = 24.0 * n + 8.0 * n;  // s + mus (gather side: h + mus)
+    double b = 2 * spin;
+    b += G * c * (occ /*z fwd w*/ + occ + half /*y fwd r w*/ + half + full /*x fwd*/ +
+                  full + full /*mult r w*/ + full + half /*x inv*/ + half + occ /*y inv*/ +
+                  occ /*z inv r*/);
+    b += (nb == 1 ? 6.0 * (c / 2) : 6.0 * nb * nb * c) * full;  // tensor
+    return b;
+  }
+};
+
+}  // namespace
+
+DipoleFFT *DipoleFFT::create(const int32_t size[3], const int32_t periodic[3], int nbasis,
+                             const double cell[9], const double *basis_frac, double a_m,
+                             double r_cutoff, const double *mus, int64_t nspins, hipStream_t st,
                              std::string *err) {
-  if (err) *err = "not available in this build";
-  return nullptr;
+  auto fail = [&](const std::string &m) -> DipoleFFT * {
+    if (err) *err = m;
+    return nullptr;
+  };
+  if (nbasis < 1 || nbasis > kMaxBasis) return fail("1 to 8 basis sites are supported");
+  if (!(a_m > 0) || !(r_cutoff > 0)) return fail("lattice parameter and r_cutoff must be positive");
+  std::unique_ptr<DipoleFFTImpl> d(new DipoleFFTImpl);
+  d->nb = nbasis;
+  d->n = nspins;
+  for (int a = 0; a < 3; ++a) {
+    if (size[a] < 1) return fail("lattice size must be positive");
+    d->L[a] = size[a];
+    d->per[a] = periodic[a] ? 1 : 0;
+    d->P[a] = periodic[a] ? size[a] : 2 * size[a];
+    if (pass_lds<double>(d->P[a], 1) > kMaxPassLds)
+      return fail("padded dimension too long for one workgroup's LDS (at most 2048)");
+    d->rad[a] = factorise(d->P[a]);
+    if ((int)d->rad[a].size() > fft::kMaxStages) return fail("too many FFT stages");
+    for (int r : d->rad[a])
+      if (r > 64) return fail("padded dimension has a prime factor above 64");
+  }
+  d->Q2 = d->P[2] / 2 + 1;
+  d->M = (int64_t)d->P[0] * d->P[1] * d->Q2;
+  if (d->mus.alloc((size_t)nspins) != hipSuccess) return fail("device allocation failed");
+  if (hipMemcpy(d->mus.p, mus, sizeof(double) * nspins, hipMemcpyHostToDevice) != hipSuccess)
+    return fail("upload failed");
+
+  // ---- real-space tensor grid, [pair][6][P0][P1][P2], normalised by 1/P ----
+  const int64_t P01 = (int64_t)d->P[0] * d->P[1], Pall = P01 * d->P[2];
+  const int ncomp = 6 * nbasis * nbasis;
+  std::vector<double2> F((size_t)ncomp * Pall);
+  for (auto &v : F) { v.x = 0.0; v.y = 0.0; }
+  const double w0 = kMu0IU / (4.0 * kPi * a_m * a_m * a_m) / (double)Pall;
+  const double rc2 = (r_cutoff + kDistTol) * (r_cutoff + kDistTol);
+  // supercell matrix (columns L_a a_a) and its inverse for the minimum image
+  double S[9], Si[9];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) S[3 * r + c] = cell[3 * r + c] * d->L[c];
+  {
+    const double det = S[0] * (S[4] * S[8] - S[5] * S[7]) - S[1] * (S[3] * S[8] - S[5] * S[6]) +
+                       S[2] * (S[3] * S[7] - S[4] * S[6]);
+    if (std::fabs(det) < 1e-300) return fail("singular unit cell");
+    Si[0] = (S[4] * S[8] - S[5] * S[7]) / det; Si[1] = (S[2] * S[7] - S[1] * S[8]) / det;
+    Si[2] = (S[1] * S[5] - S[2] * S[4]) / det; Si[3] = (S[5] * S[6] - S[3] * S[8]) / det;
+    Si[4] = (S[0] * S[8] - S[2] * S[6]) / det; Si[5] = (S[2] * S[3] - S[0] * S[5]) / det;
+    Si[6] = (S[3] * S[7] - S[4] * S[6]) / det; Si[7] = (S[1] * S[6] - S[0] * S[7]) / det;
+    Si[8] = (S[0] * S[4] - S[1] * S[3]) / det;
+  }
+  int lo[3], hi[3];
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = d->per[a] ? 0 : -(d->L[a] - 1);
+    hi[a] = d->L[a] - 1;
+  }
+  const int64_t n0cnt = hi[0] - lo[0] + 1;
+  unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::vector<std::thread> pool;
+  for (unsigned t = 0; t < nt; ++t) {
+    pool.emplace_back([&, t] {
+      for (int64_t i0 = t; i0 < n0cnt; i0 += nt) {
+        const int n0 = lo[0] + (int)i0;
+        for (int n1 = lo[1]; n1 <= hi[1]; ++n1)
+          for (int n2 = lo[2]; n2 <= hi[2]; ++n2)
+            for (int pi = 0; pi < nbasis; ++pi)
+              for (int pj = 0; pj < nbasis; ++pj) {
+                if (n0 == 0 && n1 == 0 && n2 == 0 && pi == pj) continue;
+                const double f[3] = {basis_frac[3 * pi] + n0 - basis_frac[3 * pj],
+                                     basis_frac[3 * pi + 1] + n1 - basis_frac[3 * pj + 1],
+                                     basis_frac[3 * pi + 2] + n2 - basis_frac[3 * pj + 2]};
+                double r[3];
+                for (int a = 0; a < 3; ++a)
+                  r[a] = cell[3 * a] * f[0] + cell[3 * a + 1] * f[1] + cell[3 * a + 2] * f[2];
+                bool any_per = d->per[0] || d->per[1] || d->per[2];
+                if (any_per) {
+                  double g[3];
+                  for (int a = 0; a < 3; ++a) g[a] = Si[3 * a] * r[0] + Si[3 * a + 1] * r[1] + Si[3 * a + 2] * r[2];
+                  for (int a = 0; a < 3; ++a) if (d->per[a]) g[a] -= std::nearbyint(g[a]);
+                  for (int a = 0; a < 3; ++a) r[a] = S[3 * a] * g[0] + S[3 * a + 1] * g[1] + S[3 * a + 2] * g[2];
+                }
+                const double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+                if (r2 > rc2 || r2 == 0.0) continue;
+                const double nr = std::sqrt(r2);
+                const double r5 = nr * nr * nr * nr * nr;
+                const int64_t g0 = ((int64_t)n0 % d->P[0] + d->P[0]) % d->P[0];
+                const int64_t g1 = ((int64_t)n1 % d->P[1] + d->P[1]) % d->P[1];
+                const int64_t g2 = ((int64_t)n2 % d->P[2] + d->P[2]) % d->P[2];
+                const int64_t at = (g0 * d->P[1] + g1) * d->P[2] + g2;
+                static const int ab[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+                for (int q = 0; q < 6; ++q) {
+                  const int a = ab[q][0], b = ab[q][1];
+                  const double v = w0 * (3.0 * r[a] * r[b] - (a == b ? r2 : 0.0)) / r5;
+                  F[((size_t)((pi * nbasis + pj) * 6 + q)) * Pall + at].x = v;
+                }
+              }
+      }
+    });
+  }
+  for (auto &th : pool) th.join();
+  if (d->Fsetup.alloc(F.size()) != hipSuccess) return fail("device allocation failed");
+  if (hipMemcpy(d->Fsetup.p, F.data(), sizeof(double2) * F.size(), hipMemcpyHostToDevice) != hipSuccess)
+    return fail("upload failed");
+  F.clear();
+  F.shrink_to_fit();
+  if (d->transform_tensor(st) != hipSuccess) return fail("tensor transform failed");
+  if (d->ensure(d->f64, st) != hipSuccess) return fail("tensor set-up failed");
+  if (d->ensure(d->f32, st) != hipSuccess) return fail("tensor set-up failed");
+  d->Fsetup.release();
+  return d.release();
 }
 
 }  // namespace jh

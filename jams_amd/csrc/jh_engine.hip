@@ -828,12 +828,12 @@ void do_steps(jh_ctx *c, int64_t nsteps) {
     p.step = (uint64_t)c->iteration;
     p.t = c->time;
     // predictor
-    if (dip >= 0) HIPCHK(c->dipoles[dip]->field(c->s.p, c->hext.p, c->stream));
+    if (dip >= 0) HIPCHK(c->dipoles[dip]->field(c->s.p, c->hext.p, c->prec == PR_FP32, c->stream));
     p.g32 = c->s32.p;
     p.w32 = c->sp32.p;
     HIPCHK(launch_heun(p, c->prec, c->xv, 0, c->stream));
     // corrector at t + dt
-    if (dip >= 0) HIPCHK(c->dipoles[dip]->field(c->sp.p, c->hext.p, c->stream));
+    if (dip >= 0) HIPCHK(c->dipoles[dip]->field(c->sp.p, c->hext.p, c->prec == PR_FP32, c->stream));
     p.g32 = c->sp32.p;
     p.w32 = c->s32.p;
     HIPCHK(launch_heun(p, c->prec, c->xv, 1, c->stream));
@@ -866,7 +866,7 @@ void field_into(jh_ctx *c, int term, double t, double *dst) {
   p.g32 = c->s32.p;
   const int dip = dipole_term(c);
   if (dip >= 0 && (term < 0 || c->terms[term].kind == TK_DIPOLE))
-    HIPCHK(c->dipoles[dip]->field(c->s.p, c->hext.p, c->stream));
+    HIPCHK(c->dipoles[dip]->field(c->s.p, c->hext.p, c->prec == PR_FP32, c->stream));
   HIPCHK(launch_field(p, c->prec, c->xv, term, dst, c->stream));
 }
 
@@ -1065,7 +1065,7 @@ int jh_add_dipole_fft(jh_ctx *c, const int32_t size[3], const int32_t periodic[3
     if (cells * nbasis != c->n) JH_THROW(JH_ERR_INVALID, "dipole lattice does not match N");
     std::string msg;
     std::unique_ptr<DipoleFFT> d(DipoleFFT::create(size, periodic, nbasis, cell, basis_frac, a_m,
-                                                   r_cutoff, c->mus.data(), c->stream, &msg));
+                                                   r_cutoff, c->mus.data(), c->n, c->stream, &msg));
     if (!d) JH_THROW(JH_ERR_INVALID, "dipole-fft: " + msg);
     c->dipoles.push_back(std::move(d));
     c->hext.alloc((size_t)(3 * c->n));

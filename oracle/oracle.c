@@ -558,6 +558,10 @@ void or_term_field(const or_system_t *sys, int term, const double *s, double t, 
     case OR_TERM_FIELD:
       memcpy(h, sys->ext_field, sizeof(double) * 3 * (size_t)n);
       break;
+    case OR_TERM_DIPOLE:
+      or_dipole_bruteforce(sys->dip_lat, n, sys->dip_pos, sys->mus, s, sys->dip_rcut, sys->dip_a,
+                           h, sys->nthreads);
+      break;
   }
 }
 
@@ -689,7 +693,9 @@ double or_term_energy(const or_system_t *sys, int term, const double *s, double 
   double e = 0.0;
   switch (sys->kind[term]) {
     case OR_TERM_EXCHANGE:
-      /* sparse_interaction.cc:86-100: 0.5 * sum_i (-s_i . h_i) */
+    case OR_TERM_DIPOLE:
+      /* sparse_interaction.cc:86-100, dipole_bruteforce.cc:55-58:
+       * 0.5 * sum_i (-s_i . h_i) */
       for (int64_t i = 0; i < n; ++i)
         e += -(s[3 * i] * h[3 * i] + s[3 * i + 1] * h[3 * i + 1] + s[3 * i + 2] * h[3 * i + 2]);
       e = 0.5 * e;

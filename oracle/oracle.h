@@ -57,7 +57,7 @@ double or_setup_uniform(uint64_t seed, uint64_t stream, uint64_t index);
 void or_random_spins(uint64_t seed, int64_t n, double *s);
 
 /* ---- lattice + neighbour list (core/lattice.cc, core/interactions.cc) --- */
-typedef struct {
+typedef struct or_lattice_s {
   double cell[9];        /* unit-cell matrix M, row-major; columns are a1,a2,a3 */
   int nbasis;
   const double *basis_frac;   /* nbasis x 3 fractional positions */
@@ -115,7 +115,8 @@ void or_csrmv(int64_t nrows, const int64_t *rowptr, const int32_t *col, const do
               const double *x, double *y, int nthreads);
 
 /* ---- Hamiltonian terms and the Heun step -------------------------------- */
-enum { OR_TERM_EXCHANGE = 0, OR_TERM_ZEEMAN = 1, OR_TERM_UNIAXIAL = 2, OR_TERM_FIELD = 3 };
+enum { OR_TERM_EXCHANGE = 0, OR_TERM_ZEEMAN = 1, OR_TERM_UNIAXIAL = 2, OR_TERM_FIELD = 3,
+       OR_TERM_DIPOLE = 4 };
 
 typedef struct {
   int64_t n;
@@ -130,6 +131,8 @@ typedef struct {
   const double *ext_field;
   int nterms; int kind[8];              /* terms in Hamiltonian config order */
   int nthreads;
+  /* dipole term, evaluated by brute force (or_dipole_bruteforce) */
+  const struct or_lattice_s *dip_lat; const double *dip_pos; double dip_rcut; double dip_a;
 } or_system_t;
 
 /* Field of one term at time t (meV, N x 3). */
@@ -165,6 +168,13 @@ double or_term_energy(const or_system_t *sys, int term, const double *s, double 
 void or_dipole_bruteforce(const or_lattice_t *lat, int64_t n, const double *pos,
                           const double *mus, const double *s, double r_cutoff,
                           double a_m, double *h, int nthreads);
+
+/* The FFT dipole's convolution as a direct sum over cell offsets
+ * (dipole_fft.cc:195-350).  one_sided_open = 1 keeps the reference's offsets
+ * n in [0, L) on open dimensions; 0 takes -L < n < L (= brute force). */
+void or_dipole_fft_direct(const or_lattice_t *lat, const double *mus, const double *s,
+                          double r_cutoff, double a_m, int one_sided_open, double *h,
+                          int nthreads);
 
 #ifdef __cplusplus
 }

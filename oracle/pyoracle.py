@@ -14,7 +14,7 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "_build" / "liboracle.so"
 
-TERM_EXCHANGE, TERM_ZEEMAN, TERM_UNIAXIAL, TERM_FIELD = 0, 1, 2, 3
+TERM_EXCHANGE, TERM_ZEEMAN, TERM_UNIAXIAL, TERM_FIELD, TERM_DIPOLE = 0, 1, 2, 3, 4
 
 _lib = None
 
@@ -233,7 +233,9 @@ class System(C.Structure):
                 ("val", C.c_void_p), ("zee_dc", C.c_void_p), ("zee_ac", C.c_void_p),
                 ("zee_omega", C.c_void_p), ("ani_K", C.c_void_p), ("ani_axis", C.c_void_p),
                 ("ani_power", C.c_int), ("ext_field", C.c_void_p), ("nterms", C.c_int),
-                ("kind", C.c_int * 8), ("nthreads", C.c_int)]
+                ("kind", C.c_int * 8), ("nthreads", C.c_int),
+                ("dip_lat", C.c_void_p), ("dip_pos", C.c_void_p), ("dip_rcut", C.c_double),
+                ("dip_a", C.c_double)]
 
 
 class OracleSystem:
@@ -251,6 +253,7 @@ class OracleSystem:
         self.ani_K = self.ani_axis = None
         self.ani_power = 2
         self.ext = None
+        self.dip = None
         self.nthreads = nthreads
         self._work = np.empty(12 * self.n, np.float64)
 
@@ -281,6 +284,14 @@ class OracleSystem:
         self.kinds.append(TERM_FIELD)
         return len(self.kinds) - 1
 
+    def add_dipole(self, lat: "LatticeSpec", r_cutoff: float, a_m: float):
+        """Dipole term by brute force (dipole_bruteforce.cc:75-120)."""
+        Lc = lat.c()
+        pos = np.ascontiguousarray(lat.positions(), np.float64)
+        self.dip = (lat, Lc, pos, float(r_cutoff), float(a_m))
+        self.kinds.append(TERM_DIPOLE)
+        return len(self.kinds) - 1
+
     def c(self) -> System:
         S = System()
         S.n = self.n
@@ -292,6 +303,12 @@ class OracleSystem:
         S.nterms = len(self.kinds)
         S.kind[:len(self.kinds)] = self.kinds
         S.nthreads = self.nthreads
+        if self.dip is not None:
+            _, Lc, pos, rc, a = self.dip
+            S.dip_lat = C.cast(C.pointer(Lc), C.c_void_p)
+            S.dip_pos = _p(pos)
+            S.dip_rcut = rc
+            S.dip_a = a
         return S
 
     def term_field(self, term, s, t=0.0) -> np.ndarray:
@@ -359,10 +376,27 @@ def kahan_sum3(x) -> np.ndarray:
     return out
 
 
+def dipole_fft_direct(lat: LatticeSpec, mus, s, r_cutoff, a_m, one_sided_open=False,
+                      nthreads=1) -> np.ndarray:
+    """The FFT dipole's convolution as a direct offset sum (oracle_dipole_fft.c)."""
+    mus = np.ascontiguousarray(mus, np.float64)
+    s = np.ascontiguousarray(s, np.float64).reshape(-1, 3)
+    if mus.size != lat.n or s.shape[0] != lat.n:
+        raise ValueError("mus and s must hold one entry per lattice site")
+    h = np.empty_like(s)
+    Lc = lat.c()
+    lib().or_dipole_fft_direct(C.byref(Lc), _p(mus), _p(s), C.c_double(r_cutoff),
+                               C.c_double(a_m), C.c_int(int(one_sided_open)), _p(h),
+                               C.c_int(nthreads))
+    return h
+
+
 def dipole_bruteforce(lat: LatticeSpec, mus, s, r_cutoff, a_m, nthreads=1) -> np.ndarray:
     pos = np.ascontiguousarray(lat.positions(), np.float64)
     mus = np.ascontiguousarray(mus, np.float64)
     s = np.ascontiguousarray(s, np.float64).reshape(-1, 3)
+    if mus.size != lat.n or s.shape[0] != lat.n:
+        raise ValueError("mus and s must hold one entry per lattice site")
     h = np.empty_like(s)
     Lc = lat.c()
     lib().or_dipole_bruteforce(C.byref(Lc), C.c_int64(mus.size), _p(pos), _p(mus), _p(s),

@@ -1,0 +1,170 @@
+"""GPU parity of the zero-padded FFT dipole field (hamiltonian/dipole_fft.cc)
+through the C ABI.
+
+Pins:
+  - the reference's known answers (test/hamiltonian/test_dipole.h:256-586,
+    tolerance 1e-5 meV per spin, prefactor test_dipole.h:231) on its own
+    lattices;
+  - the field against the oracle's direct offset sum (oracle_dipole_fft.c,
+    two-sided on open dimensions), which equals the brute-force sum, on
+    lattices that exercise every radix, odd and prime lengths, open, periodic
+    and mixed boundaries, one and several basis sites and a non-cubic cell;
+  - the FP32 form against FP64, and a Heun trajectory with exchange + dipole
+    against the oracle's Heun step with the brute-force dipole.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from tests import systems as S
+
+pytestmark = pytest.mark.gpu
+
+A = 0.3e-9
+SC = np.eye(3)
+
+
+def prefactor(moment=2.0, a=A):
+    """test_dipole.h:231: -(0.5 mu0 / 4 pi) (mus)^2 / a^3 in meV."""
+    mu0 = po.vacuum_permeability()
+    return -((0.5 * mu0) / (4 * math.pi)) * (moment * S.BOHR) ** 2 / a ** 3
+
+
+def dipole_engine(make, lat, mus, rc, a=A, prec=0):
+    e = make()
+    n = lat.n
+    e.set_sites(mus, np.full(n, S.gyro_e()), np.full(n, 0.1))
+    e.add_dipole_fft(lat.size, lat.periodic, lat.nbasis, lat.cell, lat.basis_frac, a, rc)
+    e.set_time_step(1e-4)
+    e.set_precision(prec)
+    return e
+
+
+def chain(nb2=False):
+    if nb2:
+        return po.LatticeSpec(np.diag([2.0, 1.0, 1.0]), [[0, 0, 0], [0.5, 0, 0]], [0, 1],
+                              (2000, 1, 1), (1, 0, 0))
+    return po.LatticeSpec(SC, [[0, 0, 0]], [0], (2000, 1, 1), (1, 0, 0))
+
+
+def along(n, v):
+    return np.tile(np.asarray(v, np.float64), (n, 1))
+
+
+def alternating(n, axis):
+    s = np.zeros((n, 3))
+    s[:, axis] = np.where(np.arange(n) % 2 == 0, -1.0, 1.0)
+    return s
+
+
+def afm_2d():
+    return po.LatticeSpec(np.diag([2.0, 2.0, 1.0]),
+                          [[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0], [0, 0.5, 0]], [0, 0, 1, 1],
+                          (256, 256, 1), (1, 1, 0))
+
+
+KNOWN = [
+    # (name, lattice, r_cutoff, spins(n), eigenvalue) -- test_dipole.h
+    ("chain_fm_x", chain, 1000.0, lambda n: along(n, (1, 0, 0)), 4.808228),
+    ("chain_fm_y", chain, 1000.0, lambda n: along(n, (0, 1, 0)), -2.404114),
+    ("chain_fm_z", chain, 1000.0, lambda n: along(n, (0, 0, 1)), -2.404114),
+    ("chain_afm_x", chain, 1000.0, lambda n: alternating(n, 0), -3.60617),
+    ("chain_afm_y", chain, 1000.0, lambda n: alternating(n, 1), 1.803085),
+    ("chain_two_atom_fm_x", lambda: chain(True), 1000.0, lambda n: along(n, (1, 0, 0)), 4.808228),
+    ("film_2d_fm_z", lambda: po.LatticeSpec(SC, [[0, 0, 0]], [0], (256, 256, 1), (1, 1, 0)),
+     128.0, lambda n: along(n, (0, 0, 1)), -9.033622 + 6.28356 * (1.0 / 128.0)),
+    ("film_2d_afm_z", afm_2d, 128.0,
+     lambda n: np.tile(np.array([[0, 0, 1.0], [0, 0, 1.0], [0, 0, -1.0], [0, 0, -1.0]]), (n // 4, 1)),
+     2.6458865),
+]
+
+
+@pytest.mark.parametrize("name,make_lat,rc,spins,value", KNOWN, ids=[k[0] for k in KNOWN])
+def test_dipole_known_answers(engine_factory, name, make_lat, rc, spins, value):
+    lat = make_lat()
+    n = lat.n
+    e = dipole_engine(engine_factory, lat, np.full(n, 2.0 * S.BOHR), rc)
+    e.set_spins(spins(n))
+    got = e.term_energy(0) / n
+    want = prefactor() * value
+    print(f"{name}: {got:.9e} meV/spin, known answer {want:.9e}")
+    assert abs(got - want) < 1e-5
+
+
+def rnd_spins(n, seed):
+    return po.random_spins(seed, n)
+
+
+FIELD_CASES = [
+    # name, lattice, moments, r_cutoff
+    ("sc_open_12x10x4", po.LatticeSpec(SC, [[0, 0, 0]], [0], (12, 10, 4), (0, 0, 0)), None, 30.0),
+    ("sc_periodic_7x6x5", po.LatticeSpec(SC, [[0, 0, 0]], [0], (7, 6, 5), (1, 1, 1)), None, 2.5),
+    ("sc_mixed_16x3x2", po.LatticeSpec(SC, [[0, 0, 0]], [0], (16, 3, 2), (1, 0, 1)), None, 12.0),
+    ("bcc_two_moment_6x5x3_mixed",
+     po.LatticeSpec(SC, [[0, 0, 0], [0.5, 0.5, 0.5]], [0, 1], (6, 5, 3), (1, 0, 1)), (2.3, 1.7), 2.9),
+    ("non_cubic_6x6x6", po.LatticeSpec(np.array([[0.5, 0.0, 0.0], [-0.25, 0.4330127019, 0.0],
+                                                 [0.0, 0.2886751346, 0.4082482905]]),
+                                       [[0, 0, 0]], [0], (6, 6, 6), (1, 1, 1)), None, 1.0),
+    ("fcc_open_5x4x3", po.LatticeSpec(SC, [[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]],
+                                      [0, 0, 0, 0], (5, 4, 3), (0, 0, 0)), None, 10.0),
+    ("chain_2000", chain(), None, 1000.0),
+]
+
+
+def case_mus(lat, moments):
+    if moments is None:
+        return np.full(lat.n, 2.0 * S.BOHR)
+    return np.tile(np.asarray(moments) * S.BOHR, lat.n // len(moments))
+
+
+@pytest.mark.parametrize("name,lat,moments,rc", FIELD_CASES, ids=[c[0] for c in FIELD_CASES])
+def test_dipole_field_matches_direct_sum(engine_factory, name, lat, moments, rc):
+    """FP64: |h_fft - h_direct| <= 1e-10 max|h| (FFT rounding only)."""
+    mus = case_mus(lat, moments)
+    s = rnd_spins(lat.n, 17)
+    e = dipole_engine(engine_factory, lat, mus, rc)
+    e.set_spins(s)
+    got = e.term_field(0)
+    ref = po.dipole_fft_direct(lat, mus, s, rc, A, nthreads=8)
+    scale = np.abs(ref).max()
+    err = np.abs(got - ref).max() / scale
+    print(f"{name}: max|h| {scale:.3e} meV, max rel err {err:.2e}")
+    assert err <= 1e-10
+
+
+@pytest.mark.parametrize("name,lat,moments,rc", FIELD_CASES[:4], ids=[c[0] for c in FIELD_CASES[:4]])
+def test_dipole_fp32_transform_error(engine_factory, name, lat, moments, rc):
+    """The precision form's FP32 transforms: error <= 1e-5 max|h|."""
+    mus = case_mus(lat, moments)
+    s = rnd_spins(lat.n, 23)
+    e64 = dipole_engine(engine_factory, lat, mus, rc, prec=0)
+    e32 = dipole_engine(engine_factory, lat, mus, rc, prec=1)
+    e64.set_spins(s)
+    e32.set_spins(s)
+    h64, h32 = e64.term_field(0), e32.term_field(0)
+    err = np.abs(h32 - h64).max() / np.abs(h64).max()
+    print(f"{name}: FP32 transform rel err {err:.2e}")
+    assert 0 < err <= 1e-5
+
+
+def test_heun_with_exchange_and_dipole_trajectory(engine_factory):
+    """sc 10 x 8 x 4 open film, J = 2 meV + dipole, T = 0, 200 steps: the
+    engine (FFT dipole) against the oracle (brute-force dipole)."""
+    lat = po.LatticeSpec(SC, [[0, 0, 0]], [0], (10, 8, 4), (0, 0, 0))
+    b = S.build(lat, S.sc_nn_rows(2.0), 2.2, 0.02, ["exchange"], dt=S.dt_ps(1e-15), spins="random",
+                seed=5)
+    rc = 20.0
+    o = b.oracle(nthreads=8)
+    o.add_dipole(lat, rc, A)
+    e = b.engine(engine_factory)
+    e.add_dipole_fft(lat.size, lat.periodic, 1, lat.cell, lat.basis_frac, A, rc)
+    e.set_spins(b.s0)
+    e.step(200)
+    s = b.s0.copy()
+    for k in range(200):
+        o.heun_step(s, b.dt, k * b.dt)
+    diff = np.abs(e.get_spins() - s).max()
+    print(f"exchange + dipole, 200 steps: max |ds| = {diff:.2e}")
+    assert diff <= 1e-9

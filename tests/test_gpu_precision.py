@@ -74,18 +74,18 @@ def test_fp32_field_trajectory_angle_c3_small(engine_factory):
 
 
 def test_fp32_uniform_slices_are_used(engine_factory):
-    """Periodic crystals get the wave-uniform tensor variant in every slice
-    (both bcc sites see the same tensors, so even a slice that straddles the
-    two sites is uniform); an open film, whose every slice mixes surface and
-    interior rows, keeps the per-lane variant; FP64 keeps the reference order."""
-    e = S.c3(8).engine(engine_factory, prec=1, use_list=True)
+    """Periodic crystals large enough for row patterns (bcc 16^3 x 2 with five
+    shells, bcc 8^3 x 2 with two) get the wave-uniform tensor variant in every
+    slice; an open film, whose every slice mixes surface and interior rows,
+    keeps the per-lane variant; FP64 keeps the reference's row order."""
+    e = S.c3(16).engine(engine_factory, prec=1, use_list=True)
     d = e.describe()
-    assert "exchange=pattern-uniform-full" in d and "uniform_slices=16/16" in d, d
-    e = S.c2(6).engine(engine_factory, prec=1)
+    assert "exchange=pattern-uniform-full" in d and "uniform_slices=128/128" in d, d
+    e = S.c2(8).engine(engine_factory, prec=1)
     d = e.describe()
-    assert "exchange=pattern-uniform-diag" in d and "uniform_slices=7/7" in d, d
+    assert "exchange=pattern-uniform-diag" in d and "uniform_slices=16/16" in d, d
     e = open_film().engine(engine_factory, prec=1)
     d = e.describe()
     assert "exchange=pattern-diag " in d, d
-    e = S.c3(8).engine(engine_factory, prec=0, use_list=True)
+    e = S.c3(16).engine(engine_factory, prec=0, use_list=True)
     assert "exchange=pattern-full " in e.describe()

@@ -299,3 +299,41 @@ def test_dipole_bruteforce_1d_afm_known_answer():
     h = po.dipole_bruteforce(lat, mus, s, 1000.0, 0.3e-9, nthreads=8)
     e = -0.5 * np.sum(s * h) / n
     assert abs(e - dipole_prefactor() * -3.60617) < 1e-5
+
+
+# ---- the FFT dipole's convolution (oracle_dipole_fft.c) ---------------------
+DIRECT_CASES = [
+    ("sc_open_6x5x4", po.LatticeSpec(np.eye(3), [[0, 0, 0]], [0], (6, 5, 4), (0, 0, 0)), 30.0),
+    ("bcc_mixed_4x3x5", po.LatticeSpec(np.eye(3), [[0, 0, 0], [0.5, 0.5, 0.5]], [0, 0], (4, 3, 5),
+                                       (1, 0, 1)), 1.9),
+    ("sc_periodic_5x5x5", po.LatticeSpec(np.eye(3), [[0, 0, 0]], [0], (5, 5, 5), (1, 1, 1)), 2.0),
+]
+
+
+@pytest.mark.parametrize("name,lat,rc", DIRECT_CASES, ids=[c[0] for c in DIRECT_CASES])
+def test_dipole_fft_direct_two_sided_equals_bruteforce(name, lat, rc):
+    """The convolution with two-sided kernels on open dimensions is the
+    brute-force sum (which the reference's known answers pin)."""
+    mus = np.full(lat.n, 2.0 * S.BOHR)
+    s = po.random_spins(3, lat.n)
+    h_d = po.dipole_fft_direct(lat, mus, s, rc, 0.3e-9, one_sided_open=False, nthreads=8)
+    h_b = po.dipole_bruteforce(lat, mus, s, rc, 0.3e-9, nthreads=8)
+    assert np.abs(h_d - h_b).max() <= 1e-12 * np.abs(h_b).max()
+
+
+def test_dipole_fft_direct_one_sided_as_written_differs_on_open_dimensions():
+    """dipole_fft.cc:227-251 as written fills only n >= 0 on an open
+    dimension: its field is not the dipole field there (not copied)."""
+    lat = DIRECT_CASES[0][1]
+    mus = np.full(lat.n, 2.0 * S.BOHR)
+    s = po.random_spins(3, lat.n)
+    h_one = po.dipole_fft_direct(lat, mus, s, 30.0, 0.3e-9, one_sided_open=True, nthreads=8)
+    h_b = po.dipole_bruteforce(lat, mus, s, 30.0, 0.3e-9, nthreads=8)
+    assert np.abs(h_one - h_b).max() > 0.1 * np.abs(h_b).max()
+    # on periodic dimensions both forms coincide
+    lat = DIRECT_CASES[2][1]
+    mus = np.full(lat.n, 2.0 * S.BOHR)
+    s = po.random_spins(4, lat.n)
+    h1 = po.dipole_fft_direct(lat, mus, s, 2.0, 0.3e-9, one_sided_open=True)
+    h2 = po.dipole_fft_direct(lat, mus, s, 2.0, 0.3e-9, one_sided_open=False)
+    assert np.array_equal(h1, h2)
