@@ -170,6 +170,56 @@ def measure_precision(L: int, steps: int, reps: int):
     return out
 
 
+def measure_dipole(steps: int, reps: int):
+    """The dipolar film (sc 256 x 256 x 8 open, 524,288 spins, NN exchange +
+    dipole-fft over every pair on the 512 x 512 x 16 padded grid, T = 0):
+    step throughput with FP64 and with FP32 transforms from the same random
+    state, the dipole evaluation's time and algorithmic bytes, and the
+    FP32-vs-FP64 difference after `steps` steps."""
+    import numpy as np
+
+    from jams_amd import configs
+    from jams_amd.engine import PREC_FP32_FIELD, PREC_FP64
+    from jams_amd.host import Simulation
+
+    t0 = time.perf_counter()
+    sim = Simulation(configs.c5(), device=0, output_dir="/tmp")
+    eng = sim.engine()
+    n = sim.n
+    setup_s = time.perf_counter() - t0
+    s0 = eng.get_spins()
+    mus = sim.sites()[0]
+    dip = [t for t in range(eng.num_terms()) if eng.term_kind(t) == 3][0]
+    out = {"n": n, "steps": steps, "setup_s": setup_s}
+    finals = {}
+    for name, mode in (("fp64", PREC_FP64), ("fp32_field", PREC_FP32_FIELD)):
+        eng.set_precision(mode)
+        eng.set_spins(s0)
+        eng.set_iteration(0)
+        eng.step(0)
+        ms = eng.time_steps(steps)
+        finals[name] = eng.get_spins()
+        eng.set_spins(s0)
+        t_dip = eng.time_term(dip, reps)
+        b_dip = eng.term_bytes(dip)
+        out[name] = {"value": n * steps / (ms * 1e-3), "ms_per_step": ms / steps,
+                     "dipole_eval_ms": t_dip, "dipole_eval_bytes": b_dip,
+                     "dipole_eval_gbs": b_dip / (t_dip * 1e-3) / 1e9, "engine": eng.describe()}
+        log(f"  C5 {name}: {ms / steps:.4f} ms/step, {out[name]['value']:.4e} spin-updates/s; "
+            f"dipole evaluation {t_dip * 1e3:.1f} us, {b_dip / 1e6:.0f} MB -> "
+            f"{out[name]['dipole_eval_gbs']:.0f} GB/s")
+    ang = _angles(finals["fp64"], finals["fp32_field"])
+    m_of = lambda s: (mus[:, None] * s).sum(0) / mus.sum()
+    out["fp32_vs_fp64"] = {"angle_rad_max": float(ang.max()),
+                           "angle_rad_rms": float(np.sqrt((ang ** 2).mean())),
+                           "dm_abs": float(np.linalg.norm(m_of(finals["fp64"]) - m_of(finals["fp32_field"])))}
+    out["speedup_fp32_vs_fp64"] = out["fp32_field"]["value"] / out["fp64"]["value"]
+    log(f"  C5 FP32 vs FP64 after {steps} steps: max angle {ang.max():.3e} rad, "
+        f"|dm| {out['fp32_vs_fp64']['dm_abs']:.3e}; speed-up {out['speedup_fp32_vs_fp64']:.3f}")
+    sim.close()
+    return out
+
+
 def copy_calibration(nbytes: int = 1 << 30, reps: int = 5) -> int:
     """A plain elementwise copy of nbytes (read nbytes, write nbytes), so that
     rocprofv3's FETCH_SIZE / WRITE_SIZE per dispatch can be scaled to bytes."""
@@ -206,6 +256,8 @@ def main() -> int:
     ap.add_argument("--c3-steps", type=int, default=10000)
     ap.add_argument("--c3-L", type=int, default=128)
     ap.add_argument("--reps", type=int, default=200)
+    ap.add_argument("--no-c5", action="store_true")
+    ap.add_argument("--c5-steps", type=int, default=2000)
     ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
     # the host layer logs set-up to stdout, as the reference does; keep this
@@ -236,6 +288,10 @@ def main() -> int:
     if not args.no_c3:
         log(f"C3 precision form: bcc {args.c3_L}^3 x 2, {args.c3_steps} steps FP64 and FP32-field")
         c3 = measure_precision(args.c3_L, args.c3_steps, min(args.reps, 50))
+    c5 = None
+    if not args.no_c5:
+        log(f"C5 dipolar film: sc 256x256x8 open, {args.c5_steps} steps FP64 and FP32 transforms")
+        c5 = measure_dipole(args.c5_steps, min(args.reps, 50))
     cpu = None
     if not args.no_cpu:
         log("CPU baseline (oracle, same system)")
@@ -280,6 +336,8 @@ def main() -> int:
                               "phases": fp32["phases"]}
     if c3 is not None:
         line["precision_c3"] = c3
+    if c5 is not None:
+        line["dipole_c5"] = c5
     print(json.dumps(line), file=json_out, flush=True)
     return 0
 

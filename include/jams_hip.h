@@ -175,6 +175,10 @@ int jh_time_phase(jh_ctx *ctx, int phase, int reps, double *ms_per_launch);
 /* Algorithmic bytes per launch of the phase kernel in the format actually
  * used, and the same quantity for the reference's FP64 scalar CSR. */
 int jh_phase_bytes(jh_ctx *ctx, int phase, double *fmt_bytes, double *csr_bytes);
+/* Average time (ms) of one evaluation of term `term`'s field on the context
+ * stream (no host copy), and its algorithmic HBM bytes. */
+int jh_time_term(jh_ctx *ctx, int term, int reps, double *ms_per_eval);
+int jh_term_bytes(jh_ctx *ctx, int term, double *bytes);
 /* Text summary of the device formats chosen (for logs). */
 int jh_describe(jh_ctx *ctx, char *buf, int64_t buflen);
 int jh_synchronize(jh_ctx *ctx);

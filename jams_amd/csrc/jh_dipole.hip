@@ -20,8 +20,9 @@
 //   x pass   inverse, outputs x < L0 only;
 //   y pass   inverse, lines x < L0, outputs y < L1;
 //   z pass   c2r, fused with the gather h_i = mus_i Re(...), outputs z < L2.
-// The 1/P normalisation is folded into the tensor.  For one basis site the
-// tensor is real (W is even in n), 6 values per k-point.
+// The 1/P normalisation is folded into the tensor.  For one basis site W is
+// even in n (except for the half-way image of an even periodic length inside
+// r_cutoff), and the transformed tensor is then real: 6 values per k-point.
 #include <algorithm>
 #include <cmath>
 #include <complex>
@@ -193,6 +194,7 @@ class DipoleFFTImpl : public DipoleFFT {
  public:
   int L[3], P[3], per[3], nb;
   int64_t n, Q2, M;  // M = P0 P1 Q2
+  bool real_tensor = false;  // W(n) = W(-n): the transformed tensor is real
   std::vector<int> rad[3];
   DBuf<double> mus;
   Plane<double> f64;
@@ -229,7 +231,7 @@ class DipoleFFTImpl : public DipoleFFT {
     }
     DCHK(pl.K.alloc((size_t)(3 * nb) * M));
     const int ncomp = 6 * nb * nb;
-    if (nb == 1) {
+    if (real_tensor) {
       DCHK(pl.Wr.alloc((size_t)ncomp * M));
     } else {
       DCHK(pl.Wc.alloc((size_t)ncomp * M));
@@ -237,7 +239,7 @@ class DipoleFFTImpl : public DipoleFFT {
     const int64_t blocks = ((int64_t)ncomp * M + 255) / 256;
     hipLaunchKernelGGL((k_dip_extract<T>), dim3((unsigned)blocks), dim3(256), 0, st,
                        (const double2 *)Fsetup.p, (int64_t)P[2], Q2, (int64_t)P[0] * P[1], ncomp,
-                       nb == 1, pl.Wr.p, pl.Wc.p);
+                       real_tensor, pl.Wr.p, pl.Wc.p);
     DCHK(hipGetLastError());
     DCHK(hipStreamSynchronize(st));
     pl.ready = true;
@@ -330,7 +332,7 @@ class DipoleFFTImpl : public DipoleFFT {
     // multiply
     {
       const int64_t blocks = (M + 255) / 256;
-      if (nb == 1)
+      if (real_tensor)
         hipLaunchKernelGGL((k_dip_mult_real<T>), dim3((unsigned)blocks), dim3(256), 0, st, pl.K.p,
                            (const T *)pl.Wr.p, M);
       else
@@ -386,7 +388,7 @@ class DipoleFFTImpl : public DipoleFFT {
     b += G * c * (occ /*z fwd w*/ + occ + half /*y fwd r w*/ + half + full /*x fwd*/ +
                   full + full /*mult r w*/ + full + half /*x inv*/ + half + occ /*y inv*/ +
                   occ /*z inv r*/);
-    b += (nb == 1 ? 6.0 * (c / 2) : 6.0 * nb * nb * c) * full;  // tensor
+    b += (real_tensor ? 6.0 * (c / 2) : 6.0 * nb * nb * c) * full;  // tensor
     return b;
   }
 };
@@ -494,6 +496,27 @@ DipoleFFT *DipoleFFT::create(const int32_t size[3], const int32_t periodic[3], i
     });
   }
   for (auto &th : pool) th.join();
+  // One basis site and W(n) = W(-n mod P): the transform is real and 6 reals
+  // per k-point suffice.  A periodic dimension of even length with
+  // r_cutoff >= L/2 breaks the symmetry (the minimum image of the half-way
+  // offset is not negated), and then the complex form is kept.
+  d->real_tensor = false;
+  if (nbasis == 1) {
+    double wmax = 0.0;
+    for (const auto &v : F) wmax = std::max(wmax, std::fabs(v.x));
+    bool sym = true;
+    for (int64_t g0 = 0; g0 < d->P[0] && sym; ++g0)
+      for (int64_t g1 = 0; g1 < d->P[1] && sym; ++g1)
+        for (int64_t g2 = 0; g2 < d->P[2] && sym; ++g2) {
+          const int64_t at = (g0 * d->P[1] + g1) * d->P[2] + g2;
+          const int64_t m0 = (d->P[0] - g0) % d->P[0], m1 = (d->P[1] - g1) % d->P[1],
+                        m2 = (d->P[2] - g2) % d->P[2];
+          const int64_t mt = (m0 * d->P[1] + m1) * d->P[2] + m2;
+          for (int q = 0; q < 6 && sym; ++q)
+            sym = std::fabs(F[(size_t)q * Pall + at].x - F[(size_t)q * Pall + mt].x) <= 1e-13 * wmax;
+        }
+    d->real_tensor = sym;
+  }
   if (d->Fsetup.alloc(F.size()) != hipSuccess) return fail("device allocation failed");
   if (hipMemcpy(d->Fsetup.p, F.data(), sizeof(double2) * F.size(), hipMemcpyHostToDevice) != hipSuccess)
     return fail("upload failed");

@@ -1299,6 +1299,48 @@ int jh_phase_bytes(jh_ctx *c, int phase, double *fmt_bytes, double *csr_bytes) {
   });
 }
 
+int jh_time_term(jh_ctx *c, int term, int reps, double *ms) {
+  return guarded(c, [&] {
+    prepare(c);
+    if (term < 0 || term >= (int)c->terms.size()) JH_THROW(JH_ERR_INVALID, "term id out of range");
+    if (reps <= 0) JH_THROW(JH_ERR_INVALID, "reps must be positive");
+    KParams p = make_params(c);
+    p.t = c->time;
+    p.g32 = c->s32.p;
+    const bool dip = c->terms[term].kind == TK_DIPOLE;
+    auto once = [&] {
+      if (dip)
+        HIPCHK(c->dipoles[c->terms[term].dipole]->field(c->s.p, c->hext.p, c->prec == PR_FP32, c->stream));
+      else
+        HIPCHK(launch_field(p, c->prec, c->xv, term, c->fout.p, c->stream));
+    };
+    once();
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    for (int r = 0; r < reps; ++r) once();
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    HIPCHK(hipEventSynchronize(c->ev1));
+    float f = 0.f;
+    HIPCHK(hipEventElapsedTime(&f, c->ev0, c->ev1));
+    *ms = (double)f / reps;
+  });
+}
+
+int jh_term_bytes(jh_ctx *c, int term, double *bytes) {
+  return guarded(c, [&] {
+    prepare(c);
+    if (term < 0 || term >= (int)c->terms.size()) JH_THROW(JH_ERR_INVALID, "term id out of range");
+    const double n = (double)c->n;
+    const int kind = c->terms[term].kind;
+    if (kind == TK_DIPOLE) {
+      *bytes = c->dipoles[c->terms[term].dipole]->bytes(c->prec == PR_FP32);
+    } else {
+      // the field kernel: spins read once, field written; exchange adds its format
+      *bytes = 24.0 * n + 24.0 * n;
+      if (kind == TK_EXCHANGE) *bytes += xv_pattern(c->xv) ? 4.0 * n : (double)c->nent * 4.0;
+    }
+  });
+}
+
 int jh_describe(jh_ctx *c, char *buf, int64_t buflen) {
   return guarded(c, [&] {
     prepare(c);

@@ -58,6 +58,8 @@ ABI = {
     "jh_time_phase": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     "jh_phase_bytes": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double),
                                  C.POINTER(C.c_double)]),
+    "jh_time_term": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+    "jh_term_bytes": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
     "jh_describe": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64]),
     "jh_synchronize": (C.c_int, [C.c_void_p]),
 }
@@ -263,6 +265,22 @@ class Engine:
         a, b = C.c_double(), C.c_double()
         self._chk(self.lib.jh_phase_bytes(self.h, int(phase), C.byref(a), C.byref(b)))
         return a.value, b.value
+
+    def num_terms(self) -> int:
+        return int(self.lib.jh_num_terms(self.h))
+
+    def term_kind(self, term: int) -> int:
+        return int(self.lib.jh_term_kind(self.h, int(term)))
+
+    def time_term(self, term: int, reps: int) -> float:
+        ms = C.c_double(0.0)
+        self._chk(self.lib.jh_time_term(self.h, int(term), int(reps), C.byref(ms)))
+        return ms.value
+
+    def term_bytes(self, term: int) -> float:
+        b = C.c_double(0.0)
+        self._chk(self.lib.jh_term_bytes(self.h, int(term), C.byref(b)))
+        return b.value
 
     def describe(self) -> str:
         buf = C.create_string_buffer(512)
