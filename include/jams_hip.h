@@ -166,6 +166,32 @@ int jh_term_energy(jh_ctx *ctx, int term_id, double t, double *energy);
  * reductions (helpers/spinops.cc:42-51, monitors/magnetisation.cc:77-104). */
 int jh_reduce_magnetisation(jh_ctx *ctx, double out[4]);
 
+/* ---- batching form: independent replicas in one context ------------------
+ * No counterpart in the reference, which runs one system per process (the
+ * north-star batching form).  The context's N spins hold nrep independent
+ * systems, replica r owning spins [offsets[r], offsets[r+1]); its sites,
+ * exchange (pairs may not cross replicas), Zeeman and anisotropy terms are
+ * given for all N spins as usual.  Each replica has its own temperature and
+ * noise seed: spin offsets[r] + i draws the thermostat stream (seeds[r],
+ * step, i), so a replica evolves bit for bit as a context holding it alone.
+ * One launch per Heun phase covers all replicas.  nrep = 0 removes them. */
+int jh_set_replicas(jh_ctx *ctx, int nrep, const int64_t *offsets, const double *temperature,
+                    const uint64_t *seeds);
+/* Per replica: sum_i mus_i s_i (3) and sum_i mus_i (1), out is nrep x 4. */
+int jh_reduce_magnetisation_replicas(jh_ctx *ctx, double *out);
+
+/* ---- launch shape -------------------------------------------------------- */
+/* Run jh_step as replays of one captured hipGraph of `steps` Heun steps
+ * (whatever is left over runs as plain launches); 0 disables graphs. */
+int jh_set_graph_steps(jh_ctx *ctx, int steps);
+/* Threads per workgroup of the step kernels: 64 .. 512, a multiple of 64. */
+int jh_set_block_size(jh_ctx *ctx, int threads);
+/* Time `steps` steps of every exchange format that fits (PATTERN, TYPED,
+ * BLOCK) at 128, 256 and 512 threads per workgroup from the current state,
+ * keep the fastest, restore the state; report (may be NULL) gets one line
+ * per candidate. */
+int jh_autotune(jh_ctx *ctx, int steps, char *report, int64_t len);
+
 /* ---- measurement -------------------------------------------------------- */
 /* Run nsteps and return the HIP-event time on the context stream (ms). */
 int jh_time_steps(jh_ctx *ctx, int64_t nsteps, double *ms);

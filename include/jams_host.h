@@ -52,6 +52,10 @@ int jams_sim_exchange_counts(const jams_sim *sim, int index, int64_t *computed, 
                              int64_t *nnz);
 int jams_sim_exchange_pairs(const jams_sim *sim, int index, int32_t *pi, int32_t *pj, double *J9);
 int jams_sim_exchange_csr(const jams_sim *sim, int index, int64_t *rowptr, int32_t *col, double *val);
+/* The same list as the engine receives it (jh_add_exchange_list): per pair
+ * the index of its tensor, and the table of distinct tensors (nvalues x 9). */
+int jams_sim_exchange_values(const jams_sim *sim, int index, int32_t *vidx, double *values9,
+                             int64_t *nvalues);
 
 /* Time step (ps), temperature (K) and thermostat seed chosen by the config. */
 int jams_sim_params(const jams_sim *sim, double *dt_ps, double *temperature, uint64_t *seed);

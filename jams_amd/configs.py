@@ -7,6 +7,9 @@ way a JAMS user would write them, plus their seeded synthetic ingredients.
       alpha 0.05, dt 0.5 fs, T = 300 K, spins +z, 10^4 steps
   C3  bcc 128^3 x 2, five shells (58 neighbours), J_n = 3 exp(-r/a) meV with
       DMI 0.1 J_n d (seeded unit d per bond pair), T = 100 K, dt 0.5 fs
+  C4  256 independent replicas, sizes drawn from a seed: simple-cubic L^3
+      or bcc (L/2)^3 x 2 with L in 8..48, NN J = 1 meV, T_r = 1000 r/255 K,
+      B_r = (0, 0, 2 r/255) T, alpha 0.1, dt 1 fs, 5000 steps
   C5  simple-cubic film 256 x 256 x 8 open, NN J = 2 meV + dipole-fft,
       mu_s = 2.2 muB, a = 0.3 nm, alpha 0.02, dt 1 fs, T = 0, 2000 steps
 
@@ -172,3 +175,34 @@ def c5(Lxy: int = 256, Lz: int = 8, precision="fp64", steps=2000, **kw) -> str:
     return sc_config(L=(Lxy, Lxy, Lz), periodic=(False, False, False), J=2.0, moment=2.2,
                      alpha=0.02, field=None, t_step=1e-15, steps=steps, temperature=0.0, seed=5,
                      dipole_rcut=rcut, precision=precision, **kw)
+
+
+def bcc_nn_rows(J: float):
+    return [("A", "A", [sx * 0.5, sy * 0.5, sz * 0.5], J)
+            for sx, sy, sz in itertools.product((1, -1), repeat=3)]
+
+
+def c4(nrep: int = 256, seed: int = 4, precision: str = "fp64", steps: int = 5000):
+    """The replica batch: a list of (config text, temperature K, noise seed,
+    applied field T, description).  The lattice kind and L are drawn from
+    numpy's default_rng(seed); replica r has T = 1000 r/(nrep-1) K and
+    B = (0, 0, 2 r/(nrep-1)) T; its spins start random from its own seed."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for r in range(nrep):
+        kind = int(rng.integers(0, 2))
+        L = int(rng.integers(8, 49))
+        frac = r / (nrep - 1) if nrep > 1 else 0.0
+        T, B = 1000.0 * frac, (0.0, 0.0, 2.0 * frac)
+        rseed = seed * 100000 + r
+        if kind == 0:
+            text = sc_config(L=(L, L, L), J=1.0, moment=2.0, alpha=0.1, field=None, t_step=1e-15,
+                             steps=steps, temperature=T, seed=rseed, precision=precision)
+            desc = f"sc {L}^3"
+        else:
+            text = bcc_config(L=L // 2, rows=bcc_nn_rows(1.0), moment=2.0, alpha=0.1, K=None,
+                              t_step=1e-15, steps=steps, temperature=T, seed=rseed,
+                              spin='"random"', precision=precision)
+            desc = f"bcc {L // 2}^3 x 2"
+        out.append((text, T, rseed, B, desc))
+    return out

@@ -199,6 +199,28 @@ struct jh_ctx {
   int ani_power = 2;
   int has_ac = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+  // replicas: independent systems in consecutive index ranges
+  int nrep = 0;
+  std::vector<int64_t> rep_off;     // nrep + 1
+  std::vector<double> rep_T;
+  std::vector<uint64_t> rep_seed;
+  DevBuf<int64_t> d_rep_off;
+  DevBuf<double> rep_out;
+
+  // launch shape and graph replay
+  int block = 256;                  // threads per workgroup of the step kernels
+  int graph_steps = 0;              // steps per captured graph (0: plain launches)
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  DevBuf<uint64_t> dstep;
+
+  void drop_graph() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    gexec = nullptr;
+    graph = nullptr;
+  }
 };
 
 namespace {
@@ -705,6 +727,20 @@ void prepare_classes(jh_ctx *c) {
       row[CF_K] = ani->K[i];
       for (int q = 0; q < 3; ++q) row[CF_AXIS + q] = ani->axis[3 * i + q];
     }
+    // thermostat of the spin's system: sqrt(T), seed, first index
+    double T = c->temperature;
+    uint64_t seed = c->seed;
+    int64_t roff = 0;
+    if (c->nrep > 0) {
+      const int r = (int)(std::upper_bound(c->rep_off.begin(), c->rep_off.end(), i) -
+                          c->rep_off.begin()) - 1;
+      T = c->rep_T[r];
+      seed = c->rep_seed[r];
+      roff = c->rep_off[r];
+    }
+    row[CF_SQRTT] = std::sqrt(T);
+    std::memcpy(&row[CF_SEED], &seed, sizeof seed);
+    row[CF_ROFF] = (double)roff;
     std::vector<uint64_t> key(CF_STRIDE);
     std::memcpy(key.data(), row.data(), sizeof(double) * CF_STRIDE);
     auto it = map.find(key);
@@ -758,7 +794,7 @@ void prepare(jh_ctx *c) {
   if (c->terms.size() > 4) JH_THROW(JH_ERR_UNSUPPORTED, "at most 4 Hamiltonian terms");
   // device order: basis-major when rows repeat with the basis period (the
   // dipole term works on the reference order)
-  const int nb = (!c->xh.empty() && dipole_term(c) < 0) ? choose_order(c->xh[0]) : 1;
+  const int nb = (!c->xh.empty() && dipole_term(c) < 0 && c->nrep == 0) ? choose_order(c->xh[0]) : 1;
   if (nb != c->order_nb) {
     std::vector<double> tmp_s((size_t)(3 * c->n)), tmp_w;
     download_aos(c, c->s.p, tmp_s.data());
@@ -771,6 +807,24 @@ void prepare(jh_ctx *c) {
     c->order_cells = c->n / nb;
     upload_soa(c, tmp_s.data(), c->s.p);
     if (inj) upload_soa(c, tmp_w.data(), c->winj.p);
+  }
+  c->drop_graph();
+  if (c->nrep > 0)
+    for (const XHost &x : c->xh)
+      for (int r = 0; r < c->nrep; ++r)
+        for (int64_t i = c->rep_off[r]; i < c->rep_off[r + 1]; ++i)
+          for (int64_t q = x.ptr[i]; q < x.ptr[i + 1]; ++q)
+            if (x.col[q] < c->rep_off[r] || x.col[q] >= c->rep_off[r + 1])
+              JH_THROW(JH_ERR_INVALID, "an exchange pair couples two replicas");
+  if (c->nrep > 0 && dipole_term(c) >= 0)
+    JH_THROW(JH_ERR_UNSUPPORTED, "the dipole term is not available with replicas");
+  if (c->noise_mode != NM_INJECTED) {
+    bool hot = c->temperature > 0.0;
+    if (c->nrep > 0) {
+      hot = false;
+      for (double T : c->rep_T) hot = hot || T > 0.0;
+    }
+    c->noise_mode = hot ? NM_PHILOX : NM_NONE;
   }
   prepare_exchange(c);
   prepare_classes(c);
@@ -810,8 +864,7 @@ KParams make_params(jh_ctx *c) {
   p.has_ac = c->has_ac;
   p.ani_power = c->ani_power;
   p.noise = c->noise_mode;
-  p.seed = c->seed;
-  p.sqrtT = std::sqrt(c->temperature);
+  p.dstep = nullptr;
   p.winj = c->winj.p;
   p.wbuf = c->wbuf.p;
   p.dt = c->dt;
@@ -821,22 +874,59 @@ KParams make_params(jh_ctx *c) {
 }
 
 
-void do_steps(jh_ctx *c, int64_t nsteps) {
-  KParams p = make_params(c);
+// One Heun step's launches (dipole fields, predictor, corrector) for step
+// counter / time in p (or on the device when p.dstep is set).
+void enqueue_step(jh_ctx *c, KParams p) {
   const int dip = dipole_term(c);
-  for (int64_t k = 0; k < nsteps; ++k) {
+  if (dip >= 0) HIPCHK(c->dipoles[dip]->field(c->s.p, c->hext.p, c->prec == PR_FP32, c->stream));
+  p.g32 = c->s32.p;
+  p.w32 = c->sp32.p;
+  HIPCHK(launch_heun(p, c->prec, c->xv, 0, c->block, c->stream));
+  // corrector at t + dt
+  if (dip >= 0) HIPCHK(c->dipoles[dip]->field(c->sp.p, c->hext.p, c->prec == PR_FP32, c->stream));
+  p.g32 = c->sp32.p;
+  p.w32 = c->s32.p;
+  HIPCHK(launch_heun(p, c->prec, c->xv, 1, c->block, c->stream));
+}
+
+// Capture graph_steps steps whose kernels read the step counter from
+// c->dstep and advance it (a one-thread kernel; no memset nodes).
+void build_graph(jh_ctx *c) {
+  c->drop_graph();
+  if (!c->dstep.p) c->dstep.alloc(1);
+  KParams p = make_params(c);
+  p.dstep = c->dstep.p;
+  HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  try {
+    for (int k = 0; k < c->graph_steps; ++k) {
+      enqueue_step(c, p);
+      HIPCHK(launch_advance(c->dstep.p, c->stream));
+    }
+  } catch (...) {
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(c->stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  HIPCHK(hipStreamEndCapture(c->stream, &c->graph));
+  HIPCHK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+}
+
+void do_steps(jh_ctx *c, int64_t nsteps) {
+  int64_t done = 0;
+  if (c->graph_steps > 0 && nsteps >= c->graph_steps) {
+    if (!c->gexec) build_graph(c);
+    HIPCHK(launch_set_step(c->dstep.p, (uint64_t)c->iteration, c->stream));
+    for (; done + c->graph_steps <= nsteps; done += c->graph_steps)
+      HIPCHK(hipGraphLaunch(c->gexec, c->stream));
+    c->iteration += done;
+    c->time = (double)c->iteration * c->dt;
+  }
+  KParams p = make_params(c);
+  for (; done < nsteps; ++done) {
     p.step = (uint64_t)c->iteration;
     p.t = c->time;
-    // predictor
-    if (dip >= 0) HIPCHK(c->dipoles[dip]->field(c->s.p, c->hext.p, c->prec == PR_FP32, c->stream));
-    p.g32 = c->s32.p;
-    p.w32 = c->sp32.p;
-    HIPCHK(launch_heun(p, c->prec, c->xv, 0, c->stream));
-    // corrector at t + dt
-    if (dip >= 0) HIPCHK(c->dipoles[dip]->field(c->sp.p, c->hext.p, c->prec == PR_FP32, c->stream));
-    p.g32 = c->sp32.p;
-    p.w32 = c->s32.p;
-    HIPCHK(launch_heun(p, c->prec, c->xv, 1, c->stream));
+    enqueue_step(c, p);
     c->iteration += 1;
     c->time = (double)c->iteration * c->dt;
   }
@@ -914,6 +1004,7 @@ int jh_destroy(jh_ctx *c) {
   if (!c) return JH_OK;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  c->drop_graph();
   c->dipoles.clear();
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -951,6 +1042,10 @@ int jh_set_sites(jh_ctx *c, int64_t n, const double *mus, const double *gyro, co
     c->order_nb = 1;
     c->order_cells = n;
     c->noise_mode = NM_NONE;
+    c->nrep = 0;
+    c->rep_off.clear();
+    c->rep_T.clear();
+    c->rep_seed.clear();
     c->iteration = 0;
     c->time = 0.0;
     c->dirty = true;
@@ -1249,10 +1344,10 @@ int jh_time_phase(jh_ctx *c, int phase, int reps, double *ms) {
       KParams q = p;
       q.g32 = c->s32.p;
       q.w32 = c->sp32.p;
-      HIPCHK(launch_heun(q, c->prec, c->xv, 0, c->stream));
+      HIPCHK(launch_heun(q, c->prec, c->xv, 0, c->block, c->stream));
     }
     HIPCHK(hipEventRecord(c->ev0, c->stream));
-    for (int r = 0; r < reps; ++r) HIPCHK(launch_heun(p, c->prec, c->xv, phase, c->stream));
+    for (int r = 0; r < reps; ++r) HIPCHK(launch_heun(p, c->prec, c->xv, phase, c->block, c->stream));
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     HIPCHK(hipEventSynchronize(c->ev1));
     float f = 0.f;
@@ -1341,6 +1436,135 @@ int jh_term_bytes(jh_ctx *c, int term, double *bytes) {
   });
 }
 
+int jh_set_replicas(jh_ctx *c, int nrep, const int64_t *offsets, const double *temperature,
+                    const uint64_t *seeds) {
+  return guarded(c, [&] {
+    require_sites(c);
+    if (nrep < 0) JH_THROW(JH_ERR_INVALID, "nrep must be >= 0");
+    c->dirty = true;
+    if (nrep == 0) {
+      c->nrep = 0;
+      c->rep_off.clear();
+      c->rep_T.clear();
+      c->rep_seed.clear();
+      return;
+    }
+    if (!offsets || !temperature || !seeds) JH_THROW(JH_ERR_INVALID, "null replica arrays");
+    if (offsets[0] != 0 || offsets[nrep] != c->n)
+      JH_THROW(JH_ERR_INVALID, "replica offsets must start at 0 and end at N");
+    for (int r = 0; r < nrep; ++r) {
+      if (offsets[r + 1] <= offsets[r]) JH_THROW(JH_ERR_INVALID, "replica offsets must increase");
+      if (!(temperature[r] >= 0.0)) JH_THROW(JH_ERR_INVALID, "temperature must be >= 0");
+    }
+    c->nrep = nrep;
+    c->rep_off.assign(offsets, offsets + nrep + 1);
+    c->rep_T.assign(temperature, temperature + nrep);
+    c->rep_seed.assign(seeds, seeds + nrep);
+    c->d_rep_off.alloc((size_t)nrep + 1);
+    HIPCHK(hipMemcpy(c->d_rep_off.p, offsets, sizeof(int64_t) * (nrep + 1), hipMemcpyHostToDevice));
+    c->rep_out.alloc((size_t)nrep * 4);
+    if (c->noise_mode == NM_INJECTED) c->noise_mode = NM_NONE;
+  });
+}
+
+int jh_reduce_magnetisation_replicas(jh_ctx *c, double *out) {
+  return guarded(c, [&] {
+    prepare(c);
+    if (c->nrep == 0) JH_THROW(JH_ERR_STATE, "no replicas (jh_set_replicas)");
+    KParams p = make_params(c);
+    HIPCHK(launch_reduce_replicas(p, c->d_rep_off.p, c->nrep, c->rep_out.p, c->stream));
+    HIPCHK(hipMemcpyAsync(out, c->rep_out.p, sizeof(double) * 4 * c->nrep, hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int jh_set_graph_steps(jh_ctx *c, int steps) {
+  return guarded(c, [&] {
+    if (steps < 0) JH_THROW(JH_ERR_INVALID, "steps must be >= 0");
+    c->graph_steps = steps;
+    c->drop_graph();
+  });
+}
+
+int jh_set_block_size(jh_ctx *c, int threads) {
+  return guarded(c, [&] {
+    if (threads < 64 || threads > kMaxBlock || threads % 64 != 0)
+      JH_THROW(JH_ERR_INVALID, "workgroup size must be a multiple of 64 in [64, 512]");
+    c->block = threads;
+    c->drop_graph();
+  });
+}
+
+int jh_autotune(jh_ctx *c, int steps, char *report, int64_t len) {
+  return guarded(c, [&] {
+    if (!c->have_dt) JH_THROW(JH_ERR_STATE, "time step not set");
+    if (steps <= 0) JH_THROW(JH_ERR_INVALID, "steps must be positive");
+    prepare(c);
+    // the state every candidate starts from, restored afterwards
+    DevBuf<double> save;
+    save.alloc((size_t)(3 * c->n));
+    HIPCHK(hipMemcpyAsync(save.p, c->s.p, sizeof(double) * 3 * c->n, hipMemcpyDeviceToDevice, c->stream));
+    const int64_t it0 = c->iteration;
+    const double t0 = c->time;
+    const int fmt0 = c->xfmt_req, bs0 = c->block, gs0 = c->graph_steps;
+    c->graph_steps = 0;
+    std::string rep;
+    double best = 1e300;
+    int best_fmt = fmt0, best_bs = bs0;
+    for (int fmt : {JH_XFMT_PATTERN, JH_XFMT_TYPED, JH_XFMT_BLOCK}) {
+      if (c->xh.empty() && fmt != JH_XFMT_PATTERN) continue;
+      c->xfmt_req = fmt;
+      c->dirty = true;
+      try {
+        prepare(c);
+      } catch (const JhError &) {
+        continue;  // this format does not fit the system
+      }
+      for (int bs : {128, 256, 512}) {
+        c->block = bs;
+        HIPCHK(hipMemcpyAsync(c->s.p, save.p, sizeof(double) * 3 * c->n, hipMemcpyDeviceToDevice, c->stream));
+        sync_f32(c);
+        c->iteration = it0;
+        c->time = t0;
+        do_steps(c, 2);
+        HIPCHK(hipEventRecord(c->ev0, c->stream));
+        do_steps(c, steps);
+        HIPCHK(hipEventRecord(c->ev1, c->stream));
+        HIPCHK(hipEventSynchronize(c->ev1));
+        float f = 0.f;
+        HIPCHK(hipEventElapsedTime(&f, c->ev0, c->ev1));
+        const double ms = (double)f / steps;
+        char line[128];
+        std::snprintf(line, sizeof line, "format=%d block=%d %.4f ms/step\n", fmt, bs, ms);
+        rep += line;
+        if (ms < best) {
+          best = ms;
+          best_fmt = fmt;
+          best_bs = bs;
+        }
+      }
+    }
+    c->xfmt_req = best_fmt;
+    c->block = best_bs;
+    c->graph_steps = gs0;
+    c->dirty = true;
+    prepare(c);
+    HIPCHK(hipMemcpyAsync(c->s.p, save.p, sizeof(double) * 3 * c->n, hipMemcpyDeviceToDevice, c->stream));
+    sync_f32(c);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->iteration = it0;
+    c->time = t0;
+    char line[128];
+    std::snprintf(line, sizeof line, "chosen format=%d block=%d", best_fmt, best_bs);
+    rep += line;
+    if (report && len > 0) {
+      std::strncpy(report, rep.c_str(), (size_t)len - 1);
+      report[len - 1] = 0;
+    }
+  });
+}
+
 int jh_describe(jh_ctx *c, char *buf, int64_t buflen) {
   return guarded(c, [&] {
     prepare(c);
@@ -1352,12 +1576,12 @@ int jh_describe(jh_ctx *c, char *buf, int64_t buflen) {
     std::snprintf(tmp, sizeof tmp,
                   "n=%lld prec=%s exchange=%s entries=%lld tensors=%d colbits=%d patterns=%d "
                   "pattern_words=%d lds=%d uniform_slices=%lld/%lld order=%d classes=%d "
-                  "terms=%d noise=%d",
+                  "terms=%d noise=%d block=%d replicas=%d graph=%d",
                   (long long)c->n, c->prec == PR_FP64 ? "fp64" : "fp32-field", xvn[c->xv],
                   (long long)c->nent, c->ntab > 0 ? c->ntab - 1 : 0, c->colbits,
                   c->npatterns, c->pat_words, c->pat_lds, (long long)c->nuniform,
                   (long long)c->nslices, c->order_nb, c->nclass,
-                  (int)c->terms.size(), c->noise_mode);
+                  (int)c->terms.size(), c->noise_mode, c->block, c->nrep, c->graph_steps);
     if (buf && buflen > 0) {
       std::strncpy(buf, tmp, (size_t)buflen - 1);
       buf[buflen - 1] = 0;

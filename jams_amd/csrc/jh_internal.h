@@ -22,8 +22,11 @@
 //    wave read contiguous neighbours; the reference (AoS) order is restored
 //    at the C ABI;
 //  - per-spin constants (mus, gyro, alpha, sigma, Zeeman and anisotropy
-//    parameters) are de-duplicated into a small table of site classes with
-//    a 16-bit class index per spin (no index at all when there is one class).
+//    parameters, and the temperature, noise seed and first index of the
+//    spin's replica) are de-duplicated into a small table of site classes
+//    with a 16-bit class index per spin (no index at all with one class);
+//  - replicas (independent systems batched in one context) are contiguous
+//    index ranges; nothing else distinguishes them on the device.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -31,7 +34,8 @@
 
 namespace jh {
 
-constexpr int kBlock = 256;          // threads per workgroup (4 wave64)
+constexpr int kBlock = 256;          // threads per workgroup of the helper kernels
+constexpr int kMaxBlock = 512;       // largest step-kernel workgroup (autotuned)
 constexpr int kSlice = 64;           // spins per ELL slice (one wavefront)
 constexpr uint32_t kPad = 0xFFFFFFFFu;
 
@@ -55,14 +59,17 @@ constexpr bool xv_diag(int v) {
   return v == XV_TDIAG || v == XV_BDIAG || v == XV_PDIAG || v == XV_GDIAG || v == XV_UDIAG;
 }
 
-// site-class table: 16 doubles per class
+// site-class table: CF_STRIDE doubles per class
 enum ClassField {
   CF_MUS = 0, CF_GYRO = 1, CF_ALPHA = 2, CF_SIGMA = 3,
   CF_ZDC = 4,   // 4..6
   CF_ZAC = 7,   // 7..9
   CF_ZOMEGA = 10, CF_K = 11,
   CF_AXIS = 12,  // 12..14
-  CF_STRIDE = 16
+  CF_SQRTT = 15,  // sqrt(temperature) of the spin's system (replica)
+  CF_SEED = 16,   // its noise seed (the 64 bits stored in the double)
+  CF_ROFF = 17,   // index of its first spin: the noise counter is index - ROFF
+  CF_STRIDE = 18
 };
 
 struct XView {
@@ -105,9 +112,8 @@ struct KParams {
   int has_ac;
   int ani_power;
   int noise;
-  uint64_t seed;
   uint64_t step;
-  double sqrtT;
+  const uint64_t *dstep;  // graph replay: the step counter lives on the device
   const double *winj;  // injected noise SoA
   double *wbuf;        // noise of the current step: written by the predictor,
                        // read by the corrector (SoA)

@@ -269,6 +269,15 @@ __device__ __forceinline__ double cparam(const KParams &p, uint32_t c, int f) {
   return p.ctab[c * CF_STRIDE + f];
 }
 
+// Noise stream of a spin: the seed of its replica and its index within it
+// (reference order), so a replica draws the same numbers batched or alone.
+__device__ __forceinline__ uint64_t noise_seed(const KParams &p, uint32_t c) {
+  return (uint64_t)__double_as_longlong(cparam(p, c, CF_SEED));
+}
+__device__ __forceinline__ uint64_t noise_index(const KParams &p, uint32_t c, int64_t i) {
+  return (uint64_t)(ref_index(p, i) - (int64_t)cparam(p, c, CF_ROFF));
+}
+
 // Local (non-exchange) term q of spin i with spin vector (s0, s1, s2).
 // GEN = false compiles only the common forms (static Zeeman, K1 anisotropy);
 // GEN = true adds the ac Zeeman field and the K2/K3 powers.
@@ -333,20 +342,25 @@ __device__ __forceinline__ void total_field(const KParams &p, const double *src6
 }
 
 template <int PREC, int XVk, int PHASE, bool GEN>
-__global__ void __launch_bounds__(kBlock, JH_WAVES_PER_EU) k_heun(const KParams p) {
+__global__ void __launch_bounds__(kMaxBlock, JH_WAVES_PER_EU) k_heun(const KParams p) {
   extern __shared__ __align__(16) unsigned char lds[];
   load_table<PREC, XVk>(p.x, lds);
   if constexpr (xv_typed(XVk)) __syncthreads();
   const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
-  const int64_t i = blk * kBlock + threadIdx.x;
+  const int64_t i = blk * blockDim.x + threadIdx.x;
   if (i >= p.n) return;
   const int64_t n = p.n;
   const uint32_t c = p.cls ? (uint32_t)p.cls[i] : 0u;
 
+  // step counter and time: from the host, or from the device in a graph
+  // replay (t = step dt, the host's own expression)
+  const uint64_t step = p.dstep ? *p.dstep : p.step;
+  const double t0 = p.dstep ? (double)step * p.dt : p.t;
+
   // spin that the field is evaluated at
   const double *src = PHASE == 0 ? p.s : p.sp;
   const double s0 = src[i], s1 = src[n + i], s2 = src[2 * n + i];
-  const double t = PHASE == 0 ? p.t : p.t + p.dt;
+  const double t = PHASE == 0 ? t0 : t0 + p.dt;
 
   double h0, h1, h2;
   total_field<PREC, XVk, GEN>(p, src, p.g32, i, c, lds, s0, s1, s2, t, h0, h1, h2);
@@ -361,9 +375,9 @@ __global__ void __launch_bounds__(kBlock, JH_WAVES_PER_EU) k_heun(const KParams 
     double w0, w1, w2;
     if (p.noise == NM_PHILOX && PHASE == 0) {
       double z0, z1, z2;
-      normals3(p.seed, p.step, (uint64_t)ref_index(p, i), z0, z1, z2);
-      const double sg = cparam(p, c, CF_SIGMA);
-      w0 = z0 * sg * p.sqrtT; w1 = z1 * sg * p.sqrtT; w2 = z2 * sg * p.sqrtT;
+      normals3(noise_seed(p, c), step, noise_index(p, c, i), z0, z1, z2);
+      const double sg = cparam(p, c, CF_SIGMA), sT = cparam(p, c, CF_SQRTT);
+      w0 = z0 * sg * sT; w1 = z1 * sg * sT; w2 = z2 * sg * sT;
       p.wbuf[i] = w0; p.wbuf[n + i] = w1; p.wbuf[2 * n + i] = w2;
     } else if (p.noise == NM_PHILOX) {
       w0 = p.wbuf[i]; w1 = p.wbuf[n + i]; w2 = p.wbuf[2 * n + i];
@@ -426,11 +440,11 @@ __global__ void __launch_bounds__(kBlock) k_noise(const KParams p, double *out) 
   if (i >= p.n) return;
   const uint32_t c = p.cls ? (uint32_t)p.cls[i] : 0u;
   double z0, z1, z2;
-  normals3(p.seed, p.step, (uint64_t)ref_index(p, i), z0, z1, z2);
-  const double sg = cparam(p, c, CF_SIGMA);
-  out[i] = z0 * sg * p.sqrtT;
-  out[p.n + i] = z1 * sg * p.sqrtT;
-  out[2 * p.n + i] = z2 * sg * p.sqrtT;
+  normals3(noise_seed(p, c), p.step, noise_index(p, c, i), z0, z1, z2);
+  const double sg = cparam(p, c, CF_SIGMA), sT = cparam(p, c, CF_SQRTT);
+  out[i] = z0 * sg * sT;
+  out[p.n + i] = z1 * sg * sT;
+  out[2 * p.n + i] = z2 * sg * sT;
 }
 
 // FP32 gather copy of SoA FP64 spins: (x, y, z, 0) per spin, 16 bytes
@@ -491,6 +505,40 @@ __global__ void __launch_bounds__(kBlock) k_reduce_final(const double *partial, 
   if (threadIdx.x < 4) out[threadIdx.x] = red[threadIdx.x][0];
 }
 
+// Per-replica magnetisation: workgroup r sums (mus s, mus) over the spins
+// [off[r], off[r+1]) in a fixed pairwise order.
+__global__ void __launch_bounds__(kBlock) k_reduce_replicas(const KParams p, const int64_t *off,
+                                                            double *out) {
+  __shared__ double red[4][kBlock];
+  const int r = blockIdx.x;
+  const int64_t a = off[r], b = off[r + 1], n = p.n;
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int64_t i = a + threadIdx.x; i < b; i += kBlock) {
+    const uint32_t c = p.cls ? (uint32_t)p.cls[i] : 0u;
+    const double mu = cparam(p, c, CF_MUS);
+    v[0] = v[0] + mu * p.s[i];
+    v[1] = v[1] + mu * p.s[n + i];
+    v[2] = v[2] + mu * p.s[2 * n + i];
+    v[3] = v[3] + mu;
+  }
+  for (int q = 0; q < 4; ++q) red[q][threadIdx.x] = v[q];
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int q = 0; q < 4; ++q) red[q][threadIdx.x] = red[q][threadIdx.x] + red[q][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) out[(int64_t)r * 4 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// One thread advances (or sets) the device step counter of graph replays.
+__global__ void k_advance(uint64_t *step) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *step = *step + 1;
+}
+__global__ void k_set_step(uint64_t *step, uint64_t value) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *step = value;
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 
@@ -501,46 +549,63 @@ static size_t table_lds(const KParams &p, int prec, int xv) {
 }
 
 template <int PREC, int XVk>
-static hipError_t launch_heun_t(const KParams &p, int phase, hipStream_t st) {
-  const int64_t nb = (p.n + kBlock - 1) / kBlock;
+static hipError_t launch_heun_t(const KParams &p, int phase, int bs, hipStream_t st) {
+  const int64_t nb = (p.n + bs - 1) / bs;
   const size_t lds = table_lds(p, PREC, XVk);
   const bool gen = p.has_ac || p.ani_power != 2;
   if (phase == 0 && !gen)
-    hipLaunchKernelGGL((k_heun<PREC, XVk, 0, false>), dim3((unsigned)nb), dim3(kBlock), lds, st, p);
+    hipLaunchKernelGGL((k_heun<PREC, XVk, 0, false>), dim3((unsigned)nb), dim3(bs), lds, st, p);
   else if (phase == 0)
-    hipLaunchKernelGGL((k_heun<PREC, XVk, 0, true>), dim3((unsigned)nb), dim3(kBlock), lds, st, p);
+    hipLaunchKernelGGL((k_heun<PREC, XVk, 0, true>), dim3((unsigned)nb), dim3(bs), lds, st, p);
   else if (!gen)
-    hipLaunchKernelGGL((k_heun<PREC, XVk, 1, false>), dim3((unsigned)nb), dim3(kBlock), lds, st, p);
+    hipLaunchKernelGGL((k_heun<PREC, XVk, 1, false>), dim3((unsigned)nb), dim3(bs), lds, st, p);
   else
-    hipLaunchKernelGGL((k_heun<PREC, XVk, 1, true>), dim3((unsigned)nb), dim3(kBlock), lds, st, p);
+    hipLaunchKernelGGL((k_heun<PREC, XVk, 1, true>), dim3((unsigned)nb), dim3(bs), lds, st, p);
   return hipGetLastError();
 }
 
 template <int PREC>
-static hipError_t launch_heun_p(const KParams &p, int xv, int phase, hipStream_t st) {
+static hipError_t launch_heun_p(const KParams &p, int xv, int phase, int bs, hipStream_t st) {
   switch (xv) {
-    case XV_NONE: return launch_heun_t<PREC, XV_NONE>(p, phase, st);
-    case XV_TDIAG: return launch_heun_t<PREC, XV_TDIAG>(p, phase, st);
-    case XV_TFULL: return launch_heun_t<PREC, XV_TFULL>(p, phase, st);
-    case XV_BDIAG: return launch_heun_t<PREC, XV_BDIAG>(p, phase, st);
-    case XV_BFULL: return launch_heun_t<PREC, XV_BFULL>(p, phase, st);
-    case XV_PDIAG: return launch_heun_t<PREC, XV_PDIAG>(p, phase, st);
-    case XV_PFULL: return launch_heun_t<PREC, XV_PFULL>(p, phase, st);
-    case XV_GDIAG: return launch_heun_t<PREC, XV_GDIAG>(p, phase, st);
-    case XV_GFULL: return launch_heun_t<PREC, XV_GFULL>(p, phase, st);
+    case XV_NONE: return launch_heun_t<PREC, XV_NONE>(p, phase, bs, st);
+    case XV_TDIAG: return launch_heun_t<PREC, XV_TDIAG>(p, phase, bs, st);
+    case XV_TFULL: return launch_heun_t<PREC, XV_TFULL>(p, phase, bs, st);
+    case XV_BDIAG: return launch_heun_t<PREC, XV_BDIAG>(p, phase, bs, st);
+    case XV_BFULL: return launch_heun_t<PREC, XV_BFULL>(p, phase, bs, st);
+    case XV_PDIAG: return launch_heun_t<PREC, XV_PDIAG>(p, phase, bs, st);
+    case XV_PFULL: return launch_heun_t<PREC, XV_PFULL>(p, phase, bs, st);
+    case XV_GDIAG: return launch_heun_t<PREC, XV_GDIAG>(p, phase, bs, st);
+    case XV_GFULL: return launch_heun_t<PREC, XV_GFULL>(p, phase, bs, st);
     case XV_UDIAG:
-      if constexpr (PREC == PR_FP32) return launch_heun_t<PREC, XV_UDIAG>(p, phase, st);
+      if constexpr (PREC == PR_FP32) return launch_heun_t<PREC, XV_UDIAG>(p, phase, bs, st);
       break;
     case XV_UFULL:
-      if constexpr (PREC == PR_FP32) return launch_heun_t<PREC, XV_UFULL>(p, phase, st);
+      if constexpr (PREC == PR_FP32) return launch_heun_t<PREC, XV_UFULL>(p, phase, bs, st);
       break;
   }
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_heun(const KParams &p, int prec, int xv, int phase, hipStream_t st) {
-  return prec == PR_FP64 ? launch_heun_p<PR_FP64>(p, xv, phase, st)
-                         : launch_heun_p<PR_FP32>(p, xv, phase, st);
+hipError_t launch_heun(const KParams &p, int prec, int xv, int phase, int bs, hipStream_t st) {
+  if (bs < 64 || bs > kMaxBlock || bs % 64 != 0) return hipErrorInvalidValue;
+  return prec == PR_FP64 ? launch_heun_p<PR_FP64>(p, xv, phase, bs, st)
+                         : launch_heun_p<PR_FP32>(p, xv, phase, bs, st);
+}
+
+hipError_t launch_reduce_replicas(const KParams &p, const int64_t *off, int nrep, double *out,
+                                  hipStream_t st) {
+  hipLaunchKernelGGL(k_reduce_replicas, dim3((unsigned)nrep), dim3(kBlock), 0, st, p, off, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_advance(uint64_t *step, hipStream_t st) {
+  hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, st, step);
+  return hipGetLastError();
+}
+
+hipError_t launch_set_step(uint64_t *step, uint64_t value, hipStream_t st) {
+  hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, st, step, value);
+  return hipGetLastError();
 }
 
 template <int PREC, int XVk>

@@ -7,7 +7,12 @@
 
 namespace jh {
 
-hipError_t launch_heun(const KParams &p, int prec, int xv, int phase, hipStream_t st);
+// bs: threads per workgroup of the step kernels (64 .. 512, multiple of 64)
+hipError_t launch_heun(const KParams &p, int prec, int xv, int phase, int bs, hipStream_t st);
+hipError_t launch_reduce_replicas(const KParams &p, const int64_t *off, int nrep, double *out,
+                                  hipStream_t st);
+hipError_t launch_advance(uint64_t *step, hipStream_t st);
+hipError_t launch_set_step(uint64_t *step, uint64_t value, hipStream_t st);
 hipError_t launch_field(const KParams &p, int prec, int xv, int term, double *out,
                         hipStream_t st);
 hipError_t launch_noise(const KParams &p, double *out, hipStream_t st);

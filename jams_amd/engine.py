@@ -58,6 +58,11 @@ ABI = {
     "jh_time_phase": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     "jh_phase_bytes": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double),
                                  C.POINTER(C.c_double)]),
+    "jh_set_replicas": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "jh_reduce_magnetisation_replicas": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "jh_set_graph_steps": (C.c_int, [C.c_void_p, C.c_int]),
+    "jh_set_block_size": (C.c_int, [C.c_void_p, C.c_int]),
+    "jh_autotune": (C.c_int, [C.c_void_p, C.c_int, C.c_char_p, C.c_int64]),
     "jh_time_term": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     "jh_term_bytes": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
     "jh_describe": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64]),
@@ -265,6 +270,33 @@ class Engine:
         a, b = C.c_double(), C.c_double()
         self._chk(self.lib.jh_phase_bytes(self.h, int(phase), C.byref(a), C.byref(b)))
         return a.value, b.value
+
+    def set_replicas(self, offsets, temperatures, seeds):
+        """Declare replicas: spins [offsets[r], offsets[r+1]) with their own
+        temperature and noise seed."""
+        off = np.ascontiguousarray(offsets, np.int64)
+        T = np.ascontiguousarray(temperatures, np.float64)
+        sd = np.ascontiguousarray(np.asarray(seeds, np.uint64))
+        if T.size != off.size - 1 or sd.size != off.size - 1:
+            raise ValueError("one temperature and one seed per replica")
+        self._chk(self.lib.jh_set_replicas(self.h, int(T.size), _ptr(off), _ptr(T), _ptr(sd)))
+        self.nrep = int(T.size)
+
+    def replica_magnetisation(self) -> np.ndarray:
+        out = np.empty((self.nrep, 4), np.float64)
+        self._chk(self.lib.jh_reduce_magnetisation_replicas(self.h, _ptr(out)))
+        return out
+
+    def set_graph_steps(self, steps: int):
+        self._chk(self.lib.jh_set_graph_steps(self.h, int(steps)))
+
+    def set_block_size(self, threads: int):
+        self._chk(self.lib.jh_set_block_size(self.h, int(threads)))
+
+    def autotune(self, steps: int = 20) -> str:
+        buf = C.create_string_buffer(4096)
+        self._chk(self.lib.jh_autotune(self.h, int(steps), buf, 4096))
+        return buf.value.decode()
 
     def num_terms(self) -> int:
         return int(self.lib.jh_num_terms(self.h))

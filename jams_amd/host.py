@@ -25,6 +25,8 @@ ABI = {
                                            C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "jams_sim_exchange_pairs": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     "jams_sim_exchange_csr": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "jams_sim_exchange_values": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                           C.POINTER(C.c_int64)]),
     "jams_sim_params": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                   C.POINTER(C.c_uint64)]),
 }
@@ -124,6 +126,20 @@ class Simulation:
         J = np.empty((npairs, 9), np.float64)
         self._chk(self.lib.jams_sim_exchange_pairs(self.h, index, _p(pi), _p(pj), _p(J)))
         return pi, pj, J
+
+    def exchange_list(self, index=0):
+        """(pi, pj, value index per pair, distinct tensors nvalues x 9): the
+        neighbour list exactly as the engine receives it."""
+        _, npairs, _ = self.exchange_counts(index)
+        pi = np.empty(npairs, np.int32)
+        pj = np.empty(npairs, np.int32)
+        self._chk(self.lib.jams_sim_exchange_pairs(self.h, index, _p(pi), _p(pj), None))
+        nv = C.c_int64()
+        self._chk(self.lib.jams_sim_exchange_values(self.h, index, None, None, C.byref(nv)))
+        vidx = np.empty(npairs, np.int32)
+        vals = np.empty((nv.value, 9), np.float64)
+        self._chk(self.lib.jams_sim_exchange_values(self.h, index, _p(vidx), _p(vals), C.byref(nv)))
+        return pi, pj, vidx, vals
 
     def exchange_csr(self, index=0):
         _, _, nnz = self.exchange_counts(index)

@@ -228,6 +228,18 @@ int jams_sim_exchange_pairs(const jams_sim *sim, int index, int32_t *pi, int32_t
   });
 }
 
+int jams_sim_exchange_values(const jams_sim *sim, int index, int32_t *vidx, double *values9,
+                             int64_t *nvalues) {
+  return guarded(const_cast<jams_sim *>(sim), [&] {
+    const NeighbourList &nl = exchange_term(sim, index)->neighbours();
+    if (nvalues) *nvalues = (int64_t)nl.values.size();
+    if (vidx) std::memcpy(vidx, nl.v.data(), sizeof(int32_t) * nl.size());
+    if (values9)
+      for (size_t v = 0; v < nl.values.size(); ++v)
+        for (int q = 0; q < 9; ++q) values9[9 * v + q] = nl.values[v][q / 3][q % 3];
+  });
+}
+
 int jams_sim_exchange_csr(const jams_sim *sim, int index, int64_t *rowptr, int32_t *col, double *val) {
   return guarded(const_cast<jams_sim *>(sim), [&] {
     std::vector<int64_t> rp;
