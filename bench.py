@@ -220,6 +220,46 @@ def measure_dipole(steps: int, reps: int):
     return out
 
 
+def measure_batch(steps: int, nrep: int = 256):
+    """The batching form: nrep independent replicas (sizes drawn from a seed,
+    sc L^3 or bcc (L/2)^3 x 2 with L in 8..48, T_r = 1000 r/(nrep-1) K,
+    B_r = 2 r/(nrep-1) T along z) in one context, one launch per Heun phase
+    for all of them, the step loop replayed from a captured hipGraph, the
+    exchange format and workgroup size picked by the autotuner; FP64 and
+    FP32-field."""
+    from jams_amd import configs
+    from jams_amd.batch import ReplicaBatch
+    from jams_amd.engine import PREC_FP32_FIELD, PREC_FP64
+
+    reps = configs.c4(nrep=nrep)
+    out = {"replicas": nrep, "steps": steps}
+    for name, mode in (("fp64", PREC_FP64), ("fp32_field", PREC_FP32_FIELD)):
+        t0 = time.perf_counter()
+        b = ReplicaBatch(reps, precision=mode)
+        setup_s = time.perf_counter() - t0
+        e = b.engine
+        tune = e.autotune(5)
+        # launches vs graph replay over the same short run, state restored after
+        s0 = e.get_spins()
+        ms_plain = e.time_steps(200)
+        e.set_graph_steps(50)
+        e.set_spins(s0)
+        e.set_iteration(0)
+        e.step(0)
+        ms = e.time_steps(steps)
+        m = b.magnetisation()
+        out[name] = {"spins": b.n, "value": b.n * steps / (ms * 1e-3), "ms_per_step": ms / steps,
+                     "ms_per_step_plain_launches": ms_plain / 200, "setup_s": setup_s,
+                     "autotune": tune.splitlines()[-1], "engine": e.describe(),
+                     "m_coldest": float(m[0]), "m_hottest": float(m[-1])}
+        log(f"  C4 {name}: {b.n} spins in {nrep} replicas, {ms / steps:.4f} ms/step with graph "
+            f"({ms_plain / 200:.4f} plain), {out[name]['value']:.4e} spin-updates/s; "
+            f"{tune.splitlines()[-1]}; |m| coldest {m[0]:.4f} hottest {m[-1]:.4f}")
+        b.close()
+    out["speedup_fp32_vs_fp64"] = out["fp32_field"]["value"] / out["fp64"]["value"]
+    return out
+
+
 def copy_calibration(nbytes: int = 1 << 30, reps: int = 5) -> int:
     """A plain elementwise copy of nbytes (read nbytes, write nbytes), so that
     rocprofv3's FETCH_SIZE / WRITE_SIZE per dispatch can be scaled to bytes."""
@@ -257,6 +297,8 @@ def main() -> int:
     ap.add_argument("--c3-L", type=int, default=128)
     ap.add_argument("--reps", type=int, default=200)
     ap.add_argument("--no-c5", action="store_true")
+    ap.add_argument("--no-c4", action="store_true")
+    ap.add_argument("--c4-steps", type=int, default=5000)
     ap.add_argument("--c5-steps", type=int, default=2000)
     ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
@@ -288,6 +330,10 @@ def main() -> int:
     if not args.no_c3:
         log(f"C3 precision form: bcc {args.c3_L}^3 x 2, {args.c3_steps} steps FP64 and FP32-field")
         c3 = measure_precision(args.c3_L, args.c3_steps, min(args.reps, 50))
+    c4 = None
+    if not args.no_c4:
+        log(f"C4 batching form: 256 replicas, {args.c4_steps} steps FP64 and FP32-field")
+        c4 = measure_batch(args.c4_steps)
     c5 = None
     if not args.no_c5:
         log(f"C5 dipolar film: sc 256x256x8 open, {args.c5_steps} steps FP64 and FP32 transforms")
@@ -336,6 +382,8 @@ def main() -> int:
                               "phases": fp32["phases"]}
     if c3 is not None:
         line["precision_c3"] = c3
+    if c4 is not None:
+        line["batch_c4"] = c4
     if c5 is not None:
         line["dipole_c5"] = c5
     print(json.dumps(line), file=json_out, flush=True)

@@ -34,6 +34,11 @@ thread_local std::string g_create_error;
 
 constexpr double kBoltzmannIU = 0.0861733326;  // meV/K, helpers/consts.h:35
 constexpr size_t kMaxTableLds = 48 * 1024;     // bytes of LDS for the tensor table
+#ifndef JH_UNIFORM_TENSOR_FP64
+#define JH_UNIFORM_TENSOR_FP64 0
+#endif
+// wave-uniform tensors of the FP32 form: rounded to FP32 (0) or exact (1)
+constexpr bool kUniformTensorFP64 = JH_UNIFORM_TENSOR_FP64 != 0;
 
 // Static partition of [0, n) over host threads (set-up work only).
 template <class F>
@@ -511,7 +516,10 @@ int64_t build_uniform(const XHost &x, int64_t n, int tbits, bool diag,
       seq_ids.push_back(ids);
       for (uint32_t k = 0; k < len; ++k) {
         const double *T = &x.tab[(size_t)ids[k] * 9];
-        for (int q = 0; q < vpt; ++q) seqtab.push_back((double)(float)(diag ? T[4 * q] : T[q]));
+        for (int q = 0; q < vpt; ++q) {
+          const double v = diag ? T[4 * q] : T[q];
+          seqtab.push_back(kUniformTensorFP64 ? v : (double)(float)v);
+        }
       }
       byhash[h].push_back(found);
     }
@@ -768,8 +776,8 @@ void prepare_classes(jh_ctx *c) {
 
 void sync_f32(jh_ctx *c) {
   if (c->prec != PR_FP32) return;
-  c->s32.alloc((size_t)(4 * c->n));
-  c->sp32.alloc((size_t)(4 * c->n));
+  c->s32.alloc((size_t)(kG32 * c->n));
+  c->sp32.alloc((size_t)(kG32 * c->n));
   HIPCHK(launch_to_f32(c->s.p, c->s32.p, c->n, c->stream));
 }
 
@@ -1331,8 +1339,8 @@ int jh_time_phase(jh_ctx *c, int phase, int reps, double *ms) {
     save.alloc((size_t)(3 * c->n));
     HIPCHK(hipMemcpyAsync(save.p, c->s.p, sizeof(double) * 3 * c->n, hipMemcpyDeviceToDevice, c->stream));
     if (c->prec == PR_FP32) {
-      save32.alloc((size_t)(4 * c->n));
-      HIPCHK(hipMemcpyAsync(save32.p, c->s32.p, sizeof(float) * 4 * c->n, hipMemcpyDeviceToDevice, c->stream));
+      save32.alloc((size_t)(kG32 * c->n));
+      HIPCHK(hipMemcpyAsync(save32.p, c->s32.p, sizeof(float) * kG32 * c->n, hipMemcpyDeviceToDevice, c->stream));
     }
     KParams p = make_params(c);
     p.step = (uint64_t)c->iteration;
@@ -1355,7 +1363,7 @@ int jh_time_phase(jh_ctx *c, int phase, int reps, double *ms) {
     *ms = (double)f / reps;
     HIPCHK(hipMemcpyAsync(c->s.p, save.p, sizeof(double) * 3 * c->n, hipMemcpyDeviceToDevice, c->stream));
     if (c->prec == PR_FP32)
-      HIPCHK(hipMemcpyAsync(c->s32.p, save32.p, sizeof(float) * 4 * c->n, hipMemcpyDeviceToDevice, c->stream));
+      HIPCHK(hipMemcpyAsync(c->s32.p, save32.p, sizeof(float) * kG32 * c->n, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
   });
 }
@@ -1369,12 +1377,12 @@ int jh_phase_bytes(jh_ctx *c, int phase, double *fmt_bytes, double *csr_bytes) {
     double mat = 0.0, gather = 0.0;
     if (xv_pattern(c->xv)) {
       mat = n * 4.0 + (c->pat_lds ? 0.0 : (double)c->pat_words * 4.0);
-      gather = c->prec == PR_FP64 ? 3.0 * n * 8 : 4.0 * n * 4;
+      gather = c->prec == PR_FP64 ? 3.0 * n * 8 : kG32 * n * 4;
     } else if (c->xv != XV_NONE) {
       mat = (double)c->nent * 4.0 + (double)(c->slice_off.n) * 8.0;
       if (c->xv == XV_BDIAG || c->xv == XV_BFULL) mat += (double)c->nent * vpb * sv;
       // every neighbour vector read once (ideal reuse of the gathers)
-      gather = c->prec == PR_FP64 ? 3.0 * n * 8 : 4.0 * n * 4;
+      gather = c->prec == PR_FP64 ? 3.0 * n * 8 : kG32 * n * 4;
     }
     // per-spin state traffic: the FP64 vector the field is evaluated at (its
     // gathers are counted above: in FP64 they are the same array, so only
@@ -1382,7 +1390,7 @@ int jh_phase_bytes(jh_ctx *c, int phase, double *fmt_bytes, double *csr_bytes) {
     double state = c->prec == PR_FP32 || c->xv == XV_NONE ? 3 * n * 8 : 0.0;
     if (phase == 0) state += 3 * n * 8 /*s* w*/ + 3 * n * 8 /*ds w*/;
     else state += 3 * n * 8 /*s_old r*/ + 3 * n * 8 /*ds r*/ + 3 * n * 8 /*s w*/;
-    if (c->prec == PR_FP32) state += 4 * n * 4;  // FP32 gather copy written
+    if (c->prec == PR_FP32) state += kG32 * n * 4;  // FP32 gather copy written
     if (c->nclass > 1) state += 2 * n;
     if (c->noise_mode == NM_INJECTED) state += 3 * n * 8;
     if (c->noise_mode == NM_PHILOX) state += 3 * n * 8;  // noise written, then read back

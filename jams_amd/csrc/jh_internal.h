@@ -38,6 +38,10 @@ constexpr int kBlock = 256;          // threads per workgroup of the helper kern
 constexpr int kMaxBlock = 512;       // largest step-kernel workgroup (autotuned)
 constexpr int kSlice = 64;           // spins per ELL slice (one wavefront)
 constexpr uint32_t kPad = 0xFFFFFFFFu;
+#ifndef JH_G32
+#define JH_G32 3
+#endif
+constexpr int kG32 = JH_G32;         // floats per spin in the FP32 gather copy
 
 enum TermKind { TK_EXCHANGE = 0, TK_ZEEMAN = 1, TK_UNIAXIAL = 2, TK_DIPOLE = 3 };
 enum NoiseMode { NM_NONE = 0, NM_PHILOX = 1, NM_INJECTED = 2 };
@@ -101,7 +105,7 @@ struct KParams {
   double *sp;          // predictor spins
   double *ds;          // ds/dt accumulator
   double *snew;        // phase 1 output (== s)
-  const float *g32;    // FP32 gather source, (x, y, z, 0) per spin (s32 / sp32)
+  const float *g32;    // FP32 gather source, kG32 floats per spin (s32 / sp32)
   float *w32;          // FP32 copy written by the phase (sp32 / s32)
   const uint16_t *cls; // per-spin class or null
   const double *ctab;  // class table

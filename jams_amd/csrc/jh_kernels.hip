@@ -26,6 +26,26 @@ namespace jh {
 
 constexpr int kGatherUnroll = JH_GATHER_UNROLL;
 
+// FP32 gather copy of the spins: kG32 floats per spin, (x, y, z[, 0])
+__device__ __forceinline__ void load_g32(const float *__restrict__ src, uint32_t j, double &x,
+                                         double &y, double &z) {
+  if constexpr (kG32 == 4) {
+    const float4 v = reinterpret_cast<const float4 *>(src)[j];
+    x = v.x; y = v.y; z = v.z;
+  } else {
+    const float *q = src + (size_t)3 * j;
+    x = q[0]; y = q[1]; z = q[2];
+  }
+}
+__device__ __forceinline__ void store_g32(float *dst, int64_t i, double x, double y, double z) {
+  if constexpr (kG32 == 4) {
+    reinterpret_cast<float4 *>(dst)[i] = make_float4((float)x, (float)y, (float)z, 0.f);
+  } else {
+    float *q = dst + 3 * i;
+    q[0] = (float)x; q[1] = (float)y; q[2] = (float)z;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // exchange field of spin i, FP64, reference order
 template <int XVk>
@@ -85,8 +105,8 @@ __device__ __forceinline__ void xfield32(const XView &x, const float *__restrict
   for (int64_t e = b0 + lane; e < b1; e += kSlice) {
     const uint32_t w = __builtin_nontemporal_load(x.ent + e);
     const uint32_t j = w & x.colmask;
-    const float4 v4 = reinterpret_cast<const float4 *>(src)[j];
-    const double sx = v4.x, sy = v4.y, sz = v4.z;
+    double sx, sy, sz;
+    load_g32(src, j, sx, sy, sz);
     if constexpr (XVk == XV_TDIAG) {
       const double *T = tab + 3 * (w >> x.colbits);
       a0 = __builtin_fma(sx, T[0], a0);
@@ -159,8 +179,8 @@ __device__ __forceinline__ void pfield32(const XView &x, const float *__restrict
     const uint32_t w = pat[o + k];
     const int64_t j = i + (int64_t)(((int32_t)w) >> x.tbits);
     const uint32_t t = w & tmask;
-    const float4 v4 = reinterpret_cast<const float4 *>(src)[j];
-    const double sx = v4.x, sy = v4.y, sz = v4.z;
+    double sx, sy, sz;
+    load_g32(src, (uint32_t)j, sx, sy, sz);
     if constexpr (xv_diag(XVk)) {
       const double *T = tab + 3 * t;
       a0 = __builtin_fma(sx, T[0], a0);
@@ -196,14 +216,13 @@ __device__ __forceinline__ void ufield32(const XView &x, const float *__restrict
   const uint32_t o = x.pofs[i];
   const int tb = x.tbits;
   const uint32_t ib = (uint32_t)i;
-  const float4 *__restrict__ s4 = reinterpret_cast<const float4 *>(src);
   double a0 = 0.0, a1 = 0.0, a2 = 0.0;
 #pragma unroll kGatherUnroll
   for (int k = 0; k < len; ++k) {
     const uint32_t w = pat[o + 1 + k];
     const uint32_t j = ib + (uint32_t)(((int32_t)w) >> tb);
-    const float4 v4 = s4[j];
-    const double sx = v4.x, sy = v4.y, sz = v4.z;
+    double sx, sy, sz;
+    load_g32(src, j, sx, sy, sz);
     if constexpr (xv_diag(XVk)) {
       const double *Tk = T + 3 * k;
       a0 = __builtin_fma(sx, Tk[0], a0);
@@ -395,9 +414,7 @@ __global__ void __launch_bounds__(kMaxBlock, JH_WAVES_PER_EU) k_heun(const KPara
     double v0 = s0 + p.dt * r0, v1 = s1 + p.dt * r1, v2 = s2 + p.dt * r2;
     unit_vector(v0, v1, v2);
     p.sp[i] = v0; p.sp[n + i] = v1; p.sp[2 * n + i] = v2;
-    if constexpr (PREC == PR_FP32) {
-      reinterpret_cast<float4 *>(p.w32)[i] = make_float4((float)v0, (float)v1, (float)v2, 0.f);
-    }
+    if constexpr (PREC == PR_FP32) store_g32(p.w32, i, v0, v1, v2);
   } else {
     const double d0 = p.ds[i] + 0.5 * r0;
     const double d1 = p.ds[n + i] + 0.5 * r1;
@@ -405,9 +422,7 @@ __global__ void __launch_bounds__(kMaxBlock, JH_WAVES_PER_EU) k_heun(const KPara
     double v0 = p.s[i] + p.dt * d0, v1 = p.s[n + i] + p.dt * d1, v2 = p.s[2 * n + i] + p.dt * d2;
     unit_vector(v0, v1, v2);
     p.snew[i] = v0; p.snew[n + i] = v1; p.snew[2 * n + i] = v2;
-    if constexpr (PREC == PR_FP32) {
-      reinterpret_cast<float4 *>(p.w32)[i] = make_float4((float)v0, (float)v1, (float)v2, 0.f);
-    }
+    if constexpr (PREC == PR_FP32) store_g32(p.w32, i, v0, v1, v2);
   }
 }
 
@@ -447,11 +462,10 @@ __global__ void __launch_bounds__(kBlock) k_noise(const KParams p, double *out) 
   out[2 * p.n + i] = z2 * sg * sT;
 }
 
-// FP32 gather copy of SoA FP64 spins: (x, y, z, 0) per spin, 16 bytes
+// FP32 gather copy of SoA FP64 spins (kG32 floats per spin)
 __global__ void __launch_bounds__(kBlock) k_to_f32(const double *a, float *b, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (i < n)
-    reinterpret_cast<float4 *>(b)[i] = make_float4((float)a[i], (float)a[n + i], (float)a[2 * n + i], 0.f);
+  if (i < n) store_g32(b, i, a[i], a[n + i], a[2 * n + i]);
 }
 
 // Deterministic pairwise block reduction of 4 FP64 quantities per spin:
