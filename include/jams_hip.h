@@ -43,9 +43,10 @@ extern "C" {
 
 /* precision forms */
 #define JH_PREC_FP64 0        /* FP64 everywhere; exchange sums in CSR order   */
-#define JH_PREC_FP32_FIELD 1  /* FP32 spins + tensors in the exchange matvec,
-                                 their exact products accumulated in FP64 with
-                                 one rounding each; FP64 Heun state,
+#define JH_PREC_FP32_FIELD 1  /* FP32 spins in the exchange matvec (and FP32
+                                 per-entry values in the BLOCK format), tensor
+                                 tables exact, products accumulated in FP64;
+                                 FP32 dipole transforms; FP64 Heun state,
                                  renormalisation, noise and reductions         */
 
 /* exchange storage formats on the device */

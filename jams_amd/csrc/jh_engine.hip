@@ -34,11 +34,15 @@ thread_local std::string g_create_error;
 
 constexpr double kBoltzmannIU = 0.0861733326;  // meV/K, helpers/consts.h:35
 constexpr size_t kMaxTableLds = 48 * 1024;     // bytes of LDS for the tensor table
-#ifndef JH_UNIFORM_TENSOR_FP64
-#define JH_UNIFORM_TENSOR_FP64 0
+// Tensor tables of the FP32 form (LDS table, wave-uniform sequences) keep
+// the exact FP64 values: they are per tensor, not per spin, so FP32 would
+// save no bandwidth, and rounding them perturbs the Hamiltonian itself (a
+// systematic error that grows along a trajectory).  Per-entry values of the
+// BLOCK format are FP32.
+#ifndef JH_FP32_TENSOR_TABLES
+#define JH_FP32_TENSOR_TABLES 0
 #endif
-// wave-uniform tensors of the FP32 form: rounded to FP32 (0) or exact (1)
-constexpr bool kUniformTensorFP64 = JH_UNIFORM_TENSOR_FP64 != 0;
+constexpr bool kRoundTables = JH_FP32_TENSOR_TABLES != 0;
 
 // Static partition of [0, n) over host threads (set-up work only).
 template <class F>
@@ -391,14 +395,13 @@ XHost xhost_from_list(int64_t n, int64_t np, const int32_t *pi, const int32_t *p
 
 // ---- prepare: device formats and site classes ----------------------------
 
-// Distinct tensors for the LDS table (diagonal or full rows); the FP32 form
-// stores them rounded to FP32, held as doubles so products stay exact.
+// Distinct tensors for the LDS table (diagonal or full rows), FP64.
 void upload_table(jh_ctx *c, const XHost &x, int ntypes, int ntab, int vpt, bool diag) {
   std::vector<double> t64((size_t)ntab * vpt, 0.0);
   for (int t = 0; t < ntypes; ++t)
     for (int q = 0; q < vpt; ++q) {
       const double v = diag ? x.tab[9 * t + 4 * q] : x.tab[9 * t + q];
-      t64[(size_t)t * vpt + q] = c->prec == PR_FP64 ? v : (double)(float)v;
+      t64[(size_t)t * vpt + q] = (c->prec == PR_FP32 && kRoundTables) ? (double)(float)v : v;
     }
   c->tab64.alloc(t64.size());
   HIPCHK(hipMemcpy(c->tab64.p, t64.data(), sizeof(double) * t64.size(), hipMemcpyHostToDevice));
@@ -481,7 +484,7 @@ bool build_patterns(const jh_ctx *c, const XHost &x, int tbits, bool by_tensor,
 
 // Slices (64 device rows) whose rows all carry the same tensor-id sequence
 // (rows ordered by tensor id): for each, the (offset, length) of that
-// sequence's tensors, rounded to FP32, in seqtab; (-1, 0) for the others.
+// sequence's tensors in seqtab; (-1, 0) for the others.
 // Returns the number of uniform slices, or -1 when the sequences would not
 // fit the table.
 int64_t build_uniform(const XHost &x, int64_t n, int tbits, bool diag,
@@ -518,7 +521,7 @@ int64_t build_uniform(const XHost &x, int64_t n, int tbits, bool diag,
         const double *T = &x.tab[(size_t)ids[k] * 9];
         for (int q = 0; q < vpt; ++q) {
           const double v = diag ? T[4 * q] : T[q];
-          seqtab.push_back(kUniformTensorFP64 ? v : (double)(float)v);
+          seqtab.push_back(kRoundTables ? (double)(float)v : v);
         }
       }
       byhash[h].push_back(found);

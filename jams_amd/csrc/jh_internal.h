@@ -82,7 +82,7 @@ struct XView {
   uint32_t colmask;
   int colbits;
   int ntab;                   // table rows incl. the trailing zero tensor
-  const double *tab64;        // TYPED: ntab x (3|9); FP32 form: values rounded to FP32
+  const double *tab64;        // TYPED: ntab x (3|9) tensors (FP64 in both forms)
   const double *val64;        // BLOCK: per entry (3|9) values, slice-major
   const float *val32;
   // PATTERN: pofs[i] = word offset of spin i's pattern in pat; pat[o] = length,
@@ -96,7 +96,7 @@ struct XView {
   // for every lane.  sseq[slice] = (offset of that sequence's tensors in
   // seqtab, length), or (-1, 0) when the slice's rows differ.
   const int2 *sseq;
-  const double *seqtab;       // per sequence: len x (3|9) values rounded to FP32
+  const double *seqtab;       // per sequence: len x (3|9) tensor values (FP64)
 };
 
 struct KParams {

@@ -8,8 +8,8 @@
 // Each thread owns one spin.  The exchange field is a row-sequential sum in
 // the reference CSR order (neighbour column ascending, tensor column
 // ascending), which is what makes the FP64 path bit-identical to the CPU
-// solver.  The FP32 form gathers FP32 spins, multiplies them by tensors
-// rounded to FP32 (each product exact in FP64) and accumulates in FP64; it
+// solver.  The FP32 form gathers FP32 spins, multiplies them by the FP64
+// tensors and accumulates in FP64 (one rounding per product and per sum); it
 // may order a row by tensor id so that tensors become wave-uniform.
 #include "jh_device.h"
 #include "jh_internal.h"
@@ -89,10 +89,10 @@ __device__ __forceinline__ void xfield64(const XView &x, const double *__restric
   h0 = a0; h1 = a1; h2 = a2;
 }
 
-// exchange field of spin i from FP32 storage: spins gathered as FP32 and
-// tensors rounded to FP32 (held as doubles), so every product is exact in
-// FP64 and one fused multiply-add per product accumulates it with a single
-// rounding; the row order is the reference's.
+// exchange field of spin i from FP32 storage: spins gathered as FP32 (the
+// per-spin bytes), tensors from the FP64 table, one fused multiply-add per
+// product into an FP64 sum; the row order is the reference's.  BLOCK values
+// are stored per entry and therefore in FP32.
 template <int XVk>
 __device__ __forceinline__ void xfield32(const XView &x, const float *__restrict__ src, int64_t n,
                                          int64_t i, const double *tab, double &h0, double &h1,
@@ -360,8 +360,17 @@ __device__ __forceinline__ void total_field(const KParams &p, const double *src6
   if (p.nterms == 0) { h0 = h1 = h2 = 0.0; }
 }
 
+// Minimum waves per SIMD the compiler keeps room for (VGPR budget): the
+// wave-uniform FP32 variants are latency-bound and gain from occupancy.
+#ifndef JH_UNIFORM_WAVES
+#define JH_UNIFORM_WAVES 1
+#endif
+template <int PREC, int XVk>
+constexpr int kMinWaves = (PREC == PR_FP32 && xv_uniform(XVk)) ? JH_UNIFORM_WAVES : JH_WAVES_PER_EU;
+
 template <int PREC, int XVk, int PHASE, bool GEN>
-__global__ void __launch_bounds__(kMaxBlock, JH_WAVES_PER_EU) k_heun(const KParams p) {
+__global__ void __launch_bounds__(kMaxBlock)
+__attribute__((amdgpu_waves_per_eu(kMinWaves<PREC, XVk>))) k_heun(const KParams p) {
   extern __shared__ __align__(16) unsigned char lds[];
   load_table<PREC, XVk>(p.x, lds);
   if constexpr (xv_typed(XVk)) __syncthreads();
