@@ -201,6 +201,7 @@ struct jh_ctx {
   int pat_words = 0, pat_lds = 0, tbits = 0, npatterns = 0;
   DevBuf<int32_t> sseq;          // UNIFORM: (offset, length) per slice
   DevBuf<double> seqtab;
+  DevBuf<int32_t> upat;          // UNIFORM: byte-offset pattern words
   int64_t nuniform = 0, nslices = 0;
   DevBuf<uint16_t> cls;
   DevBuf<double> ctab;
@@ -218,7 +219,7 @@ struct jh_ctx {
   DevBuf<double> rep_out;
 
   // launch shape and graph replay
-  int block = 256;                  // threads per workgroup of the step kernels
+  int block = 512;                  // threads per workgroup of the step kernels
   int graph_steps = 0;              // steps per captured graph (0: plain launches)
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -596,7 +597,23 @@ void prepare_exchange(jh_ctx *c) {
       std::vector<double> seqtab;
       const int64_t nu = build_uniform(x, n, tbits, diag, pofs, pat, sseq, seqtab);
       // worth its own variant when at least half of the slices are uniform
-      if (nu >= 0 && nu * 2 >= c->nslices) {
+      // byte offsets of the gather copy must fit 31 bits
+      const bool fits = (double)n * kG32 * sizeof(float) < 2147483648.0;
+      if (nu >= 0 && nu * 2 >= c->nslices && fits) {
+        // each pattern's words as byte offsets (its length word stays)
+        std::vector<int32_t> up(pat.size());
+        std::vector<char> done(pat.size(), 0);
+        for (int64_t d = 0; d < n; ++d) {
+          const uint32_t o = pofs[d];
+          if (done[o]) continue;
+          done[o] = 1;
+          const uint32_t len = pat[o];
+          up[o] = (int32_t)len;
+          for (uint32_t k = 1; k <= len; ++k)
+            up[o + k] = (((int32_t)pat[o + k]) >> tbits) * (int32_t)(kG32 * sizeof(float));
+        }
+        c->upat.alloc(up.size());
+        HIPCHK(hipMemcpy(c->upat.p, up.data(), sizeof(int32_t) * up.size(), hipMemcpyHostToDevice));
         c->nuniform = nu;
         c->sseq.alloc(sseq.size());
         HIPCHK(hipMemcpy(c->sseq.p, sseq.data(), sizeof(int32_t) * sseq.size(), hipMemcpyHostToDevice));
@@ -609,6 +626,7 @@ void prepare_exchange(jh_ctx *c) {
     if (!xv_uniform(c->xv)) {
       c->sseq.release();
       c->seqtab.release();
+      c->upat.release();
     }
     c->slice_off.release();
     c->ent.release();
@@ -869,6 +887,7 @@ KParams make_params(jh_ctx *c) {
   p.x.tbits = c->tbits;
   p.x.sseq = reinterpret_cast<const int2 *>(c->sseq.p);
   p.x.seqtab = c->seqtab.p;
+  p.x.upat = c->upat.p;
   p.nterms = (int)c->terms.size();
   for (int q = 0; q < p.nterms; ++q) p.kind[q] = c->terms[q].kind;
   p.hext = c->hext.p;

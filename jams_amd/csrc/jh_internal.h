@@ -97,6 +97,8 @@ struct XView {
   // seqtab, length), or (-1, 0) when the slice's rows differ.
   const int2 *sseq;
   const double *seqtab;       // per sequence: len x (3|9) tensor values (FP64)
+  const int32_t *upat;        // pat with each word replaced by its byte offset
+                              // (column - row) * bytes per spin of the FP32 copy
 };
 
 struct KParams {

@@ -25,6 +25,9 @@
 namespace jh {
 
 constexpr int kGatherUnroll = JH_GATHER_UNROLL;
+// full-tensor rows (9 products per neighbour) gain from more loads in flight
+template <int XVk>
+constexpr int kUnrollFor = xv_diag(XVk) ? kGatherUnroll : 2 * kGatherUnroll;
 
 // FP32 gather copy of the spins: kG32 floats per spin, (x, y, z[, 0])
 __device__ __forceinline__ void load_g32(const float *__restrict__ src, uint32_t j, double &x,
@@ -144,7 +147,7 @@ __device__ __forceinline__ void pfield64(const XView &x, const double *__restric
   const int len = (int)pat[o];
   const uint32_t tmask = (1u << x.tbits) - 1u;
   double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-#pragma unroll kGatherUnroll
+#pragma unroll kUnrollFor<XVk>
   for (int k = 1; k <= len; ++k) {
     const uint32_t w = pat[o + k];
     const int64_t j = i + (int64_t)(((int32_t)w) >> x.tbits);
@@ -174,7 +177,7 @@ __device__ __forceinline__ void pfield32(const XView &x, const float *__restrict
   const int len = (int)pat[o];
   const uint32_t tmask = (1u << x.tbits) - 1u;
   double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-#pragma unroll kGatherUnroll
+#pragma unroll kUnrollFor<XVk>
   for (int k = 1; k <= len; ++k) {
     const uint32_t w = pat[o + k];
     const int64_t j = i + (int64_t)(((int32_t)w) >> x.tbits);
@@ -200,29 +203,32 @@ __device__ __forceinline__ void pfield32(const XView &x, const float *__restrict
 // tensor sequence the tensor of slot k is wave-uniform and is read through
 // scalar loads into SGPRs (no LDS traffic for it); only the neighbour
 // offsets come from each lane's pattern.  Other slices take pfield32.
+// upat: the pattern words as byte offsets into the gather copy (LDS);
+// gpat: the (offset, tensor id) words for the non-uniform slices (global).
 template <int XVk>
 __device__ __forceinline__ void ufield32(const XView &x, const float *__restrict__ src, int64_t n,
-                                         int64_t i, const double *tab, const uint32_t *pat,
-                                         double &h0, double &h1, double &h2) {
+                                         int64_t i, const double *tab, const int32_t *upat,
+                                         const uint32_t *gpat, double &h0, double &h1,
+                                         double &h2) {
   const int sl = __builtin_amdgcn_readfirstlane((int)(i >> 6));
   const int2 info = x.sseq[sl];
   const int soff = __builtin_amdgcn_readfirstlane(info.x);
   if (soff < 0) {
-    pfield32<XVk>(x, src, n, i, tab, pat, h0, h1, h2);
+    pfield32<XVk>(x, src, n, i, tab, gpat, h0, h1, h2);
     return;
   }
   const int len = __builtin_amdgcn_readfirstlane(info.y);
   const double *__restrict__ T = x.seqtab + soff;
   const uint32_t o = x.pofs[i];
-  const int tb = x.tbits;
-  const uint32_t ib = (uint32_t)i;
+  // byte offset of this spin in the gather copy (the copy is < 2 GiB)
+  const uint32_t ib = (uint32_t)i * (uint32_t)(kG32 * sizeof(float));
+  const char *__restrict__ base = reinterpret_cast<const char *>(src);
   double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-#pragma unroll kGatherUnroll
+#pragma unroll kUnrollFor<XVk>
   for (int k = 0; k < len; ++k) {
-    const uint32_t w = pat[o + 1 + k];
-    const uint32_t j = ib + (uint32_t)(((int32_t)w) >> tb);
-    double sx, sy, sz;
-    load_g32(src, j, sx, sy, sz);
+    const uint32_t jb = ib + (uint32_t)upat[o + 1 + k];
+    const float *q = reinterpret_cast<const float *>(base + jb);
+    const double sx = q[0], sy = q[1], sz = q[2];
     if constexpr (xv_diag(XVk)) {
       const double *Tk = T + 3 * k;
       a0 = __builtin_fma(sx, Tk[0], a0);
@@ -254,7 +260,8 @@ __device__ __forceinline__ void load_table(const XView &x, unsigned char *lds) {
   }
   if constexpr (xv_pattern_lds(XVk)) {
     uint32_t *p = reinterpret_cast<uint32_t *>(lds + table_bytes(PREC, XVk, x.ntab));
-    for (int q = threadIdx.x; q < x.pat_words; q += blockDim.x) p[q] = x.pat[q];
+    const uint32_t *from = xv_uniform(XVk) ? reinterpret_cast<const uint32_t *>(x.upat) : x.pat;
+    for (int q = threadIdx.x; q < x.pat_words; q += blockDim.x) p[q] = from[q];
   }
 }
 
@@ -274,7 +281,8 @@ __device__ __forceinline__ void exchange_at(const KParams &p, const double *src6
     if constexpr (PREC == PR_FP64)
       pfield64<XVk>(p.x, src64, p.n, i, reinterpret_cast<const double *>(lds), pat, f0, f1, f2);
     else if constexpr (xv_uniform(XVk))
-      ufield32<XVk>(p.x, src32, p.n, i, reinterpret_cast<const double *>(lds), pat, f0, f1, f2);
+      ufield32<XVk>(p.x, src32, p.n, i, reinterpret_cast<const double *>(lds),
+                    reinterpret_cast<const int32_t *>(pat), p.x.pat, f0, f1, f2);
     else
       pfield32<XVk>(p.x, src32, p.n, i, reinterpret_cast<const double *>(lds), pat, f0, f1, f2);
   } else if constexpr (PREC == PR_FP64) {
