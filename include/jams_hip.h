@@ -185,7 +185,8 @@ int jh_reduce_magnetisation_replicas(jh_ctx *ctx, double *out);
 /* Run jh_step as replays of one captured hipGraph of `steps` Heun steps
  * (whatever is left over runs as plain launches); 0 disables graphs. */
 int jh_set_graph_steps(jh_ctx *ctx, int steps);
-/* Threads per workgroup of the step kernels: 64 .. 512, a multiple of 64. */
+/* Threads per workgroup of the step kernels: 64 .. 512, a multiple of 64;
+ * 0 (the default) picks 256 or 512 by exchange format. */
 int jh_set_block_size(jh_ctx *ctx, int threads);
 /* Time `steps` steps of every exchange format that fits (PATTERN, TYPED,
  * BLOCK) at 128, 256 and 512 threads per workgroup from the current state,

@@ -157,22 +157,36 @@ __global__ void __launch_bounds__(256) k_dip_extract(const double2 *F, int64_t P
 }
 
 template <class T>
-size_t pass_lds(int n, int B) {
-  return 2 * (size_t)n * B * sizeof(typename fft::C2<T>::type);
+size_t pass_lds(int n, int B, bool pingpong) {
+  return (pingpong ? 2 : 1) * (size_t)n * B * sizeof(typename fft::C2<T>::type);
 }
 
+// Lines per workgroup: as many as LDS (64 KB) and, for in-place plans, the
+// per-thread butterfly budget of every stage allow (at most 64).
 template <class T>
-int pick_lines(int n, int64_t nlines) {
-  int B = (int)std::min<size_t>(64, (64 * 1024) / pass_lds<T>(n, 1));
+int pick_lines(const Pass<T> &p) {
+  const int n = p.n;
+  int B = (int)std::min<size_t>(64, (64 * 1024) / pass_lds<T>(n, 1, p.pingpong));
+  if (!p.pingpong)
+    for (int st = 0; st < p.nst; ++st) {
+      const int R = p.radix[st];
+      const int nbf = R == 2 ? fft::kBF<2> : R == 3 ? fft::kBF<3> : R == 4 ? fft::kBF<4>
+                    : R == 5 ? fft::kBF<5> : fft::kBF<8>;
+      B = std::min<int>(B, (int)((int64_t)nbf * fft::kThreads / (n / R)));
+    }
   B = std::max(B, 1);
-  return (int)std::min<int64_t>(B, std::max<int64_t>(nlines, 1));
+  return (int)std::min<int64_t>(B, std::max<int64_t>(p.nlines, 1));
 }
 
 template <class T, int LD, int ST>
 hipError_t run_pass(Pass<T> p, hipStream_t st) {
   if (p.nlines <= 0) return hipSuccess;
-  p.B = pick_lines<T>(p.n, p.nlines);
-  const size_t lds = pass_lds<T>(p.n, p.B);
+  p.pingpong = 0;
+  for (int s = 0; s < p.nst; ++s)
+    if (p.radix[s] != 2 && p.radix[s] != 3 && p.radix[s] != 4 && p.radix[s] != 5 && p.radix[s] != 8)
+      p.pingpong = 1;
+  p.B = pick_lines<T>(p);
+  const size_t lds = pass_lds<T>(p.n, p.B, p.pingpong);
   const int64_t blocks = (p.nlines + p.B - 1) / p.B;
   hipLaunchKernelGGL((fft::k_fft_pass<T, LD, ST>), dim3((unsigned)blocks), dim3(fft::kThreads),
                      lds, st, p);
@@ -413,7 +427,7 @@ DipoleFFT *DipoleFFT::create(const int32_t size[3], const int32_t periodic[3], i
     d->L[a] = size[a];
     d->per[a] = periodic[a] ? 1 : 0;
     d->P[a] = periodic[a] ? size[a] : 2 * size[a];
-    if (pass_lds<double>(d->P[a], 1) > kMaxPassLds)
+    if (pass_lds<double>(d->P[a], 1, true) > kMaxPassLds)
       return fail("padded dimension too long for one workgroup's LDS (at most 2048)");
     d->rad[a] = factorise(d->P[a]);
     if ((int)d->rad[a].size() > fft::kMaxStages) return fail("too many FFT stages");

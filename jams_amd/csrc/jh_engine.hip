@@ -219,7 +219,15 @@ struct jh_ctx {
   DevBuf<double> rep_out;
 
   // launch shape and graph replay
-  int block = 512;                  // threads per workgroup of the step kernels
+  int block = 0;                    // threads per workgroup of the step kernels
+                                    // (0: chosen per exchange variant)
+  // measured defaults: diagonal tensors (few neighbours) run best at 256
+  // threads, full tensors and the wave-uniform FP32 variants at 512 (their
+  // occupancy is bounded by LDS per workgroup, not per thread)
+  int launch_block() const {
+    if (block) return block;
+    return (xv_diag(xv) && !xv_uniform(xv)) ? 256 : 512;
+  }
   int graph_steps = 0;              // steps per captured graph (0: plain launches)
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -911,12 +919,12 @@ void enqueue_step(jh_ctx *c, KParams p) {
   if (dip >= 0) HIPCHK(c->dipoles[dip]->field(c->s.p, c->hext.p, c->prec == PR_FP32, c->stream));
   p.g32 = c->s32.p;
   p.w32 = c->sp32.p;
-  HIPCHK(launch_heun(p, c->prec, c->xv, 0, c->block, c->stream));
+  HIPCHK(launch_heun(p, c->prec, c->xv, 0, c->launch_block(), c->stream));
   // corrector at t + dt
   if (dip >= 0) HIPCHK(c->dipoles[dip]->field(c->sp.p, c->hext.p, c->prec == PR_FP32, c->stream));
   p.g32 = c->sp32.p;
   p.w32 = c->s32.p;
-  HIPCHK(launch_heun(p, c->prec, c->xv, 1, c->block, c->stream));
+  HIPCHK(launch_heun(p, c->prec, c->xv, 1, c->launch_block(), c->stream));
 }
 
 // Capture graph_steps steps whose kernels read the step counter from
@@ -1374,10 +1382,10 @@ int jh_time_phase(jh_ctx *c, int phase, int reps, double *ms) {
       KParams q = p;
       q.g32 = c->s32.p;
       q.w32 = c->sp32.p;
-      HIPCHK(launch_heun(q, c->prec, c->xv, 0, c->block, c->stream));
+      HIPCHK(launch_heun(q, c->prec, c->xv, 0, c->launch_block(), c->stream));
     }
     HIPCHK(hipEventRecord(c->ev0, c->stream));
-    for (int r = 0; r < reps; ++r) HIPCHK(launch_heun(p, c->prec, c->xv, phase, c->block, c->stream));
+    for (int r = 0; r < reps; ++r) HIPCHK(launch_heun(p, c->prec, c->xv, phase, c->launch_block(), c->stream));
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     HIPCHK(hipEventSynchronize(c->ev1));
     float f = 0.f;
@@ -1519,8 +1527,8 @@ int jh_set_graph_steps(jh_ctx *c, int steps) {
 
 int jh_set_block_size(jh_ctx *c, int threads) {
   return guarded(c, [&] {
-    if (threads < 64 || threads > kMaxBlock || threads % 64 != 0)
-      JH_THROW(JH_ERR_INVALID, "workgroup size must be a multiple of 64 in [64, 512]");
+    if (threads != 0 && (threads < 64 || threads > kMaxBlock || threads % 64 != 0))
+      JH_THROW(JH_ERR_INVALID, "workgroup size must be 0 or a multiple of 64 in [64, 512]");
     c->block = threads;
     c->drop_graph();
   });
@@ -1611,7 +1619,7 @@ int jh_describe(jh_ctx *c, char *buf, int64_t buflen) {
                   (long long)c->nent, c->ntab > 0 ? c->ntab - 1 : 0, c->colbits,
                   c->npatterns, c->pat_words, c->pat_lds, (long long)c->nuniform,
                   (long long)c->nslices, c->order_nb, c->nclass,
-                  (int)c->terms.size(), c->noise_mode, c->block, c->nrep, c->graph_steps);
+                  (int)c->terms.size(), c->noise_mode, c->launch_block(), c->nrep, c->graph_steps);
     if (buf && buflen > 0) {
       std::strncpy(buf, tmp, (size_t)buflen - 1);
       buf[buflen - 1] = 0;

@@ -85,6 +85,7 @@ struct Pass {
   int n_in, n_out;
   int B;                 // lines per workgroup
   int inverse;
+  int pingpong;          // 1: two LDS buffers (plans with a direct-DFT factor)
   int64_t nlines;
   LineMap src, dst;
   const T2 *in;
@@ -193,6 +194,47 @@ __device__ __forceinline__ void stage_fixed(const Pass<T> &p, const typename C2<
   }
 }
 
+// Butterflies per thread of an in-place stage: R * kBF<R> <= 16 values in
+// registers.
+template <int R>
+constexpr int kBF = (16 / R) > 0 ? 16 / R : 1;
+
+// The same stage in place on one LDS buffer: every thread first reads and
+// transforms its butterflies (at most kBF<R>, which the host guarantees by
+// its choice of B), then all write.
+template <class T, int R>
+__device__ __forceinline__ void stage_inplace(const Pass<T> &p, typename C2<T>::type *A, int Ns) {
+  using T2 = typename C2<T>::type;
+  constexpr int NBF = kBF<R>;
+  const int n = p.n, B = p.B, m = n / R, total = m * B;
+  const int64_t stride_t = (int64_t)n / ((int64_t)Ns * R);
+  T2 v[NBF][R];
+#pragma unroll
+  for (int q = 0; q < NBF; ++q) {
+    const int t = threadIdx.x + q * blockDim.x;
+    if (t < total) {
+      const int b = t % B, j = t / B, jm = j % Ns;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[q][r] = A[(int64_t)(j + r * m) * B + b];
+#pragma unroll
+      for (int r = 1; r < R; ++r) v[q][r] = cmul(v[q][r], twiddle(p, (int64_t)jm * r * stride_t));
+      dft_fixed<T2, R>(v[q], p.inverse != 0);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NBF; ++q) {
+    const int t = threadIdx.x + q * blockDim.x;
+    if (t < total) {
+      const int b = t % B, j = t / B, jm = j % Ns;
+      const int d = (j / Ns) * Ns * R + jm;
+#pragma unroll
+      for (int r = 0; r < R; ++r) A[(int64_t)(d + r * Ns) * B + b] = v[q][r];
+    }
+  }
+  __syncthreads();
+}
+
 // Any other factor R: direct R-point DFT with roots from the n-table.
 template <class T>
 __device__ void stage_generic(const Pass<T> &p, const typename C2<T>::type *in,
@@ -264,9 +306,22 @@ __global__ void __launch_bounds__(kThreads) k_fft_pass(const Pass<T> p) {
   }
   __syncthreads();
 
-  // ---- stages (ping-pong between A and Bf) ----
+  // ---- stages: in place on A, or ping-pong between A and Bf ----
   T2 *in = A, *out = Bf;
   int Ns = 1;
+  if (!p.pingpong) {
+    for (int st = 0; st < p.nst; ++st) {
+      const int R = p.radix[st];
+      switch (R) {
+        case 2: stage_inplace<T, 2>(p, A, Ns); break;
+        case 3: stage_inplace<T, 3>(p, A, Ns); break;
+        case 4: stage_inplace<T, 4>(p, A, Ns); break;
+        case 5: stage_inplace<T, 5>(p, A, Ns); break;
+        default: stage_inplace<T, 8>(p, A, Ns); break;
+      }
+      Ns *= R;
+    }
+  } else
   for (int st = 0; st < p.nst; ++st) {
     const int R = p.radix[st];
     switch (R) {
