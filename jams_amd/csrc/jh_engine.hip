@@ -777,7 +777,7 @@ void prepare_classes(jh_ctx *c) {
     }
     row[CF_SQRTT] = std::sqrt(T);
     std::memcpy(&row[CF_SEED], &seed, sizeof seed);
-    row[CF_ROFF] = (double)roff;
+    std::memcpy(&row[CF_ROFF], &roff, sizeof roff);
     std::vector<uint64_t> key(CF_STRIDE);
     std::memcpy(key.data(), row.data(), sizeof(double) * CF_STRIDE);
     auto it = map.find(key);
@@ -927,8 +927,10 @@ void enqueue_step(jh_ctx *c, KParams p) {
   HIPCHK(launch_heun(p, c->prec, c->xv, 1, c->launch_block(), c->stream));
 }
 
-// Capture graph_steps steps whose kernels read the step counter from
-// c->dstep and advance it (a one-thread kernel; no memset nodes).
+// Capture graph_steps steps.  Their kernels read the step counter as
+// *c->dstep + k (k, the step's place in the graph, is baked into its
+// parameters), and the host sets *c->dstep once per replay with a one-thread
+// kernel outside the graph (no memset nodes).
 void build_graph(jh_ctx *c) {
   c->drop_graph();
   if (!c->dstep.p) c->dstep.alloc(1);
@@ -937,8 +939,8 @@ void build_graph(jh_ctx *c) {
   HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   try {
     for (int k = 0; k < c->graph_steps; ++k) {
+      p.step = (uint64_t)k;
       enqueue_step(c, p);
-      HIPCHK(launch_advance(c->dstep.p, c->stream));
     }
   } catch (...) {
     hipGraph_t g = nullptr;
@@ -954,9 +956,10 @@ void do_steps(jh_ctx *c, int64_t nsteps) {
   int64_t done = 0;
   if (c->graph_steps > 0 && nsteps >= c->graph_steps) {
     if (!c->gexec) build_graph(c);
-    HIPCHK(launch_set_step(c->dstep.p, (uint64_t)c->iteration, c->stream));
-    for (; done + c->graph_steps <= nsteps; done += c->graph_steps)
+    for (; done + c->graph_steps <= nsteps; done += c->graph_steps) {
+      HIPCHK(launch_set_step(c->dstep.p, (uint64_t)(c->iteration + done), c->stream));
       HIPCHK(hipGraphLaunch(c->gexec, c->stream));
+    }
     c->iteration += done;
     c->time = (double)c->iteration * c->dt;
   }

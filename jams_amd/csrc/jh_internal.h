@@ -72,7 +72,8 @@ enum ClassField {
   CF_AXIS = 12,  // 12..14
   CF_SQRTT = 15,  // sqrt(temperature) of the spin's system (replica)
   CF_SEED = 16,   // its noise seed (the 64 bits stored in the double)
-  CF_ROFF = 17,   // index of its first spin: the noise counter is index - ROFF
+  CF_ROFF = 17,   // index of its first spin (int64 bits): the noise counter
+                  // is index - ROFF
   CF_STRIDE = 18
 };
 
@@ -118,8 +119,8 @@ struct KParams {
   int has_ac;
   int ani_power;
   int noise;
-  uint64_t step;
-  const uint64_t *dstep;  // graph replay: the step counter lives on the device
+  uint64_t step;          // the step counter, or with dstep its offset
+  const uint64_t *dstep;  // graph replay: the replay's first step, on the device
   const double *winj;  // injected noise SoA
   double *wbuf;        // noise of the current step: written by the predictor,
                        // read by the corrector (SoA)

@@ -302,7 +302,7 @@ __device__ __forceinline__ uint64_t noise_seed(const KParams &p, uint32_t c) {
   return (uint64_t)__double_as_longlong(cparam(p, c, CF_SEED));
 }
 __device__ __forceinline__ uint64_t noise_index(const KParams &p, uint32_t c, int64_t i) {
-  return (uint64_t)(ref_index(p, i) - (int64_t)cparam(p, c, CF_ROFF));
+  return (uint64_t)ref_index(p, i) - (uint64_t)__double_as_longlong(cparam(p, c, CF_ROFF));
 }
 
 // Local (non-exchange) term q of spin i with spin vector (s0, s1, s2).
@@ -388,9 +388,10 @@ __attribute__((amdgpu_waves_per_eu(kMinWaves<PREC, XVk>))) k_heun(const KParams 
   const int64_t n = p.n;
   const uint32_t c = p.cls ? (uint32_t)p.cls[i] : 0u;
 
-  // step counter and time: from the host, or from the device in a graph
-  // replay (t = step dt, the host's own expression)
-  const uint64_t step = p.dstep ? *p.dstep : p.step;
+  // step counter and time: from the host, or in a graph replay the replay's
+  // first step (on the device) plus this step's place in the graph
+  // (t = step dt, the host's own expression)
+  const uint64_t step = p.dstep ? *p.dstep + p.step : p.step;
   const double t0 = p.dstep ? (double)step * p.dt : p.t;
 
   // spin that the field is evaluated at
@@ -562,10 +563,7 @@ __global__ void __launch_bounds__(kBlock) k_reduce_replicas(const KParams p, con
   if (threadIdx.x < 4) out[(int64_t)r * 4 + threadIdx.x] = red[threadIdx.x][0];
 }
 
-// One thread advances (or sets) the device step counter of graph replays.
-__global__ void k_advance(uint64_t *step) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) *step = *step + 1;
-}
+// One thread sets the device step counter of a graph replay.
 __global__ void k_set_step(uint64_t *step, uint64_t value) {
   if (blockIdx.x == 0 && threadIdx.x == 0) *step = value;
 }
@@ -626,11 +624,6 @@ hipError_t launch_heun(const KParams &p, int prec, int xv, int phase, int bs, hi
 hipError_t launch_reduce_replicas(const KParams &p, const int64_t *off, int nrep, double *out,
                                   hipStream_t st) {
   hipLaunchKernelGGL(k_reduce_replicas, dim3((unsigned)nrep), dim3(kBlock), 0, st, p, off, out);
-  return hipGetLastError();
-}
-
-hipError_t launch_advance(uint64_t *step, hipStream_t st) {
-  hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, st, step);
   return hipGetLastError();
 }
 

@@ -11,7 +11,6 @@ namespace jh {
 hipError_t launch_heun(const KParams &p, int prec, int xv, int phase, int bs, hipStream_t st);
 hipError_t launch_reduce_replicas(const KParams &p, const int64_t *off, int nrep, double *out,
                                   hipStream_t st);
-hipError_t launch_advance(uint64_t *step, hipStream_t st);
 hipError_t launch_set_step(uint64_t *step, uint64_t value, hipStream_t st);
 hipError_t launch_field(const KParams &p, int prec, int xv, int term, double *out,
                         hipStream_t st);
