@@ -205,6 +205,7 @@ struct jh_ctx {
   int64_t nuniform = 0, nslices = 0;
   DevBuf<uint16_t> cls;
   DevBuf<double> ctab;
+  std::vector<double> ctab_host;
   int nclass = 0;
   int ani_power = 2;
   int has_ac = 0;
@@ -793,6 +794,7 @@ void prepare_classes(jh_ctx *c) {
     ids[d] = (uint16_t)id;
   }
   c->nclass = (int)map.size();
+  c->ctab_host = table;
   c->ctab.alloc(table.size());
   HIPCHK(hipMemcpy(c->ctab.p, table.data(), sizeof(double) * table.size(), hipMemcpyHostToDevice));
   if (c->nclass > 1) {
@@ -903,6 +905,13 @@ KParams make_params(jh_ctx *c) {
   p.ani_power = c->ani_power;
   p.noise = c->noise_mode;
   p.dstep = nullptr;
+  if (c->nclass == 1) {
+    // the class-0 row as prepare_classes stored it
+    const double *row = c->ctab_host.data();
+    std::memcpy(&p.seed, &row[CF_SEED], sizeof p.seed);
+    std::memcpy(&p.roff, &row[CF_ROFF], sizeof p.roff);
+    p.sqrtT = row[CF_SQRTT];
+  }
   p.winj = c->winj.p;
   p.wbuf = c->wbuf.p;
   p.dt = c->dt;

@@ -120,6 +120,11 @@ struct KParams {
   int ani_power;
   int noise;
   uint64_t step;          // the step counter, or with dstep its offset
+  // thermostat of the only site class when cls is null (the CF_SQRTT,
+  // CF_SEED and CF_ROFF entries of class 0)
+  uint64_t seed;
+  double sqrtT;
+  uint64_t roff;
   const uint64_t *dstep;  // graph replay: the replay's first step, on the device
   const double *winj;  // injected noise SoA
   double *wbuf;        // noise of the current step: written by the predictor,

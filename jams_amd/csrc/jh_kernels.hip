@@ -373,8 +373,12 @@ __device__ __forceinline__ void total_field(const KParams &p, const double *src6
 #ifndef JH_UNIFORM_WAVES
 #define JH_UNIFORM_WAVES 1
 #endif
+#ifndef JH_DIAG_WAVES
+#define JH_DIAG_WAVES 1
+#endif
 template <int PREC, int XVk>
-constexpr int kMinWaves = (PREC == PR_FP32 && xv_uniform(XVk)) ? JH_UNIFORM_WAVES : JH_WAVES_PER_EU;
+constexpr int kMinWaves = (PREC == PR_FP32 && xv_uniform(XVk)) ? JH_UNIFORM_WAVES
+                          : xv_diag(XVk) ? JH_DIAG_WAVES : JH_WAVES_PER_EU;
 
 template <int PREC, int XVk, int PHASE, bool GEN>
 __global__ void __launch_bounds__(kMaxBlock)
@@ -399,28 +403,39 @@ __attribute__((amdgpu_waves_per_eu(kMinWaves<PREC, XVk>))) k_heun(const KParams 
   const double s0 = src[i], s1 = src[n + i], s2 = src[2 * n + i];
   const double t = PHASE == 0 ? t0 : t0 + p.dt;
 
-  double h0, h1, h2;
-  total_field<PREC, XVk, GEN>(p, src, p.g32, i, c, lds, s0, s1, s2, t, h0, h1, h2);
-
-  // h = w + h / mus  (solvers/cpu_llg_heun.cc:68-81); one draw per step is
-  // used by both phases (cpu_llg_heun.cc:53-62): the predictor generates it
-  // and leaves it for the corrector
-  const double mus = cparam(p, c, CF_MUS);
-  if (p.noise == NM_NONE) {
-    h0 = h0 / mus; h1 = h1 / mus; h2 = h2 / mus;
-  } else {
-    double w0, w1, w2;
+  // thermal field w of this step (solvers/cpu_llg_heun.cc:53-62): one draw
+  // per step serves both phases, so the predictor generates it and leaves it
+  // for the corrector
+  double w0 = 0.0, w1 = 0.0, w2 = 0.0;
+  auto thermal = [&] {
     if (p.noise == NM_PHILOX && PHASE == 0) {
-      double z0, z1, z2;
-      normals3(noise_seed(p, c), step, noise_index(p, c, i), z0, z1, z2);
-      const double sg = cparam(p, c, CF_SIGMA), sT = cparam(p, c, CF_SQRTT);
+      double z0, z1, z2, sT;
+      if (p.cls) {
+        normals3(noise_seed(p, c), step, noise_index(p, c, i), z0, z1, z2);
+        sT = cparam(p, c, CF_SQRTT);
+      } else {
+        // one site class: the key schedule stays in scalar registers
+        normals3(p.seed, step, (uint64_t)ref_index(p, i) - p.roff, z0, z1, z2);
+        sT = p.sqrtT;
+      }
+      const double sg = cparam(p, c, CF_SIGMA);
       w0 = z0 * sg * sT; w1 = z1 * sg * sT; w2 = z2 * sg * sT;
       p.wbuf[i] = w0; p.wbuf[n + i] = w1; p.wbuf[2 * n + i] = w2;
     } else if (p.noise == NM_PHILOX) {
       w0 = p.wbuf[i]; w1 = p.wbuf[n + i]; w2 = p.wbuf[2 * n + i];
-    } else {
+    } else if (p.noise == NM_INJECTED) {
       w0 = p.winj[i]; w1 = p.winj[n + i]; w2 = p.winj[2 * n + i];
     }
+  };
+  double h0, h1, h2;
+  total_field<PREC, XVk, GEN>(p, src, p.g32, i, c, lds, s0, s1, s2, t, h0, h1, h2);
+  thermal();
+
+  // h = w + h / mus  (solvers/cpu_llg_heun.cc:68-81)
+  const double mus = cparam(p, c, CF_MUS);
+  if (p.noise == NM_NONE) {
+    h0 = h0 / mus; h1 = h1 / mus; h2 = h2 / mus;
+  } else {
     h0 = w0 + h0 / mus; h1 = w1 + h1 / mus; h2 = w2 + h2 / mus;
   }
 
