@@ -137,6 +137,86 @@ __global__ void __launch_bounds__(256) k_dip_mult_cplx(typename fft::C2<T>::type
   }
 }
 
+// Short contiguous axis (P2 <= kDirectP, L2 <= kDirectL): one thread per line
+// evaluates the DFT directly in registers, fused with the spin scatter
+// (forward, real input z < L2, outputs q < Q2) or the field gather (inverse,
+// Hermitian input q < Q2, real outputs z < L2).  No LDS, no barriers: the
+// staged pass is latency-bound on such lines.
+constexpr int kDirectP = 64, kDirectL = 32;
+
+template <class T>
+__global__ void __launch_bounds__(256) k_dip_z_direct_fwd(const Pass<T> p) {
+  using T2 = typename fft::C2<T>::type;
+  const int64_t L = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (L >= p.nlines) return;
+  const int64_t cells = (int64_t)p.grid.L[0] * p.grid.L[1];
+  const int g = (int)(L / cells);
+  const int64_t xy = L % cells;
+  const int m = g / 3, comp = g % 3;
+  T v[kDirectL];
+#pragma unroll
+  for (int z = 0; z < kDirectL; ++z) {
+    v[z] = 0;
+    if (z < p.n_in) {
+      const int64_t idx = (xy * p.grid.L[2] + z) * p.grid.nb + m;
+      v[z] = (T)(p.grid.mus[idx] * p.spins[comp * p.grid.n + idx]);
+    }
+  }
+  for (int q = 0; q < p.n_out; ++q) {
+    T2 acc;
+    acc.x = 0;
+    acc.y = 0;
+    int t = 0;  // (q z) mod n
+#pragma unroll
+    for (int z = 0; z < kDirectL; ++z) {
+      if (z < p.n_in) {
+        const T2 w = p.tw[t];
+        acc.x = acc.x + v[z] * w.x;
+        acc.y = acc.y + v[z] * w.y;
+        t += q;
+        if (t >= p.n) t -= p.n;
+      }
+    }
+    p.out[p.dst.at(L, q)] = acc;
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_dip_z_direct_inv(const Pass<T> p) {
+  using T2 = typename fft::C2<T>::type;
+  const int64_t L = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (L >= p.nlines) return;
+  const int64_t cells = (int64_t)p.grid.L[0] * p.grid.L[1];
+  const int g = (int)(L / cells);
+  const int64_t xy = L % cells;
+  const int m = g / 3, comp = g % 3;
+  const int n = p.n, half = n / 2;
+  T2 X[kDirectP / 2 + 1];
+#pragma unroll
+  for (int q = 0; q <= kDirectP / 2; ++q)
+    if (q < p.n_in) X[q] = p.in[p.src.at(L, q)];
+  for (int z = 0; z < p.n_out; ++z) {
+    // Re sum_k X[k] e^{+2 pi i k z/n} over the Hermitian sequence:
+    // X[0] + 2 Re(X[k] w^{kz}) for 0 < k < n/2 (+ X[n/2] (-1)^z, n even)
+    T acc = X[0].x;
+    int t = z;  // (k z) mod n, k = 1
+#pragma unroll
+    for (int k = 1; k <= kDirectP / 2; ++k) {
+      if (k < p.n_in) {
+        const T2 w = p.tw[t];  // e^{-2 pi i t/n}; the inverse uses its conjugate
+        const T re = X[k].x * w.x + X[k].y * w.y;
+        if (2 * k == n) acc = acc + re;
+        else acc = acc + 2 * re;
+        t += z;
+        if (t >= n) t -= n;
+      }
+    }
+    (void)half;
+    const int64_t idx = (xy * p.grid.L[2] + z) * p.grid.nb + m;
+    p.field[comp * p.grid.n + idx] = p.grid.mus[idx] * (double)acc;
+  }
+}
+
 // tensor set-up: the q2 < Q2 half of the transformed grid into the tensor
 // arrays (real part for one basis site)
 template <class T>
@@ -304,6 +384,8 @@ class DipoleFFTImpl : public DipoleFFT {
     return hipStreamSynchronize(st);
   }
 
+  bool direct_z() const { return P[2] <= kDirectP && L[2] <= kDirectL; }
+
   template <class T>
   hipError_t eval(Plane<T> &pl, const double *s, double *h, hipStream_t st) {
     using T2 = typename fft::C2<T>::type;
@@ -319,7 +401,13 @@ class DipoleFFTImpl : public DipoleFFT {
       p.dst = LineMap{L[0], L[1], sa, (int64_t)P[1] * Q2, Q2, 1};
       p.out = pl.K.p;
       p.spins = s;
-      DCHK((run_pass<T, fft::LD_SPIN, fft::ST_CPLX>(p, st)));
+      if (direct_z()) {
+        hipLaunchKernelGGL((k_dip_z_direct_fwd<T>), dim3((unsigned)((p.nlines + 255) / 256)), dim3(256),
+                           0, st, p);
+        DCHK(hipGetLastError());
+      } else {
+        DCHK((run_pass<T, fft::LD_SPIN, fft::ST_CPLX>(p, st)));
+      }
     }
     // y forward: lines (g, x < L0, q2)
     {
@@ -385,7 +473,13 @@ class DipoleFFTImpl : public DipoleFFT {
       p.src = LineMap{L[0], L[1], sa, (int64_t)P[1] * Q2, Q2, 1};
       p.in = pl.K.p;
       p.field = h;
-      DCHK((run_pass<T, fft::LD_HERM, fft::ST_FIELD>(p, st)));
+      if (direct_z()) {
+        hipLaunchKernelGGL((k_dip_z_direct_inv<T>), dim3((unsigned)((p.nlines + 255) / 256)), dim3(256),
+                           0, st, p);
+        DCHK(hipGetLastError());
+      } else {
+        DCHK((run_pass<T, fft::LD_HERM, fft::ST_FIELD>(p, st)));
+      }
     }
     return hipSuccess;
   }

@@ -25,7 +25,7 @@ namespace jh {
 namespace fft {
 
 constexpr int kMaxStages = 24;
-constexpr int kThreads = 256;
+constexpr int kThreads = 512;
 
 template <class T> struct C2;
 template <> struct C2<double> { using type = double2; };
