@@ -17,64 +17,77 @@ from .engine import PREC_FP64, Engine
 from .host import Simulation
 
 
+def concatenate(replicas, output_dir: str = "/tmp") -> dict:
+    """Set the replicas up with the host layer (no device) and lay them end to
+    end: sites, spins, exchange list (indices shifted by each replica's first
+    spin, tensor indices by its first table row), Zeeman fields mus B, and the
+    replica table (offsets, temperatures, seeds)."""
+    mus, gyro, alpha, spins = [], [], [], []
+    pi, pj, vidx, vals = [], [], [], []
+    dc = []
+    offsets = [0]
+    temps, seeds, desc = [], [], []
+    dt = None
+    nvals = 0
+    for rep in replicas:
+        text, T, seed, B = rep[:4]
+        sim = Simulation(text, device=-1, output_dir=output_dir)
+        m, g, a, s = sim.sites()
+        d, _, _ = sim.params()
+        if dt is None:
+            dt = d
+        elif d != dt:
+            raise ValueError("all replicas must share one time step")
+        off = offsets[-1]
+        i, j, v, table = sim.exchange_list(0)
+        pi.append(i + off)
+        pj.append(j + off)
+        vidx.append(v + nvals)
+        vals.append(table)
+        nvals += table.shape[0]
+        mus.append(m)
+        gyro.append(g)
+        alpha.append(a)
+        spins.append(s)
+        # applied_field.cc:146-148: h = mus_i B
+        dc.append(m[:, None] * np.asarray(B, np.float64)[None, :])
+        offsets.append(off + sim.n)
+        temps.append(float(T))
+        seeds.append(int(seed))
+        desc.append(rep[4] if len(rep) > 4 else f"{sim.n} spins")
+        sim.close()
+    return {"offsets": np.asarray(offsets, np.int64), "temperatures": np.asarray(temps),
+            "seeds": seeds, "dt": dt, "desc": desc,
+            "mus": np.concatenate(mus), "gyro": np.concatenate(gyro),
+            "alpha": np.concatenate(alpha), "spins": np.ascontiguousarray(np.concatenate(spins)),
+            "pi": np.concatenate(pi), "pj": np.concatenate(pj), "vidx": np.concatenate(vidx),
+            "values9": np.concatenate(vals), "zeeman_dc": np.ascontiguousarray(np.concatenate(dc))}
+
+
 class ReplicaBatch:
     """replicas: iterable of (config text, temperature K, seed, field (3,) T
     [, description])."""
 
     def __init__(self, replicas, device: int = 0, precision: int = PREC_FP64,
                  output_dir: str = "/tmp"):
-        mus, gyro, alpha, spins = [], [], [], []
-        pi, pj, vidx, vals = [], [], [], []
-        dc = []
-        offsets = [0]
-        temps, seeds, self.desc = [], [], []
-        dt = None
-        nvals = 0
-        for rep in replicas:
-            text, T, seed, B = rep[:4]
-            sim = Simulation(text, device=-1, output_dir=output_dir)
-            m, g, a, s = sim.sites()
-            d, _, _ = sim.params()
-            if dt is None:
-                dt = d
-            elif d != dt:
-                raise ValueError("all replicas must share one time step")
-            off = offsets[-1]
-            i, j, v, table = sim.exchange_list(0)
-            pi.append(i + off)
-            pj.append(j + off)
-            vidx.append(v + nvals)
-            vals.append(table)
-            nvals += table.shape[0]
-            mus.append(m)
-            gyro.append(g)
-            alpha.append(a)
-            spins.append(s)
-            # applied_field.cc:146-148: h = mus_i B
-            dc.append(m[:, None] * np.asarray(B, np.float64)[None, :])
-            offsets.append(off + sim.n)
-            temps.append(float(T))
-            seeds.append(int(seed))
-            self.desc.append(rep[4] if len(rep) > 4 else f"{sim.n} spins")
-            sim.close()
-        self.offsets = np.asarray(offsets, np.int64)
+        cat = concatenate(replicas, output_dir)
+        self.offsets = cat["offsets"]
         self.n = int(self.offsets[-1])
-        self.nrep = len(temps)
-        self.dt = dt
+        self.nrep = len(cat["seeds"])
+        self.dt = cat["dt"]
+        self.desc = cat["desc"]
         e = Engine(device)
-        e.set_sites(np.concatenate(mus), np.concatenate(gyro), np.concatenate(alpha))
-        e.add_exchange_list(np.concatenate(pi), np.concatenate(pj), np.concatenate(vidx),
-                            np.concatenate(vals))
-        dcall = np.ascontiguousarray(np.concatenate(dc))
-        if np.any(dcall != 0.0):
-            e.add_zeeman(dcall)
-        e.set_time_step(dt)
+        e.set_sites(cat["mus"], cat["gyro"], cat["alpha"])
+        e.add_exchange_list(cat["pi"], cat["pj"], cat["vidx"], cat["values9"])
+        if np.any(cat["zeeman_dc"] != 0.0):
+            e.add_zeeman(cat["zeeman_dc"])
+        e.set_time_step(self.dt)
         e.set_precision(precision)
-        self.s0 = np.ascontiguousarray(np.concatenate(spins))
+        self.s0 = cat["spins"]
         e.set_spins(self.s0)
-        e.set_replicas(self.offsets, temps, seeds)
-        self.temperatures = np.asarray(temps)
-        self.seeds = seeds
+        e.set_replicas(self.offsets, cat["temperatures"], cat["seeds"])
+        self.temperatures = cat["temperatures"]
+        self.seeds = cat["seeds"]
         self.engine = e
 
     def spins(self, r: int | None = None) -> np.ndarray:

@@ -8,7 +8,7 @@
  * +, -, *, / and sqrt is one correctly rounded IEEE-754 double operation in
  * the order written, as in the reference's C++17 ISO build.
  *
- * How each function is pinned (tests/test_oracle_*.py):
+ * How each function is pinned (tests/test_oracle.py, tests/test_host.py):
  *   or_philox4x32_10     Random123 known-answer vectors (published KAT).
  *   or_normals3 / fill   integer stream pinned by the KAT; normals checked by
  *                        moments and a Kolmogorov-Smirnov test.
@@ -29,7 +29,13 @@
  *   or_term_energy       FM closed form -1/2 N sum_j J_ij; Zeeman/uniaxial
  *                        closed forms.
  *   or_dipole_bruteforce reference known answers (test/hamiltonian/
- *                        test_dipole.h:256-404) on periodic chains.
+ *                        test_dipole.h:256-404) on periodic chains; the
+ *                        OR_TERM_DIPOLE term of or_term_field evaluates it.
+ *   or_dipole_fft_direct (oracle_dipole_fft.c) equals or_dipole_bruteforce
+ *                        to 1e-12 in its two-sided form on open, mixed and
+ *                        periodic lattices.
+ *   or_noise_log /       libm and long-double references over 1e5 arguments.
+ *   or_noise_sincos2pi
  */
 #include "oracle.h"
 

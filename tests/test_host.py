@@ -171,3 +171,37 @@ def test_python_binding_covers_the_engine_abi():
     from jams_amd import engine
     names = declared(ROOT / "include" / "jams_hip.h")
     assert sorted(engine.ABI) == names
+
+
+def test_replica_batch_concatenation():
+    """The batching form's host side: replicas laid end to end with their
+    exchange pairs, tensor indices and fields shifted into place (no device)."""
+    from jams_amd import configs
+    from jams_amd.batch import concatenate
+    from jams_amd.host import Simulation
+
+    reps = configs.c4(nrep=5, seed=11, steps=10)
+    cat = concatenate(reps)
+    off = cat["offsets"]
+    assert off[0] == 0 and np.all(np.diff(off) > 0) and off[-1] == cat["mus"].size
+    # every pair stays inside its replica; pairs sorted by (i, j) globally
+    r_i = np.searchsorted(off, cat["pi"], side="right") - 1
+    r_j = np.searchsorted(off, cat["pj"], side="right") - 1
+    assert np.array_equal(r_i, r_j)
+    key = cat["pi"].astype(np.int64) * (1 << 32) + cat["pj"]
+    assert np.all(np.diff(key) > 0)
+    # each replica's slice equals its own set-up
+    for r, rep in enumerate(reps):
+        sim = Simulation(rep[0], device=-1, output_dir="/tmp")
+        i, j, v, table = sim.exchange_list(0)
+        sel = r_i == r
+        assert np.array_equal(cat["pi"][sel] - off[r], i)
+        assert np.array_equal(cat["pj"][sel] - off[r], j)
+        assert np.array_equal(cat["values9"][cat["vidx"][sel]], table[v])
+        m = sim.sites()[0]
+        assert np.array_equal(cat["zeeman_dc"][off[r]:off[r + 1]],
+                              m[:, None] * np.asarray(rep[3])[None, :])
+        assert cat["temperatures"][r] == rep[1] and cat["seeds"][r] == rep[2]
+        sim.close()
+    # the workload's draw: T_r = 1000 r/(nrep-1), B_r = 2 r/(nrep-1) along z
+    assert reps[-1][1] == 1000.0 and reps[-1][3] == (0.0, 0.0, 2.0)
