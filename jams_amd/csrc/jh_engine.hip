@@ -170,7 +170,6 @@ struct jh_ctx {
   int64_t order_cells = 0;
   int64_t iteration = 0;
   double time = 0.0;
-  int64_t noise_step = -1;       // step whose thermostat draw wbuf holds (-1: none)
 
   // terms
   std::vector<TermHost> terms;
@@ -869,7 +868,6 @@ void prepare(jh_ctx *c) {
   prepare_exchange(c);
   prepare_classes(c);
   if (c->noise_mode == NM_PHILOX) c->wbuf.alloc((size_t)(3 * c->n));
-  c->noise_step = -1;
   sync_f32(c);
   HIPCHK(hipStreamSynchronize(c->stream));
   c->dirty = false;
@@ -963,19 +961,7 @@ void build_graph(jh_ctx *c) {
   HIPCHK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
 }
 
-// The step kernels read the current step's draw from wbuf and the corrector
-// leaves the next one there; generate it here when wbuf does not hold it.
-void prime_noise(jh_ctx *c) {
-  if (c->noise_mode != NM_PHILOX || c->noise_step == c->iteration) return;
-  KParams p = make_params(c);
-  p.step = (uint64_t)c->iteration;
-  HIPCHK(launch_noise(p, c->wbuf.p, c->stream));
-  c->noise_step = c->iteration;
-}
-
 void do_steps(jh_ctx *c, int64_t nsteps) {
-  if (nsteps <= 0) return;
-  prime_noise(c);
   int64_t done = 0;
   if (c->graph_steps > 0 && nsteps >= c->graph_steps) {
     if (!c->gexec) build_graph(c);
@@ -985,7 +971,6 @@ void do_steps(jh_ctx *c, int64_t nsteps) {
     }
     c->iteration += done;
     c->time = (double)c->iteration * c->dt;
-    c->noise_step = c->iteration;
   }
   KParams p = make_params(c);
   for (; done < nsteps; ++done) {
@@ -995,7 +980,6 @@ void do_steps(jh_ctx *c, int64_t nsteps) {
     c->iteration += 1;
     c->time = (double)c->iteration * c->dt;
   }
-  c->noise_step = c->iteration;
 }
 
 template <class F>
@@ -1400,7 +1384,6 @@ int jh_time_phase(jh_ctx *c, int phase, int reps, double *ms) {
       save32.alloc((size_t)(kG32 * c->n));
       HIPCHK(hipMemcpyAsync(save32.p, c->s32.p, sizeof(float) * kG32 * c->n, hipMemcpyDeviceToDevice, c->stream));
     }
-    prime_noise(c);
     KParams p = make_params(c);
     p.step = (uint64_t)c->iteration;
     p.t = c->time;
@@ -1420,7 +1403,6 @@ int jh_time_phase(jh_ctx *c, int phase, int reps, double *ms) {
     float f = 0.f;
     HIPCHK(hipEventElapsedTime(&f, c->ev0, c->ev1));
     *ms = (double)f / reps;
-    c->noise_step = -1;  // the corrector launches moved wbuf on
     HIPCHK(hipMemcpyAsync(c->s.p, save.p, sizeof(double) * 3 * c->n, hipMemcpyDeviceToDevice, c->stream));
     if (c->prec == PR_FP32)
       HIPCHK(hipMemcpyAsync(c->s32.p, save32.p, sizeof(float) * kG32 * c->n, hipMemcpyDeviceToDevice, c->stream));
@@ -1453,8 +1435,7 @@ int jh_phase_bytes(jh_ctx *c, int phase, double *fmt_bytes, double *csr_bytes) {
     if (c->prec == PR_FP32) state += kG32 * n * 4;  // FP32 gather copy written
     if (c->nclass > 1) state += 2 * n;
     if (c->noise_mode == NM_INJECTED) state += 3 * n * 8;
-    // noise: both phases read this step's draw, the corrector writes the next
-    if (c->noise_mode == NM_PHILOX) state += phase == 0 ? 3 * n * 8 : 2 * 3 * n * 8;
+    if (c->noise_mode == NM_PHILOX) state += 3 * n * 8;  // noise written, then read back
     if (dipole_term(c) >= 0) state += 3 * n * 8;
     *fmt_bytes = mat + gather + state;
     // reference CSR SpMV (FP64 values, int32 indices): nnz (8+4) + (3N+1) 4 + 3N 8 + 3N 8

@@ -305,23 +305,6 @@ __device__ __forceinline__ uint64_t noise_index(const KParams &p, uint32_t c, in
   return (uint64_t)ref_index(p, i) - (uint64_t)__double_as_longlong(cparam(p, c, CF_ROFF));
 }
 
-// Scaled thermal field of spin i at `step`: w = (xi sigma) sqrt(T)
-// (solvers/cpu_llg_heun.cc:55-63).
-__device__ __forceinline__ void thermal_w(const KParams &p, uint32_t c, int64_t i, uint64_t step,
-                                          double &w0, double &w1, double &w2) {
-  double z0, z1, z2, sT;
-  if (p.cls) {
-    normals3(noise_seed(p, c), step, noise_index(p, c, i), z0, z1, z2);
-    sT = cparam(p, c, CF_SQRTT);
-  } else {
-    // one site class: the key schedule stays in scalar registers
-    normals3(p.seed, step, (uint64_t)ref_index(p, i) - p.roff, z0, z1, z2);
-    sT = p.sqrtT;
-  }
-  const double sg = cparam(p, c, CF_SIGMA);
-  w0 = z0 * sg * sT; w1 = z1 * sg * sT; w2 = z2 * sg * sT;
-}
-
 // Local (non-exchange) term q of spin i with spin vector (s0, s1, s2).
 // GEN = false compiles only the common forms (static Zeeman, K1 anisotropy);
 // GEN = true adds the ac Zeeman field and the K2/K3 powers.
@@ -420,20 +403,33 @@ __attribute__((amdgpu_waves_per_eu(kMinWaves<PREC, XVk>))) k_heun(const KParams 
   const double s0 = src[i], s1 = src[n + i], s2 = src[2 * n + i];
   const double t = PHASE == 0 ? t0 : t0 + p.dt;
 
-  // thermal field w of this step (solvers/cpu_llg_heun.cc:53-62).  One draw
-  // per step serves both phases.  It is generated one step ahead, by the
-  // previous corrector (or a priming k_noise launch), and read from wbuf by
-  // both phases; the corrector then replaces it with the next step's draw.
-  // The predictor, whose exchange sum is latency-bound, does no RNG work;
-  // the corrector's is mostly hidden behind its memory traffic.
+  // thermal field w of this step (solvers/cpu_llg_heun.cc:53-62): one draw
+  // per step serves both phases, so the predictor generates it and leaves it
+  // for the corrector
+  double w0 = 0.0, w1 = 0.0, w2 = 0.0;
+  auto thermal = [&] {
+    if (p.noise == NM_PHILOX && PHASE == 0) {
+      double z0, z1, z2, sT;
+      if (p.cls) {
+        normals3(noise_seed(p, c), step, noise_index(p, c, i), z0, z1, z2);
+        sT = cparam(p, c, CF_SQRTT);
+      } else {
+        // one site class: the key schedule stays in scalar registers
+        normals3(p.seed, step, (uint64_t)ref_index(p, i) - p.roff, z0, z1, z2);
+        sT = p.sqrtT;
+      }
+      const double sg = cparam(p, c, CF_SIGMA);
+      w0 = z0 * sg * sT; w1 = z1 * sg * sT; w2 = z2 * sg * sT;
+      p.wbuf[i] = w0; p.wbuf[n + i] = w1; p.wbuf[2 * n + i] = w2;
+    } else if (p.noise == NM_PHILOX) {
+      w0 = p.wbuf[i]; w1 = p.wbuf[n + i]; w2 = p.wbuf[2 * n + i];
+    } else if (p.noise == NM_INJECTED) {
+      w0 = p.winj[i]; w1 = p.winj[n + i]; w2 = p.winj[2 * n + i];
+    }
+  };
   double h0, h1, h2;
   total_field<PREC, XVk, GEN>(p, src, p.g32, i, c, lds, s0, s1, s2, t, h0, h1, h2);
-  double w0 = 0.0, w1 = 0.0, w2 = 0.0;
-  if (p.noise == NM_PHILOX) {
-    w0 = p.wbuf[i]; w1 = p.wbuf[n + i]; w2 = p.wbuf[2 * n + i];
-  } else if (p.noise == NM_INJECTED) {
-    w0 = p.winj[i]; w1 = p.winj[n + i]; w2 = p.winj[2 * n + i];
-  }
+  thermal();
 
   // h = w + h / mus  (solvers/cpu_llg_heun.cc:68-81)
   const double mus = cparam(p, c, CF_MUS);
@@ -459,11 +455,6 @@ __attribute__((amdgpu_waves_per_eu(kMinWaves<PREC, XVk>))) k_heun(const KParams 
     double v0 = p.s[i] + p.dt * d0, v1 = p.s[n + i] + p.dt * d1, v2 = p.s[2 * n + i] + p.dt * d2;
     unit_vector(v0, v1, v2);
     p.snew[i] = v0; p.snew[n + i] = v1; p.snew[2 * n + i] = v2;
-    if (p.noise == NM_PHILOX) {
-      double x0, x1, x2;
-      thermal_w(p, c, i, step + 1, x0, x1, x2);
-      p.wbuf[i] = x0; p.wbuf[n + i] = x1; p.wbuf[2 * n + i] = x2;
-    }
     if constexpr (PREC == PR_FP32) store_g32(p.w32, i, v0, v1, v2);
   }
 }
@@ -491,17 +482,17 @@ __global__ void __launch_bounds__(kBlock) k_field(const KParams p, int term, dou
   out[i] = h0; out[n + i] = h1; out[2 * n + i] = h2;
 }
 
-// Generated, scaled noise of one step: primes wbuf before a step loop, and
-// serves jh_get_noise.
+// Generated, scaled noise of one step (for inspection and parity tests).
 __global__ void __launch_bounds__(kBlock) k_noise(const KParams p, double *out) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= p.n) return;
   const uint32_t c = p.cls ? (uint32_t)p.cls[i] : 0u;
-  double w0, w1, w2;
-  thermal_w(p, c, i, p.step, w0, w1, w2);
-  out[i] = w0;
-  out[p.n + i] = w1;
-  out[2 * p.n + i] = w2;
+  double z0, z1, z2;
+  normals3(noise_seed(p, c), p.step, noise_index(p, c, i), z0, z1, z2);
+  const double sg = cparam(p, c, CF_SIGMA), sT = cparam(p, c, CF_SQRTT);
+  out[i] = z0 * sg * sT;
+  out[p.n + i] = z1 * sg * sT;
+  out[2 * p.n + i] = z2 * sg * sT;
 }
 
 // FP32 gather copy of SoA FP64 spins (kG32 floats per spin)
