@@ -162,6 +162,7 @@ __global__ void __launch_bounds__(256) k_dip_z_direct_fwd(const Pass<T> p) {
       v[z] = (T)(p.grid.mus[idx] * p.spins[comp * p.grid.n + idx]);
     }
   }
+  const int64_t obase = p.dst.base32((uint32_t)L);
   for (int q = 0; q < p.n_out; ++q) {
     T2 acc;
     acc.x = 0;
@@ -177,7 +178,7 @@ __global__ void __launch_bounds__(256) k_dip_z_direct_fwd(const Pass<T> p) {
         if (t >= p.n) t -= p.n;
       }
     }
-    p.out[p.dst.at(L, q)] = acc;
+    p.out[obase + (int64_t)q * p.dst.se] = acc;
   }
 }
 
@@ -192,9 +193,10 @@ __global__ void __launch_bounds__(256) k_dip_z_direct_inv(const Pass<T> p) {
   const int m = g / 3, comp = g % 3;
   const int n = p.n, half = n / 2;
   T2 X[kDirectP / 2 + 1];
+  const int64_t ibase = p.src.base32((uint32_t)L);
 #pragma unroll
   for (int q = 0; q <= kDirectP / 2; ++q)
-    if (q < p.n_in) X[q] = p.in[p.src.at(L, q)];
+    if (q < p.n_in) X[q] = p.in[ibase + (int64_t)q * p.src.se];
   for (int z = 0; z < p.n_out; ++z) {
     // Re sum_k X[k] e^{+2 pi i k z/n} over the Hermitian sequence:
     // X[0] + 2 Re(X[k] w^{kz}) for 0 < k < n/2 (+ X[n/2] (-1)^z, n even)
@@ -254,8 +256,11 @@ int pick_lines(const Pass<T> &p) {
                     : R == 5 ? fft::kBF<5> : fft::kBF<8>;
       B = std::min<int>(B, (int)((int64_t)nbf * fft::kThreads / (n / R)));
     }
-  B = std::max(B, 1);
-  return (int)std::min<int64_t>(B, std::max<int64_t>(p.nlines, 1));
+  B = (int)std::min<int64_t>(std::max(B, 1), std::max<int64_t>(p.nlines, 1));
+  // a power of two (lines map to threads by shifts and masks)
+  int P2 = 1;
+  while (2 * P2 <= B) P2 *= 2;
+  return P2;
 }
 
 template <class T, int LD, int ST>
@@ -265,11 +270,25 @@ hipError_t run_pass(Pass<T> p, hipStream_t st) {
   for (int s = 0; s < p.nst; ++s)
     if (p.radix[s] != 2 && p.radix[s] != 3 && p.radix[s] != 4 && p.radix[s] != 5 && p.radix[s] != 8)
       p.pingpong = 1;
+  if (p.nlines >= ((int64_t)1 << 31)) return hipErrorInvalidValue;
   p.B = pick_lines<T>(p);
+  p.lb = 0;
+  while ((1 << p.lb) < p.B) ++p.lb;
+  int Ns = 1;
+  for (int s = 0; s < p.nst; ++s) {
+    p.ns[s] = Ns;
+    p.ns_rcp[s] = 1.0f / (float)Ns;
+    p.tws[s] = p.n / (Ns * p.radix[s]);
+    Ns *= p.radix[s];
+  }
   const size_t lds = pass_lds<T>(p.n, p.B, p.pingpong);
   const int64_t blocks = (p.nlines + p.B - 1) / p.B;
-  hipLaunchKernelGGL((fft::k_fft_pass<T, LD, ST>), dim3((unsigned)blocks), dim3(fft::kThreads),
-                     lds, st, p);
+  if (p.inverse)
+    hipLaunchKernelGGL((fft::k_fft_pass<T, LD, ST, true>), dim3((unsigned)blocks),
+                       dim3(fft::kThreads), lds, st, p);
+  else
+    hipLaunchKernelGGL((fft::k_fft_pass<T, LD, ST, false>), dim3((unsigned)blocks),
+                       dim3(fft::kThreads), lds, st, p);
   return hipGetLastError();
 }
 

@@ -25,7 +25,8 @@ namespace jh {
 namespace fft {
 
 constexpr int kMaxStages = 24;
-constexpr int kThreads = 512;
+constexpr int kLogThreads = 9;
+constexpr int kThreads = 1 << kLogThreads;
 
 template <class T> struct C2;
 template <> struct C2<double> { using type = double2; };
@@ -63,6 +64,12 @@ struct LineMap {
     const int64_t b = ab % cb, a = ab / cb;
     return a * sa + b * sb + c * sc + k * se;
   }
+  // element 0 of line L < 2^31 (32-bit divisions; the host checks the range)
+  __device__ int64_t base32(uint32_t L) const {
+    const uint32_t c = L % (uint32_t)cc, ab = L / (uint32_t)cc;
+    const uint32_t b = ab % (uint32_t)cb, a = ab / (uint32_t)cb;
+    return (int64_t)a * sa + (int64_t)b * sb + (int64_t)c * sc;
+  }
 };
 
 // Spin-grid geometry for the fused scatter / gather: spin index of cell
@@ -83,7 +90,7 @@ struct Pass {
   int n, nst;
   int radix[kMaxStages];
   int n_in, n_out;
-  int B;                 // lines per workgroup
+  int B, lb;             // lines per workgroup, a power of two: B = 1 << lb
   int inverse;
   int pingpong;          // 1: two LDS buffers (plans with a direct-DFT factor)
   int64_t nlines;
@@ -91,6 +98,10 @@ struct Pass {
   const T2 *in;
   T2 *out;
   const T2 *tw;          // n twiddles exp(-2 pi i t / n)
+  // per stage: sub-length Ns before it, 1/Ns rounded to float, n / (Ns R)
+  int ns[kMaxStages];
+  float ns_rcp[kMaxStages];
+  int tws[kMaxStages];
   // LD_SPIN: lines are (g, x, y), g = 3 m + component; value mus * s
   // ST_FIELD: the same lines; h[comp N + i] = mus_i * Re(value)
   // LD_HERM: the line holds the first n_in = n/2 + 1 terms of a Hermitian
@@ -165,32 +176,51 @@ __device__ __forceinline__ void dft_fixed(T2 *v, bool inv) {
   }
 }
 
-template <class T>
-__device__ __forceinline__ typename C2<T>::type twiddle(const Pass<T> &p, int64_t t) {
+// q = x / d and r = x % d for 0 <= x < 2^22, 1 <= d < 2^22, with rd the
+// float nearest 1/d: the float estimate is within one of the quotient, and
+// one correction step makes it exact (no integer division instruction).
+__device__ __forceinline__ int divmod_small(int x, int d, float rd, int &r) {
+  int q = (int)((float)x * rd);
+  r = x - __mul24(q, d);
+  if (r < 0) {
+    q -= 1;
+    r += d;
+  } else if (r >= d) {
+    q += 1;
+    r -= d;
+  }
+  return q;
+}
+
+template <class T, bool INV>
+__device__ __forceinline__ typename C2<T>::type twiddle(const Pass<T> &p, int t) {
   typename C2<T>::type w = p.tw[t];
-  if (p.inverse) w.y = -w.y;
+  if (INV) w.y = -w.y;
   return w;
 }
 
-// One Stockham stage over B lines: in/out are LDS arrays [k][b].
-template <class T, int R>
+// One Stockham stage over B lines: in/out are LDS arrays [k][b].  Butterfly
+// t is line b = t mod B, index j = t / B < n / R.
+template <class T, int R, bool INV>
 __device__ __forceinline__ void stage_fixed(const Pass<T> &p, const typename C2<T>::type *in,
-                                            typename C2<T>::type *out, int Ns) {
+                                            typename C2<T>::type *out, int st) {
   using T2 = typename C2<T>::type;
-  const int n = p.n, B = p.B, m = n / R;
-  const int64_t stride_t = (int64_t)n / ((int64_t)Ns * R);
-  for (int t = threadIdx.x; t < m * B; t += blockDim.x) {
-    const int b = t % B, j = t / B;
-    const int jm = j % Ns;
+  const int n = p.n, B = p.B, lb = p.lb, m = n / R;
+  const int Ns = p.ns[st], tws = p.tws[st];
+  const float rns = p.ns_rcp[st];
+  for (int t = threadIdx.x; t < (m << lb); t += kThreads) {
+    const int b = t & (B - 1), j = t >> lb;
+    int jm;
+    const int jd = divmod_small(j, Ns, rns, jm);
     T2 v[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) v[r] = in[(int64_t)(j + r * m) * B + b];
+    for (int r = 0; r < R; ++r) v[r] = in[((j + r * m) << lb) + b];
 #pragma unroll
-    for (int r = 1; r < R; ++r) v[r] = cmul(v[r], twiddle(p, (int64_t)jm * r * stride_t));
-    dft_fixed<T2, R>(v, p.inverse != 0);
-    const int d = (j / Ns) * Ns * R + jm;
+    for (int r = 1; r < R; ++r) v[r] = cmul(v[r], twiddle<T, INV>(p, jm * r * tws));
+    dft_fixed<T2, R>(v, INV);
+    const int d = jd * Ns * R + jm;
 #pragma unroll
-    for (int r = 0; r < R; ++r) out[(int64_t)(d + r * Ns) * B + b] = v[r];
+    for (int r = 0; r < R; ++r) out[((d + r * Ns) << lb) + b] = v[r];
   }
 }
 
@@ -202,159 +232,194 @@ constexpr int kBF = (16 / R) > 0 ? 16 / R : 1;
 // The same stage in place on one LDS buffer: every thread first reads and
 // transforms its butterflies (at most kBF<R>, which the host guarantees by
 // its choice of B), then all write.
-template <class T, int R>
-__device__ __forceinline__ void stage_inplace(const Pass<T> &p, typename C2<T>::type *A, int Ns) {
+template <class T, int R, bool INV>
+__device__ __forceinline__ void stage_inplace(const Pass<T> &p, typename C2<T>::type *A, int st) {
   using T2 = typename C2<T>::type;
   constexpr int NBF = kBF<R>;
-  const int n = p.n, B = p.B, m = n / R, total = m * B;
-  const int64_t stride_t = (int64_t)n / ((int64_t)Ns * R);
+  const int n = p.n, B = p.B, lb = p.lb, m = n / R, total = m << lb;
+  const int Ns = p.ns[st], tws = p.tws[st];
+  const float rns = p.ns_rcp[st];
   T2 v[NBF][R];
+  int dq[NBF];
 #pragma unroll
   for (int q = 0; q < NBF; ++q) {
-    const int t = threadIdx.x + q * blockDim.x;
+    const int t = threadIdx.x + q * kThreads;
     if (t < total) {
-      const int b = t % B, j = t / B, jm = j % Ns;
+      const int b = t & (B - 1), j = t >> lb;
+      int jm;
+      const int jd = divmod_small(j, Ns, rns, jm);
+      dq[q] = ((jd * Ns * R + jm) << lb) + b;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[q][r] = A[(int64_t)(j + r * m) * B + b];
+      for (int r = 0; r < R; ++r) v[q][r] = A[((j + r * m) << lb) + b];
 #pragma unroll
-      for (int r = 1; r < R; ++r) v[q][r] = cmul(v[q][r], twiddle(p, (int64_t)jm * r * stride_t));
-      dft_fixed<T2, R>(v[q], p.inverse != 0);
+      for (int r = 1; r < R; ++r) v[q][r] = cmul(v[q][r], twiddle<T, INV>(p, jm * r * tws));
+      dft_fixed<T2, R>(v[q], INV);
     }
   }
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < NBF; ++q) {
-    const int t = threadIdx.x + q * blockDim.x;
+    const int t = threadIdx.x + q * kThreads;
     if (t < total) {
-      const int b = t % B, j = t / B, jm = j % Ns;
-      const int d = (j / Ns) * Ns * R + jm;
 #pragma unroll
-      for (int r = 0; r < R; ++r) A[(int64_t)(d + r * Ns) * B + b] = v[q][r];
+      for (int r = 0; r < R; ++r) A[dq[q] + ((r * Ns) << lb)] = v[q][r];
     }
   }
   __syncthreads();
 }
 
 // Any other factor R: direct R-point DFT with roots from the n-table.
-template <class T>
+template <class T, bool INV>
 __device__ void stage_generic(const Pass<T> &p, const typename C2<T>::type *in,
-                              typename C2<T>::type *out, int Ns, int R) {
+                              typename C2<T>::type *out, int st, int R) {
   using T2 = typename C2<T>::type;
-  const int n = p.n, B = p.B, m = n / R;
-  const int64_t stride_t = (int64_t)n / ((int64_t)Ns * R);
-  const int64_t root = n / R;  // exp(-2 pi i / R) = tw[n / R]
-  for (int t = threadIdx.x; t < m * B; t += blockDim.x) {
-    const int b = t % B, j = t / B;
-    const int jm = j % Ns;
-    const int d = (j / Ns) * Ns * R + jm;
+  const int n = p.n, B = p.B, lb = p.lb, m = n / R;
+  const int Ns = p.ns[st], tws = p.tws[st];
+  const float rns = p.ns_rcp[st];
+  const int root = n / R;  // exp(-2 pi i / R) = tw[n / R]
+  for (int t = threadIdx.x; t < (m << lb); t += kThreads) {
+    const int b = t & (B - 1), j = t >> lb;
+    int jm;
+    const int jd = divmod_small(j, Ns, rns, jm);
+    const int d = jd * Ns * R + jm;
     for (int q = 0; q < R; ++q) {
       T2 acc;
       acc.x = 0;
       acc.y = 0;
       for (int r = 0; r < R; ++r) {
-        T2 x = in[(int64_t)(j + r * m) * B + b];
-        if (r > 0) x = cmul(x, twiddle(p, (int64_t)jm * r * stride_t));
-        acc = cadd(acc, cmul(x, twiddle(p, ((int64_t)q * r % R) * root)));
+        T2 x = in[((j + r * m) << lb) + b];
+        if (r > 0) x = cmul(x, twiddle<T, INV>(p, jm * r * tws));
+        acc = cadd(acc, cmul(x, twiddle<T, INV>(p, ((q * r) % R) * root)));
       }
-      out[(int64_t)(d + q * Ns) * B + b] = acc;
+      out[((d + q * Ns) << lb) + b] = acc;
     }
   }
 }
 
-template <class T, int LD, int ST>
+// Thread -> (line b, first element k0, element step ks) for staging B lines
+// of n elements with the 512 threads of a workgroup: element-fastest when
+// the axis is contiguous in memory, line-fastest when it is strided.
+struct Stager {
+  int b, k0, ks;
+  __device__ Stager(bool kfast, int lb) {
+    const int tid = threadIdx.x;
+    if (kfast) {
+      const int sh = kLogThreads - lb;  // 512 / B threads per line
+      b = tid >> sh;
+      k0 = tid & ((1 << sh) - 1);
+      ks = 1 << sh;
+    } else {
+      b = tid & ((1 << lb) - 1);
+      k0 = tid >> lb;
+      ks = kThreads >> lb;
+    }
+  }
+};
+
+// Spin-grid line L = (g, x, y), g = 3 m + comp: index of its spin at z = 0
+// (a spin at z has index base + z nb) and the component.
+template <class T>
+__device__ __forceinline__ int64_t spin_line(const Pass<T> &p, uint32_t L, int &comp) {
+  const uint32_t cells = (uint32_t)p.grid.L[0] * (uint32_t)p.grid.L[1];
+  const uint32_t g = L / cells, xy = L % cells;
+  comp = (int)(g % 3u);
+  return (int64_t)xy * p.grid.L[2] * p.grid.nb + (int)(g / 3u);
+}
+
+template <class T, int LD, int ST, bool INV>
 __global__ void __launch_bounds__(kThreads) k_fft_pass(const Pass<T> p) {
   using T2 = typename C2<T>::type;
   extern __shared__ __align__(16) unsigned char smem[];
   T2 *A = reinterpret_cast<T2 *>(smem);
   T2 *Bf = A + (size_t)p.n * p.B;
-  const int n = p.n, B = p.B;
-  const int64_t L0 = (int64_t)blockIdx.x * B;
-  // element-fastest staging when the axis is contiguous, line-fastest else
-  const bool kfast_in = LD == LD_SPIN || p.src.se == 1;
-  const bool kfast_out = ST == ST_FIELD || p.dst.se == 1;
+  const int n = p.n, lb = p.lb;
+  const int64_t L0 = (int64_t)blockIdx.x << lb;
 
-  // ---- load ----
-  for (int q = threadIdx.x; q < n * B; q += blockDim.x) {
-    const int b = kfast_in ? q / n : q % B;
-    const int k = kfast_in ? q % n : q / B;
-    const int64_t L = L0 + b;
-    T2 v;
-    v.x = 0;
-    v.y = 0;
-    if (L < p.nlines) {
-      if constexpr (LD == LD_SPIN) {
-        if (k < p.n_in) {
-          const int64_t cells = (int64_t)p.grid.L[0] * p.grid.L[1];
-          const int g = (int)(L / cells);
-          const int64_t xy = L % cells;
-          const int m = g / 3, comp = g % 3;
-          const int64_t idx = (xy * p.grid.L[2] + k) * p.grid.nb + m;
-          v.x = (T)(p.grid.mus[idx] * p.spins[comp * p.grid.n + idx]);
-        }
-      } else if constexpr (LD == LD_HERM) {
-        if (k < p.n_in) {
-          v = p.in[p.src.at(L, k)];
-        } else {
-          v = p.in[p.src.at(L, n - k)];
-          v.y = -v.y;
-        }
-      } else {
-        if (k < p.n_in) v = p.in[p.src.at(L, k)];
-      }
+  // ---- load: each thread keeps one line; its address is computed once ----
+  {
+    const Stager s(LD == LD_SPIN || p.src.se == 1, lb);
+    const int64_t L = L0 + s.b;
+    const bool live = L < p.nlines;
+    int comp = 0;
+    int64_t base = 0;
+    if (live) {
+      if constexpr (LD == LD_SPIN) base = spin_line(p, (uint32_t)L, comp);
+      else base = p.src.base32((uint32_t)L);
     }
-    A[(int64_t)k * B + b] = v;
+    for (int k = s.k0; k < n; k += s.ks) {
+      T2 v;
+      v.x = 0;
+      v.y = 0;
+      if (live) {
+        if constexpr (LD == LD_SPIN) {
+          if (k < p.n_in) {
+            const int64_t idx = base + (int64_t)k * p.grid.nb;
+            v.x = (T)(p.grid.mus[idx] * p.spins[comp * p.grid.n + idx]);
+          }
+        } else if constexpr (LD == LD_HERM) {
+          if (k < p.n_in) {
+            v = p.in[base + (int64_t)k * p.src.se];
+          } else {
+            v = p.in[base + (int64_t)(n - k) * p.src.se];
+            v.y = -v.y;
+          }
+        } else {
+          if (k < p.n_in) v = p.in[base + (int64_t)k * p.src.se];
+        }
+      }
+      A[(k << lb) + s.b] = v;
+    }
   }
   __syncthreads();
 
   // ---- stages: in place on A, or ping-pong between A and Bf ----
   T2 *in = A, *out = Bf;
-  int Ns = 1;
   if (!p.pingpong) {
+    for (int st = 0; st < p.nst; ++st) {
+      switch (p.radix[st]) {
+        case 2: stage_inplace<T, 2, INV>(p, A, st); break;
+        case 3: stage_inplace<T, 3, INV>(p, A, st); break;
+        case 4: stage_inplace<T, 4, INV>(p, A, st); break;
+        case 5: stage_inplace<T, 5, INV>(p, A, st); break;
+        default: stage_inplace<T, 8, INV>(p, A, st); break;
+      }
+    }
+  } else {
     for (int st = 0; st < p.nst; ++st) {
       const int R = p.radix[st];
       switch (R) {
-        case 2: stage_inplace<T, 2>(p, A, Ns); break;
-        case 3: stage_inplace<T, 3>(p, A, Ns); break;
-        case 4: stage_inplace<T, 4>(p, A, Ns); break;
-        case 5: stage_inplace<T, 5>(p, A, Ns); break;
-        default: stage_inplace<T, 8>(p, A, Ns); break;
+        case 2: stage_fixed<T, 2, INV>(p, in, out, st); break;
+        case 3: stage_fixed<T, 3, INV>(p, in, out, st); break;
+        case 4: stage_fixed<T, 4, INV>(p, in, out, st); break;
+        case 5: stage_fixed<T, 5, INV>(p, in, out, st); break;
+        case 8: stage_fixed<T, 8, INV>(p, in, out, st); break;
+        default: stage_generic<T, INV>(p, in, out, st, R); break;
       }
-      Ns *= R;
+      __syncthreads();
+      T2 *tmp = in;
+      in = out;
+      out = tmp;
     }
-  } else
-  for (int st = 0; st < p.nst; ++st) {
-    const int R = p.radix[st];
-    switch (R) {
-      case 2: stage_fixed<T, 2>(p, in, out, Ns); break;
-      case 3: stage_fixed<T, 3>(p, in, out, Ns); break;
-      case 4: stage_fixed<T, 4>(p, in, out, Ns); break;
-      case 5: stage_fixed<T, 5>(p, in, out, Ns); break;
-      case 8: stage_fixed<T, 8>(p, in, out, Ns); break;
-      default: stage_generic<T>(p, in, out, Ns, R); break;
-    }
-    __syncthreads();
-    T2 *tmp = in;
-    in = out;
-    out = tmp;
-    Ns *= R;
   }
 
   // ---- store ----
-  for (int q = threadIdx.x; q < n * B; q += blockDim.x) {
-    const int b = kfast_out ? q / n : q % B;
-    const int k = kfast_out ? q % n : q / B;
-    const int64_t L = L0 + b;
-    if (L >= p.nlines || k >= p.n_out) continue;
-    const T2 v = in[(int64_t)k * B + b];
-    if constexpr (ST == ST_FIELD) {
-      const int64_t cells = (int64_t)p.grid.L[0] * p.grid.L[1];
-      const int g = (int)(L / cells);
-      const int64_t xy = L % cells;
-      const int m = g / 3, comp = g % 3;
-      const int64_t idx = (xy * p.grid.L[2] + k) * p.grid.nb + m;
-      p.field[comp * p.grid.n + idx] = p.grid.mus[idx] * (double)v.x;
-    } else {
-      p.out[p.dst.at(L, k)] = v;
+  {
+    const Stager s(ST == ST_FIELD || p.dst.se == 1, lb);
+    const int64_t L = L0 + s.b;
+    if (L < p.nlines) {
+      int comp = 0;
+      int64_t base;
+      if constexpr (ST == ST_FIELD) base = spin_line(p, (uint32_t)L, comp);
+      else base = p.dst.base32((uint32_t)L);
+      for (int k = s.k0; k < p.n_out; k += s.ks) {
+        const T2 v = in[(k << lb) + s.b];
+        if constexpr (ST == ST_FIELD) {
+          const int64_t idx = base + (int64_t)k * p.grid.nb;
+          p.field[comp * p.grid.n + idx] = p.grid.mus[idx] * (double)v.x;
+        } else {
+          p.out[base + (int64_t)k * p.dst.se] = v;
+        }
+      }
     }
   }
 }
