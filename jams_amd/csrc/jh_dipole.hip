@@ -252,8 +252,8 @@ int pick_lines(const Pass<T> &p) {
   if (!p.pingpong)
     for (int st = 0; st < p.nst; ++st) {
       const int R = p.radix[st];
-      const int nbf = R == 2 ? fft::kBF<2> : R == 3 ? fft::kBF<3> : R == 4 ? fft::kBF<4>
-                    : R == 5 ? fft::kBF<5> : fft::kBF<8>;
+      const int nbf = R == 2 ? fft::kBF<T, 2> : R == 3 ? fft::kBF<T, 3> : R == 4 ? fft::kBF<T, 4>
+                    : R == 5 ? fft::kBF<T, 5> : fft::kBF<T, 8>;
       B = std::min<int>(B, (int)((int64_t)nbf * fft::kThreads / (n / R)));
     }
   B = (int)std::min<int64_t>(std::max(B, 1), std::max<int64_t>(p.nlines, 1));
