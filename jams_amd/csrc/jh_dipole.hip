@@ -140,83 +140,204 @@ __global__ void __launch_bounds__(256) k_dip_mult_cplx(typename fft::C2<T>::type
 // Short contiguous axis (P2 <= kDirectP, L2 <= kDirectL): one thread per line
 // evaluates the DFT directly in registers, fused with the spin scatter
 // (forward, real input z < L2, outputs q < Q2) or the field gather (inverse,
-// Hermitian input q < Q2, real outputs z < L2).  No LDS, no barriers: the
-// staged pass is latency-bound on such lines.
+// Hermitian input q < Q2, real outputs z < L2).  A workgroup of T
+// consecutive lines stages its inputs and outputs through LDS so that global
+// loads and stores run in memory order (consecutive lines are adjacent in
+// both the spin arrays and the padded grid); each thread then reads its own
+// line from LDS (odd strides: no bank conflicts).
 constexpr int kDirectP = 64, kDirectL = 32;
 
+__host__ __device__ inline int odd_stride(int len) { return len | 1; }
+
+// LDS bytes of a direct z workgroup of T lines: twiddles, the complex side
+// (outputs forward, inputs inverse), the real side and the line tables
 template <class T>
+size_t direct_lds(int T_, int n, int n_in, int n_out, bool inverse) {
+  using T2 = typename fft::C2<T>::type;
+  const int nc = inverse ? n_in : n_out, nr = inverse ? n_out : n_in;
+  return (size_t)n * sizeof(T2) + (size_t)T_ * odd_stride(nc) * sizeof(T2) +
+         (size_t)T_ * odd_stride(nr) * sizeof(T) + (size_t)T_ * (sizeof(int64_t) * 2 + sizeof(int));
+}
+
+// LM: register bound of the line (L2 <= LM, P2 <= 2 LM), 8, 16 or 32
+template <class T, int LM>
 __global__ void __launch_bounds__(256) k_dip_z_direct_fwd(const Pass<T> p) {
   using T2 = typename fft::C2<T>::type;
-  const int64_t L = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (L >= p.nlines) return;
-  const int64_t cells = (int64_t)p.grid.L[0] * p.grid.L[1];
-  const int g = (int)(L / cells);
-  const int64_t xy = L % cells;
-  const int m = g / 3, comp = g % 3;
-  T v[kDirectL];
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int nt = blockDim.x, tid = threadIdx.x;
+  const int n = p.n, n_in = p.n_in, n_out = p.n_out;
+  const int SV = odd_stride(n_in), SO = odd_stride(n_out);
+  T2 *tw = reinterpret_cast<T2 *>(smem);
+  T2 *outs = tw + n;
+  T *vals = reinterpret_cast<T *>(outs + (size_t)nt * SO);
+  // (nt is a multiple of 64, so the int64 tables stay 8-byte aligned)
+  int64_t *lidx = reinterpret_cast<int64_t *>(vals + (size_t)nt * SV);
+  int64_t *lout = lidx + nt;
+  int *lcomp = reinterpret_cast<int *>(lout + nt);
+  const int64_t L0 = (int64_t)blockIdx.x * nt;
+  for (int t = tid; t < n; t += nt) tw[t] = p.tw[t];
+  {
+    const int64_t L = L0 + tid;
+    int comp = 0;
+    int64_t ib = -1, ob = -1;
+    if (L < p.nlines) {
+      ib = fft::spin_line(p, (uint32_t)L, comp);
+      ob = p.dst.base32((uint32_t)L);
+    }
+    lidx[tid] = ib;
+    lout[tid] = ob;
+    lcomp[tid] = comp;
+  }
+  __syncthreads();
+  // spins of the workgroup's lines, in memory order
+  const float rin = 1.0f / (float)n_in, rout = 1.0f / (float)n_out;
+  for (int e = tid; e < nt * n_in; e += nt) {
+    int z;
+    const int b = fft::divmod_small(e, n_in, rin, z);
+    const int64_t base = lidx[b];
+    T v = 0;
+    if (base >= 0) {
+      const int64_t idx = base + (int64_t)z * p.grid.nb;
+      v = (T)(p.grid.mus[idx] * p.spins[lcomp[b] * p.grid.n + idx]);
+    }
+    vals[b * SV + z] = v;
+  }
+  __syncthreads();
+  {
+    T v[LM];
 #pragma unroll
-  for (int z = 0; z < kDirectL; ++z) {
-    v[z] = 0;
-    if (z < p.n_in) {
-      const int64_t idx = (xy * p.grid.L[2] + z) * p.grid.nb + m;
-      v[z] = (T)(p.grid.mus[idx] * p.spins[comp * p.grid.n + idx]);
+    for (int z = 0; z < LM; ++z) v[z] = z < n_in ? vals[tid * SV + z] : (T)0;
+    for (int q = 0; q < n_out; ++q) {
+      T2 acc;
+      acc.x = 0;
+      acc.y = 0;
+      int t = 0;  // (q z) mod n
+#pragma unroll
+      for (int z = 0; z < LM; ++z) {
+        if (z < n_in) {
+          const T2 w = tw[t];
+          acc.x = acc.x + v[z] * w.x;
+          acc.y = acc.y + v[z] * w.y;
+          t += q;
+          if (t >= n) t -= n;
+        }
+      }
+      outs[tid * SO + q] = acc;
     }
   }
-  const int64_t obase = p.dst.base32((uint32_t)L);
-  for (int q = 0; q < p.n_out; ++q) {
-    T2 acc;
-    acc.x = 0;
-    acc.y = 0;
-    int t = 0;  // (q z) mod n
-#pragma unroll
-    for (int z = 0; z < kDirectL; ++z) {
-      if (z < p.n_in) {
-        const T2 w = p.tw[t];
-        acc.x = acc.x + v[z] * w.x;
-        acc.y = acc.y + v[z] * w.y;
-        t += q;
-        if (t >= p.n) t -= p.n;
-      }
-    }
-    p.out[obase + (int64_t)q * p.dst.se] = acc;
+  __syncthreads();
+  for (int e = tid; e < nt * n_out; e += nt) {
+    int q;
+    const int b = fft::divmod_small(e, n_out, rout, q);
+    const int64_t base = lout[b];
+    if (base >= 0) p.out[base + (int64_t)q * p.dst.se] = outs[b * SO + q];
   }
 }
 
-template <class T>
+template <class T, int LM>
 __global__ void __launch_bounds__(256) k_dip_z_direct_inv(const Pass<T> p) {
   using T2 = typename fft::C2<T>::type;
-  const int64_t L = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (L >= p.nlines) return;
-  const int64_t cells = (int64_t)p.grid.L[0] * p.grid.L[1];
-  const int g = (int)(L / cells);
-  const int64_t xy = L % cells;
-  const int m = g / 3, comp = g % 3;
-  const int n = p.n, half = n / 2;
-  T2 X[kDirectP / 2 + 1];
-  const int64_t ibase = p.src.base32((uint32_t)L);
-#pragma unroll
-  for (int q = 0; q <= kDirectP / 2; ++q)
-    if (q < p.n_in) X[q] = p.in[ibase + (int64_t)q * p.src.se];
-  for (int z = 0; z < p.n_out; ++z) {
-    // Re sum_k X[k] e^{+2 pi i k z/n} over the Hermitian sequence:
-    // X[0] + 2 Re(X[k] w^{kz}) for 0 < k < n/2 (+ X[n/2] (-1)^z, n even)
-    T acc = X[0].x;
-    int t = z;  // (k z) mod n, k = 1
-#pragma unroll
-    for (int k = 1; k <= kDirectP / 2; ++k) {
-      if (k < p.n_in) {
-        const T2 w = p.tw[t];  // e^{-2 pi i t/n}; the inverse uses its conjugate
-        const T re = X[k].x * w.x + X[k].y * w.y;
-        if (2 * k == n) acc = acc + re;
-        else acc = acc + 2 * re;
-        t += z;
-        if (t >= n) t -= n;
-      }
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int nt = blockDim.x, tid = threadIdx.x;
+  const int n = p.n, n_in = p.n_in, n_out = p.n_out;
+  const int SX = odd_stride(n_in), SZ = odd_stride(n_out);
+  T2 *tw = reinterpret_cast<T2 *>(smem);
+  T2 *xs = tw + n;
+  T *outs = reinterpret_cast<T *>(xs + (size_t)nt * SX);
+  int64_t *lidx = reinterpret_cast<int64_t *>(outs + (size_t)nt * SZ);
+  int64_t *lin = lidx + nt;
+  int *lcomp = reinterpret_cast<int *>(lin + nt);
+  const int64_t L0 = (int64_t)blockIdx.x * nt;
+  for (int t = tid; t < n; t += nt) tw[t] = p.tw[t];
+  {
+    const int64_t L = L0 + tid;
+    int comp = 0;
+    int64_t fb = -1, ib = -1;
+    if (L < p.nlines) {
+      fb = fft::spin_line(p, (uint32_t)L, comp);
+      ib = p.src.base32((uint32_t)L);
     }
-    (void)half;
-    const int64_t idx = (xy * p.grid.L[2] + z) * p.grid.nb + m;
-    p.field[comp * p.grid.n + idx] = p.grid.mus[idx] * (double)acc;
+    lidx[tid] = fb;
+    lin[tid] = ib;
+    lcomp[tid] = comp;
   }
+  __syncthreads();
+  const float rin = 1.0f / (float)n_in, rout = 1.0f / (float)n_out;
+  for (int e = tid; e < nt * n_in; e += nt) {
+    int q;
+    const int b = fft::divmod_small(e, n_in, rin, q);
+    const int64_t base = lin[b];
+    T2 v;
+    v.x = 0;
+    v.y = 0;
+    if (base >= 0) v = p.in[base + (int64_t)q * p.src.se];
+    xs[b * SX + q] = v;
+  }
+  __syncthreads();
+  {
+    T2 X[LM + 1];
+#pragma unroll
+    for (int q = 0; q <= LM; ++q)
+      if (q < n_in) X[q] = xs[tid * SX + q];
+    for (int z = 0; z < n_out; ++z) {
+      // Re sum_k X[k] e^{+2 pi i k z/n} over the Hermitian sequence:
+      // X[0] + 2 Re(X[k] w^{kz}) for 0 < k < n/2 (+ X[n/2] (-1)^z, n even)
+      T acc = X[0].x;
+      int t = z;  // (k z) mod n, k = 1
+#pragma unroll
+      for (int k = 1; k <= LM; ++k) {
+        if (k < n_in) {
+          const T2 w = tw[t];  // e^{-2 pi i t/n}; the inverse uses its conjugate
+          const T re = X[k].x * w.x + X[k].y * w.y;
+          if (2 * k == n) acc = acc + re;
+          else acc = acc + 2 * re;
+          t += z;
+          if (t >= n) t -= n;
+        }
+      }
+      outs[tid * SZ + z] = acc;
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < nt * n_out; e += nt) {
+    int z;
+    const int b = fft::divmod_small(e, n_out, rout, z);
+    const int64_t base = lidx[b];
+    if (base >= 0) {
+      const int64_t idx = base + (int64_t)z * p.grid.nb;
+      p.field[lcomp[b] * p.grid.n + idx] = p.grid.mus[idx] * (double)outs[b * SZ + z];
+    }
+  }
+}
+
+// threads (lines) per direct z workgroup: 256, or fewer so that LDS stays
+// within 64 KB
+template <class T>
+int direct_threads(int n, int n_in, int n_out, bool inverse) {
+  int t = 256;
+  while (t > 64 && direct_lds<T>(t, n, n_in, n_out, inverse) > 64 * 1024) t /= 2;
+  return t;
+}
+
+template <class T>
+hipError_t run_direct_z(const Pass<T> &p, bool inverse, hipStream_t st) {
+  if (p.nlines <= 0) return hipSuccess;
+  if (p.nlines >= ((int64_t)1 << 31)) return hipErrorInvalidValue;
+  const int t = direct_threads<T>(p.n, p.n_in, p.n_out, inverse);
+  const size_t lds = direct_lds<T>(t, p.n, p.n_in, p.n_out, inverse);
+  const unsigned blocks = (unsigned)((p.nlines + t - 1) / t);
+  const int lmax = std::max(inverse ? p.n_out : p.n_in, (p.n + 1) / 2);
+  if (lmax <= 8) {
+    if (inverse) hipLaunchKernelGGL((k_dip_z_direct_inv<T, 8>), dim3(blocks), dim3(t), lds, st, p);
+    else hipLaunchKernelGGL((k_dip_z_direct_fwd<T, 8>), dim3(blocks), dim3(t), lds, st, p);
+  } else if (lmax <= 16) {
+    if (inverse) hipLaunchKernelGGL((k_dip_z_direct_inv<T, 16>), dim3(blocks), dim3(t), lds, st, p);
+    else hipLaunchKernelGGL((k_dip_z_direct_fwd<T, 16>), dim3(blocks), dim3(t), lds, st, p);
+  } else {
+    if (inverse) hipLaunchKernelGGL((k_dip_z_direct_inv<T, 32>), dim3(blocks), dim3(t), lds, st, p);
+    else hipLaunchKernelGGL((k_dip_z_direct_fwd<T, 32>), dim3(blocks), dim3(t), lds, st, p);
+  }
+  return hipGetLastError();
 }
 
 // tensor set-up: the q2 < Q2 half of the transformed grid into the tensor
@@ -421,9 +542,7 @@ class DipoleFFTImpl : public DipoleFFT {
       p.out = pl.K.p;
       p.spins = s;
       if (direct_z()) {
-        hipLaunchKernelGGL((k_dip_z_direct_fwd<T>), dim3((unsigned)((p.nlines + 255) / 256)), dim3(256),
-                           0, st, p);
-        DCHK(hipGetLastError());
+        DCHK(run_direct_z<T>(p, false, st));
       } else {
         DCHK((run_pass<T, fft::LD_SPIN, fft::ST_CPLX>(p, st)));
       }
@@ -493,9 +612,7 @@ class DipoleFFTImpl : public DipoleFFT {
       p.in = pl.K.p;
       p.field = h;
       if (direct_z()) {
-        hipLaunchKernelGGL((k_dip_z_direct_inv<T>), dim3((unsigned)((p.nlines + 255) / 256)), dim3(256),
-                           0, st, p);
-        DCHK(hipGetLastError());
+        DCHK(run_direct_z<T>(p, true, st));
       } else {
         DCHK((run_pass<T, fft::LD_HERM, fft::ST_FIELD>(p, st)));
       }
