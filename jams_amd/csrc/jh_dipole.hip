@@ -20,6 +20,9 @@
 //   x pass   inverse, outputs x < L0 only;
 //   y pass   inverse, lines x < L0, outputs y < L1;
 //   z pass   c2r, fused with the gather h_i = mus_i Re(...), outputs z < L2.
+// After the z pass the grid is held as [g][q2][x][y] (y contiguous), so the
+// y transforms read and write whole contiguous lines and the x transforms
+// read rows of adjacent y lines.
 // The 1/P normalisation is folded into the tensor.  For one basis site W is
 // even in n (except for the half-way image of an even periodic length inside
 // r_cutoff), and the transformed tensor is then real: 6 values per k-point.
@@ -164,7 +167,8 @@ template <class T, int LM>
 __global__ void __launch_bounds__(256) k_dip_z_direct_fwd(const Pass<T> p) {
   using T2 = typename fft::C2<T>::type;
   extern __shared__ __align__(16) unsigned char smem[];
-  const int nt = blockDim.x, tid = threadIdx.x;
+  const int nt = blockDim.x, tid = threadIdx.x;  // nt: a power of two
+  const int lnt = 31 - __clz(nt);
   const int n = p.n, n_in = p.n_in, n_out = p.n_out;
   const int SV = odd_stride(n_in), SO = odd_stride(n_out);
   T2 *tw = reinterpret_cast<T2 *>(smem);
@@ -226,9 +230,12 @@ __global__ void __launch_bounds__(256) k_dip_z_direct_fwd(const Pass<T> p) {
     }
   }
   __syncthreads();
+  // grid side: element-fastest when q is contiguous, line-fastest otherwise
+  const bool qfast = p.dst.se == 1;
   for (int e = tid; e < nt * n_out; e += nt) {
-    int q;
-    const int b = fft::divmod_small(e, n_out, rout, q);
+    int q, b;
+    if (qfast) b = fft::divmod_small(e, n_out, rout, q);
+    else { q = e >> lnt; b = e & (nt - 1); }
     const int64_t base = lout[b];
     if (base >= 0) p.out[base + (int64_t)q * p.dst.se] = outs[b * SO + q];
   }
@@ -238,7 +245,8 @@ template <class T, int LM>
 __global__ void __launch_bounds__(256) k_dip_z_direct_inv(const Pass<T> p) {
   using T2 = typename fft::C2<T>::type;
   extern __shared__ __align__(16) unsigned char smem[];
-  const int nt = blockDim.x, tid = threadIdx.x;
+  const int nt = blockDim.x, tid = threadIdx.x;  // nt: a power of two
+  const int lnt = 31 - __clz(nt);
   const int n = p.n, n_in = p.n_in, n_out = p.n_out;
   const int SX = odd_stride(n_in), SZ = odd_stride(n_out);
   T2 *tw = reinterpret_cast<T2 *>(smem);
@@ -263,9 +271,11 @@ __global__ void __launch_bounds__(256) k_dip_z_direct_inv(const Pass<T> p) {
   }
   __syncthreads();
   const float rin = 1.0f / (float)n_in, rout = 1.0f / (float)n_out;
+  const bool qfast = p.src.se == 1;
   for (int e = tid; e < nt * n_in; e += nt) {
-    int q;
-    const int b = fft::divmod_small(e, n_in, rin, q);
+    int q, b;
+    if (qfast) b = fft::divmod_small(e, n_in, rin, q);
+    else { q = e >> lnt; b = e & (nt - 1); }
     const int64_t base = lin[b];
     T2 v;
     v.x = 0;
@@ -349,8 +359,9 @@ __global__ void __launch_bounds__(256) k_dip_extract(const double2 *F, int64_t P
   const int64_t M = rows * Q2;
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= M * ncomp) return;
+  // output [g][q2][row], row = x P1 + y; F is [g][row][P2]
   const int64_t g = q / M, r = q % M;
-  const double2 v = F[g * rows * P2 + (r / Q2) * P2 + r % Q2];
+  const double2 v = F[g * rows * P2 + (r % rows) * P2 + r / rows];
   if (real_only) {
     Wr[q] = (T)v.x;
   } else {
@@ -417,9 +428,9 @@ hipError_t run_pass(Pass<T> p, hipStream_t st) {
 template <class T>
 struct Plane {
   using T2 = typename fft::C2<T>::type;
-  DBuf<T2> K;            // [3 nb][P0][P1][Q2]
-  DBuf<T> Wr;            // nb == 1: [6][P0 P1 Q2]
-  DBuf<T2> Wc;           // nb > 1: [nb nb][6][P0 P1 Q2]
+  DBuf<T2> K;            // [3 nb][Q2][P0][P1]
+  DBuf<T> Wr;            // nb == 1: [6][Q2 P0 P1]
+  DBuf<T2> Wc;           // nb > 1: [nb nb][6][Q2 P0 P1]
   DBuf<T2> tw[3];
   bool ready = false;
 };
@@ -538,7 +549,7 @@ class DipoleFFTImpl : public DipoleFFT {
       p.n_in = L[2];
       p.n_out = (int)Q2;
       p.nlines = G * L[0] * L[1];
-      p.dst = LineMap{L[0], L[1], sa, (int64_t)P[1] * Q2, Q2, 1};
+      p.dst = LineMap{L[0], L[1], sa, P[1], 1, (int64_t)P[0] * P[1]};
       p.out = pl.K.p;
       p.spins = s;
       if (direct_z()) {
@@ -547,24 +558,24 @@ class DipoleFFTImpl : public DipoleFFT {
         DCHK((run_pass<T, fft::LD_SPIN, fft::ST_CPLX>(p, st)));
       }
     }
-    // y forward: lines (g, x < L0, q2)
+    // y forward: lines (g, q2, x < L0), contiguous
     {
       auto p = base_pass<T>(pl, 1, false);
       p.n_in = L[1];
       p.n_out = P[1];
       p.nlines = G * L[0] * Q2;
-      p.src = p.dst = LineMap{L[0], Q2, sa, (int64_t)P[1] * Q2, 1, Q2};
+      p.src = p.dst = LineMap{Q2, L[0], sa, (int64_t)P[0] * P[1], P[1], 1};
       p.in = pl.K.p;
       p.out = pl.K.p;
       DCHK((run_pass<T, fft::LD_CPLX, fft::ST_CPLX>(p, st)));
     }
-    // x forward: lines (g, y q2)
+    // x forward: lines (g, q2, y), stride P1
     {
       auto p = base_pass<T>(pl, 0, false);
       p.n_in = L[0];
       p.n_out = P[0];
       p.nlines = G * P[1] * Q2;
-      p.src = p.dst = LineMap{1, (int64_t)P[1] * Q2, sa, 0, 1, (int64_t)P[1] * Q2};
+      p.src = p.dst = LineMap{Q2, P[1], sa, (int64_t)P[0] * P[1], 1, P[1]};
       p.in = pl.K.p;
       p.out = pl.K.p;
       DCHK((run_pass<T, fft::LD_CPLX, fft::ST_CPLX>(p, st)));
@@ -586,18 +597,18 @@ class DipoleFFTImpl : public DipoleFFT {
       p.n_in = P[0];
       p.n_out = L[0];
       p.nlines = G * P[1] * Q2;
-      p.src = p.dst = LineMap{1, (int64_t)P[1] * Q2, sa, 0, 1, (int64_t)P[1] * Q2};
+      p.src = p.dst = LineMap{Q2, P[1], sa, (int64_t)P[0] * P[1], 1, P[1]};
       p.in = pl.K.p;
       p.out = pl.K.p;
       DCHK((run_pass<T, fft::LD_CPLX, fft::ST_CPLX>(p, st)));
     }
-    // y inverse: lines (g, x < L0, q2), outputs y < L1
+    // y inverse: lines (g, q2, x < L0), outputs y < L1
     {
       auto p = base_pass<T>(pl, 1, true);
       p.n_in = P[1];
       p.n_out = L[1];
       p.nlines = G * L[0] * Q2;
-      p.src = p.dst = LineMap{L[0], Q2, sa, (int64_t)P[1] * Q2, 1, Q2};
+      p.src = p.dst = LineMap{Q2, L[0], sa, (int64_t)P[0] * P[1], P[1], 1};
       p.in = pl.K.p;
       p.out = pl.K.p;
       DCHK((run_pass<T, fft::LD_CPLX, fft::ST_CPLX>(p, st)));
@@ -608,7 +619,7 @@ class DipoleFFTImpl : public DipoleFFT {
       p.n_in = (int)Q2;
       p.n_out = L[2];
       p.nlines = G * L[0] * L[1];
-      p.src = LineMap{L[0], L[1], sa, (int64_t)P[1] * Q2, Q2, 1};
+      p.src = LineMap{L[0], L[1], sa, P[1], 1, (int64_t)P[0] * P[1]};
       p.in = pl.K.p;
       p.field = h;
       if (direct_z()) {
