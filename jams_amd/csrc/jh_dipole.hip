@@ -374,6 +374,12 @@ template <class T>
 size_t pass_lds(int n, int B, bool pingpong) {
   return (pingpong ? 2 : 1) * (size_t)n * B * sizeof(typename fft::C2<T>::type);
 }
+// the same plus the twiddle table (at most 64 KB + 32 KB: gfx950 gives a
+// workgroup up to 160 KB)
+template <class T>
+size_t pass_lds_total(int n, int B, bool pingpong) {
+  return pass_lds<T>(n, B, pingpong) + (size_t)n * sizeof(typename fft::C2<T>::type);
+}
 
 // Lines per workgroup: as many as LDS (64 KB) and, for in-place plans, the
 // per-thread butterfly budget of every stage allow (at most 64).
@@ -413,8 +419,15 @@ hipError_t run_pass(Pass<T> p, hipStream_t st) {
     p.tws[s] = p.n / (Ns * p.radix[s]);
     Ns *= p.radix[s];
   }
-  const size_t lds = pass_lds<T>(p.n, p.B, p.pingpong);
+  const size_t lds = pass_lds_total<T>(p.n, p.B, p.pingpong);
   const int64_t blocks = (p.nlines + p.B - 1) / p.B;
+  // above 64 KB of dynamic LDS the kernel has to be told so
+  const void *fn = p.inverse ? reinterpret_cast<const void *>(&fft::k_fft_pass<T, LD, ST, true>)
+                             : reinterpret_cast<const void *>(&fft::k_fft_pass<T, LD, ST, false>);
+  if (lds > 64 * 1024) {
+    const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
   if (p.inverse)
     hipLaunchKernelGGL((fft::k_fft_pass<T, LD, ST, true>), dim3((unsigned)blocks),
                        dim3(fft::kThreads), lds, st, p);

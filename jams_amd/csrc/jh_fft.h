@@ -192,9 +192,10 @@ __device__ __forceinline__ int divmod_small(int x, int d, float rd, int &r) {
   return q;
 }
 
+// twiddle t from the workgroup's LDS copy of the table
 template <class T, bool INV>
-__device__ __forceinline__ typename C2<T>::type twiddle(const Pass<T> &p, int t) {
-  typename C2<T>::type w = p.tw[t];
+__device__ __forceinline__ typename C2<T>::type twiddle(const typename C2<T>::type *tw, int t) {
+  typename C2<T>::type w = tw[t];
   if (INV) w.y = -w.y;
   return w;
 }
@@ -202,7 +203,8 @@ __device__ __forceinline__ typename C2<T>::type twiddle(const Pass<T> &p, int t)
 // One Stockham stage over B lines: in/out are LDS arrays [k][b].  Butterfly
 // t is line b = t mod B, index j = t / B < n / R.
 template <class T, int R, bool INV>
-__device__ __forceinline__ void stage_fixed(const Pass<T> &p, const typename C2<T>::type *in,
+__device__ __forceinline__ void stage_fixed(const Pass<T> &p, const typename C2<T>::type *tw,
+                                            const typename C2<T>::type *in,
                                             typename C2<T>::type *out, int st) {
   using T2 = typename C2<T>::type;
   const int n = p.n, B = p.B, lb = p.lb, m = n / R;
@@ -216,7 +218,7 @@ __device__ __forceinline__ void stage_fixed(const Pass<T> &p, const typename C2<
 #pragma unroll
     for (int r = 0; r < R; ++r) v[r] = in[((j + r * m) << lb) + b];
 #pragma unroll
-    for (int r = 1; r < R; ++r) v[r] = cmul(v[r], twiddle<T, INV>(p, jm * r * tws));
+    for (int r = 1; r < R; ++r) v[r] = cmul(v[r], twiddle<T, INV>(tw, jm * r * tws));
     dft_fixed<T2, R>(v, INV);
     const int d = jd * Ns * R + jm;
 #pragma unroll
@@ -234,7 +236,8 @@ constexpr int kBF = (8 / R) > 0 ? 8 / R : 1;
 // transforms its butterflies (at most kBF<T, R>, which the host guarantees by
 // its choice of B), then all write.
 template <class T, int R, bool INV>
-__device__ __forceinline__ void stage_inplace(const Pass<T> &p, typename C2<T>::type *A, int st) {
+__device__ __forceinline__ void stage_inplace(const Pass<T> &p, const typename C2<T>::type *tw,
+                                              typename C2<T>::type *A, int st) {
   using T2 = typename C2<T>::type;
   constexpr int NBF = kBF<T, R>;
   const int n = p.n, B = p.B, lb = p.lb, m = n / R, total = m << lb;
@@ -253,7 +256,7 @@ __device__ __forceinline__ void stage_inplace(const Pass<T> &p, typename C2<T>::
 #pragma unroll
       for (int r = 0; r < R; ++r) v[q][r] = A[((j + r * m) << lb) + b];
 #pragma unroll
-      for (int r = 1; r < R; ++r) v[q][r] = cmul(v[q][r], twiddle<T, INV>(p, jm * r * tws));
+      for (int r = 1; r < R; ++r) v[q][r] = cmul(v[q][r], twiddle<T, INV>(tw, jm * r * tws));
       dft_fixed<T2, R>(v[q], INV);
     }
   }
@@ -271,8 +274,9 @@ __device__ __forceinline__ void stage_inplace(const Pass<T> &p, typename C2<T>::
 
 // Any other factor R: direct R-point DFT with roots from the n-table.
 template <class T, bool INV>
-__device__ void stage_generic(const Pass<T> &p, const typename C2<T>::type *in,
-                              typename C2<T>::type *out, int st, int R) {
+__device__ void stage_generic(const Pass<T> &p, const typename C2<T>::type *tw,
+                              const typename C2<T>::type *in, typename C2<T>::type *out, int st,
+                              int R) {
   using T2 = typename C2<T>::type;
   const int n = p.n, B = p.B, lb = p.lb, m = n / R;
   const int Ns = p.ns[st], tws = p.tws[st];
@@ -289,8 +293,8 @@ __device__ void stage_generic(const Pass<T> &p, const typename C2<T>::type *in,
       acc.y = 0;
       for (int r = 0; r < R; ++r) {
         T2 x = in[((j + r * m) << lb) + b];
-        if (r > 0) x = cmul(x, twiddle<T, INV>(p, jm * r * tws));
-        acc = cadd(acc, cmul(x, twiddle<T, INV>(p, ((q * r) % R) * root)));
+        if (r > 0) x = cmul(x, twiddle<T, INV>(tw, jm * r * tws));
+        acc = cadd(acc, cmul(x, twiddle<T, INV>(tw, ((q * r) % R) * root)));
       }
       out[((d + q * Ns) << lb) + b] = acc;
     }
@@ -335,6 +339,10 @@ __global__ void __launch_bounds__(kThreads) k_fft_pass(const Pass<T> p) {
   T2 *Bf = A + (size_t)p.n * p.B;
   const int n = p.n, lb = p.lb;
   const int64_t L0 = (int64_t)blockIdx.x << lb;
+  // the twiddle table follows the line buffer(s); the barrier after the
+  // load makes it visible
+  T2 *tw = (p.pingpong ? Bf : A) + ((size_t)n << lb);
+  for (int t = threadIdx.x; t < n; t += kThreads) tw[t] = p.tw[t];
 
   // ---- load: each thread keeps one line; its address is computed once ----
   {
@@ -378,23 +386,23 @@ __global__ void __launch_bounds__(kThreads) k_fft_pass(const Pass<T> p) {
   if (!p.pingpong) {
     for (int st = 0; st < p.nst; ++st) {
       switch (p.radix[st]) {
-        case 2: stage_inplace<T, 2, INV>(p, A, st); break;
-        case 3: stage_inplace<T, 3, INV>(p, A, st); break;
-        case 4: stage_inplace<T, 4, INV>(p, A, st); break;
-        case 5: stage_inplace<T, 5, INV>(p, A, st); break;
-        default: stage_inplace<T, 8, INV>(p, A, st); break;
+        case 2: stage_inplace<T, 2, INV>(p, tw, A, st); break;
+        case 3: stage_inplace<T, 3, INV>(p, tw, A, st); break;
+        case 4: stage_inplace<T, 4, INV>(p, tw, A, st); break;
+        case 5: stage_inplace<T, 5, INV>(p, tw, A, st); break;
+        default: stage_inplace<T, 8, INV>(p, tw, A, st); break;
       }
     }
   } else {
     for (int st = 0; st < p.nst; ++st) {
       const int R = p.radix[st];
       switch (R) {
-        case 2: stage_fixed<T, 2, INV>(p, in, out, st); break;
-        case 3: stage_fixed<T, 3, INV>(p, in, out, st); break;
-        case 4: stage_fixed<T, 4, INV>(p, in, out, st); break;
-        case 5: stage_fixed<T, 5, INV>(p, in, out, st); break;
-        case 8: stage_fixed<T, 8, INV>(p, in, out, st); break;
-        default: stage_generic<T, INV>(p, in, out, st, R); break;
+        case 2: stage_fixed<T, 2, INV>(p, tw, in, out, st); break;
+        case 3: stage_fixed<T, 3, INV>(p, tw, in, out, st); break;
+        case 4: stage_fixed<T, 4, INV>(p, tw, in, out, st); break;
+        case 5: stage_fixed<T, 5, INV>(p, tw, in, out, st); break;
+        case 8: stage_fixed<T, 8, INV>(p, tw, in, out, st); break;
+        default: stage_generic<T, INV>(p, tw, in, out, st, R); break;
       }
       __syncthreads();
       T2 *tmp = in;
