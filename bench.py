@@ -274,14 +274,19 @@ def copy_calibration(nbytes: int = 1 << 30, reps: int = 5) -> int:
     return 0
 
 
-def read_traffic(name: str):
+def read_traffic(name: str, phase: str):
+    """Counter-measured HBM bytes per launch of the phase kernel, from the
+    committed FETCH_SIZE / WRITE_SIZE summary (None if absent)."""
     p = ROOT / "profiles" / name
     if not p.exists():
         return None
     try:
-        return json.loads(p.read_text()).get("hbm_bytes_per_launch")
-    except Exception:
+        d = json.loads(p.read_text())
+    except ValueError:
         return None
+    if phase == "corrector":
+        d = d.get("corrector", {})
+    return d.get("hbm_bytes_per_launch")
 
 
 def main() -> int:
@@ -345,11 +350,11 @@ def main() -> int:
         log(f"  {cpu['value']:.4e} spin-updates/s on {cpu['cores']} threads")
 
     dom = max(fp64["phases"], key=lambda p: p["ms"])
-    # counter-measured HBM bytes of the predictor kernel of this workload
-    # (k_heun<FP64, pattern-diag, predictor>); null for any other kernel
+    # counter-measured HBM bytes of the dominant phase kernel of this workload
+    # (k_heun<FP64, pattern-diag, predictor or corrector>); null otherwise
     traffic = None
-    if dom["phase"] == "predictor" and "exchange=pattern-diag " in fp64["describe"]:
-        traffic = read_traffic("traffic_c2_fp64.json")
+    if "exchange=pattern-diag " in fp64["describe"] and args.L == 64:
+        traffic = read_traffic("traffic_c2_fp64.json", dom["phase"])
     line = {
         "metric": METRIC,
         "value": fp64["value"],
