@@ -49,6 +49,13 @@ extern "C" {
                                  FP32 dipole transforms; FP64 Heun state,
                                  renormalisation, noise and reductions         */
 
+/* integrators (the solver module that selects them) */
+#define JH_INTEGRATOR_HEUN 0  /* llg-heun-gpu: predictor + corrector, two field
+                                 evaluations per step (the default)            */
+#define JH_INTEGRATOR_RK4 1   /* llg-rk4-gpu: classical RK4, four field
+                                 evaluations per step, one noise draw per step,
+                                 renormalised once at the end of the step      */
+
 /* exchange storage formats on the device */
 #define JH_XFMT_AUTO 0        /* PATTERN, else TYPED, else BLOCK, whichever fits  */
 #define JH_XFMT_TYPED 1       /* per neighbour: column | tensor id in 32 bits    */
@@ -149,8 +156,12 @@ int jh_get_noise(jh_ctx *ctx, int64_t step, double *w);
 /* ---- engine options ----------------------------------------------------- */
 int jh_set_precision(jh_ctx *ctx, int mode);
 int jh_set_exchange_format(jh_ctx *ctx, int fmt);
+/* Solver::create's choice between CudaHeunLLGSolver and CUDALLGRK4Solver
+ * (core/solver.cc:61-73; solvers/cuda_llg_rk4.cu:17-35 over
+ * solvers/cuda_rk4_base.cu:57-108).  JH_INTEGRATOR_HEUN by default. */
+int jh_set_integrator(jh_ctx *ctx, int integrator);
 
-/* ---- stepping (Solver::run, one call = nsteps Heun steps) --------------- */
+/* ---- stepping (Solver::run, one call = nsteps steps of the integrator) --- */
 int jh_step(jh_ctx *ctx, int64_t nsteps);
 int jh_get_iteration(const jh_ctx *ctx, int64_t *iteration, double *time_ps);
 /* Resume: iteration counter (the thermostat stream position) and time. */

@@ -163,6 +163,8 @@ struct jh_ctx {
   // state
   DevBuf<double> s, sp, ds, winj, wbuf, hext, fout;
   DevBuf<float> s32, sp32;
+  DevBuf<double> sq;        // RK4: second stage-input buffer (stages alternate sp, sq)
+  DevBuf<float> sq32;
   DevBuf<double> red_partial, red_out;
   std::vector<double> h_stage;   // host staging for AoS <-> SoA
   bool spins_set = false;
@@ -184,6 +186,7 @@ struct jh_ctx {
 
   // options
   int prec = PR_FP64;
+  int integrator = JH_INTEGRATOR_HEUN;
   int xfmt_req = JH_XFMT_AUTO;
 
   // prepared device state
@@ -868,6 +871,13 @@ void prepare(jh_ctx *c) {
   prepare_exchange(c);
   prepare_classes(c);
   if (c->noise_mode == NM_PHILOX) c->wbuf.alloc((size_t)(3 * c->n));
+  if (c->integrator == JH_INTEGRATOR_RK4) {
+    c->sq.alloc((size_t)(3 * c->n));
+    if (c->prec == PR_FP32) c->sq32.alloc((size_t)(kG32 * c->n));
+  } else {
+    c->sq.release();
+    c->sq32.release();
+  }
   sync_f32(c);
   HIPCHK(hipStreamSynchronize(c->stream));
   c->dirty = false;
@@ -921,9 +931,33 @@ KParams make_params(jh_ctx *c) {
 }
 
 
+// One RK4 step's launches: stage k evaluates the fields at its input (s,
+// then sp, sq, sp) and writes the next input (sp, sq, sp) or, at stage 3, s.
+void enqueue_rk4(jh_ctx *c, KParams p) {
+  const int dip = dipole_term(c);
+  const bool f32 = c->prec == PR_FP32;
+  double *in64[4] = {c->s.p, c->sp.p, c->sq.p, c->sp.p};
+  float *in32[4] = {c->s32.p, c->sp32.p, c->sq32.p, c->sp32.p};
+  double *out64[3] = {c->sp.p, c->sq.p, c->sp.p};
+  float *out32[4] = {c->sp32.p, c->sq32.p, c->sp32.p, c->s32.p};
+  for (int k = 0; k < 4; ++k) {
+    if (dip >= 0) HIPCHK(c->dipoles[dip]->field(in64[k], c->hext.p, f32, c->stream));
+    p.stage = k;
+    p.sp = in64[k];
+    p.sy = k < 3 ? out64[k] : nullptr;
+    p.g32 = in32[k];
+    p.w32 = out32[k];
+    HIPCHK(launch_rk4(p, c->prec, c->xv, c->launch_block(), c->stream));
+  }
+}
+
 // One Heun step's launches (dipole fields, predictor, corrector) for step
 // counter / time in p (or on the device when p.dstep is set).
 void enqueue_step(jh_ctx *c, KParams p) {
+  if (c->integrator == JH_INTEGRATOR_RK4) {
+    enqueue_rk4(c, p);
+    return;
+  }
   const int dip = dipole_term(c);
   if (dip >= 0) HIPCHK(c->dipoles[dip]->field(c->s.p, c->hext.p, c->prec == PR_FP32, c->stream));
   p.g32 = c->s32.p;
@@ -1272,6 +1306,16 @@ int jh_set_precision(jh_ctx *c, int mode) {
   });
 }
 
+int jh_set_integrator(jh_ctx *c, int integrator) {
+  return guarded(c, [&] {
+    if (integrator != JH_INTEGRATOR_HEUN && integrator != JH_INTEGRATOR_RK4)
+      JH_THROW(JH_ERR_INVALID, "unknown integrator");
+    c->integrator = integrator;
+    c->dirty = true;
+    c->drop_graph();
+  });
+}
+
 int jh_set_exchange_format(jh_ctx *c, int fmt) {
   return guarded(c, [&] {
     if (fmt != JH_XFMT_AUTO && fmt != JH_XFMT_TYPED && fmt != JH_XFMT_BLOCK && fmt != JH_XFMT_PATTERN)
@@ -1373,6 +1417,8 @@ int jh_time_phase(jh_ctx *c, int phase, int reps, double *ms) {
   return guarded(c, [&] {
     if (!c->have_dt) JH_THROW(JH_ERR_STATE, "time step not set");
     if (reps <= 0 || (phase != 0 && phase != 1)) JH_THROW(JH_ERR_INVALID, "bad phase or reps");
+    if (c->integrator != JH_INTEGRATOR_HEUN)
+      JH_THROW(JH_ERR_UNSUPPORTED, "phase timing covers the Heun phases; time RK4 with jh_time_steps");
     prepare(c);
     // The corrector overwrites s; keep a copy and restore it afterwards so the
     // timing leaves the simulation state unchanged.
@@ -1626,12 +1672,13 @@ int jh_describe(jh_ctx *c, char *buf, int64_t buflen) {
     std::snprintf(tmp, sizeof tmp,
                   "n=%lld prec=%s exchange=%s entries=%lld tensors=%d colbits=%d patterns=%d "
                   "pattern_words=%d lds=%d uniform_slices=%lld/%lld order=%d classes=%d "
-                  "terms=%d noise=%d block=%d replicas=%d graph=%d",
+                  "terms=%d noise=%d block=%d replicas=%d graph=%d integrator=%s",
                   (long long)c->n, c->prec == PR_FP64 ? "fp64" : "fp32-field", xvn[c->xv],
                   (long long)c->nent, c->ntab > 0 ? c->ntab - 1 : 0, c->colbits,
                   c->npatterns, c->pat_words, c->pat_lds, (long long)c->nuniform,
                   (long long)c->nslices, c->order_nb, c->nclass,
-                  (int)c->terms.size(), c->noise_mode, c->launch_block(), c->nrep, c->graph_steps);
+                  (int)c->terms.size(), c->noise_mode, c->launch_block(), c->nrep, c->graph_steps,
+                  c->integrator == JH_INTEGRATOR_RK4 ? "rk4" : "heun");
     if (buf && buflen > 0) {
       std::strncpy(buf, tmp, (size_t)buflen - 1);
       buf[buflen - 1] = 0;

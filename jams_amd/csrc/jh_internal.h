@@ -108,6 +108,8 @@ struct KParams {
   double *sp;          // predictor spins
   double *ds;          // ds/dt accumulator
   double *snew;        // phase 1 output (== s)
+  double *sy;          // RK4: the next stage's input written by stages 0-2
+  int stage;           // RK4 stage 0..3
   const float *g32;    // FP32 gather source, kG32 floats per spin (s32 / sp32)
   float *w32;          // FP32 copy written by the phase (sp32 / s32)
   const uint16_t *cls; // per-spin class or null

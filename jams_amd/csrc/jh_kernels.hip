@@ -459,6 +459,91 @@ __attribute__((amdgpu_waves_per_eu(kMinWaves<PREC, XVk>))) k_heun(const KParams 
   }
 }
 
+// One stage of the classical RK4 step of llg-rk4-gpu
+// (solvers/cuda_rk4_base.cu:57-108, cuda_llg_rk4_kernel.cuh:8-58,
+// cuda_rk4_base_kernel.cuh:1-18, cuda/cuda_spin_ops.cu:4-17):
+//   k = rhs(y, h(y, t) / mus + w) at the stage input y (y = s at stage 0,
+//   else p.sp), t = t0, t0 + dt/2, t0 + dt/2, t0 + dt, with the step's noise
+//   w drawn at stage 0 and reused;
+//   stages 0-2 write the next input y' = s + c k, c = dt/2, dt/2, dt, to p.sy
+//   (the reference's copy + axpy, without renormalisation);
+//   the weighted sum k1 + 2 k2 + 2 k3 + k4 accumulates in p.ds left to right,
+//   as the reference's combination is written;
+//   stage 3 writes s = v / sqrt(v.v), v = s + (dt sum) / 6 (the reference
+//   multiplies by CUDA's rsqrt; here 1 / sqrt, each correctly rounded).
+template <int PREC, int XVk, bool GEN>
+__global__ void __launch_bounds__(kMaxBlock)
+__attribute__((amdgpu_waves_per_eu(kMinWaves<PREC, XVk>))) k_rk4(const KParams p) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  load_table<PREC, XVk>(p.x, lds);
+  if constexpr (xv_typed(XVk)) __syncthreads();
+  const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t i = blk * blockDim.x + threadIdx.x;
+  if (i >= p.n) return;
+  const int64_t n = p.n;
+  const uint32_t c = p.cls ? (uint32_t)p.cls[i] : 0u;
+  const int stage = p.stage;
+  const uint64_t step = p.dstep ? *p.dstep + p.step : p.step;
+  const double t0 = p.dstep ? (double)step * p.dt : p.t;
+  const double half = 0.5 * p.dt;
+  const double t = stage == 0 ? t0 : stage == 3 ? t0 + p.dt : t0 + half;
+
+  const double *src = stage == 0 ? p.s : p.sp;
+  const double y0 = src[i], y1 = src[n + i], y2 = src[2 * n + i];
+  double h0, h1, h2;
+  total_field<PREC, XVk, GEN>(p, src, p.g32, i, c, lds, y0, y1, y2, t, h0, h1, h2);
+
+  double w0 = 0.0, w1 = 0.0, w2 = 0.0;
+  if (p.noise == NM_PHILOX && stage == 0) {
+    double z0, z1, z2, sT;
+    if (p.cls) {
+      normals3(noise_seed(p, c), step, noise_index(p, c, i), z0, z1, z2);
+      sT = cparam(p, c, CF_SQRTT);
+    } else {
+      normals3(p.seed, step, (uint64_t)ref_index(p, i) - p.roff, z0, z1, z2);
+      sT = p.sqrtT;
+    }
+    const double sg = cparam(p, c, CF_SIGMA);
+    w0 = z0 * sg * sT; w1 = z1 * sg * sT; w2 = z2 * sg * sT;
+    p.wbuf[i] = w0; p.wbuf[n + i] = w1; p.wbuf[2 * n + i] = w2;
+  } else if (p.noise == NM_PHILOX) {
+    w0 = p.wbuf[i]; w1 = p.wbuf[n + i]; w2 = p.wbuf[2 * n + i];
+  } else if (p.noise == NM_INJECTED) {
+    w0 = p.winj[i]; w1 = p.winj[n + i]; w2 = p.winj[2 * n + i];
+  }
+  const double mus = cparam(p, c, CF_MUS);
+  if (p.noise == NM_NONE) {
+    h0 = h0 / mus; h1 = h1 / mus; h2 = h2 / mus;
+  } else {
+    h0 = h0 / mus + w0; h1 = h1 / mus + w1; h2 = h2 / mus + w2;
+  }
+  double r0, r1, r2;
+  llg_rhs(y0, y1, y2, h0, h1, h2, cparam(p, c, CF_GYRO), cparam(p, c, CF_ALPHA), r0, r1, r2);
+
+  const double s0 = p.s[i], s1 = p.s[n + i], s2 = p.s[2 * n + i];
+  if (stage < 3) {
+    if (stage == 0) {
+      p.ds[i] = r0; p.ds[n + i] = r1; p.ds[2 * n + i] = r2;
+    } else {
+      p.ds[i] = p.ds[i] + 2.0 * r0;
+      p.ds[n + i] = p.ds[n + i] + 2.0 * r1;
+      p.ds[2 * n + i] = p.ds[2 * n + i] + 2.0 * r2;
+    }
+    const double cst = stage == 2 ? p.dt : half;
+    const double v0 = s0 + cst * r0, v1 = s1 + cst * r1, v2 = s2 + cst * r2;
+    p.sy[i] = v0; p.sy[n + i] = v1; p.sy[2 * n + i] = v2;
+    if constexpr (PREC == PR_FP32) store_g32(p.w32, i, v0, v1, v2);
+  } else {
+    double v0 = s0 + (p.dt * (p.ds[i] + r0)) / 6.0;
+    double v1 = s1 + (p.dt * (p.ds[n + i] + r1)) / 6.0;
+    double v2 = s2 + (p.dt * (p.ds[2 * n + i] + r2)) / 6.0;
+    const double rn = 1.0 / sqrt(v0 * v0 + v1 * v1 + v2 * v2);
+    v0 = v0 * rn; v1 = v1 * rn; v2 = v2 * rn;
+    p.snew[i] = v0; p.snew[n + i] = v1; p.snew[2 * n + i] = v2;
+    if constexpr (PREC == PR_FP32) store_g32(p.w32, i, v0, v1, v2);
+  }
+}
+
 // Field of one term (term >= 0) or of the ordered sum (term < 0) at spins p.s,
 // time p.t, into out (SoA).
 template <int PREC, int XVk>
@@ -634,6 +719,46 @@ hipError_t launch_heun(const KParams &p, int prec, int xv, int phase, int bs, hi
   if (bs < 64 || bs > kMaxBlock || bs % 64 != 0) return hipErrorInvalidValue;
   return prec == PR_FP64 ? launch_heun_p<PR_FP64>(p, xv, phase, bs, st)
                          : launch_heun_p<PR_FP32>(p, xv, phase, bs, st);
+}
+
+template <int PREC, int XVk>
+static hipError_t launch_rk4_t(const KParams &p, int bs, hipStream_t st) {
+  const int64_t nb = (p.n + bs - 1) / bs;
+  const size_t lds = table_lds(p, PREC, XVk);
+  if (p.has_ac || p.ani_power != 2)
+    hipLaunchKernelGGL((k_rk4<PREC, XVk, true>), dim3((unsigned)nb), dim3(bs), lds, st, p);
+  else
+    hipLaunchKernelGGL((k_rk4<PREC, XVk, false>), dim3((unsigned)nb), dim3(bs), lds, st, p);
+  return hipGetLastError();
+}
+
+template <int PREC>
+static hipError_t launch_rk4_p(const KParams &p, int xv, int bs, hipStream_t st) {
+  switch (xv) {
+    case XV_NONE: return launch_rk4_t<PREC, XV_NONE>(p, bs, st);
+    case XV_TDIAG: return launch_rk4_t<PREC, XV_TDIAG>(p, bs, st);
+    case XV_TFULL: return launch_rk4_t<PREC, XV_TFULL>(p, bs, st);
+    case XV_BDIAG: return launch_rk4_t<PREC, XV_BDIAG>(p, bs, st);
+    case XV_BFULL: return launch_rk4_t<PREC, XV_BFULL>(p, bs, st);
+    case XV_PDIAG: return launch_rk4_t<PREC, XV_PDIAG>(p, bs, st);
+    case XV_PFULL: return launch_rk4_t<PREC, XV_PFULL>(p, bs, st);
+    case XV_GDIAG: return launch_rk4_t<PREC, XV_GDIAG>(p, bs, st);
+    case XV_GFULL: return launch_rk4_t<PREC, XV_GFULL>(p, bs, st);
+    case XV_UDIAG:
+      if constexpr (PREC == PR_FP32) return launch_rk4_t<PREC, XV_UDIAG>(p, bs, st);
+      break;
+    case XV_UFULL:
+      if constexpr (PREC == PR_FP32) return launch_rk4_t<PREC, XV_UFULL>(p, bs, st);
+      break;
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_rk4(const KParams &p, int prec, int xv, int bs, hipStream_t st) {
+  if (bs < 64 || bs > kMaxBlock || bs % 64 != 0 || p.stage < 0 || p.stage > 3)
+    return hipErrorInvalidValue;
+  return prec == PR_FP64 ? launch_rk4_p<PR_FP64>(p, xv, bs, st)
+                         : launch_rk4_p<PR_FP32>(p, xv, bs, st);
 }
 
 hipError_t launch_reduce_replicas(const KParams &p, const int64_t *off, int nrep, double *out,

@@ -9,6 +9,8 @@ namespace jh {
 
 // bs: threads per workgroup of the step kernels (64 .. 512, multiple of 64)
 hipError_t launch_heun(const KParams &p, int prec, int xv, int phase, int bs, hipStream_t st);
+// one RK4 stage (p.stage)
+hipError_t launch_rk4(const KParams &p, int prec, int xv, int bs, hipStream_t st);
 hipError_t launch_reduce_replicas(const KParams &p, const int64_t *off, int nrep, double *out,
                                   hipStream_t st);
 hipError_t launch_set_step(uint64_t *step, uint64_t value, hipStream_t st);

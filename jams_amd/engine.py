@@ -17,6 +17,8 @@ ENGINE_LIB = LIB_DIR / "libjamship.so"
 JH_OK = 0
 PREC_FP64 = 0
 PREC_FP32_FIELD = 1
+INTEGRATOR_HEUN = 0
+INTEGRATOR_RK4 = 1
 XFMT_AUTO, XFMT_TYPED, XFMT_BLOCK, XFMT_PATTERN = 0, 1, 2, 3
 TERM_EXCHANGE, TERM_ZEEMAN, TERM_UNIAXIAL, TERM_DIPOLE = 0, 1, 2, 3
 
@@ -46,6 +48,7 @@ ABI = {
     "jh_set_noise": (C.c_int, [C.c_void_p, C.c_void_p]),
     "jh_get_noise": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p]),
     "jh_set_precision": (C.c_int, [C.c_void_p, C.c_int]),
+    "jh_set_integrator": (C.c_int, [C.c_void_p, C.c_int]),
     "jh_set_exchange_format": (C.c_int, [C.c_void_p, C.c_int]),
     "jh_step": (C.c_int, [C.c_void_p, C.c_int64]),
     "jh_get_iteration": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
@@ -219,6 +222,10 @@ class Engine:
 
     def set_precision(self, mode: int):
         self._chk(self.lib.jh_set_precision(self.h, int(mode)))
+
+    def set_integrator(self, integrator: int):
+        """JH_INTEGRATOR_HEUN (0, llg-heun-gpu) or JH_INTEGRATOR_RK4 (1, llg-rk4-gpu)."""
+        self._chk(self.lib.jh_set_integrator(self.h, int(integrator)))
 
     def set_exchange_format(self, fmt: int):
         self._chk(self.lib.jh_set_exchange_format(self.h, int(fmt)))

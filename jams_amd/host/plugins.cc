@@ -325,12 +325,13 @@ void DipoleFFTHamiltonian::register_term(jh_ctx *ctx) {
 // ---- solver ----------------------------------------------------------------------
 
 std::unique_ptr<Solver> Solver::create(const Setting &s) {
-  // core/solver.cc:60-76: the GPU Heun solver under its reference name and
-  // an explicit alias
+  // core/solver.cc:60-76: the GPU Heun and RK4 LLG solvers under their
+  // reference names and explicit aliases
   const std::string m = lowercase(s["module"].as_string());
   if (m == "llg-heun-gpu" || m == "llg-heun-hip") return std::make_unique<HipHeunLLGSolver>();
+  if (m == "llg-rk4-gpu" || m == "llg-rk4-hip") return std::make_unique<HipRK4LLGSolver>();
   throw RuntimeError("unknown solver " + s["module"].as_string() +
-                     " (this build provides llg-heun-gpu / llg-heun-hip)");
+                     " (this build provides llg-heun-gpu / llg-heun-hip, llg-rk4-gpu / llg-rk4-hip)");
   return nullptr;
 }
 

@@ -99,6 +99,8 @@ class Solver {
   virtual void run_steps(int64_t k);      // k steps with no monitor in between
   virtual std::string name() const = 0;
   virtual bool is_cuda_solver() const { return true; }
+  // the engine integrator this solver drives (JH_INTEGRATOR_*)
+  virtual int integrator() const { return JH_INTEGRATOR_HEUN; }
   bool is_running() const { return iteration_ < max_steps_; }
   int64_t iteration() const { return iteration_; }
   double time() const { return time_; }
@@ -143,6 +145,15 @@ class HipHeunLLGSolver : public Solver {
   void initialize(const Setting &s) override;
   void run() override { run_steps(1); }
   std::string name() const override { return "llg-heun-gpu"; }
+};
+
+// CUDALLGRK4Solver (solvers/cuda_llg_rk4.cu over solvers/cuda_rk4_base.cu) on
+// the same engine: the settings it reads (cuda_rk4_base.cu:10-48) are the Heun
+// solver's, the engine runs the RK4 stages
+class HipRK4LLGSolver : public HipHeunLLGSolver {
+ public:
+  std::string name() const override { return "llg-rk4-gpu"; }
+  int integrator() const override { return JH_INTEGRATOR_RK4; }
 };
 
 class Monitor {

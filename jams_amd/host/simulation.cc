@@ -117,6 +117,7 @@ void initialize(jams_sim *s, const std::vector<std::string> &sources, int device
     if (p != "fp64" && p != "fp32-field")
       throw ConfigError("solver.precision must be \"fp64\" or \"fp32-field\"");
     engine_check(s->ctx, jh_set_precision(s->ctx, p == "fp32-field" ? JH_PREC_FP32_FIELD : JH_PREC_FP64));
+    engine_check(s->ctx, jh_set_integrator(s->ctx, s->solver->integrator()));
   }
   if (cfg.exists("monitors") && s->ctx) {
     const Setting &ms = cfg.lookup("monitors");

@@ -24,6 +24,10 @@
  *   or_heun_step         macrospin closed form (precession at gamma*B, damping
  *                        tan(theta/2) = tan(theta0/2) exp(-alpha*gamma*B*t)),
  *                        |s| = 1 invariant, energy non-increase at T = 0.
+ *   or_rk4_step          the same macrospin closed form with fourth-order
+ *                        convergence (error ratio ~16 per halving of dt;
+ *                        a wrong stage weight or time drops the order);
+ *                        |s| = 1; agreement with or_heun_step as dt -> 0.
  *   or_thermostat_sigma  closed form; Langevin-function test of <s_z>.
  *   or_kahan_*           exact sums on integer-valued data.
  *   or_term_energy       FM closed form -1/2 N sum_j J_ij; Zeeman/uniaxial
@@ -638,6 +642,56 @@ void or_heun_step(const or_system_t *sys, double *s, const double *w, double dt,
       }
       unit_vector(v);
       for (int j = 0; j < 3; ++j) s[3 * i + j] = v[j];
+    }
+  }
+}
+
+/* One CUDALLGRK4Solver::run step (solvers/cuda_rk4_base.cu:57-108 with
+ * function_kernel = cuda_llg_rk4.cu:17-30 / cuda_llg_rk4_kernel.cuh:8-58 and
+ * post_step = cuda/cuda_spin_ops.cu:4-17).  Stage k evaluates the fields at
+ * its input y_k (y_0 = s) and time t0, t0 + dt/2, t0 + dt/2, t0 + dt;
+ * k = rhs(y_k, h/mus + w) with the step's one noise draw; y_{k+1} =
+ * s_old + c k (c = dt/2, dt/2, dt; the reference's copy + axpy, unnormalised);
+ * s = v / sqrt(v.v) with v = s_old + dt (k1 + 2 k2 + 2 k3 + k4) / 6 summed
+ * left to right.  Each + - * / sqrt is one IEEE operation in that order (the
+ * reference's axpy and rsqrt are library and CUDA-math calls whose rounding
+ * this restatement does not reproduce).  work needs 5*3N doubles. */
+void or_rk4_step(const or_system_t *sys, double *s, const double *w, double dt, double t0,
+                 double *work) {
+  const int64_t n = sys->n;
+  const int nth = sys->nthreads > 0 ? sys->nthreads : 1;
+  (void)nth;
+  double *s_old = work, *acc = work + 3 * n, *h = work + 6 * n, *y = work + 9 * n,
+         *scratch = work + 12 * n;
+  memcpy(s_old, s, sizeof(double) * 3 * (size_t)n);
+  memcpy(y, s, sizeof(double) * 3 * (size_t)n);
+  const double half = 0.5 * dt;
+  for (int stage = 0; stage < 4; ++stage) {
+    const double t = stage == 0 ? t0 : stage == 3 ? t0 + dt : t0 + half;
+    or_compute_fields(sys, y, t, h, scratch);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(nth)
+#endif
+    for (int64_t i = 0; i < n; ++i) {
+      double hh[3], yi[3], k[3];
+      for (int j = 0; j < 3; ++j) {
+        hh[j] = w ? (h[3 * i + j] / sys->mus[i] + w[3 * i + j]) : h[3 * i + j] / sys->mus[i];
+        yi[j] = y[3 * i + j];
+      }
+      llg_rhs(yi, hh, sys->gyro[i], sys->alpha[i], k);
+      if (stage < 3) {
+        const double c = stage == 2 ? dt : half;
+        for (int j = 0; j < 3; ++j) {
+          acc[3 * i + j] = stage == 0 ? k[j] : acc[3 * i + j] + 2.0 * k[j];
+          y[3 * i + j] = s_old[3 * i + j] + c * k[j];
+        }
+      } else {
+        double v[3];
+        for (int j = 0; j < 3; ++j)
+          v[j] = s_old[3 * i + j] + (dt * (acc[3 * i + j] + k[j])) / 6.0;
+        const double rn = 1.0 / sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        for (int j = 0; j < 3; ++j) s[3 * i + j] = v[j] * rn;
+      }
     }
   }
 }

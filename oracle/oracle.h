@@ -145,6 +145,12 @@ void or_compute_fields(const or_system_t *sys, const double *s, double t, double
 void or_heun_step(const or_system_t *sys, double *s, const double *w, double dt, double t0,
                   double *work);
 
+/* One CUDALLGRK4Solver::run step (solvers/cuda_rk4_base.cu:57-108,
+ * cuda_llg_rk4_kernel.cuh:8-58, cuda/cuda_spin_ops.cu:4-17); w as above, the
+ * same draw for all four stages.  work needs 5*3N doubles. */
+void or_rk4_step(const or_system_t *sys, double *s, const double *w, double dt, double t0,
+                 double *work);
+
 /* sigma_i (solvers/cpu_llg_heun.cc:36-43) */
 void or_thermostat_sigma(int64_t n, const double *mus, const double *gyro,
                          const double *alpha, double dt, int gilbert, double *sigma);

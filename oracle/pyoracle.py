@@ -255,7 +255,7 @@ class OracleSystem:
         self.ext = None
         self.dip = None
         self.nthreads = nthreads
-        self._work = np.empty(12 * self.n, np.float64)
+        self._work = np.empty(15 * self.n, np.float64)
 
     def add_exchange(self, rowptr, col, val):
         self.rowptr = np.ascontiguousarray(rowptr, np.int64)
@@ -333,6 +333,14 @@ class OracleSystem:
         wp = None if w is None else np.ascontiguousarray(w, np.float64)
         lib().or_heun_step(C.byref(S), _p(s), _p(wp), C.c_double(dt), C.c_double(t0),
                            _p(self._work))
+
+    def rk4_step(self, s, dt, t0=0.0, w=None):
+        """Advance s (N x 3, modified in place) by one RK4 step (llg-rk4-gpu)."""
+        assert s.flags.c_contiguous and s.dtype == np.float64
+        S = self.c()
+        wp = None if w is None else np.ascontiguousarray(w, np.float64)
+        lib().or_rk4_step(C.byref(S), _p(s), _p(wp), C.c_double(dt), C.c_double(t0),
+                          _p(self._work))
 
     def term_energy(self, term, s, t=0.0) -> float:
         s = np.ascontiguousarray(s, np.float64).reshape(self.n, 3)

@@ -66,3 +66,20 @@ def test_energy_monitor_and_term_energies(tmp_path):
     n = 432
     assert abs(e0[0] - (-0.5 * n * (8 * 3.0 + 6 * 1.5))) < 1e-9
     assert abs(e0[1] - (-n * 0.05)) < 1e-12
+
+
+def test_rk4_solver_module_from_config(tmp_path):
+    """solver.module = "llg-rk4-gpu" (core/solver.cc:71) selects the RK4
+    integrator: the C1 lattice at 8^3, 200 steps, equal to the oracle's RK4."""
+    text = configs.c1(8, steps=200).replace('module = "llg-heun-gpu"', 'module = "llg-rk4-gpu"')
+    sim = Simulation(text, device=0, output_dir=str(tmp_path), name="rk4")
+    sim.run()
+    e = sim.engine()
+    assert "integrator=rk4" in e.describe()
+    got = e.get_spins()
+    b = S.c1(8)
+    o = b.oracle(nthreads=8)
+    s = normalised(b.s0)
+    for k in range(200):
+        o.rk4_step(s, b.dt, k * b.dt)
+    assert np.array_equal(got, s)

@@ -337,3 +337,84 @@ def test_dipole_fft_direct_one_sided_as_written_differs_on_open_dimensions():
     h1 = po.dipole_fft_direct(lat, mus, s, 2.0, 0.3e-9, one_sided_open=True)
     h2 = po.dipole_fft_direct(lat, mus, s, 2.0, 0.3e-9, one_sided_open=False)
     assert np.array_equal(h1, h2)
+
+
+# ---- RK4 (llg-rk4-gpu) ------------------------------------------------------
+
+
+def _macrospin(alpha, B, ac=None, omega=None):
+    g = S.gyro_e()
+    mus = np.array([2.0 * S.BOHR])
+    o = po.OracleSystem(mus, np.array([g]), np.array([alpha]))
+    if ac is None:
+        o.add_zeeman(mus[:, None] * np.array([[0.0, 0.0, B]]))
+    else:
+        o.add_zeeman(mus[:, None] * np.array([[0.0, 0.0, B]]), mus[:, None] * np.asarray([ac]),
+                     np.array([omega]))
+    return o, g
+
+
+def _rk4_run(o, s0, dt, nsteps):
+    s = s0.copy()
+    for k in range(nsteps):
+        o.rk4_step(s, dt, k * dt)
+    return s
+
+
+def test_rk4_macrospin_closed_form_fourth_order():
+    """Single spin in B z (closed form as for the Heun step): the RK4 error
+    falls ~16x per halving of dt.  A wrong stage weight or stage time drops
+    the order to 1 or 2 (ratio <= 4); Heun itself gives ~4."""
+    alpha, B, T = 0.1, 10.0, 1.6
+    o, g = _macrospin(alpha, B)
+    th0, ph0 = 1.1, 0.3
+    s0 = np.array([[math.sin(th0) * math.cos(ph0), math.sin(th0) * math.sin(ph0), math.cos(th0)]])
+    th = 2 * math.atan(math.tan(th0 / 2) * math.exp(-alpha * g * B * T))
+    ph = ph0 + g * B * T
+    ref = np.array([math.sin(th) * math.cos(ph), math.sin(th) * math.sin(ph), math.cos(th)])
+    errs = []
+    for nsteps in (32, 64, 128):
+        s = _rk4_run(o, s0, T / nsteps, nsteps)
+        errs.append(np.abs(s[0] - ref).max())
+        assert abs(np.linalg.norm(s[0]) - 1) < 4e-16
+    r1, r2 = errs[0] / errs[1], errs[1] / errs[2]
+    print(f"RK4 macrospin errors {errs}, ratios {r1:.2f} {r2:.2f}")
+    assert errs[2] < 1e-7
+    assert 12 < r1 < 20 and 12 < r2 < 20
+    # the Heun step on the same problem is second order (contrast)
+    s = s0.copy()
+    for k in range(128):
+        o.heun_step(s, T / 128, k * T / 128)
+    assert np.abs(s[0] - ref).max() > 100 * errs[2]
+
+
+def test_rk4_stage_times_with_an_ac_field():
+    """Time-dependent field B(t) = B z + b cos(w t) x: fourth-order convergence
+    to a run at dt/16 pins the stage times t0, t0 + dt/2, t0 + dt/2, t0 + dt."""
+    o, _ = _macrospin(0.05, 4.0, ac=(3.0, 0.0, 0.0), omega=5.0)
+    s0 = np.array([[0.0, 0.6, 0.8]])
+    T = 1.28
+    fine = _rk4_run(o, s0, T / 2048, 2048)
+    errs = [np.abs(_rk4_run(o, s0, T / m, m)[0] - fine[0]).max() for m in (32, 64, 128)]
+    r1, r2 = errs[0] / errs[1], errs[1] / errs[2]
+    print(f"RK4 ac-field errors {errs}, ratios {r1:.2f} {r2:.2f}")
+    assert 12 < r1 < 20 and 12 < r2 < 20
+
+
+def test_rk4_unit_length_and_agreement_with_heun():
+    """C1 lattice 6^3: |s| = 1 after every RK4 step; RK4 and Heun agree to
+    O(dt^2) and the gap shrinks ~4x when dt halves."""
+    b = S.c1(6)
+    o = b.oracle()
+    gaps = []
+    for scale in (1, 2):
+        dt = b.dt / scale
+        s4, s2 = b.s0.copy(), b.s0.copy()
+        for k in range(100 * scale):
+            o.rk4_step(s4, dt, k * dt)
+            o.heun_step(s2, dt, k * dt)
+            assert np.abs(np.linalg.norm(s4, axis=1) - 1).max() < 1e-15
+        gaps.append(np.abs(s4 - s2).max())
+    print(f"RK4 vs Heun gaps {gaps}")
+    assert gaps[0] < 1e-4
+    assert 3.0 < gaps[0] / gaps[1] < 5.0
