@@ -16,7 +16,7 @@ FP64 and again, from the same initial state and noise stream, in the
 FP32-field form, and states the speed-up and the per-spin angle and |dm|
 differences between the two final states.
 
-  python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-fp32]
+  python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-fp32] [--no-rk4]
                   [--no-c3] [--c3-steps S] [--reps R]
   python bench.py --copy-calibration   # a plain 1 GiB copy kernel, for
                                        # scaling rocprofv3 FETCH/WRITE_SIZE
@@ -65,6 +65,26 @@ def measure(sim_text: str, steps: int, warmup: int, reps: int = 200):
     out = {"n": n, "ms": ms, "wall_s": wall, "setup_s": setup_s, "phases": phases,
            "value": n * steps / (ms * 1e-3), "describe": eng.describe()}
     sim.close()
+    return out
+
+
+def measure_rk4(L: int, steps: int, warmup: int):
+    """The C2 system under solver module llg-rk4-gpu (four field evaluations
+    per step), FP64 and FP32-field: whole-step HIP-event time."""
+    from jams_amd import configs
+    from jams_amd.host import Simulation
+
+    out = {"integrator": "llg-rk4-gpu", "steps": steps}
+    for prec in ("fp64", "fp32-field"):
+        text = configs.c2(L, precision=prec).replace('module = "llg-heun-gpu"',
+                                                     'module = "llg-rk4-gpu"')
+        sim = Simulation(text, device=0, output_dir="/tmp")
+        eng = sim.engine()
+        eng.step(warmup)
+        ms = eng.time_steps(steps)
+        out[prec.replace("-", "_")] = {"ms_per_step": ms / steps,
+                                       "value": sim.n * steps / (ms * 1e-3)}
+        sim.close()
     return out
 
 
@@ -297,6 +317,7 @@ def main() -> int:
     ap.add_argument("--L", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-rk4", action="store_true")
     ap.add_argument("--no-c3", action="store_true")
     ap.add_argument("--c3-steps", type=int, default=10000)
     ap.add_argument("--c3-L", type=int, default=128)
@@ -331,6 +352,13 @@ def main() -> int:
         fp32 = measure(configs.c2(args.L, precision="fp32-field"), args.steps, args.warmup,
                        args.reps)
         log(f"  {fp32['value']:.4e} spin-updates/s")
+    rk4 = None
+    if not args.no_rk4:
+        log("C2 under llg-rk4-gpu (RK4), FP64 and FP32-field")
+        rk4 = measure_rk4(args.L, min(args.steps, 1000), min(args.warmup, 50))
+        log(f"  fp64 {rk4['fp64']['ms_per_step'] * 1e3:.2f} us/step "
+            f"({rk4['fp64']['value']:.4e} spin-updates/s), fp32-field "
+            f"{rk4['fp32_field']['ms_per_step'] * 1e3:.2f} us/step")
     c3 = None
     if not args.no_c3:
         log(f"C3 precision form: bcc {args.c3_L}^3 x 2, {args.c3_steps} steps FP64 and FP32-field")
@@ -381,6 +409,8 @@ def main() -> int:
         "phases": fp64["phases"],
         "cpu_baseline": cpu,
     }
+    if rk4 is not None:
+        line["rk4_c2"] = rk4
     if fp32 is not None:
         line["fp32_field"] = {"value": fp32["value"], "ms_per_step": fp32["ms"] / args.steps,
                               "speedup_vs_fp64": fp32["value"] / fp64["value"],
