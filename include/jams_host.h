@@ -5,8 +5,8 @@
  *
  * It replaces the reference driver's jams::initialize_simulation /
  * run_simulation / cleanup_simulation (core/jams++.h:20-23,
- * core/jams++.cc:206-377) for configs that select the GPU Heun solver
- * (solver.module = "llg-heun-gpu").  The command-line driver `jams` is a
+ * core/jams++.cc:206-377) for configs that select the GPU Heun or RK4 LLG
+ * solver (solver.module = "llg-heun-gpu" or "llg-rk4-gpu").  The command-line driver `jams` is a
  * thin wrapper over these calls.  Return codes are 0 on success, non-zero
  * on failure with the message in jams_sim_last_error.
  */
@@ -56,6 +56,17 @@ int jams_sim_exchange_csr(const jams_sim *sim, int index, int64_t *rowptr, int32
  * the index of its tensor, and the table of distinct tensors (nvalues x 9). */
 int jams_sim_exchange_values(const jams_sim *sim, int index, int32_t *vidx, double *values9,
                              int64_t *nvalues);
+
+/* Iteration and time (ps) of the run (Solver::iteration / Solver::time). */
+int jams_sim_iteration(const jams_sim *sim, int64_t *iteration, double *time_ps);
+
+/* Write the current spins as a checkpoint: the reference's TSV spin format
+ * (helpers/load.h:22-51, loadable as lattice.spins) in lossless %.17g, headed
+ * by '#' lines with the iteration, time and seed.  sim.resume = <file>
+ * continues the run from it, noise stream included; the reference has no
+ * counterpart (it restarts at t = 0).  The "checkpoint" monitor writes the
+ * same file every output_steps and at the end of the run. */
+int jams_sim_write_checkpoint(jams_sim *sim, const char *path);
 
 /* Time step (ps), temperature (K) and thermostat seed chosen by the config. */
 int jams_sim_params(const jams_sim *sim, double *dt_ps, double *temperature, uint64_t *seed);

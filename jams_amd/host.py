@@ -27,6 +27,8 @@ ABI = {
     "jams_sim_exchange_csr": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     "jams_sim_exchange_values": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
                                            C.POINTER(C.c_int64)]),
+    "jams_sim_iteration": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
+    "jams_sim_write_checkpoint": (C.c_int, [C.c_void_p, C.c_char_p]),
     "jams_sim_params": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                   C.POINTER(C.c_uint64)]),
 }
@@ -148,6 +150,15 @@ class Simulation:
         val = np.empty(nnz, np.float64)
         self._chk(self.lib.jams_sim_exchange_csr(self.h, index, _p(rowptr), _p(col), _p(val)))
         return rowptr, col, val
+
+    def iteration(self):
+        """(iteration, time in ps) of the run."""
+        it, t = C.c_int64(), C.c_double()
+        self._chk(self.lib.jams_sim_iteration(self.h, C.byref(it), C.byref(t)))
+        return it.value, t.value
+
+    def write_checkpoint(self, path: str):
+        self._chk(self.lib.jams_sim_write_checkpoint(self.h, str(path).encode()))
 
     def params(self):
         dt, T, seed = C.c_double(), C.c_double(), C.c_uint64()

@@ -6,6 +6,7 @@
 #include <iomanip>
 #include <iostream>
 #include <map>
+#include <sstream>
 
 namespace jams {
 
@@ -362,6 +363,13 @@ void Solver::run_steps(int64_t k) {
 
 void Solver::register_monitor(std::unique_ptr<Monitor> m) { monitors_.push_back(std::move(m)); }
 
+void Solver::resume_at(int64_t iteration) {
+  if (iteration < 0) throw RuntimeError("checkpoint iteration must not be negative");
+  iteration_ = iteration;
+  time_ = (double)iteration_ * step_size_;
+  if (ctx_) engine_check(ctx_, jh_set_iteration(ctx_, iteration));
+}
+
 void Solver::notify_monitors() {
   for (auto &m : monitors_)
     if (m->is_updating(iteration_)) m->update(*this);
@@ -380,8 +388,10 @@ std::unique_ptr<Monitor> Monitor::create(const Setting &s, const std::string &pr
   const std::string m = lowercase(s["module"].as_string());
   if (m == "magnetisation") return std::make_unique<MagnetisationMonitor>(s, prefix + "mag.tsv");
   if (m == "energy") return std::make_unique<EnergyMonitor>(s, prefix + "eng.tsv", solver);
+  if (m == "checkpoint")
+    return std::make_unique<CheckpointMonitor>(s, prefix + "checkpoint.tsv", lat.num_sites());
   throw RuntimeError("unknown monitor " + s["module"].as_string() +
-                     " (this build provides magnetisation, energy)");
+                     " (this build provides magnetisation, energy, checkpoint)");
   return nullptr;
 }
 
@@ -404,6 +414,58 @@ void MagnetisationMonitor::update(Solver &solver) {
                 mag * f);
   tsv_ << line;
   tsv_.flush();
+}
+
+void write_checkpoint(jh_ctx *ctx, int64_t n, const std::string &path, int64_t iteration,
+                      double time_ps, uint64_t seed) {
+  std::vector<double> s((size_t)(3 * n));
+  engine_check(ctx, jh_get_spins(ctx, s.data()));
+  const std::string tmp = path + ".tmp";
+  {
+    std::FILE *f = std::fopen(tmp.c_str(), "w");
+    if (!f) throw RuntimeError("failed to open file: " + tmp);
+    std::fprintf(f, "# jams checkpoint\n# num_spins %lld\n# iteration %lld\n# time_ps %.17g\n"
+                    "# seed %llu\n",
+                 (long long)n, (long long)iteration, time_ps, (unsigned long long)seed);
+    for (int64_t i = 0; i < n; ++i)
+      std::fprintf(f, "%.17g\t%.17g\t%.17g\n", s[3 * i], s[3 * i + 1], s[3 * i + 2]);
+    const bool ok = std::fflush(f) == 0;
+    std::fclose(f);
+    if (!ok) throw RuntimeError("failed to write file: " + tmp);
+  }
+  if (std::rename(tmp.c_str(), path.c_str()) != 0) throw RuntimeError("failed to write file: " + path);
+}
+
+CheckpointHeader read_checkpoint_header(const std::string &path) {
+  std::ifstream f(path);
+  if (!f.is_open()) throw RuntimeError("failed to open file: " + path);
+  CheckpointHeader h;
+  bool tagged = false, has_iteration = false;
+  for (std::string line; std::getline(f, line);) {
+    if (line.empty() || line[0] != '#') break;  // the header ends at the first data row
+    std::istringstream is(line.substr(1));
+    std::string key;
+    is >> key;
+    if (key == "jams") tagged = true;
+    else if (key == "num_spins") is >> h.num_spins;
+    else if (key == "iteration") has_iteration = static_cast<bool>(is >> h.iteration);
+    else if (key == "seed") h.has_seed = static_cast<bool>(is >> h.seed);
+  }
+  if (!tagged || !has_iteration)
+    throw ConfigError("sim.resume: " + path + " is not a checkpoint written by this build");
+  return h;
+}
+
+void CheckpointMonitor::update(Solver &solver) {
+  if (!is_updating(solver.iteration())) return;
+  write_checkpoint(solver.engine(), n_, path_, solver.iteration(), solver.time(), solver.seed());
+  written_ = solver.iteration();
+}
+
+void CheckpointMonitor::finish(Solver &solver) {
+  if (written_ == solver.iteration()) return;
+  write_checkpoint(solver.engine(), n_, path_, solver.iteration(), solver.time(), solver.seed());
+  written_ = solver.iteration();
 }
 
 EnergyMonitor::EnergyMonitor(const Setting &s, const std::string &path, Solver &solver)

@@ -109,6 +109,8 @@ class Solver {
   double temperature() const { return temperature_; }
   const Vec3 &applied_field() const { return applied_field_; }
   bool gilbert_prefactor() const { return gilbert_; }
+  uint64_t seed() const { return seed_; }
+  void set_seed(uint64_t seed) { seed_ = seed; }
   const std::string &thermostat_name() const { return thermostat_; }
 
   void attach(jh_ctx *ctx) { ctx_ = ctx; }
@@ -122,6 +124,9 @@ class Solver {
     applied_field_ = field;
   }
   void notify_monitors();
+  // continue a run from a checkpoint: iteration (and time) here and in the
+  // engine, whose thermostat stream is keyed by the step counter
+  void resume_at(int64_t iteration);
   // Solver::compute_fields (core/solver.cc:43-57) into h (N x 3)
   void compute_fields(std::vector<double> &h);
 
@@ -134,6 +139,7 @@ class Solver {
   double temperature_ = 0.0;
   Vec3 applied_field_ = {0, 0, 0};
   bool gilbert_ = false;
+  uint64_t seed_ = 0;
   std::string thermostat_ = "langevin-white-gpu";
   std::vector<std::unique_ptr<Hamiltonian>> hamiltonians_;
   std::vector<std::unique_ptr<Monitor>> monitors_;
@@ -163,12 +169,44 @@ class Monitor {
   bool is_updating(int64_t iteration) const { return iteration % output_steps_ == 0; }
   int64_t output_steps() const { return output_steps_; }
   virtual void update(Solver &solver) = 0;
+  // after the run loop (the reference's Monitor::post_process)
+  virtual void finish(Solver &solver) {}
   static std::unique_ptr<Monitor> create(const Setting &s, const std::string &prefix,
                                          Solver &solver, const Lattice &lat);
 
  protected:
   int64_t output_steps_;
 };
+
+// Checkpoint of the spin state (an addition: the reference restarts from
+// lattice.spins = <file> with the time and the noise stream reset, and its
+// HDF5 writer needs a library this build does not have).  The file is the
+// reference's TSV spin format (helpers/load.h:22-51: N rows of sx sy sz,
+// '#' lines skipped), written lossless (%.17g), headed by comment lines with
+// the iteration, time and seed; sim.resume = <file> continues from it.
+class CheckpointMonitor : public Monitor {
+ public:
+  CheckpointMonitor(const Setting &s, const std::string &path, int64_t n)
+      : Monitor(s), path_(path), n_(n) {}
+  void update(Solver &solver) override;
+  void finish(Solver &solver) override;
+
+ private:
+  std::string path_;
+  int64_t n_;
+  int64_t written_ = -1;
+};
+
+// write / read the checkpoint file (n spins read back from the engine)
+void write_checkpoint(jh_ctx *ctx, int64_t n, const std::string &path, int64_t iteration,
+                      double time_ps, uint64_t seed);
+struct CheckpointHeader {
+  int64_t iteration = 0;
+  int64_t num_spins = -1;
+  bool has_seed = false;
+  uint64_t seed = 0;
+};
+CheckpointHeader read_checkpoint_header(const std::string &path);
 
 // monitors/magnetisation.cc:77-104 (one group: all spins)
 class MagnetisationMonitor : public Monitor {

@@ -62,8 +62,21 @@ void initialize(jams_sim *s, const std::vector<std::string> &sources, int device
   const Config &cfg = s->cfg;
   // seeds: sim.seed drives the thermostat stream and "random" spins; without
   // it the seed is drawn once and printed so the run can be repeated
+  // sim.resume = <checkpoint file>: spins, iteration and (unless sim.seed
+  // names it) the thermostat seed of a run to continue
+  std::string resume;
+  CheckpointHeader ck;
+  if (cfg.exists("sim.resume")) {
+    resume = cfg.lookup("sim.resume").as_string();
+    ck = read_checkpoint_header(resume);
+  }
   if (cfg.exists("sim.seed")) {
     s->seed = (uint64_t)cfg.lookup("sim.seed").as_int64();
+    if (!resume.empty() && ck.has_seed && ck.seed != s->seed)
+      throw ConfigError("sim.seed differs from the seed of the checkpoint in sim.resume; the "
+                        "thermostat stream would not continue");
+  } else if (!resume.empty() && ck.has_seed) {
+    s->seed = ck.seed;
   } else {
     std::random_device rd;
     s->seed = ((uint64_t)rd() << 32) ^ rd();
@@ -76,10 +89,18 @@ void initialize(jams_sim *s, const std::vector<std::string> &sources, int device
   s->lattice.init_from_config(cfg, gilbert, s->seed);
   const int64_t n = s->lattice.num_sites();
   std::cout << "  computed lattice positions " << n << "\n";
+  if (!resume.empty()) {
+    if (ck.num_spins >= 0 && ck.num_spins != n)
+      throw ConfigError("sim.resume: the checkpoint holds " + std::to_string(ck.num_spins) +
+                        " spins, the lattice " + std::to_string(n));
+    s->lattice.spins = load_spins_tsv(resume, n);
+    std::cout << "  resume from " << resume << " at iteration " << ck.iteration << "\n";
+  }
 
   std::cout << "init solver\n";
   s->solver = Solver::create(solver_s);
   s->solver->initialize(solver_s);
+  s->solver->set_seed(s->seed);
   // physics (core/physics.cc): the empty module with a constant temperature
   const Setting &phys = cfg.lookup("physics");
   const std::string pm = phys.get<std::string>("module", "empty");
@@ -119,6 +140,7 @@ void initialize(jams_sim *s, const std::vector<std::string> &sources, int device
     engine_check(s->ctx, jh_set_precision(s->ctx, p == "fp32-field" ? JH_PREC_FP32_FIELD : JH_PREC_FP64));
     engine_check(s->ctx, jh_set_integrator(s->ctx, s->solver->integrator()));
   }
+  if (!resume.empty()) s->solver->resume_at(ck.iteration);
   if (cfg.exists("monitors") && s->ctx) {
     const Setting &ms = cfg.lookup("monitors");
     for (int q = 0; q < ms.length(); ++q)
@@ -146,6 +168,7 @@ void run(jams_sim *s, int64_t limit) {
     solver.run_steps(next - solver.iteration());
   }
   if (solver.iteration() == solver.max_steps()) solver.notify_monitors();
+  for (auto &m : solver.monitors()) m->finish(solver);
   const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   std::cout << "runtime " << sec << " seconds\n";
 }
@@ -250,6 +273,22 @@ int jams_sim_exchange_csr(const jams_sim *sim, int index, int64_t *rowptr, int32
     if (rowptr) std::memcpy(rowptr, rp.data(), rp.size() * 8);
     if (col) std::memcpy(col, c.data(), c.size() * 4);
     if (val) std::memcpy(val, v.data(), v.size() * 8);
+  });
+}
+
+int jams_sim_iteration(const jams_sim *sim, int64_t *iteration, double *time_ps) {
+  if (!sim) return 1;
+  if (iteration) *iteration = sim->solver->iteration();
+  if (time_ps) *time_ps = sim->solver->time();
+  return 0;
+}
+
+int jams_sim_write_checkpoint(jams_sim *sim, const char *path) {
+  return guarded(sim, [&] {
+    if (!sim->ctx) throw RuntimeError("set-up-only simulation has no state to write");
+    if (!path) throw RuntimeError("checkpoint path is null");
+    const Solver &so = *sim->solver;
+    write_checkpoint(sim->ctx, sim->lattice.num_sites(), path, so.iteration(), so.time(), sim->seed);
   });
 }
 

@@ -83,3 +83,32 @@ def test_rk4_solver_module_from_config(tmp_path):
     for k in range(200):
         o.rk4_step(s, b.dt, k * b.dt)
     assert np.array_equal(got, s)
+
+
+@pytest.mark.parametrize("module", ["llg-heun-gpu", "llg-rk4-gpu"])
+def test_checkpoint_resume_continues_bitwise(tmp_path, module):
+    """bcc 6^3 x 2 at 300 K: 120 steps straight equal 50 steps, a checkpoint
+    (the "checkpoint" monitor's file), and 70 more steps of a new simulation
+    resumed from it: the spins and the thermostat stream continue exactly."""
+    mon = 'monitors = ( { module = "checkpoint"; output_steps = 50; } );'
+    text = configs.c2(6, steps=120, monitors=mon).replace('module = "llg-heun-gpu"',
+                                                          f'module = "{module}"')
+    whole = Simulation(text, device=0, output_dir=str(tmp_path), name="whole")
+    whole.run()
+    ref = whole.engine().get_spins()
+    assert whole.iteration()[0] == 120
+    whole.close()
+    part = Simulation(text, device=0, output_dir=str(tmp_path), name="part")
+    part.run(50)
+    ck = tmp_path / "part_checkpoint.tsv"
+    assert "# iteration 50" in ck.read_text()
+    part.close()
+    resumed = Simulation([text, f'sim : {{ resume = "{ck}"; }};'], device=0,
+                         output_dir=str(tmp_path), name="resumed")
+    assert resumed.iteration()[0] == 50
+    resumed.run()
+    assert resumed.iteration()[0] == 120
+    assert np.array_equal(resumed.engine().get_spins(), ref)
+    # the explicit writer gives the same file format
+    resumed.write_checkpoint(str(tmp_path / "final.tsv"))
+    assert "# iteration 120" in (tmp_path / "final.tsv").read_text()

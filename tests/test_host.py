@@ -205,3 +205,37 @@ def test_replica_batch_concatenation():
         sim.close()
     # the workload's draw: T_r = 1000 r/(nrep-1), B_r = 2 r/(nrep-1) along z
     assert reps[-1][1] == 1000.0 and reps[-1][3] == (0.0, 0.0, 2.0)
+
+
+def _checkpoint_file(path, spins, iteration, seed=None):
+    head = ["# jams checkpoint", f"# num_spins {spins.shape[0]}", f"# iteration {iteration}",
+            "# time_ps 0"]
+    if seed is not None:
+        head.append(f"# seed {seed}")
+    rows = ["\t".join(repr(float(x)) for x in r) for r in spins]
+    path.write_text("\n".join(head + rows) + "\n")
+
+
+def test_checkpoint_resume_set_up(tmp_path):
+    """sim.resume: spins loaded bit for bit (no renormalisation), iteration
+    and time restored, the checkpoint's seed taken when sim.seed is absent,
+    and refused when sim.seed names another one."""
+    s = po.random_spins(7, 64)
+    f = tmp_path / "ck.tsv"
+    _checkpoint_file(f, s, 37, seed=4242)
+    base = configs.c1(4).replace("sim : { seed = 1; };", "")
+    sim = Simulation([base, f'sim : {{ resume = "{f}"; }};'])
+    assert np.array_equal(sim.sites()[3], s)
+    dt, _, seed = sim.params()
+    it, t = sim.iteration()
+    assert it == 37 and t == 37 * dt and seed == 4242
+    with pytest.raises(RuntimeError, match="seed"):
+        Simulation([base, f'sim : {{ resume = "{f}"; seed = 5; }};'])
+    plain = tmp_path / "plain.tsv"
+    plain.write_text("\n".join("\t".join(repr(float(x)) for x in r) for r in s))
+    with pytest.raises(RuntimeError, match="not a checkpoint"):
+        Simulation([base, f'sim : {{ resume = "{plain}"; }};'])
+    small = tmp_path / "small.tsv"
+    _checkpoint_file(small, s[:8], 1)
+    with pytest.raises(RuntimeError, match="holds 8 spins"):
+        Simulation([base, f'sim : {{ resume = "{small}"; }};'])
