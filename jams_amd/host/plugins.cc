@@ -68,12 +68,15 @@ std::unique_ptr<Hamiltonian> Hamiltonian::create(const Setting &s, const Lattice
     throw RuntimeError("dipole hamiltonians now have specific names and 'strategy' has been removed");
   const std::string m = lowercase(s["module"].as_string());
   if (m == "exchange") return std::make_unique<ExchangeHamiltonian>(s, lat);
+  if (m == "exchange-neartree") return std::make_unique<ExchangeNeartreeHamiltonian>(s, lat);
+  if (m == "exchange-functional") return std::make_unique<ExchangeFunctionalHamiltonian>(s, lat);
   if (m == "zeeman") return std::make_unique<ZeemanHamiltonian>(s, lat, false);
   if (m == "applied-field") return std::make_unique<ZeemanHamiltonian>(s, lat, true);
   if (m == "uniaxial" || m == "uniaxial-anisotropy") return std::make_unique<UniaxialHamiltonian>(s, lat);
   if (m == "dipole-fft") return std::make_unique<DipoleFFTHamiltonian>(s, lat);
   throw RuntimeError("unknown hamiltonian " + s["module"].as_string() +
-                     " (this build provides exchange, zeeman, applied-field, uniaxial, dipole-fft)");
+                     " (this build provides exchange, exchange-neartree, exchange-functional, "
+                     "zeeman, applied-field, uniaxial, dipole-fft)");
   return nullptr;
 }
 
@@ -135,6 +138,36 @@ ExchangeHamiltonian::ExchangeHamiltonian(const Setting &s, const Lattice &lat) :
   }
   if (check_symmetry && !is_symmetric())
     throw RuntimeError("sparse matrix for " + name_ + " is not symmetric");
+}
+
+ExchangeHamiltonian::ExchangeHamiltonian(const Setting &s, const Lattice &lat, int)
+    : Hamiltonian(s, lat) {}
+
+ExchangeNeartreeHamiltonian::ExchangeNeartreeHamiltonian(const Setting &s, const Lattice &lat)
+    : ExchangeHamiltonian(s, lat, 0) {
+  // hamiltonian/exchange_neartree.cc:14-155; finalize(Symmetric)
+  auto pairs = neartree_pairs(s, lat, energy_unit_conversion_, computed_);
+  std::cout << "  total interactions " << computed_ << "\n  average interactions per spin "
+            << (double)computed_ / (double)lat.num_sites() << "\n";
+  scaled_ = pairs_to_list(std::move(pairs));
+  if (!is_symmetric()) throw RuntimeError("sparse matrix for " + name_ + " is not symmetric");
+}
+
+ExchangeFunctionalHamiltonian::ExchangeFunctionalHamiltonian(const Setting &s, const Lattice &lat)
+    : ExchangeHamiltonian(s, lat, 0) {
+  // hamiltonian/exchange_functional.cc:97-258
+  std::string check = "symmetric";
+  if (s.exists("symmetry_check")) check = lowercase(s["symmetry_check"].as_string());
+  if (check != "none" && check != "symmetric" && check != "force_symmetric")
+    throw RuntimeError("invalid value for symmetry_check in ExchangeFunctionalHamiltonian");
+  auto pairs = functional_pairs(s, lat, energy_unit_conversion_, computed_);
+  std::cout << "  total interactions " << computed_ << "\n  average interactions per spin "
+            << (double)computed_ / (double)lat.num_sites() << "\n";
+  if (check == "force_symmetric") force_symmetric(pairs);
+  scaled_ = pairs_to_list(std::move(pairs));
+  if (check != "none" && !is_symmetric())
+    throw RuntimeError("sparse matrix for " + name_ + (check == "force_symmetric" ? " failed to symmetrise"
+                                                                                 : " is not symmetric"));
 }
 
 bool ExchangeHamiltonian::is_symmetric() const {

@@ -17,6 +17,7 @@
 #include "../../include/jams_hip.h"
 #include "common.h"
 #include "config.h"
+#include "exchange_pairs.h"
 #include "interactions.h"
 #include "lattice.h"
 
@@ -59,9 +60,25 @@ class ExchangeHamiltonian : public Hamiltonian {
                  std::vector<double> &val) const;
   bool is_symmetric() const;
 
- private:
+ protected:
+  // a list built by another module (the distance-based producers below)
+  ExchangeHamiltonian(const Setting &s, const Lattice &lat, int);
   NeighbourList scaled_;
   int64_t computed_ = 0;
+};
+
+// hamiltonian/exchange_neartree.cc: J I between sites of the listed material
+// pair at a given distance (a shell of width shell_width)
+class ExchangeNeartreeHamiltonian : public ExchangeHamiltonian {
+ public:
+  ExchangeNeartreeHamiltonian(const Setting &s, const Lattice &lat);
+};
+
+// hamiltonian/exchange_functional.cc: J(r_ij) I from a named functional of
+// the pair vector, for pairs of the listed material pair within r_cutoff
+class ExchangeFunctionalHamiltonian : public ExchangeHamiltonian {
+ public:
+  ExchangeFunctionalHamiltonian(const Setting &s, const Lattice &lat);
 };
 
 // hamiltonian/zeeman.cc and applied_field.cc: h = mus B (+ ac cos(omega t))

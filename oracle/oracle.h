@@ -91,6 +91,20 @@ int64_t or_neighbour_bruteforce(const or_lattice_t *lat, const or_template_t *t,
                                 double tol, int32_t *out_i, int32_t *out_j, double *out_J,
                                 int64_t max_out);
 
+/* Pairs for the distance-based exchange modules (exchange-neartree,
+ * exchange-functional): every ordered (i, image of j) of the supercell with
+ * distance d = |r_i - r_j'| in the annulus of InteractionNearTree
+ * (containers/neartree.h:359-391): !(d >> outer) && (d >> inner) with the
+ * relative test of helpers/maths.h:31-34 and tolerance eps.  Images of j:
+ * r_j + ((h A1 + k A2) + l A3) for all |h|, |k|, |l| up to ceil(outer / height)
+ * + 1 along periodic directions (brute force over all j).  Positions are
+ * M (frac + cell).  Self pairs (j = i) are included; the caller drops them.
+ * Output sorted by (i, j, d); returns the count, or -2 if max_out is too
+ * small. */
+int64_t or_annulus_pairs(const or_lattice_t *lat, double inner, double outer, double eps,
+                         int32_t *out_i, int32_t *out_j, double *out_rij, double *out_d,
+                         int64_t max_out);
+
 /* ---- sparse matrix builder (containers/sparse_matrix_builder.h) -------- */
 /* Exchange insertion (exchange.cc:162-169 -> sparse_interaction.cc:25-33) of
  * the neighbour list into COO, then sort (:109-143), merge (:146-164, with

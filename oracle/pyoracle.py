@@ -6,6 +6,7 @@ cpu_baseline leg of bench.py, never by the product package.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import subprocess
 from pathlib import Path
 
@@ -45,6 +46,7 @@ def lib() -> C.CDLL:
         _lib.or_neighbour_bruteforce.restype = C.c_int64
         _lib.or_exchange_csr.restype = C.c_int64
         _lib.or_coo_to_csr.restype = C.c_int64
+        _lib.or_annulus_pairs.restype = C.c_int64
         _lib.or_site_index.restype = C.c_int64
     return _lib
 
@@ -183,6 +185,99 @@ def neighbour_bruteforce(lat: LatticeSpec, rows, tol=1e-4, max_out=None):
     if m < 0:
         raise ValueError(f"or_neighbour_bruteforce failed with {m}")
     return pi[:m].copy(), pj[:m].copy(), pJ[:m].copy()
+
+
+def annulus_pairs(lat: LatticeSpec, inner: float, outer: float, eps: float):
+    """or_annulus_pairs: (i, j, rij (m x 3), d) sorted by (i, j, d)."""
+    cap = 64 * lat.n + 1024
+    while True:
+        pi = np.empty(cap, np.int32)
+        pj = np.empty(cap, np.int32)
+        rij = np.empty((cap, 3), np.float64)
+        d = np.empty(cap, np.float64)
+        Lc = lat.c()
+        m = lib().or_annulus_pairs(C.byref(Lc), C.c_double(inner), C.c_double(outer),
+                                   C.c_double(eps), _p(pi), _p(pj), _p(rij), _p(d),
+                                   C.c_int64(cap))
+        if m >= 0:
+            return pi[:m].copy(), pj[:m].copy(), rij[:m].copy(), d[:m].copy()
+        cap *= 4
+
+
+def _material(lat: LatticeSpec, site: int) -> int:
+    return int(lat.basis_material[site % lat.nbasis])
+
+
+def neartree_pairs(lat: LatticeSpec, rules, width: float, energy_cutoff: float):
+    """hamiltonian/exchange_neartree.cc:58-139: rules (type_a, type_b, radius,
+    J) in lattice constants and engine energy units; J I between sites of the
+    two types in the shell (radius - w/2, radius + w/2] (tolerance w/10).
+    Returns [(i, j, J)], sorted; raises on a pair found twice."""
+    ntypes = int(lat.basis_material.max()) + 1
+    shells = [[] for _ in range(ntypes)]
+    for ta, tb, radius, J in rules:
+        shells[ta].append((tb, radius, J))
+        if ta != tb:
+            shells[tb].append((ta, radius, J))
+    eps = width / 10.0
+    cand = {}
+    for ta, tb, radius, J in rules:
+        if radius not in cand:
+            pi, pj, _, _ = annulus_pairs(lat, radius - 0.5 * width, radius + 0.5 * width, eps)
+            by_i = {}
+            for i, j in zip(pi.tolist(), pj.tolist()):
+                by_i.setdefault(i, []).append(j)
+            cand[radius] = by_i
+    out = []
+    for i in range(lat.n):
+        seen = set()
+        for tb, radius, J in shells[_material(lat, i)]:
+            for j in cand[radius].get(i, []):
+                if j == i or _material(lat, j) != tb:
+                    continue
+                if j in seen:
+                    raise ValueError(f"multiple interactions between spins {i} and {j}")
+                seen.add(j)
+                if abs(J) > energy_cutoff:
+                    out.append((i, j, J))
+    return sorted(out)
+
+
+def functional_pairs(lat: LatticeSpec, table, eps: float = 1e-4):
+    """hamiltonian/exchange_functional.cc:203-250: table {(type_i, type_j):
+    (r_cutoff, f)} with f(rij) the functional; pairs within the largest
+    cutoff (annulus (eps, max]) whose own cutoff passes r - rc < eps max(r, rc).
+    Returns ([(i, j, J)] with J != 0, the reference's counter)."""
+    rmax = max(rc for rc, _ in table.values())
+    pi, pj, rij, _ = annulus_pairs(lat, eps, rmax, eps)
+    out, seen, counter = [], set(), 0
+    for i, j, r3 in zip(pi.tolist(), pj.tolist(), rij.tolist()):
+        if i == j:
+            continue
+        key = (_material(lat, i), _material(lat, j))
+        if key not in table:
+            continue
+        rc, f = table[key]
+        r = math.sqrt(r3[0] * r3[0] + r3[1] * r3[1] + r3[2] * r3[2])
+        if not (r - rc) < max(abs(r), abs(rc)) * eps:
+            continue
+        if (i, j) in seen:
+            raise ValueError(f"multiple interactions between spins {i} and {j}")
+        seen.add((i, j))
+        J = f(r3)
+        counter += 1
+        if J != 0.0:
+            out.append((i, j, J))
+    return sorted(out), counter
+
+
+def pairs_csr(n: int, pairs):
+    """CSR of scalar pairs J I (insert_interaction_scalar / _tensor, then the
+    builder of or_exchange_csr)."""
+    pi = np.array([p[0] for p in pairs], np.int32)
+    pj = np.array([p[1] for p in pairs], np.int32)
+    pJ = np.array([(np.eye(3) * p[2]).reshape(9) for p in pairs], np.float64).reshape(-1, 9)
+    return exchange_csr(n, pi, pj, pJ)
 
 
 def exchange_csr(n: int, pi, pj, pJ, scale=1.0, cutoff=0.0):

@@ -112,3 +112,34 @@ def test_checkpoint_resume_continues_bitwise(tmp_path, module):
     # the explicit writer gives the same file format
     resumed.write_checkpoint(str(tmp_path / "final.tsv"))
     assert "# iteration 120" in (tmp_path / "final.tsv").read_text()
+
+
+def test_exchange_functional_config_run_bitwise(tmp_path):
+    """A config whose exchange comes from exchange-functional (gaussian, fcc
+    4^3 periodic, two shells within r_cutoff) runs 100 Heun steps on the GPU
+    equal, bit for bit, to the oracle's Heun step on the oracle's CSR."""
+    import math
+    from tests.test_host import distance_config, spec
+    ham = ('{ module = "exchange-functional"; energy_units = "meV"; interactions = ( '
+           '("A", "A", "gaussian", 1.05, [2.0, 0.7, 0.3]) ); }')
+    text = distance_config("fcc", (4, 4, 4), (1, 1, 1), ham).replace("spin = [0.0, 0.0, 1.0]",
+                                                                      'spin = "random"')
+    sim = Simulation(text, device=0, output_dir=str(tmp_path), name="fun")
+    sim.run()
+    got = sim.engine().get_spins()
+    lat = spec("fcc", (4, 4, 4), (1, 1, 1))
+
+    def f(r3):
+        r = math.sqrt(r3[0] * r3[0] + r3[1] * r3[1] + r3[2] * r3[2])
+        return 2.0 * math.exp(-((r - 0.7) * (r - 0.7)) / (2 * (0.3 * 0.3)))
+    pairs, _ = po.functional_pairs(lat, {(0, 0): (1.05, f)})
+    rowptr, col, val = po.pairs_csr(lat.n, pairs)
+    mus, gyro, alpha, s0 = sim.sites()
+    o = po.OracleSystem(mus, gyro, alpha, nthreads=8)
+    o.add_exchange(rowptr, col, val)
+    dt, _, _ = sim.params()
+    s = np.ascontiguousarray(s0.copy())
+    for k in range(sim.max_steps):
+        o.heun_step(s, dt, k * dt)
+    assert sim.max_steps == 100
+    assert np.array_equal(got, s)

@@ -1,6 +1,7 @@
 """Host layer (include/jams_host.h) on the CPU: config parsing, lattice,
 neighbour list and CSR structure against the oracle (byte-identical), the
 reference's CrPS4 symmetry-completion count, and the C ABI exports."""
+import math
 import re
 import subprocess
 from pathlib import Path
@@ -239,3 +240,211 @@ def test_checkpoint_resume_set_up(tmp_path):
     _checkpoint_file(small, s[:8], 1)
     with pytest.raises(RuntimeError, match="holds 8 spins"):
         Simulation([base, f'sim : {{ resume = "{small}"; }};'])
+
+
+# ---- exchange-neartree / exchange-functional ------------------------------------
+
+CUBIC_BASES = {
+    "sc": [("A", [0.0, 0.0, 0.0])],
+    "bcc_ab": [("A", [0.0, 0.0, 0.0]), ("B", [0.5, 0.5, 0.5])],
+    "fcc": [("A", [0.0, 0.0, 0.0]), ("A", [0.5, 0.5, 0.0]), ("A", [0.5, 0.0, 0.5]),
+            ("A", [0.0, 0.5, 0.5])],
+}
+
+
+def distance_config(basis, size, periodic, hamiltonian, materials=("A",)):
+    mats = ", ".join(f'{{ name = "{m}"; moment = 2.0; alpha = 0.1; gyro = 1.0; spin = [0.0, 0.0, 1.0]; }}'
+                     for m in materials)
+    pos = ", ".join(f'("{t}", [{f[0]!r}, {f[1]!r}, {f[2]!r}])' for t, f in CUBIC_BASES[basis])
+    per = ", ".join("true" if p else "false" for p in periodic)
+    return f"""
+materials = ( {mats} );
+unitcell : {{ parameter = 0.3e-9; basis = ([1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]);
+             positions = ( {pos} ); symops = false; }};
+lattice : {{ size = [{size[0]}, {size[1]}, {size[2]}]; periodic = [{per}]; }};
+hamiltonians = ( {hamiltonian} );
+solver : {{ module = "llg-heun-gpu"; t_step = 1e-15; t_max = 1e-13; }};
+physics : {{ temperature = 0.0; }};
+sim : {{ seed = 1; }};
+"""
+
+
+def spec(basis, size, periodic):
+    rows = CUBIC_BASES[basis]
+    names = sorted(set(t for t, _ in rows))
+    return po.LatticeSpec(np.eye(3), [f for _, f in rows], [names.index(t) for t, _ in rows], size,
+                          periodic)
+
+
+def same_csr(sim, ref):
+    h = sim.exchange_csr()
+    for a, b in zip(h, ref):
+        assert np.array_equal(a, b)
+    return h[1].size
+
+
+def test_exchange_neartree_equals_the_template_exchange():
+    """exchange-neartree with one shell at r = 1 on sc 5^3 builds the same CSR,
+    byte for byte, as the exchange module's six nearest-neighbour vectors."""
+    ham = '{ module = "exchange-neartree"; energy_units = "meV"; interactions = ( ("A", "A", 1.0, 1.25) ); }'
+    sim = Simulation(distance_config("sc", (5, 5, 5), (1, 1, 1), ham))
+    b = S.c1(5)
+    pi, pj, pJ = po.neighbour_list(b.lat, S.sc_nn_rows(1.25))
+    nnz = same_csr(sim, po.exchange_csr(b.lat.n, pi, pj, pJ))
+    assert nnz == 125 * 6 * 3
+
+
+@pytest.mark.parametrize("periodic", [(1, 1, 1), (1, 0, 1), (0, 0, 0)])
+def test_exchange_neartree_two_materials(periodic):
+    """bcc A/B 4^3: A-B at sqrt(3)/2 (8 per site), A-A and B-B at 1 (6): the
+    host list equals the oracle's brute-force shells; periodic counts in
+    closed form."""
+    ham = ('{ module = "exchange-neartree"; energy_units = "meV"; interactions = ( '
+           '("A", "B", 0.8660254037844386, 2.0), ("A", "A", 1.0, 0.5), ("B", "B", 1.0, -0.25) ); }')
+    sim = Simulation(distance_config("bcc_ab", (4, 4, 4), periodic, ham, materials=("A", "B")))
+    lat = spec("bcc_ab", (4, 4, 4), periodic)
+    ref = po.neartree_pairs(lat, [(0, 1, 0.8660254037844386, 2.0), (0, 0, 1.0, 0.5),
+                                  (1, 1, 1.0, -0.25)], 1e-3, 1e-26)
+    nnz = same_csr(sim, po.pairs_csr(lat.n, ref))
+    if periodic == (1, 1, 1):
+        assert nnz == lat.n * 14 * 3
+    computed, _, _ = sim.exchange_counts()
+    assert computed == len(ref)
+
+
+def _fun_exponential(J0, r0, sg):
+    return lambda r3: J0 * math.exp(-(math.sqrt(r3[0] * r3[0] + r3[1] * r3[1] + r3[2] * r3[2]) - r0) / sg)
+
+
+def _fun_gauss(J0, r0, sg):
+    def f(r3):
+        r = math.sqrt(r3[0] * r3[0] + r3[1] * r3[1] + r3[2] * r3[2])
+        return J0 * math.exp(-((r - r0) * (r - r0)) / (2 * (sg * sg)))
+    return f
+
+
+def _fun_rkky(J0, r0, kf):
+    def f(r3):
+        r = math.sqrt(r3[0] * r3[0] + r3[1] * r3[1] + r3[2] * r3[2])
+        kr = 2 * kf * (r - r0)
+        return -J0 * (kr * math.cos(kr) - math.sin(kr)) / (kr * kr * kr * kr)
+    return f
+
+
+def _fun_kaneyoshi(J0, r0, sg):
+    def f(r3):
+        r = math.sqrt(r3[0] * r3[0] + r3[1] * r3[1] + r3[2] * r3[2])
+        return J0 * ((r - r0) * (r - r0)) * math.exp(-((r - r0) * (r - r0)) / (2 * (sg * sg)))
+    return f
+
+
+def _fun_step(J0, rc):
+    def f(r3):
+        r = math.sqrt(r3[0] * r3[0] + r3[1] * r3[1] + r3[2] * r3[2])
+        return J0 if (r - rc) < max(abs(r), abs(rc)) * 1e-4 else 0.0
+    return f
+
+
+FUNCTIONALS = [
+    ("exponential", "[3.0, 1.0, 0.4]", _fun_exponential(3.0, 1.0, 0.4)),
+    ("gaussian", "[2.0, 1.1, 0.3]", _fun_gauss(2.0, 1.1, 0.3)),
+    ("rkky", "[1.5, 0.0, 1.3]", _fun_rkky(1.5, 0.0, 1.3)),
+    ("kaneyoshi", "[4.0, 0.2, 0.5]", _fun_kaneyoshi(4.0, 0.2, 0.5)),
+    ("gaussian_multi", "[1.0, 0.7, 0.2, -0.5, 1.0, 0.3, 0.25, 1.4, 0.2]",
+     lambda r3: (_fun_gauss(1.0, 0.7, 0.2)(r3) + _fun_gauss(-0.5, 1.0, 0.3)(r3)) + _fun_gauss(0.25, 1.4, 0.2)(r3)),
+    ("step", "[0.75, 1.0]", _fun_step(0.75, 1.0)),
+]
+
+
+@pytest.mark.parametrize("name,params,f", FUNCTIONALS, ids=[x[0] for x in FUNCTIONALS])
+def test_exchange_functional_matches_oracle(name, params, f):
+    """fcc 3^3 (periodic in x, y), r_cutoff 1.45 (the 12 + 6 + 24 shells
+    where periodic): the host list and values equal the oracle's brute force
+    with the functional written out, bit for bit."""
+    ham = (f'{{ module = "exchange-functional"; energy_units = "meV"; interactions = ( '
+           f'("A", "A", "{name}", 1.45, {params}) ); }}')
+    sim = Simulation(distance_config("fcc", (3, 3, 3), (1, 1, 0), ham))
+    lat = spec("fcc", (3, 3, 3), (1, 1, 0))
+    ref, counter = po.functional_pairs(lat, {(0, 0): (1.45, f)})
+    same_csr(sim, po.pairs_csr(lat.n, ref))
+    computed, _, _ = sim.exchange_counts()
+    assert computed == counter
+
+
+def test_exchange_functional_c3z_and_units():
+    """c3z (sin/cos of three 120-degree rotated q vectors in the plane), with
+    distances given in nm (parameter 0.3 nm): equal to the oracle; with the
+    default symmetry check the odd sin terms are refused."""
+    a = 0.3
+    q = [2.0, 0.5, 0.0, 1.0, -1.0, 0.0]          # in 1 / nm
+    rest = [1.0, 0.4, -0.3, 0.3, 0.15, 0.2, 0.25, 0.3]  # J0 J1s J1c (meV), d0 l0 l1s l1c rstar (nm)
+    params = ", ".join(repr(x) for x in q + rest)
+    # the sin terms are odd in r, so the matrix is not symmetric: checked off
+    ham = (f'{{ module = "exchange-functional"; energy_units = "meV"; distance_units = "nm"; '
+           f'symmetry_check = "none"; interactions = ( ("A", "A", "c3z", 0.44, [{params}]) ); }}')
+    sim = Simulation(distance_config("sc", (5, 5, 3), (1, 1, 1), ham))
+    conv = 1e-9 / 0.3e-9
+    qs1 = [(1.0 / conv) * x for x in q[:3]]
+    qc1 = [(1.0 / conv) * x for x in q[3:6]]
+    J0, J1s, J1c = rest[:3]
+    d0, l0, l1s, l1c, rs = [conv * x for x in rest[3:]]
+
+    def rot(t, v):
+        c, s = math.cos(t), math.sin(t)
+        return [c * v[0] + -s * v[1] + 0.0 * v[2], s * v[0] + c * v[1] + 0.0 * v[2],
+                0.0 * v[0] + 0.0 * v[1] + 1.0 * v[2]]
+    qs = [qs1, rot(2 * math.pi / 3.0, qs1), rot(4 * math.pi / 3.0, qs1)]
+    qc = [qc1, rot(2 * math.pi / 3.0, qc1), rot(4 * math.pi / 3.0, qc1)]
+
+    def f(r3):
+        r = math.sqrt(r3[0] * r3[0] + r3[1] * r3[1] + r3[2] * r3[2])
+        rp = [r3[0], r3[1], 0.0]
+        t0 = J0 * math.exp(-abs(r - d0) / l0)
+        ss = 0.0
+        for v in qs:
+            ss += math.sin(v[0] * rp[0] + v[1] * rp[1] + v[2] * rp[2])
+        ts = J1s * math.exp(-abs(r - rs) / l1s) * ss
+        cs = 0.0
+        for v in qc:
+            cs += math.cos(v[0] * rp[0] + v[1] * rp[1] + v[2] * rp[2])
+        tc = J1c * math.exp(-abs(r - rs) / l1c) * cs
+        return (t0 + ts) + tc
+    ref, _ = po.functional_pairs(spec("sc", (5, 5, 3), (1, 1, 1)), {(0, 0): (0.44 * conv, f)})
+    same_csr(sim, po.pairs_csr(75, ref))
+    with pytest.raises(RuntimeError, match="not symmetric"):
+        Simulation(distance_config("sc", (5, 5, 3), (1, 1, 1), ham.replace('symmetry_check = "none"; ', '')))
+
+
+def test_exchange_functional_symmetry_checks_and_errors():
+    """A one-way A -> B functional is not symmetric: refused by default,
+    accepted with symmetry_check = "none", and completed by "force_symmetric"
+    (the missing B -> A entries copied); set-up errors as the reference's."""
+    one_way = '("A", "B", "exponential", 0.9, [1.0, 0.866, 0.5])'
+    mats = ("A", "B")
+
+    def ham(extra="", inter=one_way):
+        return f'{{ module = "exchange-functional"; energy_units = "meV"; {extra} interactions = ( {inter} ); }}'
+    with pytest.raises(RuntimeError, match="not symmetric"):
+        Simulation(distance_config("bcc_ab", (3, 3, 3), (1, 1, 1), ham(), mats))
+    none = Simulation(distance_config("bcc_ab", (3, 3, 3), (1, 1, 1), ham('symmetry_check = "none";'), mats))
+    assert none.exchange_csr()[1].size == 27 * 8 * 3
+    forced = Simulation(distance_config("bcc_ab", (3, 3, 3), (1, 1, 1),
+                                        ham('symmetry_check = "force_symmetric";'), mats))
+    rp, col, val = forced.exchange_csr()
+    assert col.size == 54 * 8 * 3
+    both = Simulation(distance_config("bcc_ab", (3, 3, 3), (1, 1, 1), ham(
+        inter=one_way + ', ("B", "A", "exponential", 0.9, [1.0, 0.866, 0.5])'), mats))
+    for a, b in zip(forced.exchange_csr(), both.exchange_csr()):
+        assert np.array_equal(a, b)
+    for bad, msg in [('("A", "A", "cubic", 0.9, [1.0])', "unknown exchange functional"),
+                     ('("A", "A", "gaussian", 0.9, [1.0, 1.0])', "expects 3 parameters"),
+                     ('("A", "A", "gaussian", 0.9, [1.0, 1.0, 0.0])', "non-zero parameter"),
+                     ('("A", "A", "step", 5.0, [1.0, 1.0])', "larger than the maximum cutoff"),
+                     ('("A", "A", "step", 0.9, [1.0, 1.0]), ("A", "A", "step", 0.9, [1.0, 1.0])',
+                      "defined more than once"),
+                     ('("A", "C", "step", 0.9, [1.0, 1.0])', "does not exist")]:
+        with pytest.raises(RuntimeError, match=msg):
+            Simulation(distance_config("bcc_ab", (3, 3, 3), (1, 1, 1), ham(inter=bad), mats))
+    with pytest.raises(RuntimeError, match="distance units"):
+        Simulation(distance_config("bcc_ab", (3, 3, 3), (1, 1, 1),
+                                   ham('distance_units = "A";', inter='("A", "A", "step", 0.9, [1.0, 1.0])'), mats))
