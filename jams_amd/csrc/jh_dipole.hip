@@ -149,17 +149,26 @@ __global__ void __launch_bounds__(256) k_dip_mult_cplx(typename fft::C2<T>::type
 // both the spin arrays and the padded grid); each thread then reads its own
 // line from LDS (odd strides: no bank conflicts).
 constexpr int kDirectP = 64, kDirectL = 32;
+constexpr size_t kDirectLds = 64 * 1024;
 
 __host__ __device__ inline int odd_stride(int len) { return len | 1; }
 
 // LDS bytes of a direct z workgroup of T lines: twiddles, the complex side
 // (outputs forward, inputs inverse), the real side and the line tables
 template <class T>
-size_t direct_lds(int T_, int n, int n_in, int n_out, bool inverse) {
+__host__ __device__ inline size_t direct_stage(int T_, int n_in, int n_out, bool inverse) {
   using T2 = typename fft::C2<T>::type;
   const int nc = inverse ? n_in : n_out, nr = inverse ? n_out : n_in;
-  return (size_t)n * sizeof(T2) + (size_t)T_ * odd_stride(nc) * sizeof(T2) +
-         (size_t)T_ * odd_stride(nr) * sizeof(T) + (size_t)T_ * (sizeof(int64_t) * 2 + sizeof(int));
+  const size_t c = (size_t)T_ * odd_stride(nc) * sizeof(T2), r = (size_t)T_ * odd_stride(nr) * sizeof(T);
+  return ((c > r ? c : r) + 15) & ~(size_t)15;
+}
+// the input and output staging share one area (they are live one after the
+// other, with a barrier between)
+template <class T>
+size_t direct_lds(int T_, int n, int n_in, int n_out, bool inverse) {
+  using T2 = typename fft::C2<T>::type;
+  return (size_t)n * sizeof(T2) + direct_stage<T>(T_, n_in, n_out, inverse) +
+         (size_t)T_ * (sizeof(int64_t) * 2 + sizeof(int));
 }
 
 // LM: register bound of the line (L2 <= LM, P2 <= 2 LM), 8, 16 or 32
@@ -173,9 +182,9 @@ __global__ void __launch_bounds__(256) k_dip_z_direct_fwd(const Pass<T> p) {
   const int SV = odd_stride(n_in), SO = odd_stride(n_out);
   T2 *tw = reinterpret_cast<T2 *>(smem);
   T2 *outs = tw + n;
-  T *vals = reinterpret_cast<T *>(outs + (size_t)nt * SO);
-  // (nt is a multiple of 64, so the int64 tables stay 8-byte aligned)
-  int64_t *lidx = reinterpret_cast<int64_t *>(vals + (size_t)nt * SV);
+  T *vals = reinterpret_cast<T *>(outs);  // the same area, used first
+  int64_t *lidx = reinterpret_cast<int64_t *>(reinterpret_cast<unsigned char *>(outs) +
+                                              direct_stage<T>(nt, n_in, n_out, false));
   int64_t *lout = lidx + nt;
   int *lcomp = reinterpret_cast<int *>(lout + nt);
   const int64_t L0 = (int64_t)blockIdx.x * nt;
@@ -211,6 +220,7 @@ __global__ void __launch_bounds__(256) k_dip_z_direct_fwd(const Pass<T> p) {
     T v[LM];
 #pragma unroll
     for (int z = 0; z < LM; ++z) v[z] = z < n_in ? vals[tid * SV + z] : (T)0;
+    __syncthreads();  // vals is overwritten by outs below
     for (int q = 0; q < n_out; ++q) {
       T2 acc;
       acc.x = 0;
@@ -251,8 +261,9 @@ __global__ void __launch_bounds__(256) k_dip_z_direct_inv(const Pass<T> p) {
   const int SX = odd_stride(n_in), SZ = odd_stride(n_out);
   T2 *tw = reinterpret_cast<T2 *>(smem);
   T2 *xs = tw + n;
-  T *outs = reinterpret_cast<T *>(xs + (size_t)nt * SX);
-  int64_t *lidx = reinterpret_cast<int64_t *>(outs + (size_t)nt * SZ);
+  T *outs = reinterpret_cast<T *>(xs);  // the same area, used second
+  int64_t *lidx = reinterpret_cast<int64_t *>(reinterpret_cast<unsigned char *>(xs) +
+                                              direct_stage<T>(nt, n_in, n_out, true));
   int64_t *lin = lidx + nt;
   int *lcomp = reinterpret_cast<int *>(lin + nt);
   const int64_t L0 = (int64_t)blockIdx.x * nt;
@@ -289,6 +300,7 @@ __global__ void __launch_bounds__(256) k_dip_z_direct_inv(const Pass<T> p) {
 #pragma unroll
     for (int q = 0; q <= LM; ++q)
       if (q < n_in) X[q] = xs[tid * SX + q];
+    __syncthreads();  // xs is overwritten by outs below
     for (int z = 0; z < n_out; ++z) {
       // Re sum_k X[k] e^{+2 pi i k z/n} over the Hermitian sequence:
       // X[0] + 2 Re(X[k] w^{kz}) for 0 < k < n/2 (+ X[n/2] (-1)^z, n even)
@@ -325,7 +337,7 @@ __global__ void __launch_bounds__(256) k_dip_z_direct_inv(const Pass<T> p) {
 template <class T>
 int direct_threads(int n, int n_in, int n_out, bool inverse) {
   int t = 256;
-  while (t > 64 && direct_lds<T>(t, n, n_in, n_out, inverse) > 64 * 1024) t /= 2;
+  while (t > 64 && direct_lds<T>(t, n, n_in, n_out, inverse) > kDirectLds) t /= 2;
   return t;
 }
 
