@@ -889,6 +889,9 @@ KParams make_params(jh_ctx *c) {
   p.n = c->n;
   p.s = c->s.p;
   p.sp = c->sp.p;
+  // with no dipole term nothing but the corrector reads s*, and the FP32
+  // form's corrector forms it from s and ds
+  p.spfree = c->prec == PR_FP32 && dipole_term(c) < 0 && c->integrator == JH_INTEGRATOR_HEUN;
   p.ds = c->ds.p;
   p.snew = c->s.p;
   p.cls = c->nclass > 1 ? c->cls.p : nullptr;
@@ -1475,8 +1478,11 @@ int jh_phase_bytes(jh_ctx *c, int phase, double *fmt_bytes, double *csr_bytes) {
     // per-spin state traffic: the FP64 vector the field is evaluated at (its
     // gathers are counted above: in FP64 they are the same array, so only
     // the FP32 form adds its own FP64 read), plus the Heun state
+    // (the FP32 form without a dipole term neither stores s* nor reads it back)
+    const bool spfree = make_params(c).spfree != 0;
     double state = c->prec == PR_FP32 || c->xv == XV_NONE ? 3 * n * 8 : 0.0;
-    if (phase == 0) state += 3 * n * 8 /*s* w*/ + 3 * n * 8 /*ds w*/;
+    if (phase == 1 && spfree) state = 0.0;
+    if (phase == 0) state += (spfree ? 0.0 : 3 * n * 8 /*s* w*/) + 3 * n * 8 /*ds w*/;
     else state += 3 * n * 8 /*s_old r*/ + 3 * n * 8 /*ds r*/ + 3 * n * 8 /*s w*/;
     if (c->prec == PR_FP32) state += kG32 * n * 4;  // FP32 gather copy written
     if (c->nclass > 1) state += 2 * n;

@@ -106,6 +106,9 @@ struct KParams {
   int64_t n;
   const double *s;     // current spins (phase 0 source, phase 1 s_old)
   double *sp;          // predictor spins
+  // FP32 form without a precomputed field term: the predictor does not store
+  // s* in FP64 and the corrector recomputes it from s and ds (bit for bit)
+  int spfree;
   double *ds;          // ds/dt accumulator
   double *snew;        // phase 1 output (== s)
   double *sy;          // RK4: the next stage's input written by stages 0-2

@@ -400,7 +400,17 @@ __attribute__((amdgpu_waves_per_eu(kMinWaves<PREC, XVk>))) k_heun(const KParams 
 
   // spin that the field is evaluated at
   const double *src = PHASE == 0 ? p.s : p.sp;
-  const double s0 = src[i], s1 = src[n + i], s2 = src[2 * n + i];
+  double s0, s1, s2;
+  if (PREC == PR_FP32 && PHASE == 1 && p.spfree) {
+    // s* = unit(s + dt rhs) as the predictor formed it: 2 ds is its rhs
+    // exactly (ds = rhs / 2; only a subnormal rhs would lose its last bit)
+    s0 = p.s[i] + p.dt * (2.0 * p.ds[i]);
+    s1 = p.s[n + i] + p.dt * (2.0 * p.ds[n + i]);
+    s2 = p.s[2 * n + i] + p.dt * (2.0 * p.ds[2 * n + i]);
+    unit_vector(s0, s1, s2);
+  } else {
+    s0 = src[i]; s1 = src[n + i]; s2 = src[2 * n + i];
+  }
   const double t = PHASE == 0 ? t0 : t0 + p.dt;
 
   // thermal field w of this step (solvers/cpu_llg_heun.cc:53-62): one draw
@@ -446,7 +456,9 @@ __attribute__((amdgpu_waves_per_eu(kMinWaves<PREC, XVk>))) k_heun(const KParams 
     p.ds[i] = 0.5 * r0; p.ds[n + i] = 0.5 * r1; p.ds[2 * n + i] = 0.5 * r2;
     double v0 = s0 + p.dt * r0, v1 = s1 + p.dt * r1, v2 = s2 + p.dt * r2;
     unit_vector(v0, v1, v2);
-    p.sp[i] = v0; p.sp[n + i] = v1; p.sp[2 * n + i] = v2;
+    if (!(PREC == PR_FP32 && p.spfree)) {
+      p.sp[i] = v0; p.sp[n + i] = v1; p.sp[2 * n + i] = v2;
+    }
     if constexpr (PREC == PR_FP32) store_g32(p.w32, i, v0, v1, v2);
   } else {
     const double d0 = p.ds[i] + 0.5 * r0;
