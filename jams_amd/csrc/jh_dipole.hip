@@ -11,7 +11,7 @@
 // FFT forms are one-sided (CPU) or unpadded (GPU) on open dimensions; this
 // build does not copy that.
 //
-// Per evaluation (spins s, SoA, reference order):
+// Per evaluation (spins s in the jh_layout.h layout, reference order):
 //   z pass   r2c along the contiguous axis, fused with the scatter of mus s
 //            into the zero-padded grid (only the L0 x L1 occupied lines);
 //   y pass   forward, inputs pruned to y < L1, lines x < L0 only;
@@ -211,7 +211,7 @@ __global__ void __launch_bounds__(256) k_dip_z_direct_fwd(const Pass<T> p) {
     T v = 0;
     if (base >= 0) {
       const int64_t idx = base + (int64_t)z * p.grid.nb;
-      v = (T)(p.grid.mus[idx] * p.spins[lcomp[b] * p.grid.n + idx]);
+      v = (T)(p.grid.mus[idx] * p.spins[spin_idx(p.grid.n, idx, lcomp[b])]);
     }
     vals[b * SV + z] = v;
   }

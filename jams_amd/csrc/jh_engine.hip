@@ -259,22 +259,26 @@ inline int64_t dev_of(const jh_ctx *c, int64_t i) {
   return c->order_nb == 1 ? i : (i % c->order_nb) * c->order_cells + i / c->order_nb;
 }
 
-// reference AoS (N x 3) -> device SoA in device order
-void upload_soa(jh_ctx *c, const double *aos, double *dst) {
+// index of component k of element d of a device vector array: a spin array
+// (jh_layout.h) or SoA
+inline int64_t vec_idx(int64_t n, int64_t d, int k, bool spin) {
+  return spin ? spin_idx(n, d, k) : k * n + d;
+}
+
+// reference AoS (N x 3) -> device layout in device order
+void upload_vec(jh_ctx *c, const double *aos, double *dst, bool spin) {
   const int64_t n = c->n;
   c->h_stage.resize((size_t)(3 * n));
   double *st = c->h_stage.data();
   parallel_for(n, [&](int64_t d) {
     const int64_t i = ref_of(c, d);
-    st[d] = aos[3 * i];
-    st[n + d] = aos[3 * i + 1];
-    st[2 * n + d] = aos[3 * i + 2];
+    for (int k = 0; k < 3; ++k) st[vec_idx(n, d, k, spin)] = aos[3 * i + k];
   });
   HIPCHK(hipMemcpyAsync(dst, st, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
 }
 
-void download_aos(jh_ctx *c, const double *src, double *aos) {
+void download_vec(jh_ctx *c, const double *src, double *aos, bool spin) {
   const int64_t n = c->n;
   c->h_stage.resize((size_t)(3 * n));
   double *st = c->h_stage.data();
@@ -282,9 +286,7 @@ void download_aos(jh_ctx *c, const double *src, double *aos) {
   HIPCHK(hipStreamSynchronize(c->stream));
   parallel_for(n, [&](int64_t d) {
     const int64_t i = ref_of(c, d);
-    aos[3 * i] = st[d];
-    aos[3 * i + 1] = st[n + d];
-    aos[3 * i + 2] = st[2 * n + d];
+    for (int k = 0; k < 3; ++k) aos[3 * i + k] = st[vec_idx(n, d, k, spin)];
   });
 }
 
@@ -839,16 +841,16 @@ void prepare(jh_ctx *c) {
   const int nb = (!c->xh.empty() && dipole_term(c) < 0 && c->nrep == 0) ? choose_order(c->xh[0]) : 1;
   if (nb != c->order_nb) {
     std::vector<double> tmp_s((size_t)(3 * c->n)), tmp_w;
-    download_aos(c, c->s.p, tmp_s.data());
+    download_vec(c, c->s.p, tmp_s.data(), true);
     const bool inj = c->noise_mode == NM_INJECTED && c->winj.p;
     if (inj) {
       tmp_w.resize((size_t)(3 * c->n));
-      download_aos(c, c->winj.p, tmp_w.data());
+      download_vec(c, c->winj.p, tmp_w.data(), false);
     }
     c->order_nb = nb;
     c->order_cells = c->n / nb;
-    upload_soa(c, tmp_s.data(), c->s.p);
-    if (inj) upload_soa(c, tmp_w.data(), c->winj.p);
+    upload_vec(c, tmp_s.data(), c->s.p, true);
+    if (inj) upload_vec(c, tmp_w.data(), c->winj.p, false);
   }
   c->drop_graph();
   if (c->nrep > 0)
@@ -1143,7 +1145,7 @@ int jh_set_spins(jh_ctx *c, const double *s) {
   return guarded(c, [&] {
     require_sites(c);
     if (!s) JH_THROW(JH_ERR_INVALID, "jh_set_spins: null");
-    upload_soa(c, s, c->s.p);
+    upload_vec(c, s, c->s.p, true);
     c->spins_set = true;
     if (!c->dirty) sync_f32(c);
   });
@@ -1153,7 +1155,7 @@ int jh_get_spins(jh_ctx *c, double *s) {
   return guarded(c, [&] {
     require_sites(c);
     if (!s) JH_THROW(JH_ERR_INVALID, "jh_get_spins: null");
-    download_aos(c, c->s.p, s);
+    download_vec(c, c->s.p, s, true);
   });
 }
 
@@ -1284,7 +1286,7 @@ int jh_set_noise(jh_ctx *c, const double *w) {
     require_sites(c);
     if (!w) JH_THROW(JH_ERR_INVALID, "jh_set_noise: null");
     c->winj.alloc((size_t)(3 * c->n));
-    upload_soa(c, w, c->winj.p);
+    upload_vec(c, w, c->winj.p, false);
     c->noise_mode = NM_INJECTED;
   });
 }
@@ -1296,7 +1298,7 @@ int jh_get_noise(jh_ctx *c, int64_t step, double *w) {
     KParams p = make_params(c);
     p.step = (uint64_t)step;
     HIPCHK(launch_noise(p, c->fout.p, c->stream));
-    download_aos(c, c->fout.p, w);
+    download_vec(c, c->fout.p, w, false);
   });
 }
 
@@ -1358,7 +1360,7 @@ int jh_compute_term_field(jh_ctx *c, int term, double t, double *out) {
     prepare(c);
     if (term < 0 || term >= (int)c->terms.size()) JH_THROW(JH_ERR_INVALID, "term id out of range");
     field_into(c, term, t, c->fout.p);
-    download_aos(c, c->fout.p, out);
+    download_vec(c, c->fout.p, out, false);
   });
 }
 
@@ -1366,7 +1368,7 @@ int jh_compute_fields(jh_ctx *c, double t, double *out) {
   return guarded(c, [&] {
     prepare(c);
     field_into(c, -1, t, c->fout.p);
-    download_aos(c, c->fout.p, out);
+    download_vec(c, c->fout.p, out, false);
   });
 }
 

@@ -21,6 +21,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "jh_layout.h"
+
 namespace jh {
 namespace fft {
 
@@ -107,7 +109,7 @@ struct Pass {
   // LD_HERM: the line holds the first n_in = n/2 + 1 terms of a Hermitian
   // sequence of length n
   SpinGrid grid;
-  const double *spins;   // SoA FP64
+  const double *spins;   // FP64 spin array (jh_layout.h)
   double *field;         // SoA FP64
 };
 
@@ -363,7 +365,7 @@ __global__ void __launch_bounds__(kThreads) k_fft_pass(const Pass<T> p) {
         if constexpr (LD == LD_SPIN) {
           if (k < p.n_in) {
             const int64_t idx = base + (int64_t)k * p.grid.nb;
-            v.x = (T)(p.grid.mus[idx] * p.spins[comp * p.grid.n + idx]);
+            v.x = (T)(p.grid.mus[idx] * p.spins[spin_idx(p.grid.n, idx, comp)]);
           }
         } else if constexpr (LD == LD_HERM) {
           if (k < p.n_in) {

@@ -2,8 +2,11 @@
 // kernels.
 //
 // Device layout (HBM):
-//  - spin-like vectors (s, predictor s*, ds, noise, fields) are SoA FP64:
-//    one allocation of 3N doubles, component c of spin i at [c*N + i];
+//  - FP64 spin arrays (s, predictor s*, RK4 stage inputs) are one
+//    allocation of 3N doubles with (x, y) of spin i at [2i], [2i + 1] and z
+//    at [2N + i] (jh_layout.h);
+//  - the other vectors (ds, noise, fields) are SoA FP64: component c of
+//    spin i at [c*N + i];
 //  - the exchange operator is a sliced-ELL over spins: slices of 64
 //    consecutive spins (one wavefront), each padded to the longest row in the
 //    slice and stored column-major, so lane l of slot k reads entry
@@ -31,6 +34,8 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include "jh_layout.h"
 
 namespace jh {
 

@@ -63,7 +63,8 @@ __device__ __forceinline__ void xfield64(const XView &x, const double *__restric
   for (int64_t e = b0 + lane; e < b1; e += kSlice) {
     const uint32_t w = __builtin_nontemporal_load(x.ent + e);
     const uint32_t j = w & x.colmask;
-    const double sx = src[j], sy = src[n + j], sz = src[2 * n + j];
+    double sx, sy, sz;
+    load_spin(src, n, j, sx, sy, sz);
     if constexpr (XVk == XV_TDIAG) {
       const double *T = tab + 3 * (w >> x.colbits);
       a0 = a0 + sx * T[0];
@@ -152,7 +153,8 @@ __device__ __forceinline__ void pfield64(const XView &x, const double *__restric
     const uint32_t w = pat[o + k];
     const int64_t j = i + (int64_t)(((int32_t)w) >> x.tbits);
     const uint32_t t = w & tmask;
-    const double sx = src[j], sy = src[n + j], sz = src[2 * n + j];
+    double sx, sy, sz;
+    load_spin(src, n, j, sx, sy, sz);
     if constexpr (xv_diag(XVk)) {
       const double *T = tab + 3 * t;
       a0 = a0 + sx * T[0];
@@ -404,12 +406,13 @@ __attribute__((amdgpu_waves_per_eu(kMinWaves<PREC, XVk>))) k_heun(const KParams 
   if (PREC == PR_FP32 && PHASE == 1 && p.spfree) {
     // s* = unit(s + dt rhs) as the predictor formed it: 2 ds is its rhs
     // exactly (ds = rhs / 2; only a subnormal rhs would lose its last bit)
-    s0 = p.s[i] + p.dt * (2.0 * p.ds[i]);
-    s1 = p.s[n + i] + p.dt * (2.0 * p.ds[n + i]);
-    s2 = p.s[2 * n + i] + p.dt * (2.0 * p.ds[2 * n + i]);
+    load_spin(p.s, n, i, s0, s1, s2);
+    s0 = s0 + p.dt * (2.0 * p.ds[i]);
+    s1 = s1 + p.dt * (2.0 * p.ds[n + i]);
+    s2 = s2 + p.dt * (2.0 * p.ds[2 * n + i]);
     unit_vector(s0, s1, s2);
   } else {
-    s0 = src[i]; s1 = src[n + i]; s2 = src[2 * n + i];
+    load_spin(src, n, i, s0, s1, s2);
   }
   const double t = PHASE == 0 ? t0 : t0 + p.dt;
 
@@ -456,17 +459,17 @@ __attribute__((amdgpu_waves_per_eu(kMinWaves<PREC, XVk>))) k_heun(const KParams 
     p.ds[i] = 0.5 * r0; p.ds[n + i] = 0.5 * r1; p.ds[2 * n + i] = 0.5 * r2;
     double v0 = s0 + p.dt * r0, v1 = s1 + p.dt * r1, v2 = s2 + p.dt * r2;
     unit_vector(v0, v1, v2);
-    if (!(PREC == PR_FP32 && p.spfree)) {
-      p.sp[i] = v0; p.sp[n + i] = v1; p.sp[2 * n + i] = v2;
-    }
+    if (!(PREC == PR_FP32 && p.spfree)) store_spin(p.sp, n, i, v0, v1, v2);
     if constexpr (PREC == PR_FP32) store_g32(p.w32, i, v0, v1, v2);
   } else {
     const double d0 = p.ds[i] + 0.5 * r0;
     const double d1 = p.ds[n + i] + 0.5 * r1;
     const double d2 = p.ds[2 * n + i] + 0.5 * r2;
-    double v0 = p.s[i] + p.dt * d0, v1 = p.s[n + i] + p.dt * d1, v2 = p.s[2 * n + i] + p.dt * d2;
+    double v0, v1, v2;
+    load_spin(p.s, n, i, v0, v1, v2);
+    v0 = v0 + p.dt * d0; v1 = v1 + p.dt * d1; v2 = v2 + p.dt * d2;
     unit_vector(v0, v1, v2);
-    p.snew[i] = v0; p.snew[n + i] = v1; p.snew[2 * n + i] = v2;
+    store_spin(p.snew, n, i, v0, v1, v2);
     if constexpr (PREC == PR_FP32) store_g32(p.w32, i, v0, v1, v2);
   }
 }
@@ -501,7 +504,8 @@ __attribute__((amdgpu_waves_per_eu(kMinWaves<PREC, XVk>))) k_rk4(const KParams p
   const double t = stage == 0 ? t0 : stage == 3 ? t0 + p.dt : t0 + half;
 
   const double *src = stage == 0 ? p.s : p.sp;
-  const double y0 = src[i], y1 = src[n + i], y2 = src[2 * n + i];
+  double y0, y1, y2;
+  load_spin(src, n, i, y0, y1, y2);
   double h0, h1, h2;
   total_field<PREC, XVk, GEN>(p, src, p.g32, i, c, lds, y0, y1, y2, t, h0, h1, h2);
 
@@ -532,7 +536,8 @@ __attribute__((amdgpu_waves_per_eu(kMinWaves<PREC, XVk>))) k_rk4(const KParams p
   double r0, r1, r2;
   llg_rhs(y0, y1, y2, h0, h1, h2, cparam(p, c, CF_GYRO), cparam(p, c, CF_ALPHA), r0, r1, r2);
 
-  const double s0 = p.s[i], s1 = p.s[n + i], s2 = p.s[2 * n + i];
+  double s0, s1, s2;
+  load_spin(p.s, n, i, s0, s1, s2);
   if (stage < 3) {
     if (stage == 0) {
       p.ds[i] = r0; p.ds[n + i] = r1; p.ds[2 * n + i] = r2;
@@ -543,7 +548,7 @@ __attribute__((amdgpu_waves_per_eu(kMinWaves<PREC, XVk>))) k_rk4(const KParams p
     }
     const double cst = stage == 2 ? p.dt : half;
     const double v0 = s0 + cst * r0, v1 = s1 + cst * r1, v2 = s2 + cst * r2;
-    p.sy[i] = v0; p.sy[n + i] = v1; p.sy[2 * n + i] = v2;
+    store_spin(p.sy, n, i, v0, v1, v2);
     if constexpr (PREC == PR_FP32) store_g32(p.w32, i, v0, v1, v2);
   } else {
     double v0 = s0 + (p.dt * (p.ds[i] + r0)) / 6.0;
@@ -551,7 +556,7 @@ __attribute__((amdgpu_waves_per_eu(kMinWaves<PREC, XVk>))) k_rk4(const KParams p
     double v2 = s2 + (p.dt * (p.ds[2 * n + i] + r2)) / 6.0;
     const double rn = 1.0 / sqrt(v0 * v0 + v1 * v1 + v2 * v2);
     v0 = v0 * rn; v1 = v1 * rn; v2 = v2 * rn;
-    p.snew[i] = v0; p.snew[n + i] = v1; p.snew[2 * n + i] = v2;
+    store_spin(p.snew, n, i, v0, v1, v2);
     if constexpr (PREC == PR_FP32) store_g32(p.w32, i, v0, v1, v2);
   }
 }
@@ -567,7 +572,8 @@ __global__ void __launch_bounds__(kBlock) k_field(const KParams p, int term, dou
   if (i >= p.n) return;
   const int64_t n = p.n;
   const uint32_t c = p.cls ? (uint32_t)p.cls[i] : 0u;
-  const double s0 = p.s[i], s1 = p.s[n + i], s2 = p.s[2 * n + i];
+  double s0, s1, s2;
+  load_spin(p.s, n, i, s0, s1, s2);
   double h0, h1, h2;
   if (term < 0) {
     total_field<PREC, XVk, true>(p, p.s, p.g32, i, c, lds, s0, s1, s2, p.t, h0, h1, h2);
@@ -592,10 +598,14 @@ __global__ void __launch_bounds__(kBlock) k_noise(const KParams p, double *out) 
   out[2 * p.n + i] = z2 * sg * sT;
 }
 
-// FP32 gather copy of SoA FP64 spins (kG32 floats per spin)
+// FP32 gather copy of FP64 spins (kG32 floats per spin)
 __global__ void __launch_bounds__(kBlock) k_to_f32(const double *a, float *b, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (i < n) store_g32(b, i, a[i], a[n + i], a[2 * n + i]);
+  if (i < n) {
+    double x, y, z;
+    load_spin(a, n, i, x, y, z);
+    store_g32(b, i, x, y, z);
+  }
 }
 
 // Deterministic pairwise block reduction of 4 FP64 quantities per spin:
@@ -610,7 +620,8 @@ __global__ void __launch_bounds__(kBlock) k_reduce(const KParams p, int mode, co
   if (i < p.n) {
     const int64_t n = p.n;
     const uint32_t c = p.cls ? (uint32_t)p.cls[i] : 0u;
-    const double s0 = p.s[i], s1 = p.s[n + i], s2 = p.s[2 * n + i];
+    double s0, s1, s2;
+    load_spin(p.s, n, i, s0, s1, s2);
     if (mode == 0) {
       const double mu = cparam(p, c, CF_MUS);
       v[0] = mu * s0; v[1] = mu * s1; v[2] = mu * s2; v[3] = mu;
@@ -660,9 +671,11 @@ __global__ void __launch_bounds__(kBlock) k_reduce_replicas(const KParams p, con
   for (int64_t i = a + threadIdx.x; i < b; i += kBlock) {
     const uint32_t c = p.cls ? (uint32_t)p.cls[i] : 0u;
     const double mu = cparam(p, c, CF_MUS);
-    v[0] = v[0] + mu * p.s[i];
-    v[1] = v[1] + mu * p.s[n + i];
-    v[2] = v[2] + mu * p.s[2 * n + i];
+    double s0, s1, s2;
+    load_spin(p.s, n, i, s0, s1, s2);
+    v[0] = v[0] + mu * s0;
+    v[1] = v[1] + mu * s1;
+    v[2] = v[2] + mu * s2;
     v[3] = v[3] + mu;
   }
   for (int q = 0; q < 4; ++q) red[q][threadIdx.x] = v[q];
