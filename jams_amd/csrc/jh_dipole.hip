@@ -34,6 +34,7 @@
 #include <thread>
 #include <vector>
 
+#include "jh_device.h"
 #include "jh_dipole.h"
 #include "jh_fft.h"
 
@@ -138,6 +139,159 @@ __global__ void __launch_bounds__(256) k_dip_mult_cplx(typename fft::C2<T>::type
     }
     for (int a = 0; a < 3; ++a) K[(int64_t)(3 * i + a) * M + p] = h[a];
   }
+}
+
+// ---------------------------------------------------------------------------
+// x transforms and the multiply in one kernel (one basis site, real tensor).
+// A workgroup owns `by` adjacent y lines of one q2 plane for the three
+// components (lw = 3 by lines): it loads their x < L0 terms, runs the forward
+// x transform in LDS, multiplies by W, runs the inverse and stores x < L0.
+// Every line goes through the stages, twiddles and operations of the
+// separate passes, so the field is the same to the bit; the full-length x
+// spectra never leave LDS (for a 256 x 256 x 8 film padded to 512 x 512 x 16
+// that removes two thirds of the bytes the x passes and the multiply move).
+template <class T>
+struct XFused {
+  using T2 = typename fft::C2<T>::type;
+  int n, nst;
+  int radix[fft::kMaxStages];
+  int ns[fft::kMaxStages];
+  float ns_rcp[fft::kMaxStages];
+  int tws[fft::kMaxStages];
+  int L0;          // occupied x (inputs and outputs)
+  int by, lby;     // y lines per workgroup, by = 1 << lby
+  int lw;          // 3 by lines in LDS
+  float lw_rcp;
+  int P1, nyb;     // y lines per plane, y blocks per plane
+  int64_t M, sq;   // component stride, q2-plane stride (P0 P1)
+  T2 *K;
+  const T *W;      // [y block][6][n][by]
+  const T2 *tw;
+};
+
+// one in-place stage over the lw lines of A ([x][line]); thread t handles
+// line t mod lw, butterfly t / lw (the host keeps every thread within kBF)
+template <class T, int R, bool INV>
+__device__ __forceinline__ void xf_stage(const XFused<T> &p, const typename fft::C2<T>::type *tw,
+                                         typename fft::C2<T>::type *A, int st) {
+  using T2 = typename fft::C2<T>::type;
+  constexpr int NBF = fft::kBF<T, R>;
+  const int n = p.n, lw = p.lw, m = n / R, total = m * lw, nt = blockDim.x;
+  const int Ns = p.ns[st], tws = p.tws[st];
+  const float rns = p.ns_rcp[st];
+  T2 v[NBF][R];
+  int dq[NBF];
+#pragma unroll
+  for (int q = 0; q < NBF; ++q) {
+    const int t = threadIdx.x + q * nt;
+    if (t < total) {
+      int l, jm;
+      const int j = fft::divmod_small(t, lw, p.lw_rcp, l);
+      const int jd = fft::divmod_small(j, Ns, rns, jm);
+      dq[q] = (jd * Ns * R + jm) * lw + l;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[q][r] = A[(j + r * m) * lw + l];
+#pragma unroll
+      for (int r = 1; r < R; ++r) v[q][r] = fft::cmul(v[q][r], fft::twiddle<T, INV>(tw, jm * r * tws));
+      fft::dft_fixed<T2, R>(v[q], INV);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NBF; ++q) {
+    const int t = threadIdx.x + q * nt;
+    if (t < total) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) A[dq[q] + r * Ns * lw] = v[q][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <class T, bool INV>
+__device__ __forceinline__ void xf_stages(const XFused<T> &p, const typename fft::C2<T>::type *tw,
+                                          typename fft::C2<T>::type *A) {
+  for (int st = 0; st < p.nst; ++st) {
+    switch (p.radix[st]) {
+      case 2: xf_stage<T, 2, INV>(p, tw, A, st); break;
+      case 3: xf_stage<T, 3, INV>(p, tw, A, st); break;
+      case 4: xf_stage<T, 4, INV>(p, tw, A, st); break;
+      case 5: xf_stage<T, 5, INV>(p, tw, A, st); break;
+      default: xf_stage<T, 8, INV>(p, tw, A, st); break;
+    }
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(512) k_dip_x_fused(const XFused<T> p) {
+  using T2 = typename fft::C2<T>::type;
+  extern __shared__ __align__(16) unsigned char smem[];
+  T2 *A = reinterpret_cast<T2 *>(smem);
+  T2 *tw = A + (size_t)p.n * p.lw;
+  const int n = p.n, lw = p.lw, by = p.by, lby = p.lby, nt = blockDim.x;
+  for (int t = threadIdx.x; t < n; t += nt) tw[t] = p.tw[t];
+  // consecutive blocks on one XCD share the L2 lines of adjacent y
+  const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const int q2 = (int)(blk / p.nyb), y0 = (int)(blk % p.nyb) << lby;
+  T2 *K = p.K + (int64_t)q2 * p.sq + y0;
+  // load x < L0 (zero padding above), element e = x lw + c by + yb
+  for (int e = threadIdx.x; e < n * lw; e += nt) {
+    int l;
+    const int x = fft::divmod_small(e, lw, p.lw_rcp, l);
+    const int c = l >> lby, yb = l & (by - 1);
+    T2 v;
+    v.x = 0;
+    v.y = 0;
+    if (x < p.L0 && y0 + yb < p.P1) v = K[c * p.M + (int64_t)x * p.P1 + yb];
+    A[e] = v;
+  }
+  __syncthreads();
+  xf_stages<T, false>(p, tw, A);
+  // H(k) = W(k) S(k) with the arithmetic of k_dip_mult_real
+  {
+    const int nby = n * by;
+    const T *W = p.W + blk * 6 * nby;
+    for (int e = threadIdx.x; e < nby; e += nt) {
+      T2 *a = A + (e >> lby) * lw + (e & (by - 1));
+      const T2 sx = a[0], sy = a[by], sz = a[2 * by];
+      const T xx = W[e], xy = W[nby + e], xz = W[2 * nby + e], yy = W[3 * nby + e],
+              yz = W[4 * nby + e], zz = W[5 * nby + e];
+      T2 hx, hy, hz;
+      hx.x = xx * sx.x + xy * sy.x + xz * sz.x; hx.y = xx * sx.y + xy * sy.y + xz * sz.y;
+      hy.x = xy * sx.x + yy * sy.x + yz * sz.x; hy.y = xy * sx.y + yy * sy.y + yz * sz.y;
+      hz.x = xz * sx.x + yz * sy.x + zz * sz.x; hz.y = xz * sx.y + yz * sy.y + zz * sz.y;
+      a[0] = hx;
+      a[by] = hy;
+      a[2 * by] = hz;
+    }
+  }
+  __syncthreads();
+  xf_stages<T, true>(p, tw, A);
+  // store x < L0
+  for (int e = threadIdx.x; e < p.L0 * lw; e += nt) {
+    int l;
+    const int x = fft::divmod_small(e, lw, p.lw_rcp, l);
+    const int c = l >> lby, yb = l & (by - 1);
+    if (y0 + yb < p.P1) K[c * p.M + (int64_t)x * p.P1 + yb] = A[e];
+  }
+}
+
+// the tensor in the fused kernel's order: [y block][6][x][by] from [6][q2][x][y]
+template <class T>
+__global__ void __launch_bounds__(256) k_dip_w_fused(const T *Wr, T *Wf, int64_t M, int n, int P1,
+                                                     int lby, int nyb, int64_t total) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= total) return;
+  const int by = 1 << lby;
+  const int yb = (int)(q & (by - 1));
+  int64_t r = q >> lby;
+  const int x = (int)(r % n);
+  r /= n;
+  const int c = (int)(r % 6);
+  const int64_t blk = r / 6;
+  const int64_t q2 = blk / nyb;
+  const int y = (int)(blk % nyb) * by + yb;
+  Wf[q] = y < P1 ? Wr[c * M + q2 * n * P1 + (int64_t)x * P1 + y] : (T)0;
 }
 
 // Short contiguous axis (P2 <= kDirectP, L2 <= kDirectL): one thread per line
@@ -456,6 +610,8 @@ struct Plane {
   DBuf<T2> K;            // [3 nb][Q2][P0][P1]
   DBuf<T> Wr;            // nb == 1: [6][Q2 P0 P1]
   DBuf<T2> Wc;           // nb > 1: [nb nb][6][Q2 P0 P1]
+  DBuf<T> Wf;            // fused x kernel: [Q2 y blocks][6][P0][by] (Wr released)
+  int by = 0;            // y lines per fused workgroup, 0: separate passes
   DBuf<T2> tw[3];
   bool ready = false;
 };
@@ -489,6 +645,66 @@ class DipoleFFTImpl : public DipoleFFT {
     return p;
   }
 
+  // threads of the fused x kernel for by y lines: enough that no thread
+  // holds more than kBF butterflies of any stage
+  template <class T>
+  int xf_threads(int by) const {
+    const int n = P[0], lw = 3 * by;
+    int need = 64;
+    for (int R : rad[0]) {
+      const int kbf = R == 2 ? fft::kBF<T, 2> : R == 3 ? fft::kBF<T, 3> : R == 4 ? fft::kBF<T, 4>
+                    : R == 5 ? fft::kBF<T, 5> : fft::kBF<T, 8>;
+      need = std::max(need, (n / R * lw + kbf - 1) / kbf);
+    }
+    return (need + 63) / 64 * 64;
+  }
+  // y lines per workgroup of the fused x kernel (0: not applicable): one
+  // basis site with a real tensor, in-place radices, LDS within 64 KB (two
+  // workgroups per CU), at most 512 threads
+  template <class T>
+  int fused_lines() const {
+    if (nb != 1 || !real_tensor) return 0;
+    for (int r : rad[0])
+      if (r != 2 && r != 3 && r != 4 && r != 5 && r != 8) return 0;
+    const size_t c = sizeof(typename fft::C2<T>::type);
+    for (int by = 8; by >= 1; by /= 2)
+      if ((size_t)P[0] * (3 * by + 1) * c <= kMaxPassLds && xf_threads<T>(by) <= 512) return by;
+    return 0;
+  }
+
+  template <class T>
+  hipError_t run_x_fused(Plane<T> &pl, hipStream_t st) {
+    using T2 = typename fft::C2<T>::type;
+    XFused<T> q;
+    std::memset(&q, 0, sizeof q);
+    q.n = P[0];
+    q.nst = (int)rad[0].size();
+    int Ns = 1;
+    for (int k = 0; k < q.nst; ++k) {
+      q.radix[k] = rad[0][k];
+      q.ns[k] = Ns;
+      q.ns_rcp[k] = 1.0f / (float)Ns;
+      q.tws[k] = q.n / (Ns * q.radix[k]);
+      Ns *= q.radix[k];
+    }
+    q.L0 = L[0];
+    q.by = pl.by;
+    while ((1 << q.lby) < q.by) ++q.lby;
+    q.lw = 3 * q.by;
+    q.lw_rcp = 1.0f / (float)q.lw;
+    q.P1 = P[1];
+    q.nyb = (P[1] + q.by - 1) / q.by;
+    q.M = M;
+    q.sq = (int64_t)P[0] * P[1];
+    q.K = pl.K.p;
+    q.W = pl.Wf.p;
+    q.tw = pl.tw[0].p;
+    const size_t lds = (size_t)q.n * (q.lw + 1) * sizeof(T2);
+    const unsigned blocks = (unsigned)(Q2 * q.nyb);
+    hipLaunchKernelGGL((k_dip_x_fused<T>), dim3(blocks), dim3(xf_threads<T>(q.by)), lds, st, q);
+    return hipGetLastError();
+  }
+
   template <class T>
   hipError_t ensure(Plane<T> &pl, hipStream_t st) {
     if (pl.ready) return hipSuccess;
@@ -512,6 +728,20 @@ class DipoleFFTImpl : public DipoleFFT {
                        real_tensor, pl.Wr.p, pl.Wc.p);
     DCHK(hipGetLastError());
     DCHK(hipStreamSynchronize(st));
+    const int by = fused_lines<T>();
+    if (by > 0) {
+      int lby = 0;
+      while ((1 << lby) < by) ++lby;
+      const int nyb = (P[1] + by - 1) / by;
+      const int64_t total = Q2 * nyb * 6 * (int64_t)P[0] * by;
+      DCHK(pl.Wf.alloc((size_t)total));
+      hipLaunchKernelGGL((k_dip_w_fused<T>), dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                         st, (const T *)pl.Wr.p, pl.Wf.p, M, P[0], P[1], lby, nyb, total);
+      DCHK(hipGetLastError());
+      DCHK(hipStreamSynchronize(st));
+      pl.Wr.release();
+      pl.by = by;
+    }
     pl.ready = true;
     return hipSuccess;
   }
@@ -594,38 +824,43 @@ class DipoleFFTImpl : public DipoleFFT {
       p.out = pl.K.p;
       DCHK((run_pass<T, fft::LD_CPLX, fft::ST_CPLX>(p, st)));
     }
-    // x forward: lines (g, q2, y), stride P1
-    {
-      auto p = base_pass<T>(pl, 0, false);
-      p.n_in = L[0];
-      p.n_out = P[0];
-      p.nlines = G * P[1] * Q2;
-      p.src = p.dst = LineMap{Q2, P[1], sa, (int64_t)P[0] * P[1], 1, P[1]};
-      p.in = pl.K.p;
-      p.out = pl.K.p;
-      DCHK((run_pass<T, fft::LD_CPLX, fft::ST_CPLX>(p, st)));
-    }
-    // multiply
-    {
-      const int64_t blocks = (M + 255) / 256;
-      if (real_tensor)
-        hipLaunchKernelGGL((k_dip_mult_real<T>), dim3((unsigned)blocks), dim3(256), 0, st, pl.K.p,
-                           (const T *)pl.Wr.p, M);
-      else
-        hipLaunchKernelGGL((k_dip_mult_cplx<T>), dim3((unsigned)blocks), dim3(256), 0, st, pl.K.p,
-                           (const T2 *)pl.Wc.p, M, nb);
-      DCHK(hipGetLastError());
-    }
-    // x inverse, outputs x < L0
-    {
-      auto p = base_pass<T>(pl, 0, true);
-      p.n_in = P[0];
-      p.n_out = L[0];
-      p.nlines = G * P[1] * Q2;
-      p.src = p.dst = LineMap{Q2, P[1], sa, (int64_t)P[0] * P[1], 1, P[1]};
-      p.in = pl.K.p;
-      p.out = pl.K.p;
-      DCHK((run_pass<T, fft::LD_CPLX, fft::ST_CPLX>(p, st)));
+    if (pl.by > 0) {
+      // x forward, multiply, x inverse in one kernel
+      DCHK(run_x_fused<T>(pl, st));
+    } else {
+      // x forward: lines (g, q2, y), stride P1
+      {
+        auto p = base_pass<T>(pl, 0, false);
+        p.n_in = L[0];
+        p.n_out = P[0];
+        p.nlines = G * P[1] * Q2;
+        p.src = p.dst = LineMap{Q2, P[1], sa, (int64_t)P[0] * P[1], 1, P[1]};
+        p.in = pl.K.p;
+        p.out = pl.K.p;
+        DCHK((run_pass<T, fft::LD_CPLX, fft::ST_CPLX>(p, st)));
+      }
+      // multiply
+      {
+        const int64_t blocks = (M + 255) / 256;
+        if (real_tensor)
+          hipLaunchKernelGGL((k_dip_mult_real<T>), dim3((unsigned)blocks), dim3(256), 0, st, pl.K.p,
+                             (const T *)pl.Wr.p, M);
+        else
+          hipLaunchKernelGGL((k_dip_mult_cplx<T>), dim3((unsigned)blocks), dim3(256), 0, st, pl.K.p,
+                             (const T2 *)pl.Wc.p, M, nb);
+        DCHK(hipGetLastError());
+      }
+      // x inverse, outputs x < L0
+      {
+        auto p = base_pass<T>(pl, 0, true);
+        p.n_in = P[0];
+        p.n_out = L[0];
+        p.nlines = G * P[1] * Q2;
+        p.src = p.dst = LineMap{Q2, P[1], sa, (int64_t)P[0] * P[1], 1, P[1]};
+        p.in = pl.K.p;
+        p.out = pl.K.p;
+        DCHK((run_pass<T, fft::LD_CPLX, fft::ST_CPLX>(p, st)));
+      }
     }
     // y inverse: lines (g, q2, x < L0), outputs y < L1
     {
@@ -665,8 +900,10 @@ class DipoleFFTImpl : public DipoleFFT {
     const double occ = (double)L[0] * L[1] * Q2, half = (double)L[0] * P[1] * Q2, full = (double)M;
     const double spin = 24.0 * n + 8.0 * n;  // s + mus (gather side: h + mus)
     double b = 2 * spin;
-    b += G * c * (occ /*z fwd w*/ + occ + half /*y fwd r w*/ + half + full /*x fwd*/ +
-                  full + full /*mult r w*/ + full + half /*x inv*/ + half + occ /*y inv*/ +
+    const bool fused = fp32 ? fused_lines<float>() > 0 : fused_lines<double>() > 0;
+    const double xm = fused ? half + half /*x fused r w*/
+                            : half + full /*x fwd*/ + full + full /*mult r w*/ + full + half /*x inv*/;
+    b += G * c * (occ /*z fwd w*/ + occ + half /*y fwd r w*/ + xm + half + occ /*y inv*/ +
                   occ /*z inv r*/);
     b += (real_tensor ? 6.0 * (c / 2) : 6.0 * nb * nb * c) * full;  // tensor
     return b;
