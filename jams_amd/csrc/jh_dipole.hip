@@ -169,10 +169,11 @@ struct XFused {
   const T2 *tw;
 };
 
-// one in-place stage over the lw lines of A ([x][line]); thread t handles
-// line t mod lw, butterfly t / lw (the host keeps every thread within kBF)
-template <class T, int R, bool INV>
-__device__ __forceinline__ void xf_stage(const XFused<T> &p, const typename fft::C2<T>::type *tw,
+// one in-place stage over the lw lines of A ([k][line]) of n points; thread
+// t handles line t mod lw, butterfly t / lw (the host keeps every thread
+// within kBF).  PP: XFused or ZYFused (n, lw, lw_rcp, ns, ns_rcp, tws).
+template <class T, int R, bool INV, class PP>
+__device__ __forceinline__ void xf_stage(const PP &p, const typename fft::C2<T>::type *tw,
                                          typename fft::C2<T>::type *A, int st) {
   using T2 = typename fft::C2<T>::type;
   constexpr int NBF = fft::kBF<T, R>;
@@ -208,8 +209,8 @@ __device__ __forceinline__ void xf_stage(const XFused<T> &p, const typename fft:
   __syncthreads();
 }
 
-template <class T, bool INV>
-__device__ __forceinline__ void xf_stages(const XFused<T> &p, const typename fft::C2<T>::type *tw,
+template <class T, bool INV, class PP>
+__device__ __forceinline__ void xf_stages(const PP &p, const typename fft::C2<T>::type *tw,
                                           typename fft::C2<T>::type *A) {
   for (int st = 0; st < p.nst; ++st) {
     switch (p.radix[st]) {
@@ -516,6 +517,111 @@ hipError_t run_direct_z(const Pass<T> &p, bool inverse, hipStream_t st) {
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// z forward and y forward in one kernel (one workgroup per (g, x < L0) and
+// q2 range): the direct r2c DFT of the workgroup's L1 spin lines along z,
+// written straight into LDS as lw lines along y (one per q2), zero padded to
+// P1, then the y transform, then a contiguous store of each q2 line.  The
+// occupied planes the separate z pass would write and the y pass read back
+// never leave LDS.  The DFT and the stages repeat the separate kernels'
+// operations, so the grid is the same to the bit.
+// workgroups per (g, x) of the fused z + y forward kernel, at least
+#ifndef JH_ZY_SPLIT
+#define JH_ZY_SPLIT 3
+#endif
+template <class T>
+struct ZYFused {
+  using T2 = typename fft::C2<T>::type;
+  // y transform (in place, lw lines of n = P1 points)
+  int n, nst;
+  int radix[fft::kMaxStages];
+  int ns[fft::kMaxStages];
+  float ns_rcp[fft::kMaxStages];
+  int tws[fft::kMaxStages];
+  int lw;          // q2 lines per workgroup
+  float lw_rcp;
+  int nsplit;      // workgroups per (g, x): q2 in [s lw, s lw + lw) ∩ [0, Q2)
+  // z transform and the spin grid
+  int P2, L2, Q2, L1, L0, nb;
+  int64_t nspin;   // spins (component stride of the spin arrays)
+  int64_t sa, sq;  // K strides of g and q2 (P0 P1 Q2, P0 P1)
+  const double *spins, *mus;
+  T2 *K;
+  const T2 *twy, *twz;
+};
+
+template <class T, int LM>
+__global__ void __launch_bounds__(512) k_dip_zy_fwd(const ZYFused<T> p) {
+  using T2 = typename fft::C2<T>::type;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int n = p.n, lw = p.lw, nt = blockDim.x, tid = threadIdx.x;
+  T2 *twy = reinterpret_cast<T2 *>(smem);
+  T2 *twz = twy + n;
+  T2 *A = twz + p.P2;                   // [y][line], P1 x lw
+  T *vals = reinterpret_cast<T *>(A);   // the same area, used first: [y][z]
+  for (int t = tid; t < n; t += nt) twy[t] = p.twy[t];
+  for (int t = tid; t < p.P2; t += nt) twz[t] = p.twz[t];
+  const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const int sp = (int)(blk % p.nsplit);
+  const int64_t gx = blk / p.nsplit;
+  const int x = (int)(gx % p.L0), g = (int)(gx / p.L0);
+  const int m = g / 3, comp = g % 3;
+  const int qa = sp * lw;
+  const int nq = min(lw, p.Q2 - qa);
+  const int L1 = p.L1, L2 = p.L2, SV = odd_stride(L2);
+  // spins of cells (x, y < L1, z < L2), basis m, in memory order
+  const int64_t base = (int64_t)x * L1 * L2 * p.nb + m;
+  for (int e = tid; e < L1 * L2; e += nt) {
+    const int64_t idx = base + (int64_t)e * p.nb;
+    vals[(e / L2) * SV + e % L2] = (T)(p.mus[idx] * p.spins[spin_idx(p.nspin, idx, comp)]);
+  }
+  __syncthreads();
+  // direct DFT along z of line y (the arithmetic of k_dip_z_direct_fwd)
+  T v[LM];
+  if (tid < L1) {
+#pragma unroll
+    for (int z = 0; z < LM; ++z) v[z] = z < L2 ? vals[tid * SV + z] : (T)0;
+  }
+  __syncthreads();  // vals is overwritten by A below
+  if (tid < L1) {
+    for (int l = 0; l < lw; ++l) {
+      const int q = qa + l;
+      T2 acc;
+      acc.x = 0;
+      acc.y = 0;
+      if (l < nq) {
+        int t = 0;  // (q z) mod P2
+#pragma unroll
+        for (int z = 0; z < LM; ++z) {
+          if (z < L2) {
+            const T2 w = twz[t];
+            acc.x = acc.x + v[z] * w.x;
+            acc.y = acc.y + v[z] * w.y;
+            t += q;
+            if (t >= p.P2) t -= p.P2;
+          }
+        }
+      }
+      A[tid * lw + l] = acc;
+    }
+  }
+  // zero padding y >= L1
+  for (int e = L1 * lw + tid; e < n * lw; e += nt) {
+    T2 z0;
+    z0.x = 0;
+    z0.y = 0;
+    A[e] = z0;
+  }
+  __syncthreads();
+  xf_stages<T, false>(p, twy, A);
+  // q2 lines, y contiguous
+  T2 *K = p.K + (int64_t)g * p.sa + (int64_t)qa * p.sq + (int64_t)x * n;
+  for (int e = tid; e < n * nq; e += nt) {
+    const int l = e / n, y = e - l * n;
+    K[(int64_t)l * p.sq + y] = A[y * lw + l];
+  }
+}
+
 // tensor set-up: the q2 < Q2 half of the transformed grid into the tensor
 // arrays (real part for one basis site)
 template <class T>
@@ -792,37 +898,122 @@ class DipoleFFTImpl : public DipoleFFT {
 
   bool direct_z() const { return P[2] <= kDirectP && L[2] <= kDirectL; }
 
+  // threads of the fused z + y forward kernel for lw q2 lines: one per y
+  // line for the z DFT, and no more than kBF butterflies per thread in any
+  // y stage
+  template <class T>
+  int zy_threads(int lw) const {
+    int need = std::max(64, L[1]);
+    for (int R : rad[1]) {
+      const int kbf = R == 2 ? fft::kBF<T, 2> : R == 3 ? fft::kBF<T, 3> : R == 4 ? fft::kBF<T, 4>
+                    : R == 5 ? fft::kBF<T, 5> : fft::kBF<T, 8>;
+      need = std::max(need, (P[1] / R * lw + kbf - 1) / kbf);
+    }
+    return (need + 63) / 64 * 64;
+  }
+  template <class T>
+  size_t zy_lds(int lw) const {
+    using T2 = typename fft::C2<T>::type;
+    const size_t a = std::max((size_t)P[1] * lw * sizeof(T2),
+                              (size_t)L[1] * (size_t)odd_stride(L[2]) * sizeof(T));
+    return (size_t)(P[1] + P[2]) * sizeof(T2) + a;
+  }
+  // workgroups per (g, x) of the fused z + y forward kernel, 0 when it does
+  // not apply (direct z axis, in-place y radices, LDS within 64 KB, at most
+  // 512 threads); JH_ZY_SPLIT or more when Q2 >= 4, for more workgroups in
+  // flight (at a 256 x 256 x 8 film, 3 q2 lines per workgroup ran faster
+  // than 5 or 2)
+  template <class T>
+  int zy_split() const {
+    if (!direct_z()) return 0;
+    for (int r : rad[1])
+      if (r != 2 && r != 3 && r != 4 && r != 5 && r != 8) return 0;
+    for (int ns = Q2 >= 4 ? JH_ZY_SPLIT : 1; ns <= Q2; ++ns) {
+      const int lw = (int)((Q2 + ns - 1) / ns);
+      if (zy_lds<T>(lw) <= kMaxPassLds && zy_threads<T>(lw) <= 512) return ns;
+    }
+    return 0;
+  }
+
+  template <class T>
+  hipError_t run_zy_fwd(Plane<T> &pl, const double *s, hipStream_t st) {
+    ZYFused<T> q;
+    std::memset(&q, 0, sizeof q);
+    q.n = P[1];
+    q.nst = (int)rad[1].size();
+    int Ns = 1;
+    for (int k = 0; k < q.nst; ++k) {
+      q.radix[k] = rad[1][k];
+      q.ns[k] = Ns;
+      q.ns_rcp[k] = 1.0f / (float)Ns;
+      q.tws[k] = q.n / (Ns * q.radix[k]);
+      Ns *= q.radix[k];
+    }
+    q.nsplit = zy_split<T>();
+    q.lw = (int)((Q2 + q.nsplit - 1) / q.nsplit);
+    q.lw_rcp = 1.0f / (float)q.lw;
+    q.P2 = P[2];
+    q.L2 = L[2];
+    q.Q2 = (int)Q2;
+    q.L1 = L[1];
+    q.L0 = L[0];
+    q.nb = nb;
+    q.nspin = n;
+    q.sa = (int64_t)P[0] * P[1] * Q2;
+    q.sq = (int64_t)P[0] * P[1];
+    q.spins = s;
+    q.mus = mus.p;
+    q.K = pl.K.p;
+    q.twy = pl.tw[1].p;
+    q.twz = pl.tw[2].p;
+    const unsigned blocks = (unsigned)((int64_t)3 * nb * L[0] * q.nsplit);
+    const int nt = zy_threads<T>(q.lw);
+    const size_t lds = zy_lds<T>(q.lw);
+    if (L[2] <= 8)
+      hipLaunchKernelGGL((k_dip_zy_fwd<T, 8>), dim3(blocks), dim3(nt), lds, st, q);
+    else if (L[2] <= 16)
+      hipLaunchKernelGGL((k_dip_zy_fwd<T, 16>), dim3(blocks), dim3(nt), lds, st, q);
+    else
+      hipLaunchKernelGGL((k_dip_zy_fwd<T, 32>), dim3(blocks), dim3(nt), lds, st, q);
+    return hipGetLastError();
+  }
+
   template <class T>
   hipError_t eval(Plane<T> &pl, const double *s, double *h, hipStream_t st) {
     using T2 = typename fft::C2<T>::type;
     DCHK(ensure(pl, st));
     const int64_t G = 3 * nb;
     const int64_t sa = (int64_t)P[0] * P[1] * Q2;
-    // z forward, scatter
-    {
-      auto p = base_pass<T>(pl, 2, false);
-      p.n_in = L[2];
-      p.n_out = (int)Q2;
-      p.nlines = G * L[0] * L[1];
-      p.dst = LineMap{L[0], L[1], sa, P[1], 1, (int64_t)P[0] * P[1]};
-      p.out = pl.K.p;
-      p.spins = s;
-      if (direct_z()) {
-        DCHK(run_direct_z<T>(p, false, st));
-      } else {
-        DCHK((run_pass<T, fft::LD_SPIN, fft::ST_CPLX>(p, st)));
+    if (zy_split<T>() > 0) {
+      // z forward and y forward in one kernel
+      DCHK(run_zy_fwd<T>(pl, s, st));
+    } else {
+      // z forward, scatter
+      {
+        auto p = base_pass<T>(pl, 2, false);
+        p.n_in = L[2];
+        p.n_out = (int)Q2;
+        p.nlines = G * L[0] * L[1];
+        p.dst = LineMap{L[0], L[1], sa, P[1], 1, (int64_t)P[0] * P[1]};
+        p.out = pl.K.p;
+        p.spins = s;
+        if (direct_z()) {
+          DCHK(run_direct_z<T>(p, false, st));
+        } else {
+          DCHK((run_pass<T, fft::LD_SPIN, fft::ST_CPLX>(p, st)));
+        }
       }
-    }
-    // y forward: lines (g, q2, x < L0), contiguous
-    {
-      auto p = base_pass<T>(pl, 1, false);
-      p.n_in = L[1];
-      p.n_out = P[1];
-      p.nlines = G * L[0] * Q2;
-      p.src = p.dst = LineMap{Q2, L[0], sa, (int64_t)P[0] * P[1], P[1], 1};
-      p.in = pl.K.p;
-      p.out = pl.K.p;
-      DCHK((run_pass<T, fft::LD_CPLX, fft::ST_CPLX>(p, st)));
+      // y forward: lines (g, q2, x < L0), contiguous
+      {
+        auto p = base_pass<T>(pl, 1, false);
+        p.n_in = L[1];
+        p.n_out = P[1];
+        p.nlines = G * L[0] * Q2;
+        p.src = p.dst = LineMap{Q2, L[0], sa, (int64_t)P[0] * P[1], P[1], 1};
+        p.in = pl.K.p;
+        p.out = pl.K.p;
+        DCHK((run_pass<T, fft::LD_CPLX, fft::ST_CPLX>(p, st)));
+      }
     }
     if (pl.by > 0) {
       // x forward, multiply, x inverse in one kernel
@@ -903,8 +1094,9 @@ class DipoleFFTImpl : public DipoleFFT {
     const bool fused = fp32 ? fused_lines<float>() > 0 : fused_lines<double>() > 0;
     const double xm = fused ? half + half /*x fused r w*/
                             : half + full /*x fwd*/ + full + full /*mult r w*/ + full + half /*x inv*/;
-    b += G * c * (occ /*z fwd w*/ + occ + half /*y fwd r w*/ + xm + half + occ /*y inv*/ +
-                  occ /*z inv r*/);
+    const bool zy = fp32 ? zy_split<float>() > 0 : zy_split<double>() > 0;
+    const double zf = zy ? half /*z + y fwd w*/ : occ /*z fwd w*/ + occ + half /*y fwd r w*/;
+    b += G * c * (zf + xm + half + occ /*y inv*/ + occ /*z inv r*/);
     b += (real_tensor ? 6.0 * (c / 2) : 6.0 * nb * nb * c) * full;  // tensor
     return b;
   }
