@@ -525,6 +525,13 @@ hipError_t run_direct_z(const Pass<T> &p, bool inverse, hipStream_t st) {
 // occupied planes the separate z pass would write and the y pass read back
 // never leave LDS.  The DFT and the stages repeat the separate kernels'
 // operations, so the grid is the same to the bit.
+// LDS bound of the fused y + z inverse kernel: within 48 KB three workgroups
+// fit per CU.  At a 256 x 256 x 8 film the FP32 kernel fits and beats the two
+// separate kernels; the FP64 one needs 54 KB or more (the [y][q2] staging
+// alone is 36 KB) and with 64 or 80 KB ran 3-8 % slower than them.
+#ifndef JH_ZYI_LDS
+#define JH_ZYI_LDS (48 * 1024)
+#endif
 // workgroups per (g, x) of the fused z + y forward kernel, at least
 #ifndef JH_ZY_SPLIT
 #define JH_ZY_SPLIT 3
@@ -546,6 +553,7 @@ struct ZYFused {
   int64_t nspin;   // spins (component stride of the spin arrays)
   int64_t sa, sq;  // K strides of g and q2 (P0 P1 Q2, P0 P1)
   const double *spins, *mus;
+  double *field;   // inverse: SoA field out
   T2 *K;
   const T2 *twy, *twz;
 };
@@ -619,6 +627,77 @@ __global__ void __launch_bounds__(512) k_dip_zy_fwd(const ZYFused<T> p) {
   for (int e = tid; e < n * nq; e += nt) {
     const int l = e / n, y = e - l * n;
     K[(int64_t)l * p.sq + y] = A[y * lw + l];
+  }
+}
+
+// y inverse and z inverse in one kernel: one workgroup per (g, x < L0)
+// transforms the Q2 q2 lines along y in batches of lw (all of them are
+// needed for one y's c2r along z), keeps the y < L1 outputs in LDS as
+// [y][q2], then evaluates the direct c2r of each y line (the arithmetic of
+// k_dip_z_direct_inv) and stores the field in memory order.
+template <class T, int LM>
+__global__ void __launch_bounds__(512) k_dip_zy_inv(const ZYFused<T> p) {
+  using T2 = typename fft::C2<T>::type;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int n = p.n, lw = p.lw, nt = blockDim.x, tid = threadIdx.x;
+  const int L1 = p.L1, L2 = p.L2, Q2 = p.Q2, P2 = p.P2;
+  const int SX = odd_stride(Q2), SZ = odd_stride(L2);
+  T2 *twy = reinterpret_cast<T2 *>(smem);
+  T2 *twz = twy + n;
+  T2 *S = twz + P2;                      // [y < L1][SX]
+  T2 *A = S + (size_t)L1 * SX;           // [y][line], P1 x lw; later outs
+  T *outs = reinterpret_cast<T *>(A);    // [y < L1][SZ]
+  for (int t = tid; t < n; t += nt) twy[t] = p.twy[t];
+  for (int t = tid; t < P2; t += nt) twz[t] = p.twz[t];
+  const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const int x = (int)(blk % p.L0), g = (int)(blk / p.L0);
+  const int m = g / 3, comp = g % 3;
+  const T2 *K = p.K + (int64_t)g * p.sa + (int64_t)x * n;
+  for (int q0 = 0; q0 < Q2; q0 += lw) {
+    const int nq = min(lw, Q2 - q0);
+    for (int e = tid; e < n * lw; e += nt) {
+      const int l = e / n, y = e - l * n;
+      T2 v;
+      v.x = 0;
+      v.y = 0;
+      if (l < nq) v = K[(int64_t)(q0 + l) * p.sq + y];
+      A[y * lw + l] = v;
+    }
+    __syncthreads();
+    xf_stages<T, true>(p, twy, A);
+    for (int l = 0; l < nq; ++l)
+      for (int y = tid; y < L1; y += nt) S[y * SX + q0 + l] = A[y * lw + l];
+    __syncthreads();
+  }
+  // c2r along z of each y line
+  for (int y = tid; y < L1; y += nt) {
+    T2 X[LM + 1];
+#pragma unroll
+    for (int q = 0; q <= LM; ++q)
+      if (q < Q2) X[q] = S[y * SX + q];
+    for (int z = 0; z < L2; ++z) {
+      T acc = X[0].x;
+      int t = z;  // (k z) mod P2, k = 1
+#pragma unroll
+      for (int k = 1; k <= LM; ++k) {
+        if (k < Q2) {
+          const T2 w = twz[t];
+          const T re = X[k].x * w.x + X[k].y * w.y;
+          if (2 * k == P2) acc = acc + re;
+          else acc = acc + 2 * re;
+          t += z;
+          if (t >= P2) t -= P2;
+        }
+      }
+      outs[y * SZ + z] = acc;
+    }
+  }
+  __syncthreads();
+  const int64_t base = (int64_t)x * L1 * L2 * p.nb + m;
+  for (int e = tid; e < L1 * L2; e += nt) {
+    const int y = e / L2, z = e - y * L2;
+    const int64_t idx = base + (int64_t)e * p.nb;
+    p.field[comp * p.nspin + idx] = p.mus[idx] * (double)outs[y * SZ + z];
   }
 }
 
@@ -936,7 +1015,27 @@ class DipoleFFTImpl : public DipoleFFT {
   }
 
   template <class T>
-  hipError_t run_zy_fwd(Plane<T> &pl, const double *s, hipStream_t st) {
+  size_t zy_inv_lds(int lw) const {
+    using T2 = typename fft::C2<T>::type;
+    const size_t a = std::max((size_t)P[1] * lw * sizeof(T2),
+                              (size_t)L[1] * (size_t)odd_stride(L[2]) * sizeof(T));
+    return (size_t)(P[1] + P[2]) * sizeof(T2) + (size_t)L[1] * odd_stride((int)Q2) * sizeof(T2) + a;
+  }
+  // q2 lines per batch of the fused y + z inverse kernel, 0 when it does not
+  // apply: LDS within JH_ZYI_LDS, at most 512 threads; batches of equal size
+  template <class T>
+  int zy_inv_lines() const {
+    if (zy_split<T>() == 0) return 0;
+    int lmax = 0;
+    for (int lw = (int)Q2; lw >= 1 && !lmax; --lw)
+      if (zy_inv_lds<T>(lw) <= (size_t)JH_ZYI_LDS && zy_threads<T>(lw) <= 512) lmax = lw;
+    if (!lmax) return 0;
+    const int nbatch = (int)((Q2 + lmax - 1) / lmax);
+    return (int)((Q2 + nbatch - 1) / nbatch);
+  }
+
+  template <class T>
+  ZYFused<T> zy_params(Plane<T> &pl, const double *s, double *h, int lw) const {
     ZYFused<T> q;
     std::memset(&q, 0, sizeof q);
     q.n = P[1];
@@ -949,9 +1048,8 @@ class DipoleFFTImpl : public DipoleFFT {
       q.tws[k] = q.n / (Ns * q.radix[k]);
       Ns *= q.radix[k];
     }
-    q.nsplit = zy_split<T>();
-    q.lw = (int)((Q2 + q.nsplit - 1) / q.nsplit);
-    q.lw_rcp = 1.0f / (float)q.lw;
+    q.lw = lw;
+    q.lw_rcp = 1.0f / (float)lw;
     q.P2 = P[2];
     q.L2 = L[2];
     q.Q2 = (int)Q2;
@@ -963,9 +1061,40 @@ class DipoleFFTImpl : public DipoleFFT {
     q.sq = (int64_t)P[0] * P[1];
     q.spins = s;
     q.mus = mus.p;
+    q.field = h;
     q.K = pl.K.p;
     q.twy = pl.tw[1].p;
     q.twz = pl.tw[2].p;
+    return q;
+  }
+
+  template <class T>
+  hipError_t run_zy_inv(Plane<T> &pl, double *h, hipStream_t st) {
+    const int lw = zy_inv_lines<T>();
+    const ZYFused<T> q = zy_params<T>(pl, nullptr, h, lw);
+    const unsigned blocks = (unsigned)((int64_t)3 * nb * L[0]);
+    const int nt = zy_threads<T>(lw);
+    const size_t lds = zy_inv_lds<T>(lw);
+    const int lmax = std::max(L[2], (P[2] + 1) / 2);
+    const void *fn = lmax <= 8    ? reinterpret_cast<const void *>(&k_dip_zy_inv<T, 8>)
+                     : lmax <= 16 ? reinterpret_cast<const void *>(&k_dip_zy_inv<T, 16>)
+                                  : reinterpret_cast<const void *>(&k_dip_zy_inv<T, 32>);
+    if (lds > 64 * 1024)
+      DCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    if (lmax <= 8)
+      hipLaunchKernelGGL((k_dip_zy_inv<T, 8>), dim3(blocks), dim3(nt), lds, st, q);
+    else if (lmax <= 16)
+      hipLaunchKernelGGL((k_dip_zy_inv<T, 16>), dim3(blocks), dim3(nt), lds, st, q);
+    else
+      hipLaunchKernelGGL((k_dip_zy_inv<T, 32>), dim3(blocks), dim3(nt), lds, st, q);
+    return hipGetLastError();
+  }
+
+  template <class T>
+  hipError_t run_zy_fwd(Plane<T> &pl, const double *s, hipStream_t st) {
+    const int nsplit = zy_split<T>();
+    ZYFused<T> q = zy_params<T>(pl, s, nullptr, (int)((Q2 + nsplit - 1) / nsplit));
+    q.nsplit = nsplit;
     const unsigned blocks = (unsigned)((int64_t)3 * nb * L[0] * q.nsplit);
     const int nt = zy_threads<T>(q.lw);
     const size_t lds = zy_lds<T>(q.lw);
@@ -1053,30 +1182,35 @@ class DipoleFFTImpl : public DipoleFFT {
         DCHK((run_pass<T, fft::LD_CPLX, fft::ST_CPLX>(p, st)));
       }
     }
-    // y inverse: lines (g, q2, x < L0), outputs y < L1
-    {
-      auto p = base_pass<T>(pl, 1, true);
-      p.n_in = P[1];
-      p.n_out = L[1];
-      p.nlines = G * L[0] * Q2;
-      p.src = p.dst = LineMap{Q2, L[0], sa, (int64_t)P[0] * P[1], P[1], 1};
-      p.in = pl.K.p;
-      p.out = pl.K.p;
-      DCHK((run_pass<T, fft::LD_CPLX, fft::ST_CPLX>(p, st)));
-    }
-    // z inverse (c2r), gather
-    {
-      auto p = base_pass<T>(pl, 2, true);
-      p.n_in = (int)Q2;
-      p.n_out = L[2];
-      p.nlines = G * L[0] * L[1];
-      p.src = LineMap{L[0], L[1], sa, P[1], 1, (int64_t)P[0] * P[1]};
-      p.in = pl.K.p;
-      p.field = h;
-      if (direct_z()) {
-        DCHK(run_direct_z<T>(p, true, st));
-      } else {
-        DCHK((run_pass<T, fft::LD_HERM, fft::ST_FIELD>(p, st)));
+    if (zy_inv_lines<T>() > 0) {
+      // y inverse and z inverse in one kernel
+      DCHK(run_zy_inv<T>(pl, h, st));
+    } else {
+      // y inverse: lines (g, q2, x < L0), outputs y < L1
+      {
+        auto p = base_pass<T>(pl, 1, true);
+        p.n_in = P[1];
+        p.n_out = L[1];
+        p.nlines = G * L[0] * Q2;
+        p.src = p.dst = LineMap{Q2, L[0], sa, (int64_t)P[0] * P[1], P[1], 1};
+        p.in = pl.K.p;
+        p.out = pl.K.p;
+        DCHK((run_pass<T, fft::LD_CPLX, fft::ST_CPLX>(p, st)));
+      }
+      // z inverse (c2r), gather
+      {
+        auto p = base_pass<T>(pl, 2, true);
+        p.n_in = (int)Q2;
+        p.n_out = L[2];
+        p.nlines = G * L[0] * L[1];
+        p.src = LineMap{L[0], L[1], sa, P[1], 1, (int64_t)P[0] * P[1]};
+        p.in = pl.K.p;
+        p.field = h;
+        if (direct_z()) {
+          DCHK(run_direct_z<T>(p, true, st));
+        } else {
+          DCHK((run_pass<T, fft::LD_HERM, fft::ST_FIELD>(p, st)));
+        }
       }
     }
     return hipSuccess;
@@ -1096,7 +1230,9 @@ class DipoleFFTImpl : public DipoleFFT {
                             : half + full /*x fwd*/ + full + full /*mult r w*/ + full + half /*x inv*/;
     const bool zy = fp32 ? zy_split<float>() > 0 : zy_split<double>() > 0;
     const double zf = zy ? half /*z + y fwd w*/ : occ /*z fwd w*/ + occ + half /*y fwd r w*/;
-    b += G * c * (zf + xm + half + occ /*y inv*/ + occ /*z inv r*/);
+    const bool zyi = fp32 ? zy_inv_lines<float>() > 0 : zy_inv_lines<double>() > 0;
+    const double zi = zyi ? half /*y + z inv r*/ : half + occ /*y inv r w*/ + occ /*z inv r*/;
+    b += G * c * (zf + xm + zi);
     b += (real_tensor ? 6.0 * (c / 2) : 6.0 * nb * nb * c) * full;  // tensor
     return b;
   }

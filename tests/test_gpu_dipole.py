@@ -110,7 +110,13 @@ FIELD_CASES = [
     ("fcc_open_5x4x3", po.LatticeSpec(SC, [[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]],
                                       [0, 0, 0, 0], (5, 4, 3), (0, 0, 0)), None, 10.0),
     ("chain_2000", chain(), None, 1000.0),
+    # z longer than the direct z DFT takes (P2 = 80): separate z and y passes
+    ("sc_open_4x3x40", po.LatticeSpec(SC, [[0, 0, 0]], [0], (4, 3, 40), (0, 0, 0)), None, 50.0),
+    # y of prime length 7 (a generic radix): separate z and y passes, fused x
+    ("sc_y7_5x7x3", po.LatticeSpec(SC, [[0, 0, 0]], [0], (5, 7, 3), (0, 1, 0)), None, 3.0),
 ]
+# the FP32 transform check: the first four and the two separate-pass cases
+FP32_CASES = FIELD_CASES[:4] + FIELD_CASES[-2:]
 
 
 def case_mus(lat, moments):
@@ -134,7 +140,7 @@ def test_dipole_field_matches_direct_sum(engine_factory, name, lat, moments, rc)
     assert err <= 1e-10
 
 
-@pytest.mark.parametrize("name,lat,moments,rc", FIELD_CASES[:4], ids=[c[0] for c in FIELD_CASES[:4]])
+@pytest.mark.parametrize("name,lat,moments,rc", FP32_CASES, ids=[c[0] for c in FP32_CASES])
 def test_dipole_fp32_transform_error(engine_factory, name, lat, moments, rc):
     """The precision form's FP32 transforms: error <= 1e-5 max|h|."""
     mus = case_mus(lat, moments)
