@@ -526,9 +526,9 @@ hipError_t run_direct_z(const Pass<T> &p, bool inverse, hipStream_t st) {
 // never leave LDS.  The DFT and the stages repeat the separate kernels'
 // operations, so the grid is the same to the bit.
 // LDS bound of the fused y + z inverse kernel: within 48 KB three workgroups
-// fit per CU.  At a 256 x 256 x 8 film the FP32 kernel fits and beats the two
-// separate kernels; the FP64 one needs 54 KB or more (the [y][q2] staging
-// alone is 36 KB) and with 64 or 80 KB ran 3-8 % slower than them.
+// fit per CU (with a [y][q2] staging area in LDS instead of register sums
+// the FP64 kernel needed 54-80 KB and ran 3-8 % slower than two separate
+// kernels at a 256 x 256 x 8 film).
 #ifndef JH_ZYI_LDS
 #define JH_ZYI_LDS (48 * 1024)
 #endif
@@ -631,21 +631,20 @@ __global__ void __launch_bounds__(512) k_dip_zy_fwd(const ZYFused<T> p) {
 }
 
 // y inverse and z inverse in one kernel: one workgroup per (g, x < L0)
-// transforms the Q2 q2 lines along y in batches of lw (all of them are
-// needed for one y's c2r along z), keeps the y < L1 outputs in LDS as
-// [y][q2], then evaluates the direct c2r of each y line (the arithmetic of
-// k_dip_z_direct_inv) and stores the field in memory order.
+// transforms the Q2 q2 lines along y in batches of lw; after each batch the
+// thread of line y < L1 adds the batch's terms to its c2r sums along z
+// (registers, in the order k = 0, 1, ... of k_dip_z_direct_inv, so the sums
+// are the same to the bit); then the field is stored in memory order.
 template <class T, int LM>
 __global__ void __launch_bounds__(512) k_dip_zy_inv(const ZYFused<T> p) {
   using T2 = typename fft::C2<T>::type;
   extern __shared__ __align__(16) unsigned char smem[];
   const int n = p.n, lw = p.lw, nt = blockDim.x, tid = threadIdx.x;
   const int L1 = p.L1, L2 = p.L2, Q2 = p.Q2, P2 = p.P2;
-  const int SX = odd_stride(Q2), SZ = odd_stride(L2);
+  const int SZ = odd_stride(L2);
   T2 *twy = reinterpret_cast<T2 *>(smem);
   T2 *twz = twy + n;
-  T2 *S = twz + P2;                      // [y < L1][SX]
-  T2 *A = S + (size_t)L1 * SX;           // [y][line], P1 x lw; later outs
+  T2 *A = twz + P2;                      // [y][line], P1 x lw; later outs
   T *outs = reinterpret_cast<T *>(A);    // [y < L1][SZ]
   for (int t = tid; t < n; t += nt) twy[t] = p.twy[t];
   for (int t = tid; t < P2; t += nt) twz[t] = p.twz[t];
@@ -653,6 +652,9 @@ __global__ void __launch_bounds__(512) k_dip_zy_inv(const ZYFused<T> p) {
   const int x = (int)(blk % p.L0), g = (int)(blk / p.L0);
   const int m = g / 3, comp = g % 3;
   const T2 *K = p.K + (int64_t)g * p.sa + (int64_t)x * n;
+  // c2r sums of line y = tid: X[0] + 2 Re(X[k] w^{kz}) for 0 < k < P2/2
+  // (+ X[P2/2] (-1)^z, P2 even)
+  T acc[LM];
   for (int q0 = 0; q0 < Q2; q0 += lw) {
     const int nq = min(lw, Q2 - q0);
     for (int e = tid; e < n * lw; e += nt) {
@@ -665,32 +667,31 @@ __global__ void __launch_bounds__(512) k_dip_zy_inv(const ZYFused<T> p) {
     }
     __syncthreads();
     xf_stages<T, true>(p, twy, A);
-    for (int l = 0; l < nq; ++l)
-      for (int y = tid; y < L1; y += nt) S[y * SX + q0 + l] = A[y * lw + l];
-    __syncthreads();
-  }
-  // c2r along z of each y line
-  for (int y = tid; y < L1; y += nt) {
-    T2 X[LM + 1];
+    if (tid < L1) {
+      for (int l = 0; l < nq; ++l) {
+        const int k = q0 + l;
+        const T2 X = A[tid * lw + l];
 #pragma unroll
-    for (int q = 0; q <= LM; ++q)
-      if (q < Q2) X[q] = S[y * SX + q];
-    for (int z = 0; z < L2; ++z) {
-      T acc = X[0].x;
-      int t = z;  // (k z) mod P2, k = 1
-#pragma unroll
-      for (int k = 1; k <= LM; ++k) {
-        if (k < Q2) {
-          const T2 w = twz[t];
-          const T re = X[k].x * w.x + X[k].y * w.y;
-          if (2 * k == P2) acc = acc + re;
-          else acc = acc + 2 * re;
-          t += z;
-          if (t >= P2) t -= P2;
+        for (int z = 0; z < LM; ++z) {
+          if (z < L2) {
+            if (k == 0) {
+              acc[z] = X.x;
+            } else {
+              const T2 w = twz[(k * z) % P2];  // e^{-2 pi i kz/P2}; conjugated below
+              const T re = X.x * w.x + X.y * w.y;
+              if (2 * k == P2) acc[z] = acc[z] + re;
+              else acc[z] = acc[z] + 2 * re;
+            }
+          }
         }
       }
-      outs[y * SZ + z] = acc;
     }
+    __syncthreads();
+  }
+  if (tid < L1) {
+#pragma unroll
+    for (int z = 0; z < LM; ++z)
+      if (z < L2) outs[tid * SZ + z] = acc[z];
   }
   __syncthreads();
   const int64_t base = (int64_t)x * L1 * L2 * p.nb + m;
@@ -1019,7 +1020,7 @@ class DipoleFFTImpl : public DipoleFFT {
     using T2 = typename fft::C2<T>::type;
     const size_t a = std::max((size_t)P[1] * lw * sizeof(T2),
                               (size_t)L[1] * (size_t)odd_stride(L[2]) * sizeof(T));
-    return (size_t)(P[1] + P[2]) * sizeof(T2) + (size_t)L[1] * odd_stride((int)Q2) * sizeof(T2) + a;
+    return (size_t)(P[1] + P[2]) * sizeof(T2) + a;
   }
   // q2 lines per batch of the fused y + z inverse kernel, 0 when it does not
   // apply: LDS within JH_ZYI_LDS, at most 512 threads; batches of equal size
