@@ -23,6 +23,12 @@
 
 #include "jh_layout.h"
 
+// The transforms need no particular rounding sequence (the dipole's parity is
+// a tolerance against the direct sum), so multiply-adds may fuse here even
+// though the build disables contraction for the step kernels' bit-exact
+// arithmetic.
+#pragma clang fp contract(fast)
+
 namespace jh {
 namespace fft {
 
@@ -437,3 +443,5 @@ __global__ void __launch_bounds__(kThreads) k_fft_pass(const Pass<T> p) {
 
 }  // namespace fft
 }  // namespace jh
+
+#pragma clang fp contract(off)
