@@ -23,6 +23,10 @@
 // After the z pass the grid is held as [g][q2][x][y] (y contiguous), so the
 // y transforms read and write whole contiguous lines and the x transforms
 // read rows of adjacent y lines.
+// Where they apply, three kernels do this work with fewer trips to memory:
+// k_dip_zy_fwd (z + y forward), k_dip_x_fused (x forward, multiply, x
+// inverse; one basis site, real tensor) and k_dip_zy_inv (y + z inverse).
+// They repeat the separate passes' operations line by line.
 // The 1/P normalisation is folded into the tensor.  For one basis site W is
 // even in n (except for the half-way image of an even periodic length inside
 // r_cutoff), and the transformed tensor is then real: 6 values per k-point.
