@@ -68,6 +68,54 @@ def measure(sim_text: str, steps: int, warmup: int, reps: int = 200):
     return out
 
 
+def precision_errors(a, b, mus):
+    """Per-spin angle between two spin sets and |dm| of their magnetisations."""
+    import numpy as np
+    ang = _angles(a, b)
+    m_of = lambda s: (mus[:, None] * s).sum(0) / mus.sum()
+    return {"angle_rad_max": float(ang.max()), "angle_rad_rms": float(np.sqrt((ang ** 2).mean())),
+            "dm_abs": float(np.linalg.norm(m_of(a) - m_of(b)))}
+
+
+def c2_accuracy(L: int, steps: int, early: int = 100):
+    """C2 run for `steps` steps (untimed) in FP64, in the FP32-field form and
+    in FP64 from the start state rotated by ~1e-12 rad per spin, all on the
+    same noise stream: the FP32-field and the perturbed run against FP64,
+    after `early` steps and after `steps`.  At 300 K this lattice is
+    paramagnetic and its stochastic trajectories are chaotic: the perturbed
+    run shows how far the dynamics alone carries a tiny difference."""
+    import numpy as np
+
+    from jams_amd import configs
+    from jams_amd.engine import PREC_FP32_FIELD, PREC_FP64
+    from jams_amd.host import Simulation
+
+    sim = Simulation(configs.c2(L), device=0, output_dir="/tmp")
+    eng = sim.engine()
+    s0 = eng.get_spins()
+    mus = sim.sites()[0]
+    d = np.random.default_rng(11).normal(size=s0.shape) * 1e-12
+    s_pert = s0 + d - np.einsum("ij,ij->i", d, s0)[:, None] * s0
+    s_pert /= np.linalg.norm(s_pert, axis=1)[:, None]
+    at = {early: {}, steps: {}}
+    for name, mode, init in (("fp64", PREC_FP64, s0), ("fp32_field", PREC_FP32_FIELD, s0),
+                             ("fp64_perturbed", PREC_FP64, s_pert)):
+        eng.set_precision(mode)
+        eng.set_spins(init)
+        eng.set_iteration(0)
+        eng.step(early)
+        at[early][name] = eng.get_spins()
+        eng.step(steps - early)
+        at[steps][name] = eng.get_spins()
+    sim.close()
+    out = {}
+    for k, f in at.items():
+        out[f"after_{k}_steps"] = {
+            "fp32_field_vs_fp64": precision_errors(f["fp64"], f["fp32_field"], mus),
+            "fp64_perturbed_vs_fp64": precision_errors(f["fp64"], f["fp64_perturbed"], mus)}
+    return out
+
+
 def measure_rk4(L: int, steps: int, warmup: int):
     """The C2 system under solver module llg-rk4-gpu (four field evaluations
     per step), FP64 and FP32-field: whole-step HIP-event time."""
@@ -253,6 +301,7 @@ def measure_batch(steps: int, nrep: int = 256):
 
     reps = configs.c4(nrep=nrep)
     out = {"replicas": nrep, "steps": steps}
+    finals = {}
     for name, mode in (("fp64", PREC_FP64), ("fp32_field", PREC_FP32_FIELD)):
         t0 = time.perf_counter()
         b = ReplicaBatch(reps, precision=mode)
@@ -268,6 +317,7 @@ def measure_batch(steps: int, nrep: int = 256):
         e.step(0)
         ms = e.time_steps(steps)
         m = b.magnetisation()
+        finals[name] = (e.get_spins(), m)
         out[name] = {"spins": b.n, "value": b.n * steps / (ms * 1e-3), "ms_per_step": ms / steps,
                      "ms_per_step_plain_launches": ms_plain / 200, "setup_s": setup_s,
                      "autotune": tune.splitlines()[-1], "engine": e.describe(),
@@ -277,6 +327,17 @@ def measure_batch(steps: int, nrep: int = 256):
             f"{tune.splitlines()[-1]}; |m| coldest {m[0]:.4f} hottest {m[-1]:.4f}")
         b.close()
     out["speedup_fp32_vs_fp64"] = out["fp32_field"]["value"] / out["fp64"]["value"]
+    # FP32-field against FP64 after the same steps: the per-spin angle over all
+    # replicas and the largest per-replica |m| difference (hot replicas amplify
+    # any perturbation, as a random start does at C3)
+    import numpy as np
+    (s64, m64), (s32, m32) = finals["fp64"], finals["fp32_field"]
+    ang = _angles(s64, s32)
+    out["fp32_vs_fp64"] = {"angle_rad_max": float(ang.max()),
+                           "angle_rad_rms": float(np.sqrt((ang ** 2).mean())),
+                           "replica_dm_abs_max": float(np.abs(np.asarray(m64) - np.asarray(m32)).max())}
+    log(f"  C4 FP32-field vs FP64 after {steps} steps: max angle {ang.max():.3e} rad, "
+        f"largest replica |d|m|| {out['fp32_vs_fp64']['replica_dm_abs_max']:.3e}")
     return out
 
 
@@ -326,6 +387,9 @@ def main() -> int:
     ap.add_argument("--no-c4", action="store_true")
     ap.add_argument("--c4-steps", type=int, default=5000)
     ap.add_argument("--c5-steps", type=int, default=2000)
+    # C2: FP32-field and perturbed-FP64 runs of this many steps against FP64,
+    # untimed (0: none)
+    ap.add_argument("--accuracy-steps", type=int, default=10000)
     ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
     # the host layer logs set-up to stdout, as the reference does; keep this
@@ -352,6 +416,12 @@ def main() -> int:
         fp32 = measure(configs.c2(args.L, precision="fp32-field"), args.steps, args.warmup,
                        args.reps)
         log(f"  {fp32['value']:.4e} spin-updates/s")
+        if args.accuracy_steps > 100:
+            fp32["vs_fp64"] = c2_accuracy(args.L, args.accuracy_steps)
+            for when, res in fp32["vs_fp64"].items():
+                for k, v in res.items():
+                    log(f"  C2 {k} {when}: max angle {v['angle_rad_max']:.3e} rad "
+                        f"(rms {v['angle_rad_rms']:.3e}), |dm| {v['dm_abs']:.3e}")
     rk4 = None
     if not args.no_rk4:
         log("C2 under llg-rk4-gpu (RK4), FP64 and FP32-field")
@@ -415,6 +485,8 @@ def main() -> int:
         line["fp32_field"] = {"value": fp32["value"], "ms_per_step": fp32["ms"] / args.steps,
                               "speedup_vs_fp64": fp32["value"] / fp64["value"],
                               "phases": fp32["phases"]}
+        if "vs_fp64" in fp32:
+            line["fp32_field"]["vs_fp64"] = fp32["vs_fp64"]
     if c3 is not None:
         line["precision_c3"] = c3
     if c4 is not None:
