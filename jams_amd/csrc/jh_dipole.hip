@@ -227,13 +227,77 @@ __device__ __forceinline__ void xf_stages(const PP &p, const typename fft::C2<T>
   }
 }
 
-template <class T>
+// 512-point lines (radix 8, 8, 8) and LW lines known at compile time: the
+// thread's line l = t mod LW and butterfly j = t / LW are the same in every
+// stage, the stage's sub-length is a power of two, and the LDS reads and
+// writes of a butterfly sit at constant offsets from one base address.  The
+// operations on the data are those of xf_stage.
+template <class T, int NS, bool INV, int LW>
+__device__ __forceinline__ void xf_stage512(const typename fft::C2<T>::type *tw,
+                                            typename fft::C2<T>::type *A, int j, int l) {
+  using T2 = typename fft::C2<T>::type;
+  constexpr int R = 8, m = 512 / R, tws = 512 / (NS * R);
+  const bool live = j < m;
+  T2 v[R];
+  int d = 0;
+  if (live) {
+    const int jm = j & (NS - 1), jd = j / NS;
+    d = (jd * NS * R + jm) * LW + l;
+    const T2 *a = A + j * LW + l;
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = a[r * m * LW];
+#pragma unroll
+    for (int r = 1; r < R; ++r) v[r] = fft::cmul(v[r], fft::twiddle<T, INV>(tw, jm * r * tws));
+    fft::dft_fixed<T2, R>(v, INV);
+  }
+  __syncthreads();
+  if (live) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) A[d + r * NS * LW] = v[r];
+  }
+  __syncthreads();
+}
+
+template <class T, bool INV, int LW>
+__device__ __forceinline__ void xf_stages512(const typename fft::C2<T>::type *tw,
+                                             typename fft::C2<T>::type *A) {
+  const int t = threadIdx.x;
+  const int j = t / LW, l = t - j * LW;
+  xf_stage512<T, 1, INV, LW>(tw, A, j, l);
+  xf_stage512<T, 8, INV, LW>(tw, A, j, l);
+  xf_stage512<T, 64, INV, LW>(tw, A, j, l);
+}
+
+// e = q lw + l with a compile-time lw when LW > 0
+template <int LW>
+__device__ __forceinline__ int divmod_lw(int e, int lw, float rcp, int &l) {
+  if constexpr (LW > 0) {
+    const int q = e / LW;
+    l = e - q * LW;
+    return q;
+  } else {
+    return fft::divmod_small(e, lw, rcp, l);
+  }
+}
+
+// the y or x transform of a fused kernel: LW > 0 selects the 512-point stages
+template <class T, bool INV, int LW, class PP>
+__device__ __forceinline__ void line_stages(const PP &p, const typename fft::C2<T>::type *tw,
+                                            typename fft::C2<T>::type *A) {
+  if constexpr (LW > 0) xf_stages512<T, INV, LW>(tw, A);
+  else xf_stages<T, INV>(p, tw, A);
+}
+
+// LWK > 0: 512-point x lines and LWK = 3 by lines (compile-time geometry)
+template <class T, int LWK>
 __global__ void __launch_bounds__(512) k_dip_x_fused(const XFused<T> p) {
   using T2 = typename fft::C2<T>::type;
   extern __shared__ __align__(16) unsigned char smem[];
+  const int n = LWK ? 512 : p.n, lw = LWK ? LWK : p.lw;
+  const int by = LWK ? LWK / 3 : p.by, lby = LWK ? (LWK == 6 ? 1 : LWK == 12 ? 2 : 0) : p.lby;
+  const int nt = blockDim.x;
   T2 *A = reinterpret_cast<T2 *>(smem);
-  T2 *tw = A + (size_t)p.n * p.lw;
-  const int n = p.n, lw = p.lw, by = p.by, lby = p.lby, nt = blockDim.x;
+  T2 *tw = A + (size_t)n * lw;
   for (int t = threadIdx.x; t < n; t += nt) tw[t] = p.tw[t];
   // consecutive blocks on one XCD share the L2 lines of adjacent y
   const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
@@ -242,7 +306,7 @@ __global__ void __launch_bounds__(512) k_dip_x_fused(const XFused<T> p) {
   // load x < L0 (zero padding above), element e = x lw + c by + yb
   for (int e = threadIdx.x; e < n * lw; e += nt) {
     int l;
-    const int x = fft::divmod_small(e, lw, p.lw_rcp, l);
+    const int x = divmod_lw<LWK>(e, lw, p.lw_rcp, l);
     const int c = l >> lby, yb = l & (by - 1);
     T2 v;
     v.x = 0;
@@ -251,7 +315,7 @@ __global__ void __launch_bounds__(512) k_dip_x_fused(const XFused<T> p) {
     A[e] = v;
   }
   __syncthreads();
-  xf_stages<T, false>(p, tw, A);
+  line_stages<T, false, LWK>(p, tw, A);
   // H(k) = W(k) S(k) with the arithmetic of k_dip_mult_real
   {
     const int nby = n * by;
@@ -271,11 +335,11 @@ __global__ void __launch_bounds__(512) k_dip_x_fused(const XFused<T> p) {
     }
   }
   __syncthreads();
-  xf_stages<T, true>(p, tw, A);
+  line_stages<T, true, LWK>(p, tw, A);
   // store x < L0
   for (int e = threadIdx.x; e < p.L0 * lw; e += nt) {
     int l;
-    const int x = fft::divmod_small(e, lw, p.lw_rcp, l);
+    const int x = divmod_lw<LWK>(e, lw, p.lw_rcp, l);
     const int c = l >> lby, yb = l & (by - 1);
     if (y0 + yb < p.P1) K[c * p.M + (int64_t)x * p.P1 + yb] = A[e];
   }
@@ -562,11 +626,12 @@ struct ZYFused {
   const T2 *twy, *twz;
 };
 
-template <class T, int LM>
+// LWK > 0: 512-point y lines and LWK q2 lines per workgroup (compile time)
+template <class T, int LM, int LWK>
 __global__ void __launch_bounds__(512) k_dip_zy_fwd(const ZYFused<T> p) {
   using T2 = typename fft::C2<T>::type;
   extern __shared__ __align__(16) unsigned char smem[];
-  const int n = p.n, lw = p.lw, nt = blockDim.x, tid = threadIdx.x;
+  const int n = LWK ? 512 : p.n, lw = LWK ? LWK : p.lw, nt = blockDim.x, tid = threadIdx.x;
   T2 *twy = reinterpret_cast<T2 *>(smem);
   T2 *twz = twy + n;
   T2 *A = twz + p.P2;                   // [y][line], P1 x lw
@@ -625,7 +690,7 @@ __global__ void __launch_bounds__(512) k_dip_zy_fwd(const ZYFused<T> p) {
     A[e] = z0;
   }
   __syncthreads();
-  xf_stages<T, false>(p, twy, A);
+  line_stages<T, false, LWK>(p, twy, A);
   // q2 lines, y contiguous
   T2 *K = p.K + (int64_t)g * p.sa + (int64_t)qa * p.sq + (int64_t)x * n;
   for (int e = tid; e < n * nq; e += nt) {
@@ -639,11 +704,12 @@ __global__ void __launch_bounds__(512) k_dip_zy_fwd(const ZYFused<T> p) {
 // thread of line y < L1 adds the batch's terms to its c2r sums along z
 // (registers, in the order k = 0, 1, ... of k_dip_z_direct_inv, so the sums
 // are the same to the bit); then the field is stored in memory order.
-template <class T, int LM>
+// LWK > 0: 512-point y lines and batches of LWK q2 lines (compile time)
+template <class T, int LM, int LWK>
 __global__ void __launch_bounds__(512) k_dip_zy_inv(const ZYFused<T> p) {
   using T2 = typename fft::C2<T>::type;
   extern __shared__ __align__(16) unsigned char smem[];
-  const int n = p.n, lw = p.lw, nt = blockDim.x, tid = threadIdx.x;
+  const int n = LWK ? 512 : p.n, lw = LWK ? LWK : p.lw, nt = blockDim.x, tid = threadIdx.x;
   const int L1 = p.L1, L2 = p.L2, Q2 = p.Q2, P2 = p.P2;
   const int SZ = odd_stride(L2);
   T2 *twy = reinterpret_cast<T2 *>(smem);
@@ -670,7 +736,7 @@ __global__ void __launch_bounds__(512) k_dip_zy_inv(const ZYFused<T> p) {
       A[y * lw + l] = v;
     }
     __syncthreads();
-    xf_stages<T, true>(p, twy, A);
+    line_stages<T, true, LWK>(p, twy, A);
     if (tid < L1) {
       for (int l = 0; l < nq; ++l) {
         const int k = q0 + l;
@@ -793,6 +859,21 @@ hipError_t run_pass(Pass<T> p, hipStream_t st) {
   return hipGetLastError();
 }
 
+// the fused y + z inverse kernel for a compile-time lwk (3, 5) or none (0)
+template <class T, int LM>
+hipError_t launch_zy_inv(const ZYFused<T> &q, int lwk, unsigned blocks, int nt, size_t lds,
+                         hipStream_t st) {
+  const void *fn = lwk == 3   ? reinterpret_cast<const void *>(&k_dip_zy_inv<T, LM, 3>)
+                   : lwk == 5 ? reinterpret_cast<const void *>(&k_dip_zy_inv<T, LM, 5>)
+                              : reinterpret_cast<const void *>(&k_dip_zy_inv<T, LM, 0>);
+  if (lds > 64 * 1024)
+    DCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  if (lwk == 3) hipLaunchKernelGGL((k_dip_zy_inv<T, LM, 3>), dim3(blocks), dim3(nt), lds, st, q);
+  else if (lwk == 5) hipLaunchKernelGGL((k_dip_zy_inv<T, LM, 5>), dim3(blocks), dim3(nt), lds, st, q);
+  else hipLaunchKernelGGL((k_dip_zy_inv<T, LM, 0>), dim3(blocks), dim3(nt), lds, st, q);
+  return hipGetLastError();
+}
+
 // per-precision device state
 template <class T>
 struct Plane {
@@ -848,6 +929,10 @@ class DipoleFFTImpl : public DipoleFFT {
     }
     return (need + 63) / 64 * 64;
   }
+  // axis of 512 points factored 8 x 8 x 8: the fused kernels' compile-time
+  // stage geometry applies
+  bool lines512(int axis) const { return P[axis] == 512 && rad[axis] == std::vector<int>{8, 8, 8}; }
+
   // y lines per workgroup of the fused x kernel (0: not applicable): one
   // basis site with a real tensor, in-place radices, LDS within 64 KB (two
   // workgroups per CU), at most 512 threads
@@ -891,7 +976,10 @@ class DipoleFFTImpl : public DipoleFFT {
     q.tw = pl.tw[0].p;
     const size_t lds = (size_t)q.n * (q.lw + 1) * sizeof(T2);
     const unsigned blocks = (unsigned)(Q2 * q.nyb);
-    hipLaunchKernelGGL((k_dip_x_fused<T>), dim3(blocks), dim3(xf_threads<T>(q.by)), lds, st, q);
+    if (lines512(0) && q.lw == 6)
+      hipLaunchKernelGGL((k_dip_x_fused<T, 6>), dim3(blocks), dim3(xf_threads<T>(q.by)), lds, st, q);
+    else
+      hipLaunchKernelGGL((k_dip_x_fused<T, 0>), dim3(blocks), dim3(xf_threads<T>(q.by)), lds, st, q);
     return hipGetLastError();
   }
 
@@ -1081,18 +1169,11 @@ class DipoleFFTImpl : public DipoleFFT {
     const int nt = zy_threads<T>(lw);
     const size_t lds = zy_inv_lds<T>(lw);
     const int lmax = std::max(L[2], (P[2] + 1) / 2);
-    const void *fn = lmax <= 8    ? reinterpret_cast<const void *>(&k_dip_zy_inv<T, 8>)
-                     : lmax <= 16 ? reinterpret_cast<const void *>(&k_dip_zy_inv<T, 16>)
-                                  : reinterpret_cast<const void *>(&k_dip_zy_inv<T, 32>);
-    if (lds > 64 * 1024)
-      DCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    if (lmax <= 8)
-      hipLaunchKernelGGL((k_dip_zy_inv<T, 8>), dim3(blocks), dim3(nt), lds, st, q);
-    else if (lmax <= 16)
-      hipLaunchKernelGGL((k_dip_zy_inv<T, 16>), dim3(blocks), dim3(nt), lds, st, q);
-    else
-      hipLaunchKernelGGL((k_dip_zy_inv<T, 32>), dim3(blocks), dim3(nt), lds, st, q);
-    return hipGetLastError();
+    const int lwk = lines512(1) && (lw == 3 || lw == 5) ? lw : 0;
+    if (lmax <= 8) DCHK((launch_zy_inv<T, 8>(q, lwk, blocks, nt, lds, st)));
+    else if (lmax <= 16) DCHK((launch_zy_inv<T, 16>(q, lwk, blocks, nt, lds, st)));
+    else DCHK((launch_zy_inv<T, 32>(q, lwk, blocks, nt, lds, st)));
+    return hipSuccess;
   }
 
   template <class T>
@@ -1103,12 +1184,17 @@ class DipoleFFTImpl : public DipoleFFT {
     const unsigned blocks = (unsigned)((int64_t)3 * nb * L[0] * q.nsplit);
     const int nt = zy_threads<T>(q.lw);
     const size_t lds = zy_lds<T>(q.lw);
-    if (L[2] <= 8)
-      hipLaunchKernelGGL((k_dip_zy_fwd<T, 8>), dim3(blocks), dim3(nt), lds, st, q);
-    else if (L[2] <= 16)
-      hipLaunchKernelGGL((k_dip_zy_fwd<T, 16>), dim3(blocks), dim3(nt), lds, st, q);
-    else
-      hipLaunchKernelGGL((k_dip_zy_fwd<T, 32>), dim3(blocks), dim3(nt), lds, st, q);
+    const bool k3 = lines512(1) && q.lw == 3;
+    if (L[2] <= 8) {
+      if (k3) hipLaunchKernelGGL((k_dip_zy_fwd<T, 8, 3>), dim3(blocks), dim3(nt), lds, st, q);
+      else hipLaunchKernelGGL((k_dip_zy_fwd<T, 8, 0>), dim3(blocks), dim3(nt), lds, st, q);
+    } else if (L[2] <= 16) {
+      if (k3) hipLaunchKernelGGL((k_dip_zy_fwd<T, 16, 3>), dim3(blocks), dim3(nt), lds, st, q);
+      else hipLaunchKernelGGL((k_dip_zy_fwd<T, 16, 0>), dim3(blocks), dim3(nt), lds, st, q);
+    } else {
+      if (k3) hipLaunchKernelGGL((k_dip_zy_fwd<T, 32, 3>), dim3(blocks), dim3(nt), lds, st, q);
+      else hipLaunchKernelGGL((k_dip_zy_fwd<T, 32, 0>), dim3(blocks), dim3(nt), lds, st, q);
+    }
     return hipGetLastError();
   }
 
