@@ -741,17 +741,20 @@ __global__ void __launch_bounds__(512) k_dip_zy_inv(const ZYFused<T> p) {
       for (int l = 0; l < nq; ++l) {
         const int k = q0 + l;
         const T2 X = A[tid * lw + l];
+        int t = 0;  // (k z) mod P2
 #pragma unroll
         for (int z = 0; z < LM; ++z) {
           if (z < L2) {
             if (k == 0) {
               acc[z] = X.x;
             } else {
-              const T2 w = twz[(k * z) % P2];  // e^{-2 pi i kz/P2}; conjugated below
+              const T2 w = twz[t];  // e^{-2 pi i kz/P2}; conjugated below
               const T re = X.x * w.x + X.y * w.y;
               if (2 * k == P2) acc[z] = acc[z] + re;
               else acc[z] = acc[z] + 2 * re;
             }
+            t += k;
+            if (t >= P2) t -= P2;
           }
         }
       }
