@@ -650,7 +650,9 @@ __global__ void __launch_bounds__(512) k_dip_zy_fwd(const ZYFused<T> p) {
   const int64_t base = (int64_t)x * L1 * L2 * p.nb + m;
   for (int e = tid; e < L1 * L2; e += nt) {
     const int64_t idx = base + (int64_t)e * p.nb;
-    vals[(e / L2) * SV + e % L2] = (T)(p.mus[idx] * p.spins[spin_idx(p.nspin, idx, comp)]);
+    // (y, z) of element e: a shift when L2 fills the register bound LM
+    const int y = L2 == LM ? e / LM : e / L2, z = e - y * L2;
+    vals[y * SV + z] = (T)(p.mus[idx] * p.spins[spin_idx(p.nspin, idx, comp)]);
   }
   __syncthreads();
   // direct DFT along z of line y (the arithmetic of k_dip_z_direct_fwd)
@@ -769,7 +771,7 @@ __global__ void __launch_bounds__(512) k_dip_zy_inv(const ZYFused<T> p) {
   __syncthreads();
   const int64_t base = (int64_t)x * L1 * L2 * p.nb + m;
   for (int e = tid; e < L1 * L2; e += nt) {
-    const int y = e / L2, z = e - y * L2;
+    const int y = L2 == LM ? e / LM : e / L2, z = e - y * L2;
     const int64_t idx = base + (int64_t)e * p.nb;
     p.field[comp * p.nspin + idx] = p.mus[idx] * (double)outs[y * SZ + z];
   }
