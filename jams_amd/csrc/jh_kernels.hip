@@ -25,6 +25,8 @@
 namespace jh {
 
 constexpr int kGatherUnroll = JH_GATHER_UNROLL;
+// Tensor-table rows are indexed with 24-bit multiplies (__mul24, full rate;
+// a 32-bit multiply is a quarter-rate instruction): table ids are < 2^24.
 // full-tensor rows (9 products per neighbour) gain from more loads in flight
 template <int XVk>
 constexpr int kUnrollFor = xv_diag(XVk) ? kGatherUnroll : 2 * kGatherUnroll;
@@ -59,19 +61,30 @@ __device__ __forceinline__ void xfield64(const XView &x, const double *__restric
   const int lane = (int)(i & 63);
   const int64_t b0 = x.slice_off[sl], b1 = x.slice_off[sl + 1];
   double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  // (x, y) pairs and z by 32-bit byte offsets from scalar bases when they fit
+  const bool off32 = n < ((int64_t)1 << 28);
+  const char *__restrict__ xy = reinterpret_cast<const char *>(src);
+  const char *__restrict__ zs = reinterpret_cast<const char *>(src + 2 * n);
 #pragma unroll 4
   for (int64_t e = b0 + lane; e < b1; e += kSlice) {
     const uint32_t w = __builtin_nontemporal_load(x.ent + e);
     const uint32_t j = w & x.colmask;
     double sx, sy, sz;
-    load_spin(src, n, j, sx, sy, sz);
+    if (off32) {
+      const double2 v = *reinterpret_cast<const double2 *>(xy + (j << 4));
+      sx = v.x;
+      sy = v.y;
+      sz = *reinterpret_cast<const double *>(zs + (j << 3));
+    } else {
+      load_spin(src, n, j, sx, sy, sz);
+    }
     if constexpr (XVk == XV_TDIAG) {
-      const double *T = tab + 3 * (w >> x.colbits);
+      const double *T = tab + __mul24((int)(w >> x.colbits), 3);
       a0 = a0 + sx * T[0];
       a1 = a1 + sy * T[1];
       a2 = a2 + sz * T[2];
     } else if constexpr (XVk == XV_TFULL) {
-      const double *T = tab + 9 * (w >> x.colbits);
+      const double *T = tab + __mul24((int)(w >> x.colbits), 9);
       a0 = a0 + sx * T[0]; a0 = a0 + sy * T[1]; a0 = a0 + sz * T[2];
       a1 = a1 + sx * T[3]; a1 = a1 + sy * T[4]; a1 = a1 + sz * T[5];
       a2 = a2 + sx * T[6]; a2 = a2 + sy * T[7]; a2 = a2 + sz * T[8];
@@ -112,12 +125,12 @@ __device__ __forceinline__ void xfield32(const XView &x, const float *__restrict
     double sx, sy, sz;
     load_g32(src, j, sx, sy, sz);
     if constexpr (XVk == XV_TDIAG) {
-      const double *T = tab + 3 * (w >> x.colbits);
+      const double *T = tab + __mul24((int)(w >> x.colbits), 3);
       a0 = __builtin_fma(sx, T[0], a0);
       a1 = __builtin_fma(sy, T[1], a1);
       a2 = __builtin_fma(sz, T[2], a2);
     } else if constexpr (XVk == XV_TFULL) {
-      const double *T = tab + 9 * (w >> x.colbits);
+      const double *T = tab + __mul24((int)(w >> x.colbits), 9);
       a0 = __builtin_fma(sx, T[0], a0); a0 = __builtin_fma(sy, T[1], a0); a0 = __builtin_fma(sz, T[2], a0);
       a1 = __builtin_fma(sx, T[3], a1); a1 = __builtin_fma(sy, T[4], a1); a1 = __builtin_fma(sz, T[5], a1);
       a2 = __builtin_fma(sx, T[6], a2); a2 = __builtin_fma(sy, T[7], a2); a2 = __builtin_fma(sz, T[8], a2);
@@ -148,23 +161,41 @@ __device__ __forceinline__ void pfield64(const XView &x, const double *__restric
   const int len = (int)pat[o];
   const uint32_t tmask = (1u << x.tbits) - 1u;
   double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-#pragma unroll kUnrollFor<XVk>
-  for (int k = 1; k <= len; ++k) {
-    const uint32_t w = pat[o + k];
-    const int64_t j = i + (int64_t)(((int32_t)w) >> x.tbits);
-    const uint32_t t = w & tmask;
-    double sx, sy, sz;
-    load_spin(src, n, j, sx, sy, sz);
+  // the products of one neighbour, in the reference order
+  auto accum = [&](uint32_t t, double sx, double sy, double sz) {
     if constexpr (xv_diag(XVk)) {
-      const double *T = tab + 3 * t;
+      const double *T = tab + __mul24((int)t, 3);
       a0 = a0 + sx * T[0];
       a1 = a1 + sy * T[1];
       a2 = a2 + sz * T[2];
     } else {
-      const double *T = tab + 9 * t;
+      const double *T = tab + __mul24((int)t, 9);
       a0 = a0 + sx * T[0]; a0 = a0 + sy * T[1]; a0 = a0 + sz * T[2];
       a1 = a1 + sx * T[3]; a1 = a1 + sy * T[4]; a1 = a1 + sz * T[5];
       a2 = a2 + sx * T[6]; a2 = a2 + sy * T[7]; a2 = a2 + sz * T[8];
+    }
+  };
+  if (n < ((int64_t)1 << 28)) {
+    // (x, y) pairs and z addressed by 32-bit byte offsets from scalar bases
+    const char *__restrict__ xy = reinterpret_cast<const char *>(src);
+    const char *__restrict__ zs = reinterpret_cast<const char *>(src + 2 * n);
+    const uint32_t ii = (uint32_t)i;
+#pragma unroll kUnrollFor<XVk>
+    for (int k = 1; k <= len; ++k) {
+      const uint32_t w = pat[o + k];
+      const uint32_t j = ii + (uint32_t)(((int32_t)w) >> x.tbits);
+      const double2 v = *reinterpret_cast<const double2 *>(xy + (j << 4));
+      const double sz = *reinterpret_cast<const double *>(zs + (j << 3));
+      accum(w & tmask, v.x, v.y, sz);
+    }
+  } else {
+#pragma unroll kUnrollFor<XVk>
+    for (int k = 1; k <= len; ++k) {
+      const uint32_t w = pat[o + k];
+      const int64_t j = i + (int64_t)(((int32_t)w) >> x.tbits);
+      double sx, sy, sz;
+      load_spin(src, n, j, sx, sy, sz);
+      accum(w & tmask, sx, sy, sz);
     }
   }
   h0 = a0; h1 = a1; h2 = a2;
@@ -187,12 +218,12 @@ __device__ __forceinline__ void pfield32(const XView &x, const float *__restrict
     double sx, sy, sz;
     load_g32(src, (uint32_t)j, sx, sy, sz);
     if constexpr (xv_diag(XVk)) {
-      const double *T = tab + 3 * t;
+      const double *T = tab + __mul24((int)t, 3);
       a0 = __builtin_fma(sx, T[0], a0);
       a1 = __builtin_fma(sy, T[1], a1);
       a2 = __builtin_fma(sz, T[2], a2);
     } else {
-      const double *T = tab + 9 * t;
+      const double *T = tab + __mul24((int)t, 9);
       a0 = __builtin_fma(sx, T[0], a0); a0 = __builtin_fma(sy, T[1], a0); a0 = __builtin_fma(sz, T[2], a0);
       a1 = __builtin_fma(sx, T[3], a1); a1 = __builtin_fma(sy, T[4], a1); a1 = __builtin_fma(sz, T[5], a1);
       a2 = __builtin_fma(sx, T[6], a2); a2 = __builtin_fma(sy, T[7], a2); a2 = __builtin_fma(sz, T[8], a2);
