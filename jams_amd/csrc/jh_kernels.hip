@@ -61,30 +61,19 @@ __device__ __forceinline__ void xfield64(const XView &x, const double *__restric
   const int lane = (int)(i & 63);
   const int64_t b0 = x.slice_off[sl], b1 = x.slice_off[sl + 1];
   double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-  // (x, y) pairs and z by 32-bit byte offsets from scalar bases when they fit
-  const bool off32 = n < ((int64_t)1 << 28);
-  const char *__restrict__ xy = reinterpret_cast<const char *>(src);
-  const char *__restrict__ zs = reinterpret_cast<const char *>(src + 2 * n);
 #pragma unroll 4
   for (int64_t e = b0 + lane; e < b1; e += kSlice) {
     const uint32_t w = __builtin_nontemporal_load(x.ent + e);
     const uint32_t j = w & x.colmask;
     double sx, sy, sz;
-    if (off32) {
-      const double2 v = *reinterpret_cast<const double2 *>(xy + (j << 4));
-      sx = v.x;
-      sy = v.y;
-      sz = *reinterpret_cast<const double *>(zs + (j << 3));
-    } else {
-      load_spin(src, n, j, sx, sy, sz);
-    }
+    load_spin(src, n, j, sx, sy, sz);
     if constexpr (XVk == XV_TDIAG) {
-      const double *T = tab + __mul24((int)(w >> x.colbits), 3);
+      const double *T = tab + 3 * (w >> x.colbits);
       a0 = a0 + sx * T[0];
       a1 = a1 + sy * T[1];
       a2 = a2 + sz * T[2];
     } else if constexpr (XVk == XV_TFULL) {
-      const double *T = tab + __mul24((int)(w >> x.colbits), 9);
+      const double *T = tab + 9 * (w >> x.colbits);
       a0 = a0 + sx * T[0]; a0 = a0 + sy * T[1]; a0 = a0 + sz * T[2];
       a1 = a1 + sx * T[3]; a1 = a1 + sy * T[4]; a1 = a1 + sz * T[5];
       a2 = a2 + sx * T[6]; a2 = a2 + sy * T[7]; a2 = a2 + sz * T[8];
