@@ -227,6 +227,24 @@ __device__ __forceinline__ void xf_stages(const PP &p, const typename fft::C2<T>
   }
 }
 
+// Twiddle t < 512 of a 512-point line from the first half of the table:
+// exp(-2 pi i (t + 256) / 512) = -exp(-2 pi i t / 512), and the table's
+// rounded entries keep that symmetry exactly, so the value is the full
+// table's.  The half table (4 KB in FP64) lets a third fused x workgroup fit
+// per CU.
+constexpr int kTw512 = 256;
+template <class T, bool INV>
+__device__ __forceinline__ typename fft::C2<T>::type twiddle512(const typename fft::C2<T>::type *tw,
+                                                                 int t) {
+  typename fft::C2<T>::type w = tw[t & (kTw512 - 1)];
+  if (t & kTw512) {
+    w.x = -w.x;
+    w.y = -w.y;
+  }
+  if (INV) w.y = -w.y;
+  return w;
+}
+
 // 512-point lines (radix 8, 8, 8) and LW lines known at compile time: the
 // thread's line l = t mod LW and butterfly j = t / LW are the same in every
 // stage, the stage's sub-length is a power of two, and the LDS reads and
@@ -247,7 +265,7 @@ __device__ __forceinline__ void xf_stage512(const typename fft::C2<T>::type *tw,
 #pragma unroll
     for (int r = 0; r < R; ++r) v[r] = a[r * m * LW];
 #pragma unroll
-    for (int r = 1; r < R; ++r) v[r] = fft::cmul(v[r], fft::twiddle<T, INV>(tw, jm * r * tws));
+    for (int r = 1; r < R; ++r) v[r] = fft::cmul(v[r], twiddle512<T, INV>(tw, jm * r * tws));
     fft::dft_fixed<T2, R>(v, INV);
   }
   __syncthreads();
@@ -298,7 +316,7 @@ __global__ void __launch_bounds__(512) k_dip_x_fused(const XFused<T> p) {
   const int nt = blockDim.x;
   T2 *A = reinterpret_cast<T2 *>(smem);
   T2 *tw = A + (size_t)n * lw;
-  for (int t = threadIdx.x; t < n; t += nt) tw[t] = p.tw[t];
+  for (int t = threadIdx.x; t < (LWK ? kTw512 : n); t += nt) tw[t] = p.tw[t];
   // consecutive blocks on one XCD share the L2 lines of adjacent y
   const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
   const int q2 = (int)(blk / p.nyb), y0 = (int)(blk % p.nyb) << lby;
@@ -633,10 +651,10 @@ __global__ void __launch_bounds__(512) k_dip_zy_fwd(const ZYFused<T> p) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int n = LWK ? 512 : p.n, lw = LWK ? LWK : p.lw, nt = blockDim.x, tid = threadIdx.x;
   T2 *twy = reinterpret_cast<T2 *>(smem);
-  T2 *twz = twy + n;
+  T2 *twz = twy + (LWK ? kTw512 : n);
   T2 *A = twz + p.P2;                   // [y][line], P1 x lw
   T *vals = reinterpret_cast<T *>(A);   // the same area, used first: [y][z]
-  for (int t = tid; t < n; t += nt) twy[t] = p.twy[t];
+  for (int t = tid; t < (LWK ? kTw512 : n); t += nt) twy[t] = p.twy[t];
   for (int t = tid; t < p.P2; t += nt) twz[t] = p.twz[t];
   const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
   const int sp = (int)(blk % p.nsplit);
@@ -715,10 +733,10 @@ __global__ void __launch_bounds__(512) k_dip_zy_inv(const ZYFused<T> p) {
   const int L1 = p.L1, L2 = p.L2, Q2 = p.Q2, P2 = p.P2;
   const int SZ = odd_stride(L2);
   T2 *twy = reinterpret_cast<T2 *>(smem);
-  T2 *twz = twy + n;
+  T2 *twz = twy + (LWK ? kTw512 : n);
   T2 *A = twz + P2;                      // [y][line], P1 x lw; later outs
   T *outs = reinterpret_cast<T *>(A);    // [y < L1][SZ]
-  for (int t = tid; t < n; t += nt) twy[t] = p.twy[t];
+  for (int t = tid; t < (LWK ? kTw512 : n); t += nt) twy[t] = p.twy[t];
   for (int t = tid; t < P2; t += nt) twz[t] = p.twz[t];
   const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
   const int x = (int)(blk % p.L0), g = (int)(blk / p.L0);
@@ -937,6 +955,11 @@ class DipoleFFTImpl : public DipoleFFT {
   // axis of 512 points factored 8 x 8 x 8: the fused kernels' compile-time
   // stage geometry applies
   bool lines512(int axis) const { return P[axis] == 512 && rad[axis] == std::vector<int>{8, 8, 8}; }
+  // the compile-time line count of each fused kernel (0: general stages); the
+  // LDS sizes and the launches both follow these
+  int x_lwk(int lw) const { return lines512(0) && lw == 6 ? 6 : 0; }
+  int zy_fwd_lwk(int lw) const { return lines512(1) && lw == 3 ? 3 : 0; }
+  int zy_inv_lwk(int lw) const { return lines512(1) && (lw == 3 || lw == 5) ? lw : 0; }
 
   // y lines per workgroup of the fused x kernel (0: not applicable): one
   // basis site with a real tensor, in-place radices, LDS within 64 KB (two
@@ -979,9 +1002,9 @@ class DipoleFFTImpl : public DipoleFFT {
     q.K = pl.K.p;
     q.W = pl.Wf.p;
     q.tw = pl.tw[0].p;
-    const size_t lds = (size_t)q.n * (q.lw + 1) * sizeof(T2);
+    const size_t lds = ((size_t)q.n * q.lw + (x_lwk(q.lw) ? kTw512 : q.n)) * sizeof(T2);
     const unsigned blocks = (unsigned)(Q2 * q.nyb);
-    if (lines512(0) && q.lw == 6)
+    if (x_lwk(q.lw) == 6)
       hipLaunchKernelGGL((k_dip_x_fused<T, 6>), dim3(blocks), dim3(xf_threads<T>(q.by)), lds, st, q);
     else
       hipLaunchKernelGGL((k_dip_x_fused<T, 0>), dim3(blocks), dim3(xf_threads<T>(q.by)), lds, st, q);
@@ -1093,7 +1116,7 @@ class DipoleFFTImpl : public DipoleFFT {
     using T2 = typename fft::C2<T>::type;
     const size_t a = std::max((size_t)P[1] * lw * sizeof(T2),
                               (size_t)L[1] * (size_t)odd_stride(L[2]) * sizeof(T));
-    return (size_t)(P[1] + P[2]) * sizeof(T2) + a;
+    return (size_t)((zy_fwd_lwk(lw) ? kTw512 : P[1]) + P[2]) * sizeof(T2) + a;
   }
   // workgroups per (g, x) of the fused z + y forward kernel, 0 when it does
   // not apply (direct z axis, in-place y radices, LDS within 64 KB, at most
@@ -1117,7 +1140,7 @@ class DipoleFFTImpl : public DipoleFFT {
     using T2 = typename fft::C2<T>::type;
     const size_t a = std::max((size_t)P[1] * lw * sizeof(T2),
                               (size_t)L[1] * (size_t)odd_stride(L[2]) * sizeof(T));
-    return (size_t)(P[1] + P[2]) * sizeof(T2) + a;
+    return (size_t)((zy_inv_lwk(lw) ? kTw512 : P[1]) + P[2]) * sizeof(T2) + a;
   }
   // q2 lines per batch of the fused y + z inverse kernel, 0 when it does not
   // apply: LDS within JH_ZYI_LDS, at most 512 threads; batches of equal size
@@ -1174,7 +1197,7 @@ class DipoleFFTImpl : public DipoleFFT {
     const int nt = zy_threads<T>(lw);
     const size_t lds = zy_inv_lds<T>(lw);
     const int lmax = std::max(L[2], (P[2] + 1) / 2);
-    const int lwk = lines512(1) && (lw == 3 || lw == 5) ? lw : 0;
+    const int lwk = zy_inv_lwk(lw);
     if (lmax <= 8) DCHK((launch_zy_inv<T, 8>(q, lwk, blocks, nt, lds, st)));
     else if (lmax <= 16) DCHK((launch_zy_inv<T, 16>(q, lwk, blocks, nt, lds, st)));
     else DCHK((launch_zy_inv<T, 32>(q, lwk, blocks, nt, lds, st)));
@@ -1189,7 +1212,7 @@ class DipoleFFTImpl : public DipoleFFT {
     const unsigned blocks = (unsigned)((int64_t)3 * nb * L[0] * q.nsplit);
     const int nt = zy_threads<T>(q.lw);
     const size_t lds = zy_lds<T>(q.lw);
-    const bool k3 = lines512(1) && q.lw == 3;
+    const bool k3 = zy_fwd_lwk(q.lw) == 3;
     if (L[2] <= 8) {
       if (k3) hipLaunchKernelGGL((k_dip_zy_fwd<T, 8, 3>), dim3(blocks), dim3(nt), lds, st, q);
       else hipLaunchKernelGGL((k_dip_zy_fwd<T, 8, 0>), dim3(blocks), dim3(nt), lds, st, q);
