@@ -8,7 +8,9 @@ Pins:
   - the field against the oracle's direct offset sum (oracle_dipole_fft.c,
     two-sided on open dimensions), which equals the brute-force sum, on
     lattices that exercise every radix, odd and prime lengths, open, periodic
-    and mixed boundaries, one and several basis sites and a non-cubic cell;
+    and mixed boundaries, one and several basis sites and a non-cubic cell,
+    and the 256 x 256 x 8 open film whose 512-point lines take the
+    compile-time stage geometry;
   - the FP32 form against FP64, and a Heun trajectory with exchange + dipole
     against the oracle's Heun step with the brute-force dipole.
 """
@@ -153,6 +155,75 @@ def test_dipole_fp32_transform_error(engine_factory, name, lat, moments, rc):
     err = np.abs(h32 - h64).max() / np.abs(h64).max()
     print(f"{name}: FP32 transform rel err {err:.2e}")
     assert 0 < err <= 1e-5
+
+
+def offset_kernel(rc, mu):
+    """mu^2 W(r) for the offsets |r_a| <= floor(rc) of a simple-cubic lattice,
+    read off the oracle's direct sum on a small open cube with one spin at its
+    centre (the sum is linear in the moments): K[m + r][:, b] is the field at
+    centre + r from a unit spin along b at the centre."""
+    m = int(math.floor(rc))
+    L = 2 * m + 1
+    lat = po.LatticeSpec(SC, [[0, 0, 0]], [0], (L, L, L), (0, 0, 0))
+    c = (m * L + m) * L + m
+    K = np.zeros((L, L, L, 3, 3))
+    for b in range(3):
+        s = np.zeros((lat.n, 3))
+        s[c, b] = 1.0
+        K[..., :, b] = po.dipole_fft_direct(lat, np.full(lat.n, mu), s, rc, A).reshape(L, L, L, 3)
+    return K
+
+
+def stencil_field(size, K, s):
+    """h(x) = sum_r K(r) s(x - r) on an open simple-cubic lattice (sites in
+    the oracle's index order, zero moment outside)."""
+    m = (K.shape[0] - 1) // 2
+    L0, L1, L2 = size
+    S_ = np.pad(s.reshape(L0, L1, L2, 3), ((m, m), (m, m), (m, m), (0, 0)))
+    h = np.zeros((L0, L1, L2, 3))
+    for r0 in range(-m, m + 1):
+        for r1 in range(-m, m + 1):
+            for r2 in range(-m, m + 1):
+                k = K[m + r0, m + r1, m + r2]
+                if not k.any():
+                    continue
+                v = S_[m - r0:m - r0 + L0, m - r1:m - r1 + L1, m - r2:m - r2 + L2]
+                h += v @ k.T
+    return h.reshape(-1, 3)
+
+
+def test_dipole_512_point_film(engine_factory):
+    """The 256 x 256 x 8 open film (padded 512 x 512 x 16: the fused kernels
+    with the compile-time 512-point stages and the direct 16-point z DFT),
+    r_cutoff 2.5: the FP64 field against the oracle's tensor applied as a
+    stencil, <= 1e-10 max|h|, and the FP32 transforms against FP64,
+    <= 1e-5 max|h|. The oracle's direct sum is O(N x padded grid), too slow
+    at this size; the stencil is checked against it on a 12 x 10 x 4 film
+    first (<= 1e-13 max|h|)."""
+    rc, mu = 2.5, 2.0 * S.BOHR
+    K = offset_kernel(rc, mu)
+    small = po.LatticeSpec(SC, [[0, 0, 0]], [0], (12, 10, 4), (0, 0, 0))
+    s = rnd_spins(small.n, 5)
+    ref = po.dipole_fft_direct(small, np.full(small.n, mu), s, rc, A, nthreads=8)
+    got = stencil_field(small.size, K, s)
+    assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max()
+
+    lat = po.LatticeSpec(SC, [[0, 0, 0]], [0], (256, 256, 8), (0, 0, 0))
+    mus = np.full(lat.n, mu)
+    s = rnd_spins(lat.n, 17)
+    ref = stencil_field(lat.size, K, s)
+    scale = np.abs(ref).max()
+    e64 = dipole_engine(engine_factory, lat, mus, rc, prec=0)
+    e64.set_spins(s)
+    h64 = e64.term_field(0)
+    err64 = np.abs(h64 - ref).max() / scale
+    e32 = dipole_engine(engine_factory, lat, mus, rc, prec=1)
+    e32.set_spins(s)
+    err32 = np.abs(e32.term_field(0) - h64).max() / np.abs(h64).max()
+    print(f"256x256x8 film: max|h| {scale:.3e} meV, FP64 rel err {err64:.2e}, "
+          f"FP32 transform rel err {err32:.2e}")
+    assert err64 <= 1e-10
+    assert 0 < err32 <= 1e-5
 
 
 def test_heun_with_exchange_and_dipole_trajectory(engine_factory):
