@@ -418,3 +418,26 @@ def test_rk4_unit_length_and_agreement_with_heun():
     print(f"RK4 vs Heun gaps {gaps}")
     assert gaps[0] < 1e-4
     assert 3.0 < gaps[0] / gaps[1] < 5.0
+
+
+# ---- dipole stencil ------------------------------------------------------------
+def test_dipole_stencil_matches_direct_sum():
+    """The dipole tensor read off the direct offset sum with one spin at the
+    centre of a small cube, applied as a stencil, is the reference of the GPU
+    test on the 256 x 256 x 8 open film: it equals the direct sum on a
+    12 x 10 x 4 open film (r_cutoff 2.5). Catches a dropped or misplaced
+    offset, a wrong boundary (wrap instead of zero) and a wrong kernel size."""
+    from tests.test_gpu_dipole import A, SC, offset_kernel, stencil_field
+    rc, mu = 2.5, 2.0 * S.BOHR
+    K = offset_kernel(rc, mu)
+    assert K.shape == (5, 5, 5, 3, 3)
+    lat = po.LatticeSpec(SC, [[0, 0, 0]], [0], (12, 10, 4), (0, 0, 0))
+    s = po.random_spins(5, lat.n)
+    ref = po.dipole_fft_direct(lat, np.full(lat.n, mu), s, rc, A, nthreads=4)
+    scale = np.abs(ref).max()
+    assert np.abs(stencil_field(lat.size, K, s) - ref).max() <= 1e-13 * scale
+    dropped = K.copy()
+    dropped[2, 2, 4] = 0.0
+    assert np.abs(stencil_field(lat.size, dropped, s) - ref).max() > 1e-3 * scale
+    wrapped = np.roll(s.reshape(12, 10, 4, 3), 1, axis=2).reshape(-1, 3)
+    assert np.abs(stencil_field(lat.size, K, wrapped) - ref).max() > 1e-1 * scale
