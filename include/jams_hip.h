@@ -62,6 +62,9 @@ extern "C" {
 #define JH_XFMT_BLOCK 2       /* per neighbour: column + explicit 3x3 tensor     */
 #define JH_XFMT_PATTERN 3     /* per spin: offset of its row pattern, a shared
                                  sequence of (column - row, tensor id) pairs     */
+#define JH_XFMT_PATTERN_GLOBAL 4 /* PATTERN with the pattern table read from
+                                 global memory (what PATTERN falls back to when
+                                 the table does not fit in LDS)                  */
 
 /* term kinds reported by jh_term_kind */
 #define JH_TERM_EXCHANGE 0

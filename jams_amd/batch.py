@@ -88,6 +88,8 @@ class ReplicaBatch:
         e.set_replicas(self.offsets, cat["temperatures"], cat["seeds"])
         self.temperatures = cat["temperatures"]
         self.seeds = cat["seeds"]
+        self.mus, self.gyro, self.alpha = cat["mus"], cat["gyro"], cat["alpha"]
+        self.zeeman_dc = cat["zeeman_dc"]
         self.engine = e
 
     def spins(self, r: int | None = None) -> np.ndarray:

@@ -24,6 +24,11 @@ class DipoleFFT {
   virtual hipError_t field(const double *s, double *h, bool fp32, hipStream_t st) = 0;
   // algorithmic HBM bytes of one field evaluation
   virtual double bytes(bool fp32) const = 0;
+  // the kernel variants an evaluation launches: the x axis (fused x forward +
+  // multiply + x inverse, or separate passes), the z + y forward and the
+  // y + z inverse (fused, or separate passes); "512" marks the compile-time
+  // 512-point stages, the number the lines per workgroup
+  virtual std::string describe(bool fp32) const = 0;
 };
 
 }  // namespace jh

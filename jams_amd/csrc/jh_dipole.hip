@@ -89,14 +89,22 @@ std::vector<int> factorise(int n) {
   return r;
 }
 
+// exp(-2 pi i t / n), t < n.  Entries of the first half are evaluated in long
+// double and rounded once; for even n the second half is their exact
+// negation, w(t + n/2) = -w(t), so the rounded table keeps that symmetry by
+// construction (the 512-point kernels hold only the first half).
 template <class T2>
 std::vector<T2> twiddles(int n) {
   std::vector<T2> w((size_t)n);
-  for (int t = 0; t < n; ++t) {
-    // exp(-2 pi i t / n), angle reduced to the first octant in long double
+  const int half = (n % 2 == 0) ? n / 2 : n;
+  for (int t = 0; t < half; ++t) {
     const long double a = -2.0L * 3.141592653589793238462643383279502884L * (long double)t / n;
     w[t].x = (decltype(w[t].x))cosl(a);
     w[t].y = (decltype(w[t].y))sinl(a);
+  }
+  for (int t = half; t < n; ++t) {
+    w[t].x = -w[t - half].x;
+    w[t].y = -w[t - half].y;
   }
   return w;
 }
@@ -228,10 +236,10 @@ __device__ __forceinline__ void xf_stages(const PP &p, const typename fft::C2<T>
 }
 
 // Twiddle t < 512 of a 512-point line from the first half of the table:
-// exp(-2 pi i (t + 256) / 512) = -exp(-2 pi i t / 512), and the table's
-// rounded entries keep that symmetry exactly, so the value is the full
-// table's.  The half table (4 KB in FP64) lets a third fused x workgroup fit
-// per CU.
+// exp(-2 pi i (t + 256) / 512) = -exp(-2 pi i t / 512), and twiddles() builds
+// the second half as the exact negation of the first, so the value is the
+// full table's.  The half table (4 KB in FP64) lets a third fused x
+// workgroup fit per CU.
 constexpr int kTw512 = 256;
 template <class T, bool INV>
 __device__ __forceinline__ typename fft::C2<T>::type twiddle512(const typename fft::C2<T>::type *tw,
@@ -1337,6 +1345,24 @@ class DipoleFFTImpl : public DipoleFFT {
 
   hipError_t field(const double *s, double *h, bool fp32, hipStream_t st) override {
     return fp32 ? eval(f32, s, h, st) : eval(f64, s, h, st);
+  }
+
+  template <class T>
+  std::string describe_t() const {
+    const int by = fused_lines<T>();
+    const int ns = zy_split<T>();
+    const int lwf = ns ? (int)((Q2 + ns - 1) / ns) : 0;
+    const int lwi = zy_inv_lines<T>();
+    auto part = [](const char *name, int lw, int lwk) {
+      return std::string(name) + "=" +
+             (lw ? (lwk ? "fused512/" : "fused/") + std::to_string(lw) : std::string("passes"));
+    };
+    return part("dip_x", by ? 3 * by : 0, by ? x_lwk(3 * by) : 0) + " " +
+           part("dip_zyf", lwf, lwf ? zy_fwd_lwk(lwf) : 0) + " " +
+           part("dip_zyi", lwi, lwi ? zy_inv_lwk(lwi) : 0);
+  }
+  std::string describe(bool fp32) const override {
+    return fp32 ? describe_t<float>() : describe_t<double>();
   }
 
   double bytes(bool fp32) const override {

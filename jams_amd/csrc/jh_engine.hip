@@ -582,23 +582,26 @@ void prepare_exchange(jh_ctx *c) {
   // row patterns: identical sequences of (column - row, tensor id)
   std::vector<uint32_t> pofs, pat;
   const bool by_tensor = c->prec == PR_FP32;
-  bool pattern_ok = typed_ok && (c->xfmt_req == JH_XFMT_AUTO || c->xfmt_req == JH_XFMT_PATTERN) &&
+  const bool want_pattern = c->xfmt_req == JH_XFMT_AUTO || c->xfmt_req == JH_XFMT_PATTERN ||
+                            c->xfmt_req == JH_XFMT_PATTERN_GLOBAL;
+  bool pattern_ok = typed_ok && want_pattern &&
                     build_patterns(c, x, tbits, by_tensor, pofs, pat, c->npatterns);
   int fmt = c->xfmt_req;
   if (fmt == JH_XFMT_AUTO) fmt = pattern_ok ? JH_XFMT_PATTERN : typed_ok ? JH_XFMT_TYPED : JH_XFMT_BLOCK;
-  if ((fmt == JH_XFMT_TYPED || fmt == JH_XFMT_PATTERN) && !typed_ok)
+  if (fmt != JH_XFMT_BLOCK && !typed_ok)
     JH_THROW(JH_ERR_UNSUPPORTED, "TYPED exchange format does not fit (too many distinct tensors)");
-  if (fmt == JH_XFMT_PATTERN && !pattern_ok)
+  if ((fmt == JH_XFMT_PATTERN || fmt == JH_XFMT_PATTERN_GLOBAL) && !pattern_ok)
     JH_THROW(JH_ERR_UNSUPPORTED, "PATTERN exchange format does not fit (too many distinct rows)");
   c->xfmt = fmt;
-  if (fmt == JH_XFMT_PATTERN) {
+  if (fmt == JH_XFMT_PATTERN || fmt == JH_XFMT_PATTERN_GLOBAL) {
     upload_table(c, x, ntypes, ntab, vpt, diag);
     c->pofs.alloc(pofs.size());
     c->pat.alloc(pat.size());
     HIPCHK(hipMemcpy(c->pofs.p, pofs.data(), sizeof(uint32_t) * pofs.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->pat.p, pat.data(), sizeof(uint32_t) * pat.size(), hipMemcpyHostToDevice));
     c->pat_words = (int)pat.size();
-    c->pat_lds = (((lds_need + 15) & ~(size_t)15) + pat.size() * 4 <= kMaxTableLds) ? 1 : 0;
+    c->pat_lds = (fmt == JH_XFMT_PATTERN &&
+                  ((lds_need + 15) & ~(size_t)15) + pat.size() * 4 <= kMaxTableLds) ? 1 : 0;
     c->tbits = tbits;
     c->ntab = ntab;
     c->nent = x.ptr[n];
@@ -1323,7 +1326,8 @@ int jh_set_integrator(jh_ctx *c, int integrator) {
 
 int jh_set_exchange_format(jh_ctx *c, int fmt) {
   return guarded(c, [&] {
-    if (fmt != JH_XFMT_AUTO && fmt != JH_XFMT_TYPED && fmt != JH_XFMT_BLOCK && fmt != JH_XFMT_PATTERN)
+    if (fmt != JH_XFMT_AUTO && fmt != JH_XFMT_TYPED && fmt != JH_XFMT_BLOCK && fmt != JH_XFMT_PATTERN &&
+        fmt != JH_XFMT_PATTERN_GLOBAL)
       JH_THROW(JH_ERR_INVALID, "unknown exchange format");
     c->xfmt_req = fmt;
     c->dirty = true;
@@ -1687,8 +1691,11 @@ int jh_describe(jh_ctx *c, char *buf, int64_t buflen) {
                   (long long)c->nslices, c->order_nb, c->nclass,
                   (int)c->terms.size(), c->noise_mode, c->launch_block(), c->nrep, c->graph_steps,
                   c->integrator == JH_INTEGRATOR_RK4 ? "rk4" : "heun");
+    std::string out = tmp;
+    const int dip = dipole_term(c);
+    if (dip >= 0) out += " " + c->dipoles[dip]->describe(c->prec == PR_FP32);
     if (buf && buflen > 0) {
-      std::strncpy(buf, tmp, (size_t)buflen - 1);
+      std::strncpy(buf, out.c_str(), (size_t)buflen - 1);
       buf[buflen - 1] = 0;
     }
   });

@@ -19,7 +19,7 @@ PREC_FP64 = 0
 PREC_FP32_FIELD = 1
 INTEGRATOR_HEUN = 0
 INTEGRATOR_RK4 = 1
-XFMT_AUTO, XFMT_TYPED, XFMT_BLOCK, XFMT_PATTERN = 0, 1, 2, 3
+XFMT_AUTO, XFMT_TYPED, XFMT_BLOCK, XFMT_PATTERN, XFMT_PATTERN_GLOBAL = 0, 1, 2, 3, 4
 TERM_EXCHANGE, TERM_ZEEMAN, TERM_UNIAXIAL, TERM_DIPOLE = 0, 1, 2, 3
 
 # every entry point declared in include/jams_hip.h
@@ -322,8 +322,8 @@ class Engine:
         return b.value
 
     def describe(self) -> str:
-        buf = C.create_string_buffer(512)
-        self._chk(self.lib.jh_describe(self.h, buf, 512))
+        buf = C.create_string_buffer(1024)
+        self._chk(self.lib.jh_describe(self.h, buf, 1024))
         return buf.value.decode()
 
     def synchronize(self):

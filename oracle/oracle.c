@@ -875,13 +875,45 @@ double or_term_energy(const or_system_t *sys, int term, const double *s, double 
 /* ======================================================================= */
 /* Dipole brute force (hamiltonian/dipole_bruteforce.cc:75-120)              */
 
-void or_dipole_bruteforce(const or_lattice_t *lat, int64_t n, const double *pos, const double *mus,
-                          const double *s, double r_cutoff, double a_m, double *h, int nthreads) {
-  /* supercell matrix and its inverse for Smith's minimum image */
-  double S[9], Si[9];
+/* dipole_bruteforce.cc:75-120: the field at site i from every other site
+ * within r_cutoff (Smith's minimum image on periodic axes) */
+static void dipole_site(const or_lattice_t *lat, const double S[9], const double Si[9], int64_t n,
+                        const double *pos, const double *mus, const double *s, double rc2,
+                        double mu0, double a3, int64_t i, double out[3]) {
+  const double w0 = mus[i] * mu0 / (4.0 * OR_PI * a3);
+  double hx = 0, hy = 0, hz = 0;
+  for (int64_t j = 0; j < n; ++j) {
+    if (j == i) continue;
+    double r[3], f[3];
+    for (int q = 0; q < 3; ++q) r[q] = pos[3 * i + q] - pos[3 * j + q];
+    mat3_mul_vec(Si, r, f);
+    for (int q = 0; q < 3; ++q) if (lat->periodic[q]) f[q] = f[q] - nearbyint(f[q]);
+    mat3_mul_vec(S, f, r);
+    const double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+    const double mx = r2 > rc2 ? r2 : rc2;
+    if ((r2 - rc2) > mx * 1e-4) continue;
+    const double sj[3] = {s[3 * j], s[3 * j + 1], s[3 * j + 2]};
+    const double sr = sj[0] * r[0] + sj[1] * r[1] + sj[2] * r[2];
+    const double nr = sqrt(r2);
+    const double r5 = nr * nr * nr * nr * nr;
+    hx += w0 * mus[j] * (3.0 * r[0] * sr - r2 * sj[0]) / r5;
+    hy += w0 * mus[j] * (3.0 * r[1] * sr - r2 * sj[1]) / r5;
+    hz += w0 * mus[j] * (3.0 * r[2] * sr - r2 * sj[2]) / r5;
+  }
+  out[0] = hx; out[1] = hy; out[2] = hz;
+}
+
+/* supercell matrix and its inverse for Smith's minimum image */
+static void dipole_setup(const or_lattice_t *lat, double S[9], double Si[9]) {
   for (int r = 0; r < 3; ++r)
     for (int c = 0; c < 3; ++c) S[3 * r + c] = lat->cell[3 * r + c] * lat->size[c];
   mat3_inverse(S, Si);
+}
+
+void or_dipole_bruteforce(const or_lattice_t *lat, int64_t n, const double *pos, const double *mus,
+                          const double *s, double r_cutoff, double a_m, double *h, int nthreads) {
+  double S[9], Si[9];
+  dipole_setup(lat, S, Si);
   const double rc2 = r_cutoff * r_cutoff;
   const double mu0 = or_vacuum_permeability();
   const double a3 = a_m * a_m * a_m;
@@ -889,27 +921,21 @@ void or_dipole_bruteforce(const or_lattice_t *lat, int64_t n, const double *pos,
 #ifdef _OPENMP
 #pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads > 0 ? nthreads : 1)
 #endif
-  for (int64_t i = 0; i < n; ++i) {
-    const double w0 = mus[i] * mu0 / (4.0 * OR_PI * a3);
-    double hx = 0, hy = 0, hz = 0;
-    for (int64_t j = 0; j < n; ++j) {
-      if (j == i) continue;
-      double r[3], f[3];
-      for (int q = 0; q < 3; ++q) r[q] = pos[3 * i + q] - pos[3 * j + q];
-      mat3_mul_vec(Si, r, f);
-      for (int q = 0; q < 3; ++q) if (lat->periodic[q]) f[q] = f[q] - nearbyint(f[q]);
-      mat3_mul_vec(S, f, r);
-      const double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
-      const double mx = r2 > rc2 ? r2 : rc2;
-      if ((r2 - rc2) > mx * 1e-4) continue;
-      const double sj[3] = {s[3 * j], s[3 * j + 1], s[3 * j + 2]};
-      const double sr = sj[0] * r[0] + sj[1] * r[1] + sj[2] * r[2];
-      const double nr = sqrt(r2);
-      const double r5 = nr * nr * nr * nr * nr;
-      hx += w0 * mus[j] * (3.0 * r[0] * sr - r2 * sj[0]) / r5;
-      hy += w0 * mus[j] * (3.0 * r[1] * sr - r2 * sj[1]) / r5;
-      hz += w0 * mus[j] * (3.0 * r[2] * sr - r2 * sj[2]) / r5;
-    }
-    h[3 * i] = hx; h[3 * i + 1] = hy; h[3 * i + 2] = hz;
-  }
+  for (int64_t i = 0; i < n; ++i) dipole_site(lat, S, Si, n, pos, mus, s, rc2, mu0, a3, i, h + 3 * i);
+}
+
+void or_dipole_bruteforce_sites(const or_lattice_t *lat, int64_t n, const double *pos,
+                                const double *mus, const double *s, double r_cutoff, double a_m,
+                                int64_t nsites, const int64_t *sites, double *h, int nthreads) {
+  double S[9], Si[9];
+  dipole_setup(lat, S, Si);
+  const double rc2 = r_cutoff * r_cutoff;
+  const double mu0 = or_vacuum_permeability();
+  const double a3 = a_m * a_m * a_m;
+  (void)nthreads;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads > 0 ? nthreads : 1)
+#endif
+  for (int64_t q = 0; q < nsites; ++q)
+    dipole_site(lat, S, Si, n, pos, mus, s, rc2, mu0, a3, sites[q], h + 3 * q);
 }

@@ -188,6 +188,11 @@ double or_term_energy(const or_system_t *sys, int term, const double *s, double 
 void or_dipole_bruteforce(const or_lattice_t *lat, int64_t n, const double *pos,
                           const double *mus, const double *s, double r_cutoff,
                           double a_m, double *h, int nthreads);
+/* The same sum at the listed sites only (h: nsites x 3), for lattices whose
+ * full O(N^2) sum is out of reach. */
+void or_dipole_bruteforce_sites(const or_lattice_t *lat, int64_t n, const double *pos,
+                                const double *mus, const double *s, double r_cutoff, double a_m,
+                                int64_t nsites, const int64_t *sites, double *h, int nthreads);
 
 /* The FFT dipole's convolution as a direct sum over cell offsets
  * (dipole_fft.cc:195-350).  one_sided_open = 1 keeps the reference's offsets

@@ -505,3 +505,22 @@ def dipole_bruteforce(lat: LatticeSpec, mus, s, r_cutoff, a_m, nthreads=1) -> np
     lib().or_dipole_bruteforce(C.byref(Lc), C.c_int64(mus.size), _p(pos), _p(mus), _p(s),
                                C.c_double(r_cutoff), C.c_double(a_m), _p(h), C.c_int(nthreads))
     return h
+
+
+def dipole_bruteforce_sites(lat: LatticeSpec, mus, s, r_cutoff, a_m, sites,
+                            nthreads=1) -> np.ndarray:
+    """or_dipole_bruteforce at the listed sites only (nsites x 3)."""
+    pos = np.ascontiguousarray(lat.positions(), np.float64)
+    mus = np.ascontiguousarray(mus, np.float64)
+    s = np.ascontiguousarray(s, np.float64).reshape(-1, 3)
+    sites = np.ascontiguousarray(sites, np.int64)
+    if mus.size != lat.n or s.shape[0] != lat.n:
+        raise ValueError("mus and s must hold one entry per lattice site")
+    if sites.size and (sites.min() < 0 or sites.max() >= lat.n):
+        raise ValueError("site index out of range")
+    h = np.empty((sites.size, 3), np.float64)
+    Lc = lat.c()
+    lib().or_dipole_bruteforce_sites(C.byref(Lc), C.c_int64(mus.size), _p(pos), _p(mus), _p(s),
+                                     C.c_double(r_cutoff), C.c_double(a_m), C.c_int64(sites.size),
+                                     _p(sites), _p(h), C.c_int(nthreads))
+    return h

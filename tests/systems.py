@@ -150,3 +150,105 @@ def c3(L=128, with_csr=False):
     return build(bcc_lattice(L), bcc_five_shell_dmi_rows(), 2.0, 0.05, ["exchange"],
                  dt=dt_ps(0.5e-15), spins="random", seed=3, temperature=100.0,
                  with_csr=with_csr)
+
+
+# ---------------------------------------------------------------------------
+# Reference CSR rows of a few sites, found geometrically.
+#
+# The exchange CSR row 3i+m holds, for every neighbour j of site i in
+# ascending order and every tensor column n in 0..2 with J_ij[m][n] != 0, the
+# entry (3j+n, J_ij[m][n]) (hamiltonian/sparse_interaction.cc:25-33 and the
+# builder's (row, col) sort, containers/sparse_matrix_builder.h:109-248).
+# Here the neighbours come straight from the positions: site i at cell c,
+# basis m sits at (c + b_m) A (A = cell matrix, columns a1 a2 a3); the partner
+# of template vector r is the site whose position is that + r within 1e-4
+# lattice constants, wrapped on periodic axes and dropped on open ones.  No
+# oracle code is involved, so the rows of a full-size lattice can be formed
+# for the sites a test samples without building its whole CSR.
+
+def _site_cells(lat, sites):
+    sites = np.asarray(sites, np.int64)
+    nb = lat.nbasis
+    m = sites % nb
+    cell = sites // nb
+    Lx, Ly, Lz = lat.size
+    ci = cell // (Ly * Lz)
+    cj = (cell // Lz) % Ly
+    ck = cell % Lz
+    return np.stack([ci, cj, ck], axis=1), m
+
+
+def template_neighbours(lat, rows, sites, tol=1e-4):
+    """For each site (reference index) the neighbour indices and tensors of
+    the template rows that apply to it: lists of (j, J 3x3) per site."""
+    cells, mb = _site_cells(lat, sites)
+    A = lat.cell.T                      # columns a1 a2 a3
+    Ainv = np.linalg.inv(A)
+    size = np.asarray(lat.size)
+    out = []
+    for q in range(len(sites)):
+        mi = int(mb[q])
+        p = A @ (cells[q] + lat.basis_frac[mi])
+        nbrs = []
+        for ti, tj, r, J in rows:
+            if ti != lat.basis_material[mi]:
+                continue
+            f = Ainv @ (p + np.asarray(r, np.float64))
+            for mj in range(lat.nbasis):
+                if lat.basis_material[mj] != tj:
+                    continue
+                d = f - lat.basis_frac[mj]
+                c = np.rint(d)
+                if np.abs(d - c).max() >= tol:
+                    continue
+                c = c.astype(np.int64)
+                ok = True
+                for ax in range(3):
+                    if lat.periodic[ax]:
+                        c[ax] %= size[ax]
+                    elif not 0 <= c[ax] < size[ax]:
+                        ok = False
+                if not ok:
+                    continue
+                j = int(((c[0] * size[1] + c[1]) * size[2] + c[2]) * lat.nbasis + mj)
+                Jv = np.asarray(J, np.float64)
+                Jm = float(Jv) * np.eye(3) if Jv.size == 1 else Jv.reshape(3, 3)
+                nbrs.append((j, Jm))
+        nbrs.sort(key=lambda t: t[0])
+        out.append(nbrs)
+    return out
+
+
+def rows_csr(n, lat, rows, sites):
+    """The reference CSR (3n rows) restricted to the rows of `sites`; every
+    other row is empty."""
+    sites = np.unique(np.asarray(sites, np.int64))
+    nbrs = template_neighbours(lat, rows, sites)
+    counts = np.zeros(3 * n + 1, np.int64)
+    cols, vals = [], []
+    for i, nb in zip(sites.tolist(), nbrs):
+        for m in range(3):
+            k = 0
+            for j, Jm in nb:
+                for c in range(3):
+                    if Jm[m, c] != 0.0:
+                        cols.append(3 * j + c)
+                        vals.append(Jm[m, c])
+                        k += 1
+            counts[3 * i + m + 1] = k
+    return (np.cumsum(counts), np.asarray(cols, np.int32), np.asarray(vals, np.float64))
+
+
+def neighbourhood(lat, rows, sites, hops):
+    """The sites within `hops` exchange hops of `sites` (sites included)."""
+    cur = set(int(i) for i in sites)
+    frontier = list(cur)
+    for _ in range(hops):
+        nxt = []
+        for nb in template_neighbours(lat, rows, frontier):
+            for j, _ in nb:
+                if j not in cur:
+                    cur.add(j)
+                    nxt.append(j)
+        frontier = nxt
+    return np.array(sorted(cur), np.int64)

@@ -13,6 +13,7 @@ import pytest
 
 from jams_amd import configs
 from jams_amd.batch import ReplicaBatch
+from jams_amd import engine as E
 from jams_amd.engine import Engine
 from jams_amd.host import Simulation
 
@@ -132,3 +133,96 @@ def test_block_sizes_give_the_same_bits(engine_factory, bs):
         e.set_thermostat(300.0, 3)
         e.step(20)
     assert np.array_equal(e1.get_spins(), e2.get_spins())
+
+
+# ---------------------------------------------------------------------------
+# The 256-replica workload against the oracle (solvers/cpu_llg_heun.cc:45-148
+# per replica; the batch itself has no reference counterpart)
+
+def replica_oracle(b, r, desc):
+    """The oracle for replica r of a batch of configs.c4(): its lattice and
+    nearest-neighbour rows restated from the description, its sites, Zeeman
+    field mus B (after the exchange, as the batch orders the terms) and its
+    initial spins."""
+    from oracle import pyoracle as po
+    from tests import systems as S
+    a, z = b.offsets[r], b.offsets[r + 1]
+    kind, size = desc.split()[0], int(desc.split()[1].split("^")[0])
+    if kind == "sc":
+        lat = po.LatticeSpec(S.SC_CELL, [[0, 0, 0]], [0], (size,) * 3, (1, 1, 1))
+        rows = S.sc_nn_rows(1.0)
+    else:
+        lat = S.bcc_lattice(size)
+        rows = [(0, 0, [sx * 0.5, sy * 0.5, sz * 0.5], 1.0)
+                for sx in (1, -1) for sy in (1, -1) for sz in (1, -1)]
+    assert lat.n == z - a
+    o = po.OracleSystem(b.mus[a:z], b.gyro[a:z], b.alpha[a:z], nthreads=8)
+    o.add_exchange(*po.exchange_csr(lat.n, *po.neighbour_list(lat, rows)))
+    o.add_zeeman(b.zeeman_dc[a:z])
+    sigma = po.thermostat_sigma(b.mus[a:z], b.gyro[a:z], b.alpha[a:z], b.dt)
+    T, seed = float(b.temperatures[r]), int(b.seeds[r])
+    noise = (lambda k: po.scale_noise(po.normals(seed, k, lat.n), sigma, T)) if T > 0 else (lambda k: None)
+    return o, b.s0[a:z].copy(), noise
+
+
+def chosen_replicas(reps):
+    sizes = [(int(d.split()[1].split("^")[0]), r) for r, (*_, d) in enumerate(reps)]
+    sc = max((s, r) for (s, r), (*_, d) in zip(sizes, reps) if d.startswith("sc"))[1]
+    bcc = max((s, r) for (s, r), (*_, d) in zip(sizes, reps) if d.startswith("bcc"))[1]
+    return sorted({0, len(reps) - 1, sc, bcc, 97})
+
+
+def test_c4_workload_replicas_match_oracle_bitwise():
+    """configs.c4(): 256 seeded replicas (sc L^3 and bcc (L/2)^3 x 2, L in
+    8..48, T 0-1000 K, B 0-2 T), 6,356,301 spins in one context, 100 FP64 Heun
+    steps with the in-kernel stream.  The coldest, the hottest, the largest sc
+    and bcc replicas and one in the middle equal or_heun_step on their own
+    Philox streams (seed, local index), bit for bit."""
+    reps = configs.c4()
+    b = ReplicaBatch(reps)
+    assert b.nrep == 256 and b.n == 6356301
+    b.engine.step(100)
+    got = b.spins()
+    for r in chosen_replicas(reps):
+        o, s, noise = replica_oracle(b, r, reps[r][4])
+        for k in range(100):
+            o.heun_step(s, b.dt, k * b.dt, noise(k))
+        g = got[b.offsets[r]:b.offsets[r + 1]]
+        print(f"replica {r} ({reps[r][4]}, T = {reps[r][1]:.1f} K, B = {reps[r][3][2]:.3f} T, "
+              f"first spin {b.offsets[r]}): max |ds| = {np.abs(g - s).max():.3e}")
+        assert np.array_equal(g, s), r
+    b.close()
+
+
+def test_c4_workload_fp32_field_against_fp64():
+    """The FP32-field batch of the same workload after 100 steps on the same
+    streams: every replica within 1e-6 rad of FP64 per spin; the exchange
+    field of the chosen replicas within 2^-22 sum_j |J_ij| |s_j| of the
+    oracle's FP64 field."""
+    from oracle import pyoracle as po
+    reps = configs.c4()
+    finals, fields = {}, {}
+    for prec in (E.PREC_FP64, E.PREC_FP32_FIELD):
+        b = ReplicaBatch(reps, precision=prec)
+        fields[prec] = b.engine.term_field(0)
+        b.engine.step(100)
+        finals[prec] = b.spins()
+        offsets = b.offsets
+        if prec == E.PREC_FP64:
+            keep = b
+        else:
+            b.close()
+    a64, a32 = finals[E.PREC_FP64], finals[E.PREC_FP32_FIELD]
+    cos = np.einsum("ij,ij->i", a64, a32)
+    ang = np.arctan2(np.linalg.norm(np.cross(a64, a32), axis=1), cos)
+    worst = max(ang[offsets[r]:offsets[r + 1]].max() for r in range(len(reps)))
+    print(f"FP32-field vs FP64 after 100 steps: max angle {worst:.3e} rad")
+    assert worst < 1e-6
+    for r in chosen_replicas(reps):
+        o, s, _ = replica_oracle(keep, r, reps[r][4])
+        sl = slice(offsets[r], offsets[r + 1])
+        ref = o.term_field(0, s)
+        assert np.array_equal(fields[E.PREC_FP64][sl], ref), r
+        bound = po.csrmv(o.rowptr, o.col, np.abs(o.val), np.abs(s).reshape(-1)).reshape(-1, 3)
+        assert np.all(np.abs(fields[E.PREC_FP32_FIELD][sl] - ref) <= bound * 2.0**-22), r
+    keep.close()

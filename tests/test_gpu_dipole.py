@@ -219,6 +219,13 @@ def test_dipole_512_point_film(engine_factory):
     err64 = np.abs(h64 - ref).max() / scale
     e32 = dipole_engine(engine_factory, lat, mus, rc, prec=1)
     e32.set_spins(s)
+    # the fused kernels with the compile-time 512-point stages are the ones
+    # that ran: x forward + multiply + x inverse on 6 lines, z + y forward on
+    # 3 q2 lines, y + z inverse on 3 or 5
+    for e in (e64, e32):
+        d = e.describe()
+        assert "dip_x=fused512/6" in d and "dip_zyf=fused512/3" in d, d
+        assert "dip_zyi=fused512/3" in d or "dip_zyi=fused512/5" in d, d
     err32 = np.abs(e32.term_field(0) - h64).max() / np.abs(h64).max()
     print(f"256x256x8 film: max|h| {scale:.3e} meV, FP64 rel err {err64:.2e}, "
           f"FP32 transform rel err {err32:.2e}")
@@ -245,3 +252,38 @@ def test_heun_with_exchange_and_dipole_trajectory(engine_factory):
     diff = np.abs(e.get_spins() - s).max()
     print(f"exchange + dipole, 200 steps: max |ds| = {diff:.2e}")
     assert diff <= 1e-9
+
+
+def test_c5_all_pairs_field_at_sampled_sites():
+    """The dipolar film workload as configured (256 x 256 x 8 open,
+    mu_s = 2.2 muB, a = 0.3 nm, r_cutoff the supercell's longest diagonal, so
+    every pair interacts) through the host layer: the FFT dipole field at
+    256 sampled sites against the brute-force sum over all 524,287 partners
+    (dipole_bruteforce.cc:75-120), <= 1e-10 of max|h| in FP64; the FP32
+    transforms within 1e-5."""
+    from jams_amd import configs
+    from jams_amd.engine import PREC_FP32_FIELD
+    from jams_amd.host import Simulation
+    sim = Simulation(configs.c5(), device=0, output_dir="/tmp")
+    e = sim.engine()
+    mus = sim.sites()[0]
+    dip = [t for t in range(e.num_terms()) if e.term_kind(t) == 3][0]
+    lat = po.LatticeSpec(SC, [[0, 0, 0]], [0], (256, 256, 8), (0, 0, 0))
+    s = rnd_spins(lat.n, 29)
+    rng = np.random.default_rng(6)
+    sites = np.unique(np.concatenate([[0, lat.n - 1, 255 * 8, 128 * 256 * 8 + 128 * 8 + 4],
+                                      rng.integers(0, lat.n, 252)])).astype(np.int64)
+    rc = float(np.sqrt(256 * 256 * 2 + 8 * 8))
+    ref = po.dipole_bruteforce_sites(lat, mus, s, rc, A, sites, nthreads=16)
+    e.set_spins(s)
+    h64 = e.term_field(dip)
+    scale = np.abs(h64).max()
+    err64 = np.abs(h64[sites] - ref).max() / scale
+    e.set_precision(PREC_FP32_FIELD)
+    e.set_spins(s)
+    err32 = np.abs(e.term_field(dip)[sites] - ref).max() / scale
+    print(f"C5 all pairs, {sites.size} sites: max|h| {scale:.3e} meV, FP64 rel err {err64:.2e}, "
+          f"FP32 rel err {err32:.2e}")
+    sim.close()
+    assert err64 <= 1e-10
+    assert err32 <= 1e-5

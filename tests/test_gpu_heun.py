@@ -139,3 +139,52 @@ def test_ragged_and_tiny_systems(engine_factory, n_cells):
     e = b.engine(engine_factory)
     e.step(30)
     assert np.array_equal(e.get_spins(), run_oracle(b, 30))
+
+
+@pytest.mark.parametrize("fmt", [0, 1, 2])
+def test_vacancies_zero_length_spins_bitwise(engine_factory, fmt):
+    """Zero-length spins (vacancies), for which the reference switches to its
+    zero-safe kernels (solvers/cuda_llg_heun.cu:42-52,
+    cuda_llg_heun_kernel.cuh:116-211): bcc 8^3 x 2, two shells + K1, 10 % of
+    the sites empty, T = 300 K with the generated stream, 200 steps.  The
+    trajectory equals the oracle's bit for bit, the empty sites stay exactly
+    zero (unit_vector leaves |v| <= epsilon alone, containers/vec3.h:276-283)
+    and every other spin keeps unit length."""
+    b = S.c2(8)
+    rng = np.random.default_rng(8)
+    empty = rng.choice(b.n, b.n // 10, replace=False)
+    b.s0 = po.random_spins(8, b.n)
+    b.s0[empty] = 0.0
+    e = b.engine(engine_factory, fmt=fmt)
+    e.set_thermostat(b.temperature, 55)
+    e.step(200)
+    got = e.get_spins()
+    sigma = po.thermostat_sigma(b.mus, b.gyro, b.alpha, b.dt)
+    noise = lambda k: po.scale_noise(po.normals(55, k, b.n), sigma, b.temperature)
+    ref = run_oracle(b, 200, noise)
+    assert np.array_equal(got, ref)
+    assert np.all(got[empty] == 0.0)
+    full = np.setdiff1d(np.arange(b.n), empty)
+    assert np.abs(np.linalg.norm(got[full], axis=1) - 1.0).max() < 1e-15
+
+
+def test_vacancies_fp32_field_form(engine_factory):
+    """The FP32-field form on the same vacancy lattice: empty sites stay
+    exactly zero and the others stay within 1e-6 rad of FP64 after 200
+    steps on the same stream."""
+    b = S.c2(8)
+    rng = np.random.default_rng(8)
+    empty = rng.choice(b.n, b.n // 10, replace=False)
+    b.s0 = po.random_spins(8, b.n)
+    b.s0[empty] = 0.0
+    out = []
+    for prec in (0, 1):
+        e = b.engine(engine_factory, prec=prec)
+        e.set_thermostat(b.temperature, 55)
+        e.step(200)
+        out.append(e.get_spins())
+    assert np.all(out[1][empty] == 0.0)
+    full = np.setdiff1d(np.arange(b.n), empty)
+    a, c = out[0][full], out[1][full]
+    ang = np.arctan2(np.linalg.norm(np.cross(a, c), axis=1), np.einsum("ij,ij->i", a, c))
+    assert ang.max() < 1e-6

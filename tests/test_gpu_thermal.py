@@ -94,3 +94,38 @@ def test_mean_m_matches_oracle_on_independent_stream(engine_factory):
     print(f"<|m|> GPU {mg:.5f} +- {eg:.1e}, oracle {mc:.5f} +- {ec:.1e}")
     assert 0.3 < mg < 0.99
     assert abs(mg - mc) <= 3.0 * math.sqrt(eg * eg + ec * ec)
+
+
+@pytest.mark.parametrize("T", [4.0, 8.0, 24.0, 40.0])
+def test_mean_m_curve_across_tc(engine_factory, T):
+    """<|m|>(T) of the sc 8^3 ferromagnet (J = 1 meV, classical T_c about
+    16.7 K) at two temperatures below T_c and two above: the GPU's time
+    average on one stream (seed 101) and the oracle's on another (seed 202),
+    |m| every 10 steps over steps 2000-22000, agree within 4 combined
+    standard errors from an 8-block analysis (the critical region itself,
+    whose correlation times outlast such a run, is left out)."""
+    lat = po.LatticeSpec(S.SC_CELL, [[0, 0, 0]], [0], (8, 8, 8), (1, 1, 1))
+    b = S.build(lat, S.sc_nn_rows(1.0), 2.0, 0.5, ["exchange"], dt=S.dt_ps(1.0e-15), spins="+z",
+                temperature=T)
+    sigma = po.thermostat_sigma(b.mus, b.gyro, b.alpha, b.dt)
+    e = b.engine(engine_factory)
+    e.set_thermostat(T, 101)
+    e.step(2000)
+    g = []
+    for _ in range(2000):
+        e.step(10)
+        m = e.magnetisation()
+        g.append(np.linalg.norm(m[:3]) / m[3])
+    o = b.oracle(nthreads=1)
+    s = b.s0.copy()
+    c = []
+    for k in range(22000):
+        o.heun_step(s, b.dt, k * b.dt, po.scale_noise(po.normals(202, k, b.n), sigma, T))
+        if k >= 2000 and (k + 1) % 10 == 0:
+            c.append(np.linalg.norm((b.mus[:, None] * s).sum(axis=0)) / b.mus.sum())
+    mg, eg = block_mean_err(g, 8)
+    mc, ec = block_mean_err(c, 8)
+    print(f"T = {T} K: <|m|> GPU {mg:.5f} +- {eg:.1e}, oracle {mc:.5f} +- {ec:.1e}")
+    assert abs(mg - mc) <= 4.0 * math.sqrt(eg * eg + ec * ec)
+    lo, hi = {4.0: (0.85, 0.95), 8.0: (0.70, 0.82), 24.0: (0.03, 0.15), 40.0: (0.02, 0.10)}[T]
+    assert lo < mg < hi

@@ -448,3 +448,29 @@ def test_exchange_functional_symmetry_checks_and_errors():
     with pytest.raises(RuntimeError, match="distance units"):
         Simulation(distance_config("bcc_ab", (3, 3, 3), (1, 1, 1),
                                    ham('distance_units = "A";', inter='("A", "A", "step", 0.9, [1.0, 1.0])'), mats))
+
+
+# The CSR byte-equality at the workloads' full sizes
+# (containers/sparse_matrix_builder.h:187-248): the whole bcc 64^3 x 2
+# two-shell system (22,020,096 non-zeros), the 256 x 256 x 8 open film
+# (9,019,392) and the five-shell DMI template at bcc 16^3 x 2 (4,276,224).
+FULL_SIZE = {
+    "c2_bcc64": (lambda: configs.c2(64), lambda: S.bcc_lattice(64),
+                 lambda: S.bcc_two_shell_rows(3.0, 1.5), 22020096),
+    "c5_film": (lambda: configs.sc_config(L=(256, 256, 8), periodic=(False, False, False), J=2.0,
+                                          field=None).replace(
+                    "symops = false; interactions",
+                    "symops = false; check_identical_motif_neighbour_count = false; interactions"),
+                lambda: po.LatticeSpec(np.eye(3), [[0, 0, 0]], [0], (256, 256, 8), (0, 0, 0)),
+                lambda: S.sc_nn_rows(2.0), 9019392),
+    "c3_bcc16": (lambda: configs.c3(16), lambda: S.bcc_lattice(16),
+                 lambda: S.bcc_five_shell_dmi_rows(), 4276224),
+}
+
+
+@pytest.mark.parametrize("name", list(FULL_SIZE))
+def test_full_size_csr_matches_oracle_bytewise(name):
+    text, lat, rows, nnz = (f() if callable(f) else f for f in FULL_SIZE[name])
+    sim = Simulation(text)
+    assert assert_same_exchange(sim, lat, rows) == nnz
+    sim.close()

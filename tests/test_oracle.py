@@ -321,6 +321,17 @@ def test_dipole_fft_direct_two_sided_equals_bruteforce(name, lat, rc):
     assert np.abs(h_d - h_b).max() <= 1e-12 * np.abs(h_b).max()
 
 
+@pytest.mark.parametrize("name,lat,rc", DIRECT_CASES, ids=[c[0] for c in DIRECT_CASES])
+def test_dipole_bruteforce_sites_equals_full_sum(name, lat, rc):
+    """The sampled-site form is the brute-force sum at those sites, bit for
+    bit (the full sum is pinned by the reference's known answers)."""
+    mus = np.full(lat.n, 2.0 * S.BOHR)
+    s = po.random_spins(4, lat.n)
+    full = po.dipole_bruteforce(lat, mus, s, rc, 0.3e-9, nthreads=8)
+    sites = np.array([0, lat.n - 1, lat.n // 2, 3], np.int64)
+    assert np.array_equal(po.dipole_bruteforce_sites(lat, mus, s, rc, 0.3e-9, sites, 4), full[sites])
+
+
 def test_dipole_fft_direct_one_sided_as_written_differs_on_open_dimensions():
     """dipole_fft.cc:227-251 as written fills only n >= 0 on an open
     dimension: its field is not the dipole field there (not copied)."""
@@ -441,3 +452,93 @@ def test_dipole_stencil_matches_direct_sum():
     assert np.abs(stencil_field(lat.size, dropped, s) - ref).max() > 1e-3 * scale
     wrapped = np.roll(s.reshape(12, 10, 4, 3), 1, axis=2).reshape(-1, 3)
     assert np.abs(stencil_field(lat.size, K, wrapped) - ref).max() > 1e-1 * scale
+
+
+# ---- uniaxial anisotropy: field against the energy it derives from -----------
+def _uniaxial_system(n, power, seed):
+    rng = np.random.default_rng(seed)
+    mus = np.full(n, 2.0 * S.BOHR)
+    o = po.OracleSystem(mus, np.full(n, S.gyro_e()), np.full(n, 0.1))
+    K = rng.uniform(0.02, 0.5, n)
+    axis = rng.normal(size=(n, 3))
+    axis /= np.linalg.norm(axis, axis=1)[:, None]
+    o.add_uniaxial(K, axis, power)
+    return o, K, axis
+
+
+def _energy_gradient_field(o, s, eps=1e-6):
+    """-dE/ds by central differences of or_term_energy, per spin (each site's
+    energy depends on its own spin only)."""
+    h = np.empty_like(s)
+    for c in range(3):
+        sp, sm = s.copy(), s.copy()
+        sp[:, c] += eps
+        sm[:, c] -= eps
+        # per-site energies: the term energy of one site at a time is the sum
+        # over a system whose other spins are perpendicular to their axes
+        # (zero energy for even powers), so sum differences site by site
+        ep = np.array([o.term_energy(0, _only(sp, i, o)) for i in range(s.shape[0])])
+        em = np.array([o.term_energy(0, _only(sm, i, o)) for i in range(s.shape[0])])
+        h[:, c] = -(ep - em) / (2 * eps)
+    return h
+
+
+def _only(s, i, o):
+    """s with every spin but i turned perpendicular to its axis."""
+    out = np.cross(o.ani_axis, np.array([0.0, 0.0, 1.0]))
+    bad = np.linalg.norm(out, axis=1) < 1e-6
+    out[bad] = np.cross(o.ani_axis[bad], np.array([1.0, 0.0, 0.0]))
+    out[i] = s[i]
+    return out
+
+
+def _field_matches_gradient(h, g):
+    return bool(np.all(np.abs(h - g) <= 1e-7 * (1.0 + np.abs(g))))
+
+
+@pytest.mark.parametrize("power", [2, 4, 6])
+def test_uniaxial_field_is_minus_energy_gradient(power):
+    """uniaxial_anisotropy.cc:116-123 (E = -K (s.e)^p) and :155-163 (the field
+    K p (s.e)^(p-1) e): the oracle's field equals -dE/ds of its own energy by
+    central differences at random spins on tilted axes.  The same check
+    rejects the plausible slips: the factor p dropped, the exponent p instead
+    of p - 1, the sign flipped."""
+    o, K, axis = _uniaxial_system(12, power, 40 + power)
+    s = po.random_spins(41, 12)
+    h = o.term_field(0, s)
+    g = _energy_gradient_field(o, s)
+    assert _field_matches_gradient(h, g)
+    d = np.einsum("ij,ij->i", s, axis)
+    wrong = {"no factor p": (K * d ** (power - 1))[:, None] * axis,
+             "exponent p": (K * power * d ** power)[:, None] * axis,
+             "sign": -h}
+    for name, hw in wrong.items():
+        assert not _field_matches_gradient(hw, g), name
+
+
+@pytest.mark.parametrize("power", [2, 4, 6])
+def test_uniaxial_precession_closed_form(power):
+    """One spin, easy axis z, no damping, T = 0: s precesses about z at
+    omega = gamma p K cos^(p-1)(theta) / mus with theta fixed (the field is
+    along z with magnitude p K cos^(p-1) theta in meV, / mus in tesla)."""
+    g = S.gyro_e()
+    K = 0.8 if power == 2 else 2.0
+    mu = 2.0 * S.BOHR
+    o = po.OracleSystem(np.array([mu]), np.array([g]), np.array([0.0]))
+    o.add_uniaxial(np.array([K]), np.array([[0.0, 0.0, 1.0]]), power)
+    th0, ph0 = 0.6, 0.2
+    s = np.array([[math.sin(th0) * math.cos(ph0), math.sin(th0) * math.sin(ph0), math.cos(th0)]])
+    omega = g * power * K * math.cos(th0) ** (power - 1) / mu
+    dt = 2e-4
+    nsteps = int(round(2.5 / (omega * dt)))
+    for k in range(nsteps):
+        o.heun_step(s, dt, k * dt)
+    ph = ph0 + omega * nsteps * dt
+    ref = np.array([math.sin(th0) * math.cos(ph), math.sin(th0) * math.sin(ph), math.cos(th0)])
+    err = np.abs(s[0] - ref).max()
+    assert err < 1e-6, err
+    # a field short of the factor p would precess p times slower
+    ph_wrong = ph0 + omega / power * nsteps * dt
+    wrong = np.array([math.sin(th0) * math.cos(ph_wrong), math.sin(th0) * math.sin(ph_wrong),
+                      math.cos(th0)])
+    assert np.abs(s[0] - wrong).max() > 1e-2
