@@ -180,6 +180,18 @@ int jh_term_energy(jh_ctx *ctx, int term_id, double t, double *energy);
 /* sum_i mus_i s_i (3) and sum_i mus_i (1): the magnetisation monitor's
  * reductions (helpers/spinops.cc:42-51, monitors/magnetisation.cc:77-104). */
 int jh_reduce_magnetisation(jh_ctx *ctx, double out[4]);
+/* jh_reduce_magnetisation without any device allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate (tight serving or
+ * monitoring loops, harnesses that forbid hipMalloc).  The same reduction
+ * kernels and result, with their partial sums in a static device array of the
+ * library (2 MiB + 32 bytes per GPU, kept until the process ends) instead of
+ * the context's buffers; at most 16,777,216 spins (JH_ERR_INVALID beyond).
+ * The context must be laid out already (one jh_step, jh_compute_fields or
+ * jh_reduce_magnetisation since the last set-up change; JH_ERR_STATE
+ * otherwise), so that nothing is allocated.  All calls on one GPU share the
+ * array: calls that overlap on one GPU (contexts on several host threads or
+ * streams) race. */
+int jh_reduce_magnetisation_noalloc(jh_ctx *ctx, double out[4]);
 
 /* ---- batching form: independent replicas in one context ------------------
  * No counterpart in the reference, which runs one system per process (the

@@ -15,6 +15,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <unordered_map>
@@ -28,7 +29,43 @@
 
 using namespace jh;
 
+// ---------------------------------------------------------------------------
+// Device scratch of jh_reduce_magnetisation_noalloc: the per-workgroup partial
+// sums of k_reduce (4 doubles per kBlock spins) followed by the 4-double
+// result, for at most kNoAllocMaxSpins spins: 2 MiB + 32 bytes that every GPU
+// the library runs on holds until the process ends.  k_reduce writes every
+// partial that k_reduce_final reads, so nothing in it has to start at zero.
+// In host code `scratch` is only a symbol: its device address comes from
+// hipGetSymbolAddress, once per device (bind_scratch, called by jh_create),
+// and that address is what kernels and copies are given.
+constexpr int64_t kNoAllocMaxSpins = (int64_t)1 << 24;
+constexpr int64_t kNoAllocBlocks = (kNoAllocMaxSpins + kBlock - 1) / kBlock;
+static __device__ __align__(256) double scratch[4 * kNoAllocBlocks + 4];
+
 namespace {
+
+std::mutex g_scratch_mu;
+std::unordered_map<int, double *> g_scratch_addr;  // device -> address of scratch
+
+// Looks up the device address of `scratch` on `device`, which must be the
+// current device; once per device.
+hipError_t bind_scratch(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_addr.count(device)) return hipSuccess;
+  double *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(reinterpret_cast<void **>(&ptr), HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  if (!ptr) return hipErrorInvalidSymbol;
+  g_scratch_addr[device] = ptr;
+  return hipSuccess;
+}
+
+double *scratch_on(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  const auto it = g_scratch_addr.find(device);
+  return it == g_scratch_addr.end() ? nullptr : it->second;
+}
+
 
 thread_local std::string g_create_error;
 
@@ -1079,6 +1116,11 @@ int jh_create(int device, jh_ctx **out) {
     g_create_error = std::string("hipSetDevice: ") + hipGetErrorString(e);
     return JH_ERR_HIP;
   }
+  e = bind_scratch(device);
+  if (e != hipSuccess) {
+    g_create_error = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+    return JH_ERR_HIP;
+  }
   jh_ctx *c = new (std::nothrow) jh_ctx();
   if (!c) return JH_ERR_NOMEM;
   c->device = device;
@@ -1404,6 +1446,33 @@ int jh_reduce_magnetisation(jh_ctx *c, double out[4]) {
     KParams p = make_params(c);
     HIPCHK(launch_reduce(p, 0, nullptr, c->red_partial.p, c->red_out.p, c->stream));
     HIPCHK(hipMemcpyAsync(out, c->red_out.p, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int jh_reduce_magnetisation_noalloc(jh_ctx *c, double out[4]) {
+  return guarded(c, [&] {
+    require_sites(c);
+    if (!out) JH_THROW(JH_ERR_INVALID, "jh_reduce_magnetisation_noalloc: out is null");
+    if (c->n > kNoAllocMaxSpins)
+      JH_THROW(JH_ERR_INVALID, "jh_reduce_magnetisation_noalloc: " + std::to_string(c->n) +
+                                   " spins exceed its scratch limit of " +
+                                   std::to_string(kNoAllocMaxSpins) +
+                                   "; use jh_reduce_magnetisation");
+    if (!c->spins_set) JH_THROW(JH_ERR_STATE, "spins have not been set (jh_set_spins)");
+    if (c->dirty)
+      JH_THROW(JH_ERR_STATE,
+               "jh_reduce_magnetisation_noalloc: the context has set-up changes that are not laid "
+               "out on the device yet (that allocates); call jh_step, jh_compute_fields or "
+               "jh_reduce_magnetisation once first");
+    double *scr = scratch_on(c->device);
+    if (!scr) JH_THROW(JH_ERR_STATE, "jh_reduce_magnetisation_noalloc: no scratch address for this device");
+    HIPCHK(hipSetDevice(c->device));
+    KParams p = make_params(c);
+    const int64_t nb = (c->n + kBlock - 1) / kBlock;
+    double *result = scr + 4 * nb;
+    HIPCHK(launch_reduce(p, 0, nullptr, scr, result, c->stream));
+    HIPCHK(hipMemcpyAsync(out, result, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
   });
 }

@@ -57,6 +57,7 @@ ABI = {
     "jh_compute_fields": (C.c_int, [C.c_void_p, C.c_double, C.c_void_p]),
     "jh_term_energy": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.POINTER(C.c_double)]),
     "jh_reduce_magnetisation": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "jh_reduce_magnetisation_noalloc": (C.c_int, [C.c_void_p, C.c_void_p]),
     "jh_time_steps": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_double)]),
     "jh_time_phase": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     "jh_phase_bytes": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double),
