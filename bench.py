@@ -1,23 +1,28 @@
 """Benchmark of the gfx950 Heun-LLG step (one JSON line on stdout).
 
-Workload (default): bcc 64^3 x 2 = 524,288 spins, exchange to the second
-shell (8 x 3.0 meV + 6 x 1.5 meV), uniaxial K1 = 0.05 meV, alpha = 0.05,
-dt = 0.5 fs, Langevin thermostat at 300 K from the Philox stream, FP64.
-It is set up through the JAMS-compatible host layer from config text, the
-way a JAMS user selects the solver, and runs on one GPU.
+Workload (headline): bcc 128^3 x 2 = 4,194,304 spins, full 3x3 exchange
+tensors to the fifth shell (58 neighbours, J_n = 3.0 exp(-r/a) meV plus a DMI
+part 0.1 J_n along seeded unit vectors), alpha = 0.05, dt = 0.5 fs, Langevin
+thermostat at 100 K from the Philox stream, random initial spins from seed 3,
+FP64; the same system in the FP32-field form beside it.  It is set up through
+the JAMS-compatible host layer from config text, the way a JAMS user selects
+the solver, and runs on one GPU.
 
 A "step" is one full Heun step (predictor + corrector, both field
 evaluations, in-kernel noise).  value = spins x steps / event time of the
 step loop, inputs resident in HBM.
 
-The precision section runs the 4,194,304-spin system (bcc 128^3 x 2, full
-3x3 tensors to the fifth shell with DMI, T = 100 K) for --c3-steps steps in
-FP64 and again, from the same initial state and noise stream, in the
-FP32-field form, and states the speed-up and the per-spin angle and |dm|
-differences between the two final states.
+The precision section runs the same system for --accuracy-steps steps in
+FP64, in the FP32-field form and in FP64 from a state rotated by ~1e-12 rad,
+from the ordered (+z) and from the random start, and states the per-spin
+angle and |dm| differences and the <|m|> and exchange-energy statistics over
+the last 10^3 steps.  Further legs: the bcc 64^3 x 2 two-shell system (C2),
+RK4, the 256-replica batch, the dipolar film, and the CPU baseline (the
+oracle's restatement of llg-heun-cpu at 1 thread and at every host thread).
 
-  python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-fp32] [--no-rk4]
-                  [--no-c3] [--c3-steps S] [--reps R]
+  python bench.py [--steps K] [--warmup W] [--gpus 1] [--headline-only]
+                  [--no-cpu] [--no-c2] [--no-rk4] [--no-c4] [--no-c5]
+                  [--accuracy-steps S] [--reps R]
   python bench.py --copy-calibration   # a plain 1 GiB copy kernel, for
                                        # scaling rocprofv3 FETCH/WRITE_SIZE
 """
@@ -26,6 +31,8 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import platform
+import subprocess
 import sys
 import time
 from pathlib import Path
@@ -136,16 +143,29 @@ def measure_rk4(L: int, steps: int, warmup: int):
     return out
 
 
-def cpu_baseline(L: int, budget_s: float = 15.0):
-    """The oracle (C restatement of llg-heun-cpu) on the same C2 system, timed
-    on this host's cores over a bounded number of steps."""
-    import numpy as np
+def _cpu_model() -> str:
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return platform.processor() or "unknown"
 
+
+def _nproc() -> int:
+    try:
+        return int(subprocess.run(["nproc"], capture_output=True, text=True, timeout=10).stdout)
+    except (OSError, subprocess.SubprocessError, ValueError):
+        return os.cpu_count() or 1
+
+
+def _time_oracle(b, nthreads: int, budget_s: float, max_steps: int = 10000):
+    """Heun steps of the oracle on system b at `nthreads` threads, with the
+    noise drawn serially on the host each step, for about budget_s seconds."""
     from oracle import pyoracle as po
-    from tests import systems as S
 
-    nthreads = max(1, min(16, os.cpu_count() or 1))
-    b = S.c2(L)
     o = b.oracle(nthreads=nthreads)
     sigma = po.thermostat_sigma(b.mus, b.gyro, b.alpha, b.dt)
     s = b.s0.copy()
@@ -156,11 +176,37 @@ def cpu_baseline(L: int, budget_s: float = 15.0):
         o.heun_step(s, b.dt, k * b.dt, w)
         k += 1
         el = time.perf_counter() - t0
-        if el > budget_s or k >= 10000:
+        if el > budget_s or k >= max_steps:
             break
-    return {"value": b.n * k / el, "unit": "spin-updates/s", "cores": nthreads, "kind": "port",
-            "sample": f"bcc {L}^3 x 2 ({b.n} spins), {k} Heun steps at 300 K with serial "
-                      f"noise generation (as solvers/cpu_llg_heun.cc:55), {el:.1f} s"}
+    return {"value": b.n * k / el, "steps": k, "seconds": el}
+
+
+def cpu_baseline(L3: int, L2: int, budget_s: float = 7.0):
+    """The oracle (C restatement of llg-heun-cpu, solvers/cpu_llg_heun.cc:15-148)
+    timed on this host's cores: the headline system scaled to bcc L3^3 x 2
+    (its full 2.19e9-entry CSR does not fit a CPU run) and the C2 system, each
+    at 1 thread (the reference's default build, JAMS_BUILD_OMP=OFF) and at
+    every thread nproc reports (its OpenMP build), noise drawn serially as in
+    cpu_llg_heun.cc:55."""
+    from tests import systems as S
+
+    nproc = _nproc()
+    model = _cpu_model()
+    runs = {}
+    for name, b in (("c3_scaled", S.c3(L3, with_csr=True)), ("c2", S.c2(L2))):
+        for nt in (1, nproc):
+            r = _time_oracle(b, nt, budget_s)
+            runs[f"{name}_{nt}_threads"] = {"spins": b.n, **r}
+            log(f"  CPU {name} ({b.n} spins) at {nt} threads: {r['value']:.4e} spin-updates/s "
+                f"({r['steps']} steps in {r['seconds']:.1f} s)")
+    head = runs[f"c3_scaled_{nproc}_threads"]
+    return {"value": head["value"], "unit": "spin-updates/s", "cores": nproc, "kind": "port",
+            "sample": f"the headline system scaled to bcc {L3}^3 x 2 ({head['spins']} spins, "
+                      f"same 58-neighbour DMI tensors, T = 100 K): {head['steps']} Heun steps "
+                      f"in {head['seconds']:.1f} s at {nproc} OpenMP threads, noise drawn "
+                      f"serially (solvers/cpu_llg_heun.cc:55)",
+            "nproc": nproc, "cpu_model": model,
+            "one_thread_value": runs["c3_scaled_1_threads"]["value"], "runs": runs}
 
 
 def _angles(a, b):
@@ -170,14 +216,57 @@ def _angles(a, b):
     return np.arctan2(np.linalg.norm(np.cross(a, b), axis=1), cosang)
 
 
-def measure_precision(L: int, steps: int, reps: int):
-    """The 4.2M-spin DMI system (T = 100 K) run for `steps` steps in FP64 and
-    in the FP32-field form from the same state with the same noise stream,
-    from two initial states: ordered (+z) and random (seed 3).  A third run,
-    FP64 from the initial state rotated by ~1e-12 rad per spin, measures how
-    far the dynamics itself carries a perturbation of that size: where the
-    FP32-vs-FP64 difference is of the same order, it is the trajectory's
-    sensitivity, not the precision form, that sets it."""
+def _phase_times(eng, reps):
+    phases = []
+    for ph in (0, 1):
+        t_ms = eng.time_phase(ph, reps)
+        fmt_b, csr_b = eng.phase_bytes(ph)
+        phases.append({"phase": ("predictor", "corrector")[ph], "ms": t_ms, "bytes": fmt_b,
+                       "gbs": fmt_b / (t_ms * 1e-3) / 1e9,
+                       "csr_equiv_gbs": csr_b / (t_ms * 1e-3) / 1e9})
+    return phases
+
+
+def _tail_run(eng, mus, steps, tail, every):
+    """steps Heun steps; over the last `tail` of them, every `every` steps,
+    |m| and the exchange energy per spin (meV)."""
+    import numpy as np
+
+    eng.step(steps - tail)
+    mabs, energy = [], []
+    for _ in range(tail // every):
+        eng.step(every)
+        m = eng.magnetisation()
+        mabs.append(float(np.linalg.norm(m[:3]) / m[3]))
+        t = eng.iteration()[1]
+        energy.append(eng.term_energy(0, t) / mus.size)
+    return np.asarray(mabs), np.asarray(energy)
+
+
+def _blocked_mean(x, nblocks=10):
+    """Mean and its standard error from nblocks contiguous blocks (the samples
+    of a trajectory are correlated)."""
+    import numpy as np
+
+    k = x.size // nblocks
+    blocks = x[:k * nblocks].reshape(nblocks, k).mean(1)
+    return float(x.mean()), float(blocks.std(ddof=1) / np.sqrt(nblocks))
+
+
+def measure_c3(L: int, steps: int, warmup: int, reps: int, accuracy_steps: int,
+               tail: int = 1000, every: int = 10):
+    """The headline system (bcc L^3 x 2, five shells with DMI, T = 100 K).
+
+    Speed: `steps` Heun steps after `warmup`, from the workload's own random
+    start, in FP64 and then in the FP32-field form, plus HIP-event times of
+    each phase kernel.
+
+    Accuracy (untimed, accuracy_steps > 0): from the ordered start (+z) and
+    from the random start, FP64, FP32-field and FP64 from the start rotated by
+    ~1e-12 rad per spin, all on the same noise stream: the per-spin angle and
+    |dm| of each against FP64 after accuracy_steps, and <|m|> and the exchange
+    energy over the last `tail` steps (a statistical comparison, which is the
+    meaningful one where the trajectory itself amplifies a 1e-12 difference)."""
     import numpy as np
 
     from jams_amd import configs
@@ -191,49 +280,70 @@ def measure_precision(L: int, steps: int, reps: int):
     setup_s = time.perf_counter() - t0
     s_random = eng.get_spins()
     mus = sim.sites()[0]
-    rng = np.random.default_rng(11)
-    out = {"n": n, "steps": steps, "setup_s": setup_s, "temperature_K": 100.0}
-    m_of = lambda s: (mus[:, None] * s).sum(0) / mus.sum()
-    for label, s0 in (("ordered", np.tile([0.0, 0.0, 1.0], (n, 1))), ("random", s_random)):
-        d = rng.normal(size=s0.shape) * 1e-12
-        s_pert = s0 + d - np.einsum("ij,ij->i", d, s0)[:, None] * s0
-        s_pert /= np.linalg.norm(s_pert, axis=1)[:, None]
-        finals, res = {}, {}
-        for name, mode, init in (("fp64", PREC_FP64, s0), ("fp32_field", PREC_FP32_FIELD, s0),
-                                 ("fp64_perturbed", PREC_FP64, s_pert)):
-            eng.set_precision(mode)
-            eng.set_spins(init)
-            eng.set_iteration(0)
-            eng.step(0)  # lays out the device formats outside the timed region
-            ms = eng.time_steps(steps)
-            finals[name] = eng.get_spins()
-            res[name] = {"value": n * steps / (ms * 1e-3), "ms_per_step": ms / steps}
-            if label == "ordered" and name != "fp64_perturbed":
-                phases = []
-                for ph in (0, 1):
-                    t_ms = eng.time_phase(ph, reps)
-                    fmt_b, csr_b = eng.phase_bytes(ph)
-                    phases.append({"phase": ("predictor", "corrector")[ph], "ms": t_ms,
-                                   "bytes": fmt_b, "gbs": fmt_b / (t_ms * 1e-3) / 1e9,
-                                   "csr_equiv_gbs": csr_b / (t_ms * 1e-3) / 1e9})
-                out[name] = {**res[name], "phases": phases, "engine": eng.describe()}
-            log(f"  C3 {label} {name}: {ms / steps:.4f} ms/step, "
-                f"{res[name]['value']:.4e} spin-updates/s")
-        m64 = m_of(finals["fp64"])
-        st = {}
-        for other in ("fp32_field", "fp64_perturbed"):
-            ang = _angles(finals["fp64"], finals[other])
-            st[other + "_vs_fp64"] = {
-                "angle_rad_max": float(ang.max()), "angle_rad_rms": float(np.sqrt((ang ** 2).mean())),
-                "dm_abs": float(np.linalg.norm(m64 - m_of(finals[other])))}
-        st["m_fp64"] = float(np.linalg.norm(m64))
-        out[label] = st
-        a, c = st["fp32_field_vs_fp64"], st["fp64_perturbed_vs_fp64"]
-        log(f"  C3 {label} start, after {steps} steps: FP32-field vs FP64 max angle "
-            f"{a['angle_rad_max']:.3e} rad (rms {a['angle_rad_rms']:.3e}), |dm| {a['dm_abs']:.3e}; "
-            f"FP64 1e-12-perturbed vs FP64 max angle {c['angle_rad_max']:.3e}, |dm| {c['dm_abs']:.3e}")
+    out = {"n": n, "setup_s": setup_s, "temperature_K": 100.0}
+    log(f"  set-up {setup_s:.1f} s")
+    for name, mode in (("fp64", PREC_FP64), ("fp32_field", PREC_FP32_FIELD)):
+        eng.set_precision(mode)
+        eng.set_spins(s_random)
+        eng.set_iteration(0)
+        eng.step(warmup)
+        eng.synchronize()
+        w0 = time.perf_counter()
+        ms = eng.time_steps(steps)
+        wall = time.perf_counter() - w0
+        out[name] = {"value": n * steps / (ms * 1e-3), "ms": ms, "ms_per_step": ms / steps,
+                     "wall_s": wall, "phases": _phase_times(eng, reps),
+                     "describe": eng.describe()}
+        log(f"  C3 {name}: {ms / steps:.4f} ms/step, {out[name]['value']:.4e} spin-updates/s; "
+            f"{eng.describe()}")
     out["speedup_fp32_vs_fp64"] = out["fp32_field"]["value"] / out["fp64"]["value"]
     log(f"  C3 speed-up FP32-field / FP64: {out['speedup_fp32_vs_fp64']:.3f}")
+    if accuracy_steps > 0:
+        tail = min(tail, accuracy_steps)
+        rng = np.random.default_rng(11)
+        m_of = lambda s: (mus[:, None] * s).sum(0) / mus.sum()
+        acc = {"steps": accuracy_steps, "tail_steps": tail, "sample_every": every}
+        for label, s0 in (("ordered", np.tile([0.0, 0.0, 1.0], (n, 1))), ("random", s_random)):
+            d = rng.normal(size=s0.shape) * 1e-12
+            s_pert = s0 + d - np.einsum("ij,ij->i", d, s0)[:, None] * s0
+            s_pert /= np.linalg.norm(s_pert, axis=1)[:, None]
+            finals, stats = {}, {}
+            for name, mode, init in (("fp64", PREC_FP64, s0), ("fp32_field", PREC_FP32_FIELD, s0),
+                                     ("fp64_perturbed", PREC_FP64, s_pert)):
+                eng.set_precision(mode)
+                eng.set_spins(init)
+                eng.set_iteration(0)
+                mabs, energy = _tail_run(eng, mus, accuracy_steps, tail, every)
+                finals[name] = eng.get_spins()
+                mm, me = _blocked_mean(mabs)
+                em, ee = _blocked_mean(energy)
+                stats[name] = {"mean_abs_m": mm, "mean_abs_m_stderr": me,
+                               "exchange_energy_meV_per_spin": em,
+                               "exchange_energy_stderr": ee}
+            m64 = m_of(finals["fp64"])
+            st = {"m_fp64": float(np.linalg.norm(m64)), "tail": stats}
+            for other in ("fp32_field", "fp64_perturbed"):
+                ang = _angles(finals["fp64"], finals[other])
+                a, b = stats[other], stats["fp64"]
+                st[other + "_vs_fp64"] = {
+                    "angle_rad_max": float(ang.max()),
+                    "angle_rad_rms": float(np.sqrt((ang ** 2).mean())),
+                    "dm_abs": float(np.linalg.norm(m64 - m_of(finals[other]))),
+                    "tail_mean_abs_m_diff": a["mean_abs_m"] - b["mean_abs_m"],
+                    "tail_mean_abs_m_diff_sigma": float(np.hypot(a["mean_abs_m_stderr"],
+                                                                 b["mean_abs_m_stderr"])),
+                    "tail_energy_diff": a["exchange_energy_meV_per_spin"]
+                    - b["exchange_energy_meV_per_spin"],
+                    "tail_energy_diff_sigma": float(np.hypot(a["exchange_energy_stderr"],
+                                                             b["exchange_energy_stderr"]))}
+            acc[label] = st
+            f, c = st["fp32_field_vs_fp64"], st["fp64_perturbed_vs_fp64"]
+            log(f"  C3 {label} start, after {accuracy_steps} steps: FP32-field vs FP64 max angle "
+                f"{f['angle_rad_max']:.3e} rad (rms {f['angle_rad_rms']:.3e}), |dm| {f['dm_abs']:.3e}, "
+                f"tail <|m|> diff {f['tail_mean_abs_m_diff']:.2e} +- {f['tail_mean_abs_m_diff_sigma']:.1e}, "
+                f"energy diff {f['tail_energy_diff']:.2e} +- {f['tail_energy_diff_sigma']:.1e} meV; "
+                f"FP64 1e-12-perturbed vs FP64 max angle {c['angle_rad_max']:.3e}, |dm| {c['dm_abs']:.3e}")
+        out["accuracy"] = acc
     sim.close()
     return out
 
@@ -258,6 +368,8 @@ def measure_dipole(steps: int, reps: int):
     s0 = eng.get_spins()
     mus = sim.sites()[0]
     dip = [t for t in range(eng.num_terms()) if eng.term_kind(t) == 3][0]
+    p_c = 512 * 512 * (16 // 2 + 1)
+    b_formula = 48.0 * n + 6 * p_c * 16 + 12 * p_c * 16
     out = {"n": n, "steps": steps, "setup_s": setup_s}
     finals = {}
     for name, mode in (("fp64", PREC_FP64), ("fp32_field", PREC_FP32_FIELD)):
@@ -272,7 +384,13 @@ def measure_dipole(steps: int, reps: int):
         b_dip = eng.term_bytes(dip)
         out[name] = {"value": n * steps / (ms * 1e-3), "ms_per_step": ms / steps,
                      "dipole_eval_ms": t_dip, "dipole_eval_bytes": b_dip,
-                     "dipole_eval_gbs": b_dip / (t_dip * 1e-3) / 1e9, "engine": eng.describe()}
+                     "dipole_eval_gbs": b_dip / (t_dip * 1e-3) / 1e9,
+                     # the scope table's fixed formula for one evaluation,
+                     # 48 N + 6 P_c 16 (tensor) + 12 P_c 16 (k-space spin and
+                     # field, written and read), P_c = 512 x 512 x (16/2 + 1)
+                     "dipole_eval_bytes_formula": b_formula,
+                     "dipole_eval_gbs_formula": b_formula / (t_dip * 1e-3) / 1e9,
+                     "engine": eng.describe()}
         log(f"  C5 {name}: {ms / steps:.4f} ms/step, {out[name]['value']:.4e} spin-updates/s; "
             f"dipole evaluation {t_dip * 1e3:.1f} us, {b_dip / 1e6:.0f} MB -> "
             f"{out[name]['dipole_eval_gbs']:.0f} GB/s")
@@ -375,21 +493,22 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--L", type=int, default=64)
+    ap.add_argument("--L3", type=int, default=128, help="headline lattice: bcc L3^3 x 2")
+    ap.add_argument("--L", type=int, default=64, help="C2 lattice: bcc L^3 x 2")
+    ap.add_argument("--reps", type=int, default=50, help="launches per phase timing")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the headline system's FP64 and FP32-field timings")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-c2", action="store_true")
     ap.add_argument("--no-rk4", action="store_true")
-    ap.add_argument("--no-c3", action="store_true")
-    ap.add_argument("--c3-steps", type=int, default=10000)
-    ap.add_argument("--c3-L", type=int, default=128)
-    ap.add_argument("--reps", type=int, default=200)
-    ap.add_argument("--no-c5", action="store_true")
     ap.add_argument("--no-c4", action="store_true")
+    ap.add_argument("--no-c5", action="store_true")
     ap.add_argument("--c4-steps", type=int, default=5000)
     ap.add_argument("--c5-steps", type=int, default=2000)
-    # C2: FP32-field and perturbed-FP64 runs of this many steps against FP64,
-    # untimed (0: none)
+    # FP32-field and perturbed-FP64 runs of this many steps against FP64,
+    # untimed, on the headline system and on C2 (0: none)
     ap.add_argument("--accuracy-steps", type=int, default=10000)
+    ap.add_argument("--cpu-L3", type=int, default=32)
     ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
     # the host layer logs set-up to stdout, as the reference does; keep this
@@ -407,88 +526,90 @@ def main() -> int:
 
     from jams_amd import configs
 
-    log(f"C2 FP64: bcc {args.L}^3 x 2, {args.steps} timed steps after {args.warmup} warm-up")
-    fp64 = measure(configs.c2(args.L), args.steps, args.warmup, args.reps)
-    log(f"  {fp64['value']:.4e} spin-updates/s, {fp64['ms'] / args.steps * 1e3:.2f} us/step")
-    fp32 = None
-    if not args.no_fp32:
-        log("C2 FP32-field")
-        fp32 = measure(configs.c2(args.L, precision="fp32-field"), args.steps, args.warmup,
-                       args.reps)
-        log(f"  {fp32['value']:.4e} spin-updates/s")
+    extras = args.headline_only
+    log(f"C3 (headline): bcc {args.L3}^3 x 2, {args.steps} timed steps after {args.warmup} "
+        f"warm-up, FP64 and FP32-field")
+    c3 = measure_c3(args.L3, args.steps, args.warmup, args.reps,
+                    0 if extras else args.accuracy_steps)
+    c2 = None
+    if not (extras or args.no_c2):
+        log(f"C2: bcc {args.L}^3 x 2, FP64 and FP32-field")
+        c2 = {"fp64": measure(configs.c2(args.L), args.steps, args.warmup, args.reps),
+              "fp32_field": measure(configs.c2(args.L, precision="fp32-field"), args.steps,
+                                    args.warmup, args.reps)}
+        c2["speedup_fp32_vs_fp64"] = c2["fp32_field"]["value"] / c2["fp64"]["value"]
+        log(f"  C2 fp64 {c2['fp64']['value']:.4e}, fp32-field {c2['fp32_field']['value']:.4e} "
+            f"spin-updates/s")
         if args.accuracy_steps > 100:
-            fp32["vs_fp64"] = c2_accuracy(args.L, args.accuracy_steps)
-            for when, res in fp32["vs_fp64"].items():
+            c2["accuracy"] = c2_accuracy(args.L, args.accuracy_steps)
+            for when, res in c2["accuracy"].items():
                 for k, v in res.items():
                     log(f"  C2 {k} {when}: max angle {v['angle_rad_max']:.3e} rad "
                         f"(rms {v['angle_rad_rms']:.3e}), |dm| {v['dm_abs']:.3e}")
     rk4 = None
-    if not args.no_rk4:
+    if not (extras or args.no_rk4):
         log("C2 under llg-rk4-gpu (RK4), FP64 and FP32-field")
         rk4 = measure_rk4(args.L, min(args.steps, 1000), min(args.warmup, 50))
         log(f"  fp64 {rk4['fp64']['ms_per_step'] * 1e3:.2f} us/step "
             f"({rk4['fp64']['value']:.4e} spin-updates/s), fp32-field "
             f"{rk4['fp32_field']['ms_per_step'] * 1e3:.2f} us/step")
-    c3 = None
-    if not args.no_c3:
-        log(f"C3 precision form: bcc {args.c3_L}^3 x 2, {args.c3_steps} steps FP64 and FP32-field")
-        c3 = measure_precision(args.c3_L, args.c3_steps, min(args.reps, 50))
     c4 = None
-    if not args.no_c4:
+    if not (extras or args.no_c4):
         log(f"C4 batching form: 256 replicas, {args.c4_steps} steps FP64 and FP32-field")
         c4 = measure_batch(args.c4_steps)
     c5 = None
-    if not args.no_c5:
+    if not (extras or args.no_c5):
         log(f"C5 dipolar film: sc 256x256x8 open, {args.c5_steps} steps FP64 and FP32 transforms")
-        c5 = measure_dipole(args.c5_steps, min(args.reps, 50))
+        c5 = measure_dipole(args.c5_steps, args.reps)
     cpu = None
-    if not args.no_cpu:
-        log("CPU baseline (oracle, same system)")
-        cpu = cpu_baseline(args.L)
-        log(f"  {cpu['value']:.4e} spin-updates/s on {cpu['cores']} threads")
+    if not (extras or args.no_cpu):
+        log("CPU baseline (oracle): headline system scaled, and C2")
+        cpu = cpu_baseline(args.cpu_L3, args.L)
 
-    dom = max(fp64["phases"], key=lambda p: p["ms"])
+    h = c3["fp64"]
+    dom = max(h["phases"], key=lambda p: p["ms"])
     # counter-measured HBM bytes of the dominant phase kernel of this workload
-    # (k_heun<FP64, pattern-diag, predictor or corrector>); null otherwise
+    # (k_heun<FP64, pattern-full, predictor or corrector>); null otherwise
     traffic = None
-    if "exchange=pattern-diag " in fp64["describe"] and args.L == 64:
-        traffic = read_traffic("traffic_c2_fp64.json", dom["phase"])
+    if "exchange=pattern-full " in h["describe"] and args.L3 == 128:
+        traffic = read_traffic("traffic_c3_fp64.json", dom["phase"])
+    f32 = c3["fp32_field"]
     line = {
         "metric": METRIC,
-        "value": fp64["value"],
+        "value": h["value"],
         "unit": "spin-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": fp64["ms"] / args.steps,
+        "ms_per_step": h["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"C2: bcc {args.L}^3x2 ({fp64['n']} spins), J1=3.0 meV x8 + "
-                               f"J2=1.5 meV x6, K1=0.05 meV, alpha=0.05, dt=0.5 fs, T=300 K "
-                               f"(Philox noise), FP64 Heun",
-                   "spins": fp64["n"], "engine": fp64["describe"]},
-        "roofline": {"bound": "hbm", "kernel": f"k_heun {dom['phase']}",
+        "config": {"workload": f"C3: bcc {args.L3}^3x2 ({c3['n']} spins), five shells "
+                               f"(58 neighbours) J_n=3.0 exp(-r/a) meV + DMI D_n=0.1 J_n, "
+                               f"alpha=0.05, dt=0.5 fs, T=100 K (Philox noise), random start, "
+                               f"FP64 Heun",
+                   "spins": c3["n"], "engine": h["describe"]},
+        "roofline": {"bound": "hbm", "kernel": f"k_heun<FP64, pattern-full> {dom['phase']}",
                      "achieved": dom["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": dom["gbs"] / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": dom["bytes"],
                      "launch_ms": dom["ms"],
                      "csr_equivalent_gbs": dom["csr_equiv_gbs"]},
-        "phases": fp64["phases"],
+        "phases": h["phases"],
+        "fp32_field": {"value": f32["value"], "ms_per_step": f32["ms_per_step"],
+                       "speedup_vs_fp64": c3["speedup_fp32_vs_fp64"], "phases": f32["phases"],
+                       "engine": f32["describe"]},
         "cpu_baseline": cpu,
     }
+    if "accuracy" in c3:
+        line["precision_c3"] = c3["accuracy"]
+    if c2 is not None:
+        line["c2"] = c2
     if rk4 is not None:
         line["rk4_c2"] = rk4
-    if fp32 is not None:
-        line["fp32_field"] = {"value": fp32["value"], "ms_per_step": fp32["ms"] / args.steps,
-                              "speedup_vs_fp64": fp32["value"] / fp64["value"],
-                              "phases": fp32["phases"]}
-        if "vs_fp64" in fp32:
-            line["fp32_field"]["vs_fp64"] = fp32["vs_fp64"]
-    if c3 is not None:
-        line["precision_c3"] = c3
     if c4 is not None:
         line["batch_c4"] = c4
     if c5 is not None:
