@@ -65,6 +65,11 @@ extern "C" {
 #define JH_XFMT_PATTERN_GLOBAL 4 /* PATTERN with the pattern table read from
                                  global memory (what PATTERN falls back to when
                                  the table does not fit in LDS)                  */
+#define JH_XFMT_STENCIL 5     /* PATTERN, and the Heun step kernels stage each
+                                 workgroup's neighbourhood in LDS: a crystal
+                                 given by jh_set_lattice whose rows all
+                                 translate one stencil per basis site (AUTO
+                                 takes it where it applies)                      */
 
 /* term kinds reported by jh_term_kind */
 #define JH_TERM_EXCHANGE 0
@@ -91,6 +96,12 @@ int jh_set_sites(jh_ctx *ctx, int64_t n, const double *mus, const double *gyro,
                  const double *alpha);
 
 /* globals::s, N x 3.  jh_get_spins synchronises. */
+/* Lattice shape of the sites (after jh_set_sites): spin index
+ * ((x*size[1] + y)*size[2] + z)*nbasis + b, periodic[a] != 0 on periodic
+ * axes (Lattice::generate_supercell, core/lattice.cc:577-749, :622-649;
+ * boundaries :987-1007).  Optional: it lets a crystal's exchange run as a
+ * stencil over LDS tiles (JH_XFMT_STENCIL). */
+int jh_set_lattice(jh_ctx *ctx, const int32_t size[3], const int32_t periodic[3], int nbasis);
 int jh_set_spins(jh_ctx *ctx, const double *s);
 int jh_get_spins(jh_ctx *ctx, double *s);
 

@@ -140,6 +140,18 @@ __device__ __forceinline__ void llg_rhs(double s0, double s1, double s2, double 
   r2 = g * (c2 + alpha * d2);
 }
 
+// v / |v| as one reciprocal and three products (the FP32-field form, whose
+// arithmetic need not reproduce the reference's operation for operation):
+// within 1.5 ulp of unit_vector, two FP64 divisions fewer.
+__device__ __forceinline__ void unit_vector_rcp(double &v0, double &v1, double &v2) {
+  const double len = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
+  if (fabs(len) <= 2.220446049250313e-16) return;
+  const double r = 1.0 / len;
+  v0 = v0 * r;
+  v1 = v1 * r;
+  v2 = v2 * r;
+}
+
 // unit_vector (containers/vec3.h:276-283): v / |v| unless |v| <= DBL_EPSILON.
 __device__ __forceinline__ void unit_vector(double &v0, double &v1, double &v2) {
   const double len = sqrt(v0 * v0 + v1 * v1 + v2 * v2);

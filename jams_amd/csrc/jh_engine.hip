@@ -9,6 +9,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -71,6 +73,7 @@ thread_local std::string g_create_error;
 
 constexpr double kBoltzmannIU = 0.0861733326;  // meV/K, helpers/consts.h:35
 constexpr size_t kMaxTableLds = 48 * 1024;     // bytes of LDS for the tensor table
+constexpr size_t kStMaxTileLds = 160 * 1024;   // bytes of LDS for a STENCIL tile (a CU's LDS)
 // Tensor tables of the FP32 form (LDS table, wave-uniform sequences) keep
 // the exact FP64 values: they are per tensor, not per spin, so FP32 would
 // save no bandwidth, and rounding them perturbs the Hamiltonian itself (a
@@ -80,6 +83,11 @@ constexpr size_t kMaxTableLds = 48 * 1024;     // bytes of LDS for the tensor ta
 #define JH_FP32_TENSOR_TABLES 0
 #endif
 constexpr bool kRoundTables = JH_FP32_TENSOR_TABLES != 0;
+// FP32 form: pair programs for isotropic + DMI tensors (build_pairs)
+#ifndef JH_PAIR_PROGRAMS
+#define JH_PAIR_PROGRAMS 1
+#endif
+constexpr bool kPairPrograms = JH_PAIR_PROGRAMS != 0 && !kRoundTables;
 
 // Static partition of [0, n) over host threads (set-up work only).
 template <class F>
@@ -242,6 +250,9 @@ struct jh_ctx {
   DevBuf<int32_t> sseq;          // UNIFORM: (offset, length) per slice
   DevBuf<double> seqtab;
   DevBuf<int32_t> upat;          // UNIFORM: byte-offset pattern words
+  DevBuf<int32_t> spair, pgrp;   // pair programs (XV_QISO / XV_QDMI)
+  DevBuf<double> pval;
+  int64_t npaired = 0;
   int64_t nuniform = 0, nslices = 0;
   DevBuf<uint16_t> cls;
   DevBuf<double> ctab;
@@ -250,6 +261,19 @@ struct jh_ctx {
   int ani_power = 2;
   int has_ac = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+  // lattice shape (jh_set_lattice): spin ((x Ly + y) Lz + z) nb + b
+  bool have_lattice = false;
+  int lsize[3] = {0, 0, 0}, lper[3] = {0, 0, 0}, lnb = 0;
+  // STENCIL step kernels (a layer over the PATTERN format, which k_field,
+  // RK4 and the FP64 row-list launch keep using)
+  bool st_on = false;
+  StView st{};
+  DevBuf<int32_t> st_seq, st_off, st_wseq, st_rows;
+  DevBuf<double> st_val;
+  DevBuf<uint64_t> st_fixmask;
+  int64_t st_nfix = 0;
+  int st_nseq = 0;
 
   // replicas: independent systems in consecutive index ranges
   int nrep = 0;
@@ -266,7 +290,7 @@ struct jh_ctx {
   // threads, full tensors and the wave-uniform FP32 variants at 512 (their
   // occupancy is bounded by LDS per workgroup, not per thread)
   int launch_block() const {
-    if (block) return block;
+    if (block) return std::min(block, kMaxBlock);  // 1024 applies to the STENCIL launch
     return (xv_diag(xv) && !xv_uniform(xv)) ? 256 : 512;
   }
   int graph_steps = 0;              // steps per captured graph (0: plain launches)
@@ -541,7 +565,9 @@ bool build_patterns(const jh_ctx *c, const XHost &x, int tbits, bool by_tensor,
 // fit the table.
 int64_t build_uniform(const XHost &x, int64_t n, int tbits, bool diag,
                       const std::vector<uint32_t> &pofs, const std::vector<uint32_t> &pat,
-                      std::vector<int32_t> &sseq, std::vector<double> &seqtab) {
+                      std::vector<int32_t> &sseq, std::vector<double> &seqtab,
+                      std::vector<std::vector<uint32_t>> &seq_ids_out,
+                      std::vector<int32_t> &slice_seq) {
   const uint32_t tmask = (1u << tbits) - 1u;
   const int vpt = diag ? 3 : 9;
   // tensor sequence of each pattern, de-duplicated by its id sequence
@@ -582,6 +608,7 @@ int64_t build_uniform(const XHost &x, int64_t n, int tbits, bool diag,
   }
   const int64_t nsl = (n + kSlice - 1) / kSlice;
   sseq.assign((size_t)nsl * 2, 0);
+  slice_seq.assign((size_t)nsl, -1);
   int64_t nu = 0;
   for (int64_t sl = 0; sl < nsl; ++sl) {
     const int32_t first = pat_seq[pofs[(size_t)sl * kSlice]];
@@ -590,13 +617,355 @@ int64_t build_uniform(const XHost &x, int64_t n, int tbits, bool diag,
       uni = pat_seq[pofs[d]] == first;
     sseq[2 * sl] = uni ? seqs[first].first : -1;
     sseq[2 * sl + 1] = uni ? seqs[first].second : 0;
+    slice_seq[sl] = uni ? first : -1;
     nu += uni;
   }
+  seq_ids_out = std::move(seq_ids);
   return nu;
+}
+
+// Pair programs of the FP32 form (XV_QISO / XV_QDMI).  Where every tensor of
+// a uniform sequence is J I + [D]x (isotropic plus antisymmetric: T[m][m] = J,
+// T[m][q] = -T[q][m]) and the entries pair up as (T, T^T) (a symmetric
+// Hamiltonian's bonds r and -r), a pair's field is
+//     T s1 + T^T s2 = J (s1 + s2) + D x (s1 - s2),
+// so the pairs of one J are summed two at a time in FP32 (|s1 + s2| <= 2,
+// their sum <= 4: rounding of the size of the FP32 spins themselves) and
+// multiplied by the exact FP64 J once, and D x (s1 - s2) takes six FP64
+// multiply-adds: about 10.75 operations per neighbour against 13 for a full
+// tensor.  Per group of one or two pairs sharing J: pgrp = the four slots
+// (k1A, k2A, k1B, k2B; k1B = -1 for one pair) of the sequence (the per-lane
+// pattern words give each slot's offset) and pval = (J, DA, DB, 0); the
+// two-pair groups come first.  Per slice: spair = (first group, two-pair
+// groups | one-pair groups << 16), (-1, 0) without a program.
+// Returns the number of slices with a program; dmi is set when some D != 0.
+int64_t build_pairs(const XHost &x, const std::vector<std::vector<uint32_t>> &seq_ids,
+                    const std::vector<int32_t> &slice_seq, std::vector<int32_t> &spair,
+                    std::vector<int32_t> &pgrp, std::vector<double> &pval, bool &dmi) {
+  dmi = false;
+  std::vector<std::pair<int32_t, int32_t>> prog(seq_ids.size(), {-1, 0});
+  for (size_t q = 0; q < seq_ids.size(); ++q) {
+    const auto &ids = seq_ids[q];
+    if (ids.size() >= 0x7FFF) continue;
+    struct Iso { double J, D[3]; };
+    std::vector<Iso> iso(ids.size());
+    bool ok = true;
+    for (size_t k = 0; k < ids.size() && ok; ++k) {
+      const double *T = &x.tab[(size_t)ids[k] * 9];
+      ok = T[0] == T[4] && T[4] == T[8] && T[1] == -T[3] && T[2] == -T[6] && T[5] == -T[7];
+      // [D]x = ((0, -Dz, Dy), (Dz, 0, -Dx), (-Dy, Dx, 0))
+      iso[k] = {T[0], {T[7], T[2], T[3]}};
+    }
+    if (!ok) continue;
+    // pair each entry with an unpaired one of tensor T^T: same J, -D
+    auto key = [](double J, double a, double b, double c) {
+      // canonical zeros so that D = 0 pairs with itself
+      auto z = [](double v) { return v == 0.0 ? 0.0 : v; };
+      return std::array<double, 4>{z(J), z(a), z(b), z(c)};
+    };
+    std::map<std::array<double, 4>, std::vector<int>> open;
+    std::vector<std::pair<int, int>> pairs;
+    for (size_t k = 0; k < ids.size(); ++k) {
+      const Iso &e = iso[k];
+      auto &cands = open[key(e.J, -e.D[0], -e.D[1], -e.D[2])];
+      if (!cands.empty()) {
+        pairs.push_back({cands.back(), (int)k});
+        cands.pop_back();
+      } else {
+        open[key(e.J, e.D[0], e.D[1], e.D[2])].push_back((int)k);
+      }
+    }
+    if (pairs.size() * 2 != ids.size()) continue;  // an entry without its partner
+    // groups of two pairs of one J, then the pairs left over one by one (the
+    // kernel runs the two kinds as two loops without branches)
+    std::stable_sort(pairs.begin(), pairs.end(), [&](const auto &a, const auto &b) {
+      return iso[a.first].J < iso[b.first].J;
+    });
+    std::vector<std::array<int, 4>> twos, ones;
+    for (size_t p = 0; p < pairs.size();) {
+      if (p + 1 < pairs.size() && iso[pairs[p + 1].first].J == iso[pairs[p].first].J) {
+        twos.push_back({pairs[p].first, pairs[p].second, pairs[p + 1].first, pairs[p + 1].second});
+        p += 2;
+      } else {
+        ones.push_back({pairs[p].first, pairs[p].second, -1, -1});
+        p += 1;
+      }
+    }
+    const int32_t g0 = (int32_t)(pgrp.size() / 4);
+    for (const auto *list : {&twos, &ones})
+      for (const auto &g : *list) {
+        for (int k : g) pgrp.push_back(k);
+        pval.push_back(iso[g[0]].J);
+        for (int a = 0; a < 3; ++a) pval.push_back(iso[g[0]].D[a]);
+        for (int a = 0; a < 3; ++a) pval.push_back(g[2] >= 0 ? iso[g[2]].D[a] : 0.0);
+        pval.push_back(0.0);
+        for (int a = 0; a < 3; ++a)
+          dmi = dmi || iso[g[0]].D[a] != 0.0 || (g[2] >= 0 && iso[g[2]].D[a] != 0.0);
+      }
+    if (twos.size() >= 0x8000 || ones.size() >= 0x8000) continue;
+    prog[q] = {g0, (int32_t)(twos.size() | (ones.size() << 16))};
+  }
+  spair.assign(slice_seq.size() * 2, 0);
+  int64_t np = 0;
+  for (size_t sl = 0; sl < slice_seq.size(); ++sl) {
+    const int32_t q = slice_seq[sl];
+    const auto pr = q >= 0 ? prog[q] : std::pair<int32_t, int32_t>{-1, 0};
+    spair[2 * sl] = pr.first;
+    spair[2 * sl + 1] = pr.second;
+    np += pr.first >= 0;
+  }
+  return np;
+}
+
+// ---- STENCIL layout ------------------------------------------------------
+//
+// Available when the context holds one crystal whose shape jh_set_lattice
+// gave, whose device order is basis-major with the lattice's basis sites,
+// with Lz a multiple of 64, and whose exchange rows are all the translates of
+// one stencil per basis site (every neighbour (dx, dy, dz, b', tensor) of the
+// row at the middle of the lattice, wrapped on periodic axes and dropped
+// outside open ones).  Builds the sequences, the tile geometry and, in FP64,
+// the rows the row-list launch computes; returns false (and leaves the
+// context without it) otherwise.
+bool build_stencil(jh_ctx *c, const XHost &x, bool diag) {
+  if (!c->have_lattice || c->nrep > 0 || c->integrator != JH_INTEGRATOR_HEUN) return false;
+  const int L[3] = {c->lsize[0], c->lsize[1], c->lsize[2]};
+  const int nb = c->lnb;
+  const int64_t C = (int64_t)L[0] * L[1] * L[2];
+  if (nb < 1 || nb > kStMaxBasis || C * nb != c->n || L[2] % 64 != 0) return false;
+  if (c->order_nb != nb && !(nb == 1 && c->order_nb == 1)) return false;
+  const bool fp64 = c->prec == PR_FP64;
+  const int elem = fp64 ? 24 : 16;
+  struct Ent { int d[3]; int bb; int t; };
+  std::vector<std::vector<Ent>> sten((size_t)nb);
+  int lo[kStMaxBasis][3], hi[kStMaxBasis][3];
+  auto decomp = [&](int64_t j, int cell[3], int &bb) {
+    bb = (int)(j % nb);
+    int64_t q = j / nb;
+    cell[2] = (int)(q % L[2]);
+    q /= L[2];
+    cell[1] = (int)(q % L[1]);
+    cell[0] = (int)(q / L[1]);
+  };
+  for (int b = 0; b < nb; ++b) {
+    const int mid[3] = {L[0] / 2, L[1] / 2, L[2] / 2};
+    const int64_t i = (((int64_t)mid[0] * L[1] + mid[1]) * L[2] + mid[2]) * nb + b;
+    for (int a = 0; a < 3; ++a) { lo[b][a] = 0; hi[b][a] = 0; }
+    for (int64_t q = x.ptr[i]; q < x.ptr[i + 1]; ++q) {
+      Ent e;
+      int cell[3];
+      decomp(x.col[q], cell, e.bb);
+      for (int a = 0; a < 3; ++a) {
+        int d = cell[a] - mid[a];
+        if (c->lper[a]) {
+          if (d > L[a] / 2) d -= L[a];
+          if (d < -(L[a] - 1) / 2) d += L[a];
+        }
+        // the stencil must reach less than half way round (a unique minimum
+        // image) and stay clear of the middle row's open boundaries
+        if (2 * std::abs(d) >= L[a]) return false;
+        e.d[a] = d;
+        lo[b][a] = std::min(lo[b][a], d);
+        hi[b][a] = std::max(hi[b][a], d);
+      }
+      e.t = x.typ[q];
+      sten[b].push_back(e);
+    }
+  }
+  // workgroup shape: with jh_set_block_size the shape of that many threads,
+  // else the most waves whose tile leaves the LDS for two workgroups per CU
+  // (one workgroup's tile load then overlaps the other's sums), else one
+  static const int kShapes[][2] = {{4, 4}, {2, 4}, {4, 2}, {2, 2}, {1, 4}, {4, 1},
+                                   {1, 2}, {2, 1}, {1, 1}};
+  int wx = 0, wy = 0;
+  for (int pass = 0; pass < 2 && !wx; ++pass)
+  for (const auto &sh : kShapes) {
+    if (L[0] % sh[0] || L[1] % sh[1]) continue;
+    if (c->block && sh[0] * sh[1] * 64 != c->block) continue;
+    size_t need = 0;
+    for (int b = 0; b < nb; ++b)
+      need = std::max(need, (size_t)nb * (sh[0] + hi[b][0] - lo[b][0]) *
+                                (sh[1] + hi[b][1] - lo[b][1]) * (64 + hi[b][2] - lo[b][2]) * elem);
+    const size_t cap = (pass == 0 && !c->block) ? kStMaxTileLds / 2 : kStMaxTileLds;
+    if (need <= cap) { wx = sh[0]; wy = sh[1]; break; }
+  }
+  if (!wx) return false;
+  StView v{};
+  v.lx = L[0]; v.ly = L[1]; v.lz = L[2]; v.nb = nb;
+  for (int a = 0; a < 3; ++a) v.per[a] = c->lper[a] ? 1 : 0;
+  v.wx = wx; v.wy = wy;
+  v.nbx = L[0] / wx; v.nby = L[1] / wy; v.nbz = L[2] / 64;
+  for (int b = 0; b < nb; ++b) {
+    v.ox[b] = lo[b][0]; v.oy[b] = lo[b][1]; v.oz[b] = lo[b][2];
+    v.tx[b] = wx + hi[b][0] - lo[b][0];
+    v.ty[b] = wy + hi[b][1] - lo[b][1];
+    v.tz[b] = 64 + hi[b][2] - lo[b][2];
+  }
+  // column of entry e of row (b, x, y, z), or -1 outside an open boundary
+  auto col_of = [&](const Ent &e, const int cell[3]) -> int64_t {
+    int cc[3];
+    for (int a = 0; a < 3; ++a) {
+      cc[a] = cell[a] + e.d[a];
+      if (cc[a] < 0 || cc[a] >= L[a]) {
+        if (!c->lper[a]) return -1;
+        cc[a] = (cc[a] + L[a]) % L[a];
+      }
+    }
+    return (((int64_t)cc[0] * L[1] + cc[1]) * L[2] + cc[2]) * nb + e.bb;
+  };
+  // sequences: FP32 form, the stencil order of each basis site (any order
+  // serves); FP64, per line (b, x, y) the column order of its rows away from
+  // the z ends (where no z neighbour wraps), entries outside an open
+  // boundary last
+  std::vector<std::vector<int>> seqs;
+  std::vector<int32_t> wseq;
+  if (!fp64) {
+    for (int b = 0; b < nb; ++b) {
+      std::vector<int> o(sten[b].size());
+      for (size_t k = 0; k < o.size(); ++k) o[k] = (int)k;
+      seqs.push_back(o);
+    }
+  } else {
+    wseq.assign((size_t)nb * L[0] * L[1], -1);
+    std::map<std::vector<int>, int> ids;
+    for (int b = 0; b < nb; ++b)
+      for (int xx = 0; xx < L[0]; ++xx)
+        for (int yy = 0; yy < L[1]; ++yy) {
+          const int cell[3] = {xx, yy, L[2] / 2};
+          const auto &sb = sten[b];
+          std::vector<std::pair<int64_t, int>> keyed(sb.size());
+          for (size_t k = 0; k < sb.size(); ++k) {
+            const int64_t j = col_of(sb[k], cell);
+            keyed[k] = {j < 0 ? INT64_MAX : j, (int)k};
+          }
+          std::sort(keyed.begin(), keyed.end());
+          std::vector<int> o(sb.size() + 1);
+          o[0] = b;  // the key carries the basis site (entries are per basis)
+          for (size_t k = 0; k < sb.size(); ++k) o[k + 1] = keyed[k].second;
+          auto it = ids.find(o);
+          int id;
+          if (it == ids.end()) {
+            id = (int)seqs.size();
+            ids.emplace(o, id);
+            seqs.emplace_back(o.begin() + 1, o.end());
+          } else {
+            id = it->second;
+          }
+          wseq[((size_t)b * L[0] + xx) * L[1] + yy] = id;
+        }
+    if (seqs.size() > 4096) return false;
+  }
+  std::vector<int> seq_basis(seqs.size());
+  if (!fp64) {
+    for (int b = 0; b < nb; ++b) seq_basis[b] = b;
+  } else {
+    for (int b = 0; b < nb; ++b)
+      for (int xx = 0; xx < L[0]; ++xx)
+        for (int yy = 0; yy < L[1]; ++yy)
+          seq_basis[wseq[((size_t)b * L[0] + xx) * L[1] + yy]] = b;
+  }
+  // every row must be its stencil's translate; FP64 rows whose column order
+  // is not their line's sequence (a z neighbour wraps) go to the row list
+  const int64_t n = c->n;
+  std::vector<uint8_t> fix((size_t)n, 0);
+  std::atomic<int> bad{0};
+  parallel_for(n, [&](int64_t d) {
+    const int b = (int)(d / C);
+    const int64_t cellidx = d % C;
+    int cell[3];
+    cell[2] = (int)(cellidx % L[2]);
+    cell[1] = (int)((cellidx / L[2]) % L[1]);
+    cell[0] = (int)(cellidx / ((int64_t)L[2] * L[1]));
+    const int64_t i = cellidx * nb + b;
+    const auto &sb = sten[b];
+    const std::vector<int> &o =
+        fp64 ? seqs[wseq[((size_t)b * L[0] + cell[0]) * L[1] + cell[1]]] : seqs[b];
+    thread_local std::vector<std::pair<int64_t, int32_t>> got;
+    got.clear();
+    bool ordered = true;
+    int64_t last = -1;
+    for (int k : o) {
+      const int64_t j = col_of(sb[k], cell);
+      if (j < 0) continue;
+      if (j <= last) ordered = false;
+      last = j;
+      got.push_back({j, sb[k].t});
+    }
+    if (!ordered) std::sort(got.begin(), got.end());
+    const int64_t len = x.ptr[i + 1] - x.ptr[i];
+    bool ok = len == (int64_t)got.size();
+    for (int64_t q = 0; ok && q < len; ++q)
+      ok = got[q].first == x.col[x.ptr[i] + q] && got[q].second == x.typ[x.ptr[i] + q];
+    if (!ok) bad.store(1, std::memory_order_relaxed);
+    if (fp64 && !ordered) fix[d] = 1;
+  });
+  if (bad.load()) return false;
+  // tables
+  std::vector<int32_t> seqv, offv;
+  std::vector<double> valv;
+  const int V = diag ? 3 : 9;
+  for (size_t s = 0; s < seqs.size(); ++s) {
+    const int b = seq_basis[s];
+    seqv.push_back((int32_t)offv.size());
+    seqv.push_back((int32_t)seqs[s].size());
+    for (int k : seqs[s]) {
+      const Ent &e = sten[b][k];
+      const int64_t el = (((int64_t)e.bb * v.tx[b] + e.d[0] - v.ox[b]) * v.ty[b] + e.d[1] - v.oy[b]) *
+                             v.tz[b] + e.d[2] - v.oz[b];
+      // FP64: byte offset in the z plane (the (x, y) plane's is twice it);
+      // FP32: byte offset of the 16-byte element
+      offv.push_back((int32_t)(el * (fp64 ? 8 : 16)));
+      for (int q = 0; q < V; ++q) valv.push_back(diag ? x.tab[9 * e.t + 4 * q] : x.tab[9 * e.t + q]);
+    }
+  }
+  c->st_seq.alloc(seqv.size());
+  HIPCHK(hipMemcpy(c->st_seq.p, seqv.data(), sizeof(int32_t) * seqv.size(), hipMemcpyHostToDevice));
+  c->st_off.alloc(std::max<size_t>(offv.size(), 1));
+  if (!offv.empty())
+    HIPCHK(hipMemcpy(c->st_off.p, offv.data(), sizeof(int32_t) * offv.size(), hipMemcpyHostToDevice));
+  c->st_val.alloc(std::max<size_t>(valv.size(), 1));
+  if (!valv.empty())
+    HIPCHK(hipMemcpy(c->st_val.p, valv.data(), sizeof(double) * valv.size(), hipMemcpyHostToDevice));
+  c->st_wseq.release();
+  c->st_rows.release();
+  c->st_fixmask.release();
+  c->st_nfix = 0;
+  if (fp64) {
+    c->st_wseq.alloc(wseq.size());
+    HIPCHK(hipMemcpy(c->st_wseq.p, wseq.data(), sizeof(int32_t) * wseq.size(), hipMemcpyHostToDevice));
+    std::vector<int32_t> rows;
+    for (int64_t d = 0; d < n; ++d)
+      if (fix[d]) rows.push_back((int32_t)d);
+    if (!rows.empty()) {
+      // per wave of the stencil launch: the lanes the row list owns
+      const int wpg = wx * wy;
+      const int64_t nwg = (int64_t)nb * v.nbx * v.nby * v.nbz;
+      std::vector<uint64_t> mask((size_t)(nwg * wpg), 0);
+      for (int32_t d : rows) {
+        const int b = (int)(d / C);
+        const int64_t cellidx = d % C;
+        const int z = (int)(cellidx % L[2]);
+        const int yy = (int)((cellidx / L[2]) % L[1]);
+        const int xx = (int)(cellidx / ((int64_t)L[2] * L[1]));
+        const int64_t lb = (((int64_t)b * v.nbx + xx / wx) * v.nby + yy / wy) * v.nbz + z / 64;
+        const int wave = (xx % wx) * wy + (yy % wy);
+        mask[(size_t)(lb * wpg + wave)] |= 1ull << (z % 64);
+      }
+      c->st_rows.alloc(rows.size());
+      HIPCHK(hipMemcpy(c->st_rows.p, rows.data(), sizeof(int32_t) * rows.size(), hipMemcpyHostToDevice));
+      c->st_fixmask.alloc(mask.size());
+      HIPCHK(hipMemcpy(c->st_fixmask.p, mask.data(), sizeof(uint64_t) * mask.size(), hipMemcpyHostToDevice));
+      c->st_nfix = (int64_t)rows.size();
+    }
+  }
+  c->st = v;
+  c->st_nseq = (int)seqs.size();
+  return true;
 }
 
 void prepare_exchange(jh_ctx *c) {
   c->xv = XV_NONE;
+  c->st_on = false;
   c->ntab = 0;
   c->nent = 0;
   if (c->xh.empty()) return;
@@ -620,11 +989,19 @@ void prepare_exchange(jh_ctx *c) {
   std::vector<uint32_t> pofs, pat;
   const bool by_tensor = c->prec == PR_FP32;
   const bool want_pattern = c->xfmt_req == JH_XFMT_AUTO || c->xfmt_req == JH_XFMT_PATTERN ||
-                            c->xfmt_req == JH_XFMT_PATTERN_GLOBAL;
+                            c->xfmt_req == JH_XFMT_PATTERN_GLOBAL ||
+                            c->xfmt_req == JH_XFMT_STENCIL;
   bool pattern_ok = typed_ok && want_pattern &&
                     build_patterns(c, x, tbits, by_tensor, pofs, pat, c->npatterns);
   int fmt = c->xfmt_req;
-  if (fmt == JH_XFMT_AUTO) fmt = pattern_ok ? JH_XFMT_PATTERN : typed_ok ? JH_XFMT_TYPED : JH_XFMT_BLOCK;
+  // AUTO keeps the gather kernels: at the bcc 128^3 x 2 five-shell workload the
+  // STENCIL launch measured slower (its tile staging costs more VALU than the
+  // texture-address time it saves); jh_autotune times it beside the others
+  const bool want_stencil = fmt == JH_XFMT_STENCIL;
+  if (fmt == JH_XFMT_STENCIL && !pattern_ok)
+    JH_THROW(JH_ERR_UNSUPPORTED, "STENCIL exchange format needs the row patterns of a crystal");
+  if (fmt == JH_XFMT_AUTO || fmt == JH_XFMT_STENCIL)
+    fmt = pattern_ok ? JH_XFMT_PATTERN : typed_ok ? JH_XFMT_TYPED : JH_XFMT_BLOCK;
   if (fmt != JH_XFMT_BLOCK && !typed_ok)
     JH_THROW(JH_ERR_UNSUPPORTED, "TYPED exchange format does not fit (too many distinct tensors)");
   if ((fmt == JH_XFMT_PATTERN || fmt == JH_XFMT_PATTERN_GLOBAL) && !pattern_ok)
@@ -647,9 +1024,11 @@ void prepare_exchange(jh_ctx *c) {
     c->nslices = (n + kSlice - 1) / kSlice;
     c->nuniform = 0;
     if (by_tensor && c->pat_lds) {
-      std::vector<int32_t> sseq;
+      std::vector<int32_t> sseq, slice_seq;
       std::vector<double> seqtab;
-      const int64_t nu = build_uniform(x, n, tbits, diag, pofs, pat, sseq, seqtab);
+      std::vector<std::vector<uint32_t>> seq_ids;
+      const int64_t nu = build_uniform(x, n, tbits, diag, pofs, pat, sseq, seqtab, seq_ids,
+                                       slice_seq);
       // worth its own variant when at least half of the slices are uniform
       // byte offsets of the gather copy must fit 31 bits
       const bool fits = (double)n * kG32 * sizeof(float) < 2147483648.0;
@@ -675,7 +1054,28 @@ void prepare_exchange(jh_ctx *c) {
         if (!seqtab.empty())
           HIPCHK(hipMemcpy(c->seqtab.p, seqtab.data(), sizeof(double) * seqtab.size(), hipMemcpyHostToDevice));
         c->xv = diag ? XV_UDIAG : XV_UFULL;
+        // pair programs (isotropic + DMI tensors), when most slices have one
+        std::vector<int32_t> spair, pgrp;
+        std::vector<double> pval;
+        bool dmi = false;
+        const int64_t npr = kPairPrograms ? build_pairs(x, seq_ids, slice_seq, spair, pgrp, pval, dmi) : 0;
+        if (npr * 2 >= c->nslices && !pgrp.empty()) {
+          c->spair.alloc(spair.size());
+          HIPCHK(hipMemcpy(c->spair.p, spair.data(), sizeof(int32_t) * spair.size(), hipMemcpyHostToDevice));
+          c->pgrp.alloc(pgrp.size());
+          HIPCHK(hipMemcpy(c->pgrp.p, pgrp.data(), sizeof(int32_t) * pgrp.size(), hipMemcpyHostToDevice));
+          c->pval.alloc(pval.size());
+          HIPCHK(hipMemcpy(c->pval.p, pval.data(), sizeof(double) * pval.size(), hipMemcpyHostToDevice));
+          c->npaired = npr;
+          c->xv = dmi ? XV_QDMI : XV_QISO;
+        }
       }
+    }
+    if (!xv_pair(c->xv)) {
+      c->spair.release();
+      c->pgrp.release();
+      c->pval.release();
+      c->npaired = 0;
     }
     if (!xv_uniform(c->xv)) {
       c->sseq.release();
@@ -686,6 +1086,12 @@ void prepare_exchange(jh_ctx *c) {
     c->ent.release();
     c->val64.release();
     c->val32.release();
+    c->st_on = want_stencil && build_stencil(c, x, diag);
+    if (c->xfmt_req == JH_XFMT_STENCIL && !c->st_on)
+      JH_THROW(JH_ERR_UNSUPPORTED,
+               "STENCIL exchange format does not apply (needs jh_set_lattice, one crystal whose "
+               "rows all translate one stencil per basis site, Lz a multiple of 64, Heun)");
+    if (c->st_on) c->xfmt = JH_XFMT_STENCIL;
     return;
   }
   c->pofs.release();
@@ -953,6 +1359,19 @@ KParams make_params(jh_ctx *c) {
   p.x.sseq = reinterpret_cast<const int2 *>(c->sseq.p);
   p.x.seqtab = c->seqtab.p;
   p.x.upat = c->upat.p;
+  p.x.spair = reinterpret_cast<const int2 *>(c->spair.p);
+  p.x.pgrp = reinterpret_cast<const int4 *>(c->pgrp.p);
+  p.x.pval = c->pval.p;
+  if (c->st_on) {
+    p.st = c->st;
+    p.st.seq = reinterpret_cast<const int2 *>(c->st_seq.p);
+    p.st.off = c->st_off.p;
+    p.st.val = c->st_val.p;
+    p.st.wseq = c->st_wseq.p;
+    p.st.fixmask = c->st_fixmask.p;
+  }
+  p.rowlist = nullptr;
+  p.nrowlist = 0;
   p.nterms = (int)c->terms.size();
   for (int q = 0; q < p.nterms; ++q) p.kind[q] = c->terms[q].kind;
   p.hext = c->hext.p;
@@ -998,6 +1417,22 @@ void enqueue_rk4(jh_ctx *c, KParams p) {
 
 // One Heun step's launches (dipole fields, predictor, corrector) for step
 // counter / time in p (or on the device when p.dstep is set).
+// One Heun phase: the STENCIL launch and, in FP64, the row-list launch of
+// its wrapped rows; or one launch of the context's exchange variant.
+void launch_phase(jh_ctx *c, const KParams &p, int phase) {
+  if (!c->st_on) {
+    HIPCHK(launch_heun(p, c->prec, c->xv, phase, c->launch_block(), c->stream));
+    return;
+  }
+  HIPCHK(launch_heun_st(p, c->prec, xv_diag(c->xv), phase, c->stream));
+  if (c->st_nfix > 0) {
+    KParams q = p;
+    q.rowlist = c->st_rows.p;
+    q.nrowlist = c->st_nfix;
+    HIPCHK(launch_heun(q, c->prec, c->xv, phase, c->launch_block(), c->stream));
+  }
+}
+
 void enqueue_step(jh_ctx *c, KParams p) {
   if (c->integrator == JH_INTEGRATOR_RK4) {
     enqueue_rk4(c, p);
@@ -1007,12 +1442,12 @@ void enqueue_step(jh_ctx *c, KParams p) {
   if (dip >= 0) HIPCHK(c->dipoles[dip]->field(c->s.p, c->hext.p, c->prec == PR_FP32, c->stream));
   p.g32 = c->s32.p;
   p.w32 = c->sp32.p;
-  HIPCHK(launch_heun(p, c->prec, c->xv, 0, c->launch_block(), c->stream));
+  launch_phase(c, p, 0);
   // corrector at t + dt
   if (dip >= 0) HIPCHK(c->dipoles[dip]->field(c->sp.p, c->hext.p, c->prec == PR_FP32, c->stream));
   p.g32 = c->sp32.p;
   p.w32 = c->s32.p;
-  HIPCHK(launch_heun(p, c->prec, c->xv, 1, c->launch_block(), c->stream));
+  launch_phase(c, p, 1);
 }
 
 // Capture graph_steps steps.  Their kernels read the step counter as
@@ -1173,6 +1608,7 @@ int jh_set_sites(jh_ctx *c, int64_t n, const double *mus, const double *gyro, co
     c->red_partial.alloc((size_t)(4 * nb));
     c->red_out.alloc(4);
     c->spins_set = false;
+    c->have_lattice = false;
     c->order_nb = 1;
     c->order_cells = n;
     c->noise_mode = NM_NONE;
@@ -1182,6 +1618,28 @@ int jh_set_sites(jh_ctx *c, int64_t n, const double *mus, const double *gyro, co
     c->rep_seed.clear();
     c->iteration = 0;
     c->time = 0.0;
+    c->dirty = true;
+  });
+}
+
+int jh_set_lattice(jh_ctx *c, const int32_t size[3], const int32_t periodic[3], int nbasis) {
+  return guarded(c, [&] {
+    require_sites(c);
+    if (!size || !periodic || nbasis < 1)
+      JH_THROW(JH_ERR_INVALID, "jh_set_lattice: bad arguments");
+    int64_t cells = 1;
+    for (int a = 0; a < 3; ++a) {
+      if (size[a] < 1) JH_THROW(JH_ERR_INVALID, "jh_set_lattice: sizes must be positive");
+      cells *= size[a];
+    }
+    if (cells * nbasis != c->n)
+      JH_THROW(JH_ERR_INVALID, "jh_set_lattice: size[0] size[1] size[2] nbasis must equal the number of spins");
+    c->have_lattice = true;
+    for (int a = 0; a < 3; ++a) {
+      c->lsize[a] = size[a];
+      c->lper[a] = periodic[a] ? 1 : 0;
+    }
+    c->lnb = nbasis;
     c->dirty = true;
   });
 }
@@ -1369,7 +1827,7 @@ int jh_set_integrator(jh_ctx *c, int integrator) {
 int jh_set_exchange_format(jh_ctx *c, int fmt) {
   return guarded(c, [&] {
     if (fmt != JH_XFMT_AUTO && fmt != JH_XFMT_TYPED && fmt != JH_XFMT_BLOCK && fmt != JH_XFMT_PATTERN &&
-        fmt != JH_XFMT_PATTERN_GLOBAL)
+        fmt != JH_XFMT_PATTERN_GLOBAL && fmt != JH_XFMT_STENCIL)
       JH_THROW(JH_ERR_INVALID, "unknown exchange format");
     c->xfmt_req = fmt;
     c->dirty = true;
@@ -1518,10 +1976,10 @@ int jh_time_phase(jh_ctx *c, int phase, int reps, double *ms) {
       KParams q = p;
       q.g32 = c->s32.p;
       q.w32 = c->sp32.p;
-      HIPCHK(launch_heun(q, c->prec, c->xv, 0, c->launch_block(), c->stream));
+      launch_phase(c, q, 0);
     }
     HIPCHK(hipEventRecord(c->ev0, c->stream));
-    for (int r = 0; r < reps; ++r) HIPCHK(launch_heun(p, c->prec, c->xv, phase, c->launch_block(), c->stream));
+    for (int r = 0; r < reps; ++r) launch_phase(c, p, phase);
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     HIPCHK(hipEventSynchronize(c->ev1));
     float f = 0.f;
@@ -1666,8 +2124,9 @@ int jh_set_graph_steps(jh_ctx *c, int steps) {
 
 int jh_set_block_size(jh_ctx *c, int threads) {
   return guarded(c, [&] {
-    if (threads != 0 && (threads < 64 || threads > kMaxBlock || threads % 64 != 0))
-      JH_THROW(JH_ERR_INVALID, "workgroup size must be 0 or a multiple of 64 in [64, 512]");
+    if (threads != 0 && (threads < 64 || threads > kStMaxBlock || threads % 64 != 0))
+      JH_THROW(JH_ERR_INVALID, "workgroup size must be 0 or a multiple of 64 in [64, 1024]");
+    if (threads != c->block && (c->st_on || c->have_lattice)) c->dirty = true;  // tile shape
     c->block = threads;
     c->drop_graph();
   });
@@ -1689,17 +2148,31 @@ int jh_autotune(jh_ctx *c, int steps, char *report, int64_t len) {
     std::string rep;
     double best = 1e300;
     int best_fmt = fmt0, best_bs = bs0;
-    for (int fmt : {JH_XFMT_PATTERN, JH_XFMT_TYPED, JH_XFMT_BLOCK}) {
+    for (int fmt : {JH_XFMT_PATTERN, JH_XFMT_TYPED, JH_XFMT_BLOCK, JH_XFMT_STENCIL}) {
       if (c->xh.empty() && fmt != JH_XFMT_PATTERN) continue;
+      const bool stencil = fmt == JH_XFMT_STENCIL;
+      if (stencil && (!c->have_lattice || c->integrator != JH_INTEGRATOR_HEUN)) continue;
       c->xfmt_req = fmt;
       c->dirty = true;
-      try {
-        prepare(c);
-      } catch (const JhError &) {
-        continue;  // this format does not fit the system
+      if (!stencil) {
+        try {
+          prepare(c);
+        } catch (const JhError &) {
+          continue;  // this format does not fit the system
+        }
       }
-      for (int bs : {128, 256, 512}) {
+      for (int bs : {128, 256, 512, 1024}) {
+        if (bs == 1024 && !stencil) continue;
         c->block = bs;
+        if (stencil) {
+          // the block size is the tile shape: lay it out again
+          c->dirty = true;
+          try {
+            prepare(c);
+          } catch (const JhError &) {
+            continue;  // no stencil of this many threads fits
+          }
+        }
         HIPCHK(hipMemcpyAsync(c->s.p, save.p, sizeof(double) * 3 * c->n, hipMemcpyDeviceToDevice, c->stream));
         sync_f32(c);
         c->iteration = it0;
@@ -1748,19 +2221,30 @@ int jh_describe(jh_ctx *c, char *buf, int64_t buflen) {
     static const char *xvn[] = {"none", "typed-diag", "typed-full", "block-diag", "block-full",
                                 "pattern-diag", "pattern-full", "pattern-diag-global",
                                 "pattern-full-global", "pattern-uniform-diag",
-                                "pattern-uniform-full"};
+                                "pattern-uniform-full", "pattern-uniform-pairs-iso",
+                                "pattern-uniform-pairs-dmi"};
     char tmp[512];
     std::snprintf(tmp, sizeof tmp,
                   "n=%lld prec=%s exchange=%s entries=%lld tensors=%d colbits=%d patterns=%d "
-                  "pattern_words=%d lds=%d uniform_slices=%lld/%lld order=%d classes=%d "
+                  "pattern_words=%d lds=%d uniform_slices=%lld/%lld paired_slices=%lld "
+                  "order=%d classes=%d "
                   "terms=%d noise=%d block=%d replicas=%d graph=%d integrator=%s",
                   (long long)c->n, c->prec == PR_FP64 ? "fp64" : "fp32-field", xvn[c->xv],
                   (long long)c->nent, c->ntab > 0 ? c->ntab - 1 : 0, c->colbits,
                   c->npatterns, c->pat_words, c->pat_lds, (long long)c->nuniform,
-                  (long long)c->nslices, c->order_nb, c->nclass,
+                  (long long)c->nslices, (long long)c->npaired, c->order_nb, c->nclass,
                   (int)c->terms.size(), c->noise_mode, c->launch_block(), c->nrep, c->graph_steps,
                   c->integrator == JH_INTEGRATOR_RK4 ? "rk4" : "heun");
     std::string out = tmp;
+    if (c->st_on) {
+      char st[160];
+      std::snprintf(st, sizeof st, " stencil=%dx%dx64 tile_lds=%zu sequences=%d wrapped_rows=%lld",
+                    c->st.wx, c->st.wy, st_tile_bytes(c->st, c->prec, 0), c->st_nseq,
+                    (long long)c->st_nfix);
+      out += st;
+    } else {
+      out += " stencil=off";
+    }
     const int dip = dipole_term(c);
     if (dip >= 0) out += " " + c->dipoles[dip]->describe(c->prec == PR_FP32);
     if (buf && buflen > 0) {

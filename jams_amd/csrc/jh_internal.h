@@ -54,18 +54,53 @@ enum Prec { PR_FP64 = 0, PR_FP32 = 1 };
 // exchange kernel variants
 // (PDIAG/PFULL keep the pattern table in LDS, GDIAG/GFULL in global memory;
 // UDIAG/UFULL are PDIAG/PFULL plus wave-uniform tensor sequences, FP32 form)
+// QISO/QDMI: UDIAG/UFULL plus pair programs for isotropic (+ DMI) tensors
 enum XVariant {
   XV_NONE = 0, XV_TDIAG = 1, XV_TFULL = 2, XV_BDIAG = 3, XV_BFULL = 4,
-  XV_PDIAG = 5, XV_PFULL = 6, XV_GDIAG = 7, XV_GFULL = 8, XV_UDIAG = 9, XV_UFULL = 10
+  XV_PDIAG = 5, XV_PFULL = 6, XV_GDIAG = 7, XV_GFULL = 8, XV_UDIAG = 9, XV_UFULL = 10,
+  XV_QISO = 11, XV_QDMI = 12
 };
-constexpr bool xv_uniform(int v) { return v == XV_UDIAG || v == XV_UFULL; }
+constexpr bool xv_pair(int v) { return v == XV_QISO || v == XV_QDMI; }
+
+// STENCIL (crystal lattices, Heun step kernels): a workgroup owns wx x wy
+// runs of 64 cells along z of one basis site and stages the spins its rows
+// read (the runs plus the stencil's halo, every basis site) in LDS once;
+// each neighbour is then one LDS read at a per-entry offset from the lane's
+// own tile position, and the tensor of entry k of the wave's sequence is
+// wave-uniform (scalar loads).  FP64 sums each row in the reference column
+// order: the waves of one (basis, x, y) line share it unless a z neighbour
+// wraps around a periodic boundary, and those few rows are left to a second
+// launch over a row list with their own row patterns.
+constexpr int kStMaxBasis = 4;
+constexpr int kStMaxBlock = 1024;
+struct StView {
+  int lx, ly, lz, nb;      // cells per axis, basis sites
+  int per[3];              // periodic axes
+  int wx, wy;              // waves per workgroup along x and y
+  int nbx, nby, nbz;       // workgroups along x, y and z (nbz = lz / 64)
+  // per source basis site b: cell offset of tile element 0 from the
+  // workgroup's first cell (the lowest dx, dy, dz of its stencil) and tile
+  // extents; the tile holds every basis site: nb * tx * ty * tz elements
+  // ((bb * tx + tx_) * ty + ty_) * tz + tz_
+  int ox[kStMaxBasis], oy[kStMaxBasis], oz[kStMaxBasis];
+  int tx[kStMaxBasis], ty[kStMaxBasis], tz[kStMaxBasis];
+  const int2 *seq;         // per sequence: (first entry, length)
+  const int32_t *off;      // per entry: byte offset from the lane's own element
+  const double *val;       // per entry: 3 (diagonal) or 9 tensor values
+  const int32_t *wseq;     // FP64: sequence of line (b, x, y) at (b * lx + x) * ly + y;
+                           // null: sequence b (FP32 form: one order per basis site)
+  const uint64_t *fixmask; // FP64: per wave, the lanes whose row the row-list
+                           // launch computes (their z neighbours wrap)
+};
+constexpr bool xv_uniform(int v) { return v == XV_UDIAG || v == XV_UFULL || xv_pair(v); }
 constexpr bool xv_pattern(int v) { return (v >= XV_PDIAG && v <= XV_GFULL) || xv_uniform(v); }
 constexpr bool xv_pattern_lds(int v) {
   return v == XV_PDIAG || v == XV_PFULL || xv_uniform(v);
 }
 constexpr bool xv_typed(int v) { return v == XV_TDIAG || v == XV_TFULL || xv_pattern(v); }
 constexpr bool xv_diag(int v) {
-  return v == XV_TDIAG || v == XV_BDIAG || v == XV_PDIAG || v == XV_GDIAG || v == XV_UDIAG;
+  return v == XV_TDIAG || v == XV_BDIAG || v == XV_PDIAG || v == XV_GDIAG || v == XV_UDIAG ||
+         v == XV_QISO;
 }
 
 // site-class table: CF_STRIDE doubles per class
@@ -105,6 +140,12 @@ struct XView {
   const double *seqtab;       // per sequence: len x (3|9) tensor values (FP64)
   const int32_t *upat;        // pat with each word replaced by its byte offset
                               // (column - row) * bytes per spin of the FP32 copy
+  // pair programs (QISO / QDMI): spair[slice] = (first group, groups) or
+  // (-1, 0); group g: pgrp[g] = slots (k1A, k2A, k1B, k2B), k1B < 0 for one
+  // pair; pval[8 g ..] = (J, DA, DB, 0)
+  const int2 *spair;
+  const int4 *pgrp;
+  const double *pval;
 };
 
 struct KParams {
@@ -143,6 +184,11 @@ struct KParams {
   double t;
   int order_nb;         // basis-major period (1: reference order)
   int64_t order_cells;  // N / order_nb
+  StView st;            // STENCIL step kernels
+  // row-list launch (the FP64 stencil's wrapped rows): thread j computes
+  // device row rowlist[j] (pattern format) instead of row j
+  const int32_t *rowlist;
+  int64_t nrowlist;
 };
 
 // reference spin index of device index d (the noise counter uses it, so the

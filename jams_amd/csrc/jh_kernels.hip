@@ -11,6 +11,8 @@
 // solver.  The FP32 form gathers FP32 spins, multiplies them by the FP64
 // tensors and accumulates in FP64 (one rounding per product and per sum); it
 // may order a row by tensor id so that tensors become wave-uniform.
+#include <algorithm>
+
 #include "jh_device.h"
 #include "jh_internal.h"
 #include "jh_launch.h"
@@ -266,6 +268,82 @@ __device__ __forceinline__ void ufield32(const XView &x, const float *__restrict
   h0 = a0; h1 = a1; h2 = a2;
 }
 
+// D x b into (a0, a1, a2), b = s1 - s2 from FP32 spins, six FP64 multiply-adds
+__device__ __forceinline__ void dmi_add(const double *__restrict__ D, float bx, float by, float bz,
+                                        double &a0, double &a1, double &a2) {
+  const double X = bx, Y = by, Z = bz;
+  a0 = __builtin_fma(D[1], Z, a0); a0 = __builtin_fma(-D[2], Y, a0);
+  a1 = __builtin_fma(D[2], X, a1); a1 = __builtin_fma(-D[0], Z, a1);
+  a2 = __builtin_fma(D[0], Y, a2); a2 = __builtin_fma(-D[1], X, a2);
+}
+
+// FP32 form, isotropic (+ DMI) tensors in slices with a pair program
+// (XV_QISO / XV_QDMI): per group of one or two bond pairs (T, T^T) sharing J,
+// J (s1 + s2 [+ s3 + s4]) with the spin sums in FP32 and J exact in FP64,
+// plus D x (s1 - s2) per pair in FP64 (QDMI).  Other slices take ufield32.
+template <int XVk>
+__device__ __forceinline__ void pairfield32(const XView &x, const float *__restrict__ src,
+                                            int64_t n, int64_t i, const double *tab,
+                                            const int32_t *upat, const uint32_t *gpat,
+                                            double &h0, double &h1, double &h2) {
+  const int sl = __builtin_amdgcn_readfirstlane((int)(i >> 6));
+  const int2 info = x.spair[sl];
+  const int g0 = __builtin_amdgcn_readfirstlane(info.x);
+  if (g0 < 0) {
+    ufield32<XVk>(x, src, n, i, tab, upat, gpat, h0, h1, h2);
+    return;
+  }
+  const int cnt = __builtin_amdgcn_readfirstlane(info.y);
+  const int n2 = cnt & 0xFFFF, n1 = cnt >> 16;
+  const int4 *__restrict__ G = x.pgrp + g0;
+  const double *__restrict__ V = x.pval + (size_t)8 * g0;
+  const uint32_t o = x.pofs[i] + 1;
+  // byte offset of this spin in the gather copy (the copy is < 2 GiB)
+  const uint32_t ib = (uint32_t)i * (uint32_t)(kG32 * sizeof(float));
+  const char *__restrict__ base = reinterpret_cast<const char *>(src);
+  auto spin = [&](int k, float &sx, float &sy, float &sz) {
+    const float *q = reinterpret_cast<const float *>(base + (ib + (uint32_t)upat[o + k]));
+    sx = q[0]; sy = q[1]; sz = q[2];
+  };
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  // groups of two pairs: four gathers each, no branch
+#pragma unroll 2
+  for (int g = 0; g < n2; ++g) {
+    const int4 k = G[g];
+    const double *__restrict__ Vg = V + 8 * g;
+    float x1, y1, z1, x2, y2, z2, x3, y3, z3, x4, y4, z4;
+    spin(k.x, x1, y1, z1);
+    spin(k.y, x2, y2, z2);
+    spin(k.z, x3, y3, z3);
+    spin(k.w, x4, y4, z4);
+    const float p0 = (x1 + x2) + (x3 + x4);
+    const float p1 = (y1 + y2) + (y3 + y4);
+    const float p2 = (z1 + z2) + (z3 + z4);
+    if constexpr (XVk == XV_QDMI) {
+      dmi_add(Vg + 1, x1 - x2, y1 - y2, z1 - z2, a0, a1, a2);
+      dmi_add(Vg + 4, x3 - x4, y3 - y4, z3 - z4, a0, a1, a2);
+    }
+    const double J = Vg[0];
+    a0 = __builtin_fma(J, (double)p0, a0);
+    a1 = __builtin_fma(J, (double)p1, a1);
+    a2 = __builtin_fma(J, (double)p2, a2);
+  }
+  // single pairs
+  for (int g = n2; g < n2 + n1; ++g) {
+    const int4 k = G[g];
+    const double *__restrict__ Vg = V + 8 * g;
+    float x1, y1, z1, x2, y2, z2;
+    spin(k.x, x1, y1, z1);
+    spin(k.y, x2, y2, z2);
+    if constexpr (XVk == XV_QDMI) dmi_add(Vg + 1, x1 - x2, y1 - y2, z1 - z2, a0, a1, a2);
+    const double J = Vg[0];
+    a0 = __builtin_fma(J, (double)(x1 + x2), a0);
+    a1 = __builtin_fma(J, (double)(y1 + y2), a1);
+    a2 = __builtin_fma(J, (double)(z1 + z2), a2);
+  }
+  h0 = a0; h1 = a1; h2 = a2;
+}
+
 // bytes of LDS used by the tensor table (the pattern table follows it)
 __host__ __device__ inline size_t table_bytes(int prec, int xv, int ntab) {
   if (!xv_typed(xv)) return 0;
@@ -273,17 +351,29 @@ __host__ __device__ inline size_t table_bytes(int prec, int xv, int ntab) {
   return (b + 15) & ~(size_t)15;
 }
 
+// Copy `words` 32-bit words global -> LDS (both 16-byte aligned) with
+// 16-byte loads: every workgroup stages the tables, so their load count
+// matters (the five-shell bcc pattern table is 7552 words).
+__device__ __forceinline__ void copy_words(uint32_t *dst, const uint32_t *__restrict__ src,
+                                           int words) {
+  const int w4 = words >> 2;
+  uint4 *d4 = reinterpret_cast<uint4 *>(dst);
+  const uint4 *s4 = reinterpret_cast<const uint4 *>(src);
+  for (int q = threadIdx.x; q < w4; q += blockDim.x) d4[q] = s4[q];
+  for (int q = (w4 << 2) + (int)threadIdx.x; q < words; q += blockDim.x) dst[q] = src[q];
+}
+
 template <int PREC, int XVk>
 __device__ __forceinline__ void load_table(const XView &x, unsigned char *lds) {
   if constexpr (xv_typed(XVk)) {
     const int cnt = x.ntab * (xv_diag(XVk) ? 3 : 9);
-    double *t = reinterpret_cast<double *>(lds);
-    for (int q = threadIdx.x; q < cnt; q += blockDim.x) t[q] = x.tab64[q];
+    copy_words(reinterpret_cast<uint32_t *>(lds), reinterpret_cast<const uint32_t *>(x.tab64),
+               2 * cnt);
   }
   if constexpr (xv_pattern_lds(XVk)) {
     uint32_t *p = reinterpret_cast<uint32_t *>(lds + table_bytes(PREC, XVk, x.ntab));
     const uint32_t *from = xv_uniform(XVk) ? reinterpret_cast<const uint32_t *>(x.upat) : x.pat;
-    for (int q = threadIdx.x; q < x.pat_words; q += blockDim.x) p[q] = from[q];
+    copy_words(p, from, x.pat_words);
   }
 }
 
@@ -302,6 +392,9 @@ __device__ __forceinline__ void exchange_at(const KParams &p, const double *src6
       pat = p.x.pat;
     if constexpr (PREC == PR_FP64)
       pfield64<XVk>(p.x, src64, p.n, i, reinterpret_cast<const double *>(lds), pat, f0, f1, f2);
+    else if constexpr (xv_pair(XVk))
+      pairfield32<XVk>(p.x, src32, p.n, i, reinterpret_cast<const double *>(lds),
+                       reinterpret_cast<const int32_t *>(pat), p.x.pat, f0, f1, f2);
     else if constexpr (xv_uniform(XVk))
       ufield32<XVk>(p.x, src32, p.n, i, reinterpret_cast<const double *>(lds),
                     reinterpret_cast<const int32_t *>(pat), p.x.pat, f0, f1, f2);
@@ -364,15 +457,12 @@ __device__ __forceinline__ void local_term(const KParams &p, int kind, uint32_t 
 }
 
 // Total field (meV) of spin i, terms summed in config order
-// (core/solver.cc:43-57: h = f0; h = h + 1.0 * f_k).
-template <int PREC, int XVk, bool GEN>
-__device__ __forceinline__ void total_field(const KParams &p, const double *src64,
-                                            const float *src32, int64_t i, uint32_t c,
-                                            const unsigned char *lds, double s0, double s1,
-                                            double s2, double t, double &h0, double &h1,
-                                            double &h2) {
-  double x0, x1, x2;
-  exchange_at<PREC, XVk>(p, src64, src32, i, lds, x0, x1, x2);
+// (core/solver.cc:43-57: h = f0; h = h + 1.0 * f_k), with the exchange term's
+// field (x0, x1, x2) already computed.
+template <bool GEN>
+__device__ __forceinline__ void sum_terms(const KParams &p, uint32_t c, int64_t i, double s0,
+                                          double s1, double s2, double t, double x0, double x1,
+                                          double x2, double &h0, double &h1, double &h2) {
   for (int q = 0; q < p.nterms; ++q) {
     double f0, f1, f2;
     const int k = p.kind[q];
@@ -388,6 +478,134 @@ __device__ __forceinline__ void total_field(const KParams &p, const double *src6
     }
   }
   if (p.nterms == 0) { h0 = h1 = h2 = 0.0; }
+}
+
+template <int PREC, int XVk, bool GEN>
+__device__ __forceinline__ void total_field(const KParams &p, const double *src64,
+                                            const float *src32, int64_t i, uint32_t c,
+                                            const unsigned char *lds, double s0, double s1,
+                                            double s2, double t, double &h0, double &h1,
+                                            double &h2) {
+  double x0, x1, x2;
+  exchange_at<PREC, XVk>(p, src64, src32, i, lds, x0, x1, x2);
+  sum_terms<GEN>(p, c, i, s0, s1, s2, t, x0, x1, x2, h0, h1, h2);
+}
+
+// step counter and time of this launch: from the host, or in a graph replay
+// the replay's first step (on the device) plus this step's place in the
+// graph (t = step dt, the host's own expression)
+__device__ __forceinline__ uint64_t launch_step(const KParams &p) {
+  return p.dstep ? *p.dstep + p.step : p.step;
+}
+__device__ __forceinline__ double launch_time(const KParams &p, uint64_t step) {
+  return p.dstep ? (double)step * p.dt : p.t;
+}
+
+// Renormalisation: the reference's unit_vector in FP64; one reciprocal and
+// three products in the FP32-field form (within 1.5 ulp; the predictor and
+// the corrector's recomputation of s* use the same one)
+template <int PREC>
+__device__ __forceinline__ void renorm(double &v0, double &v1, double &v2) {
+  if constexpr (PREC == PR_FP64)
+    unit_vector(v0, v1, v2);
+  else
+    unit_vector_rcp(v0, v1, v2);
+}
+
+// What a corrector reads of its own spin besides s*: s (the step's start)
+// and ds, loaded once.
+struct OwnState {
+  double s0, s1, s2, d0, d1, d2;
+};
+
+// The spin a Heun phase evaluates the field at: s (predictor) or s*
+// (corrector); the corrector also loads its s and ds into `own`.
+template <int PREC, int PHASE>
+__device__ __forceinline__ void phase_spin(const KParams &p, int64_t i, double &s0, double &s1,
+                                           double &s2, OwnState &own) {
+  const int64_t n = p.n;
+  if constexpr (PHASE == 1) {
+    load_spin(p.s, n, i, own.s0, own.s1, own.s2);
+    own.d0 = p.ds[i]; own.d1 = p.ds[n + i]; own.d2 = p.ds[2 * n + i];
+  }
+  if (PREC == PR_FP32 && PHASE == 1 && p.spfree) {
+    // s* = unit(s + dt rhs) as the predictor formed it: 2 ds is its rhs
+    // exactly (ds = rhs / 2; only a subnormal rhs would lose its last bit)
+    s0 = own.s0 + p.dt * (2.0 * own.d0);
+    s1 = own.s1 + p.dt * (2.0 * own.d1);
+    s2 = own.s2 + p.dt * (2.0 * own.d2);
+    renorm<PREC>(s0, s1, s2);
+  } else {
+    load_spin(PHASE == 0 ? p.s : p.sp, n, i, s0, s1, s2);
+  }
+}
+
+// Everything of a Heun phase after the exchange field (x0, x1, x2) of spin i
+// at (s0, s1, s2): the ordered field sum, the thermal field, h / mus, the LLG
+// right-hand side and the phase's update and stores.
+template <int PREC, int PHASE, bool GEN>
+__device__ __forceinline__ void heun_spin(const KParams &p, int64_t i, uint32_t c, uint64_t step,
+                                          double t, double s0, double s1, double s2, double x0,
+                                          double x1, double x2, const OwnState &own) {
+  const int64_t n = p.n;
+  double h0, h1, h2;
+  sum_terms<GEN>(p, c, i, s0, s1, s2, t, x0, x1, x2, h0, h1, h2);
+
+  // thermal field w of this step (solvers/cpu_llg_heun.cc:53-62): one draw
+  // per step serves both phases, so the predictor generates it and leaves it
+  // for the corrector
+  double w0 = 0.0, w1 = 0.0, w2 = 0.0;
+  if (p.noise == NM_PHILOX && PHASE == 0) {
+    double z0, z1, z2, sT;
+    if (p.cls) {
+      normals3(noise_seed(p, c), step, noise_index(p, c, i), z0, z1, z2);
+      sT = cparam(p, c, CF_SQRTT);
+    } else {
+      // one site class: the key schedule stays in scalar registers
+      normals3(p.seed, step, (uint64_t)ref_index(p, i) - p.roff, z0, z1, z2);
+      sT = p.sqrtT;
+    }
+    const double sg = cparam(p, c, CF_SIGMA);
+    w0 = z0 * sg * sT; w1 = z1 * sg * sT; w2 = z2 * sg * sT;
+    p.wbuf[i] = w0; p.wbuf[n + i] = w1; p.wbuf[2 * n + i] = w2;
+  } else if (p.noise == NM_PHILOX) {
+    w0 = p.wbuf[i]; w1 = p.wbuf[n + i]; w2 = p.wbuf[2 * n + i];
+  } else if (p.noise == NM_INJECTED) {
+    w0 = p.winj[i]; w1 = p.winj[n + i]; w2 = p.winj[2 * n + i];
+  }
+
+  // h = w + h / mus  (solvers/cpu_llg_heun.cc:68-81); the FP32-field form
+  // multiplies by one reciprocal
+  const double mus = cparam(p, c, CF_MUS);
+  if constexpr (PREC == PR_FP32) {
+    const double rm = 1.0 / mus;
+    h0 = h0 * rm; h1 = h1 * rm; h2 = h2 * rm;
+    if (p.noise != NM_NONE) { h0 = w0 + h0; h1 = w1 + h1; h2 = w2 + h2; }
+  } else if (p.noise == NM_NONE) {
+    h0 = h0 / mus; h1 = h1 / mus; h2 = h2 / mus;
+  } else {
+    h0 = w0 + h0 / mus; h1 = w1 + h1 / mus; h2 = w2 + h2 / mus;
+  }
+
+  double r0, r1, r2;
+  llg_rhs(s0, s1, s2, h0, h1, h2, cparam(p, c, CF_GYRO), cparam(p, c, CF_ALPHA), r0, r1, r2);
+
+  if constexpr (PHASE == 0) {
+    p.ds[i] = 0.5 * r0; p.ds[n + i] = 0.5 * r1; p.ds[2 * n + i] = 0.5 * r2;
+    double v0 = s0 + p.dt * r0, v1 = s1 + p.dt * r1, v2 = s2 + p.dt * r2;
+    renorm<PREC>(v0, v1, v2);
+    if (!(PREC == PR_FP32 && p.spfree)) store_spin(p.sp, n, i, v0, v1, v2);
+    if constexpr (PREC == PR_FP32) store_g32(p.w32, i, v0, v1, v2);
+  } else {
+    const double d0 = own.d0 + 0.5 * r0;
+    const double d1 = own.d1 + 0.5 * r1;
+    const double d2 = own.d2 + 0.5 * r2;
+    double v0 = own.s0, v1 = own.s1, v2 = own.s2;
+    v0 = v0 + p.dt * d0; v1 = v1 + p.dt * d1; v2 = v2 + p.dt * d2;
+    renorm<PREC>(v0, v1, v2);
+    store_spin(p.snew, n, i, v0, v1, v2);
+    if constexpr (PREC == PR_FP32) store_g32(p.w32, i, v0, v1, v2);
+  }
 }
 
 // Minimum waves per SIMD the compiler keeps room for (VGPR budget): the
@@ -409,89 +627,186 @@ __attribute__((amdgpu_waves_per_eu(kMinWaves<PREC, XVk>))) k_heun(const KParams 
   load_table<PREC, XVk>(p.x, lds);
   if constexpr (xv_typed(XVk)) __syncthreads();
   const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
-  const int64_t i = blk * blockDim.x + threadIdx.x;
-  if (i >= p.n) return;
-  const int64_t n = p.n;
+  int64_t i = blk * blockDim.x + threadIdx.x;
+  if (p.rowlist) {
+    if (i >= p.nrowlist) return;
+    i = p.rowlist[i];
+  } else if (i >= p.n) {
+    return;
+  }
   const uint32_t c = p.cls ? (uint32_t)p.cls[i] : 0u;
-
-  // step counter and time: from the host, or in a graph replay the replay's
-  // first step (on the device) plus this step's place in the graph
-  // (t = step dt, the host's own expression)
-  const uint64_t step = p.dstep ? *p.dstep + p.step : p.step;
-  const double t0 = p.dstep ? (double)step * p.dt : p.t;
-
-  // spin that the field is evaluated at
-  const double *src = PHASE == 0 ? p.s : p.sp;
+  const uint64_t step = launch_step(p);
+  const double t0 = launch_time(p, step);
   double s0, s1, s2;
-  if (PREC == PR_FP32 && PHASE == 1 && p.spfree) {
-    // s* = unit(s + dt rhs) as the predictor formed it: 2 ds is its rhs
-    // exactly (ds = rhs / 2; only a subnormal rhs would lose its last bit)
-    load_spin(p.s, n, i, s0, s1, s2);
-    s0 = s0 + p.dt * (2.0 * p.ds[i]);
-    s1 = s1 + p.dt * (2.0 * p.ds[n + i]);
-    s2 = s2 + p.dt * (2.0 * p.ds[2 * n + i]);
-    unit_vector(s0, s1, s2);
-  } else {
-    load_spin(src, n, i, s0, s1, s2);
-  }
+  OwnState own;
+  phase_spin<PREC, PHASE>(p, i, s0, s1, s2, own);
   const double t = PHASE == 0 ? t0 : t0 + p.dt;
+  double x0, x1, x2;
+  exchange_at<PREC, XVk>(p, PHASE == 0 ? p.s : p.sp, p.g32, i, lds, x0, x1, x2);
+  heun_spin<PREC, PHASE, GEN>(p, i, c, step, t, s0, s1, s2, x0, x1, x2, own);
+}
 
-  // thermal field w of this step (solvers/cpu_llg_heun.cc:53-62): one draw
-  // per step serves both phases, so the predictor generates it and leaves it
-  // for the corrector
-  double w0 = 0.0, w1 = 0.0, w2 = 0.0;
-  auto thermal = [&] {
-    if (p.noise == NM_PHILOX && PHASE == 0) {
-      double z0, z1, z2, sT;
-      if (p.cls) {
-        normals3(noise_seed(p, c), step, noise_index(p, c, i), z0, z1, z2);
-        sT = cparam(p, c, CF_SQRTT);
+// ---------------------------------------------------------------------------
+// STENCIL step kernels (StView in jh_internal.h).
+//
+// LDS tile of E elements, one spin each: FP64 as a z plane (8 E bytes, z of
+// element e at 8 e) followed by an (x, y) plane ((x, y) at 8 E + 16 e), so
+// lanes read consecutive 8- and 16-byte words (no bank conflicts); FP32 as
+// (x, y, z, 0) at 16 e.  Cells outside an open boundary hold zero spins:
+// their products add +-0 to a sum that starts at +0, which leaves it
+// unchanged.
+template <int PREC>
+constexpr int kStElem = PREC == PR_FP64 ? 24 : 16;
+
+__device__ __forceinline__ bool st_wrap(int &a, int l, int periodic) {
+  if (a < 0) {
+    if (!periodic) return false;
+    a += l;
+  } else if (a >= l) {
+    if (!periodic) return false;
+    a -= l;
+  }
+  return true;
+}
+
+// Stage the workgroup's tile: wave w loads tile rows w, w + nw, ... (a row is
+// one (basis site, x, y) line of tz cells along z, lanes along z).
+template <int PREC>
+__device__ __forceinline__ void st_load_tile(const KParams &p, const double *__restrict__ src64,
+                                             const float *__restrict__ src32, int b, int X0,
+                                             int Y0, int Z0, unsigned char *lds) {
+  const StView &st = p.st;
+  const int TX = st.tx[b], TY = st.ty[b], TZ = st.tz[b];
+  const int X1 = X0 + st.ox[b], Y1 = Y0 + st.oy[b], Z1 = Z0 + st.oz[b];
+  const int rows = st.nb * TX * TY;
+  const int E = rows * TZ;
+  const int nw = (int)(blockDim.x >> 6);
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int lane = (int)(threadIdx.x & 63);
+  const int64_t C = (int64_t)st.lx * st.ly * st.lz;
+  const int64_t n = p.n;
+  for (int row = wave; row < rows; row += nw) {
+    const int bb = row / (TX * TY);
+    const int rem = row - bb * (TX * TY);
+    int xx = X1 + rem / TY, yy = Y1 + rem % TY;
+    const bool okxy = st_wrap(xx, st.lx, st.per[0]) && st_wrap(yy, st.ly, st.per[1]);
+    const int64_t rowbase = (int64_t)bb * C + ((int64_t)xx * st.ly + yy) * st.lz;
+    for (int tz = lane; tz < TZ; tz += 64) {
+      int zz = Z1 + tz;
+      const bool ok = okxy && st_wrap(zz, st.lz, st.per[2]);
+      const int e = row * TZ + tz;
+      if constexpr (PREC == PR_FP64) {
+        double x = 0.0, y = 0.0, z = 0.0;
+        if (ok) load_spin(src64, n, rowbase + zz, x, y, z);
+        reinterpret_cast<double *>(lds)[e] = z;
+        reinterpret_cast<double2 *>(lds + (size_t)8 * E)[e] = make_double2(x, y);
       } else {
-        // one site class: the key schedule stays in scalar registers
-        normals3(p.seed, step, (uint64_t)ref_index(p, i) - p.roff, z0, z1, z2);
-        sT = p.sqrtT;
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (ok) {
+          const float *q = src32 + (size_t)kG32 * (rowbase + zz);
+          v = make_float4(q[0], q[1], q[2], 0.f);
+        }
+        reinterpret_cast<float4 *>(lds)[e] = v;
       }
-      const double sg = cparam(p, c, CF_SIGMA);
-      w0 = z0 * sg * sT; w1 = z1 * sg * sT; w2 = z2 * sg * sT;
-      p.wbuf[i] = w0; p.wbuf[n + i] = w1; p.wbuf[2 * n + i] = w2;
-    } else if (p.noise == NM_PHILOX) {
-      w0 = p.wbuf[i]; w1 = p.wbuf[n + i]; w2 = p.wbuf[2 * n + i];
-    } else if (p.noise == NM_INJECTED) {
-      w0 = p.winj[i]; w1 = p.winj[n + i]; w2 = p.winj[2 * n + i];
     }
-  };
-  double h0, h1, h2;
-  total_field<PREC, XVk, GEN>(p, src, p.g32, i, c, lds, s0, s1, s2, t, h0, h1, h2);
-  thermal();
-
-  // h = w + h / mus  (solvers/cpu_llg_heun.cc:68-81)
-  const double mus = cparam(p, c, CF_MUS);
-  if (p.noise == NM_NONE) {
-    h0 = h0 / mus; h1 = h1 / mus; h2 = h2 / mus;
-  } else {
-    h0 = w0 + h0 / mus; h1 = w1 + h1 / mus; h2 = w2 + h2 / mus;
   }
+}
 
-  double r0, r1, r2;
-  llg_rhs(s0, s1, s2, h0, h1, h2, cparam(p, c, CF_GYRO), cparam(p, c, CF_ALPHA), r0, r1, r2);
-
-  if constexpr (PHASE == 0) {
-    p.ds[i] = 0.5 * r0; p.ds[n + i] = 0.5 * r1; p.ds[2 * n + i] = 0.5 * r2;
-    double v0 = s0 + p.dt * r0, v1 = s1 + p.dt * r1, v2 = s2 + p.dt * r2;
-    unit_vector(v0, v1, v2);
-    if (!(PREC == PR_FP32 && p.spfree)) store_spin(p.sp, n, i, v0, v1, v2);
-    if constexpr (PREC == PR_FP32) store_g32(p.w32, i, v0, v1, v2);
-  } else {
-    const double d0 = p.ds[i] + 0.5 * r0;
-    const double d1 = p.ds[n + i] + 0.5 * r1;
-    const double d2 = p.ds[2 * n + i] + 0.5 * r2;
-    double v0, v1, v2;
-    load_spin(p.s, n, i, v0, v1, v2);
-    v0 = v0 + p.dt * d0; v1 = v1 + p.dt * d1; v2 = v2 + p.dt * d2;
-    unit_vector(v0, v1, v2);
-    store_spin(p.snew, n, i, v0, v1, v2);
-    if constexpr (PREC == PR_FP32) store_g32(p.w32, i, v0, v1, v2);
+// Exchange field of the lane whose tile element starts at byte `base`, over
+// the entries of sequence sid.  FP64: the reference order with one rounding
+// per product and per sum; FP32 form: one fused multiply-add per product into
+// FP64 sums.
+template <int PREC, bool DIAG>
+__device__ __forceinline__ void st_field(const StView &st, int sid, const unsigned char *lds,
+                                         uint32_t xybase, uint32_t base, double &h0, double &h1,
+                                         double &h2) {
+  const int2 sq = st.seq[sid];
+  const int first = __builtin_amdgcn_readfirstlane(sq.x);
+  const int len = __builtin_amdgcn_readfirstlane(sq.y);
+  const int32_t *__restrict__ off = st.off + first;
+  constexpr int V = DIAG ? 3 : 9;
+  const double *__restrict__ T = st.val + (size_t)first * V;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+#pragma unroll 8
+  for (int k = 0; k < len; ++k) {
+    const uint32_t a = base + (uint32_t)off[k];
+    const unsigned char *q = lds + a;
+    double sx, sy, sz;
+    if constexpr (PREC == PR_FP64) {
+      sz = *reinterpret_cast<const double *>(q);
+      const double2 v = *reinterpret_cast<const double2 *>(lds + (xybase + 2 * a));
+      sx = v.x; sy = v.y;
+    } else {
+      const float4 v = *reinterpret_cast<const float4 *>(q);
+      sx = v.x; sy = v.y; sz = v.z;
+    }
+    const double *Tk = T + V * k;
+    if constexpr (PREC == PR_FP64) {
+      if constexpr (DIAG) {
+        a0 = a0 + sx * Tk[0];
+        a1 = a1 + sy * Tk[1];
+        a2 = a2 + sz * Tk[2];
+      } else {
+        a0 = a0 + sx * Tk[0]; a0 = a0 + sy * Tk[1]; a0 = a0 + sz * Tk[2];
+        a1 = a1 + sx * Tk[3]; a1 = a1 + sy * Tk[4]; a1 = a1 + sz * Tk[5];
+        a2 = a2 + sx * Tk[6]; a2 = a2 + sy * Tk[7]; a2 = a2 + sz * Tk[8];
+      }
+    } else {
+      if constexpr (DIAG) {
+        a0 = __builtin_fma(sx, Tk[0], a0);
+        a1 = __builtin_fma(sy, Tk[1], a1);
+        a2 = __builtin_fma(sz, Tk[2], a2);
+      } else {
+        a0 = __builtin_fma(sx, Tk[0], a0); a0 = __builtin_fma(sy, Tk[1], a0); a0 = __builtin_fma(sz, Tk[2], a0);
+        a1 = __builtin_fma(sx, Tk[3], a1); a1 = __builtin_fma(sy, Tk[4], a1); a1 = __builtin_fma(sz, Tk[5], a1);
+        a2 = __builtin_fma(sx, Tk[6], a2); a2 = __builtin_fma(sy, Tk[7], a2); a2 = __builtin_fma(sz, Tk[8], a2);
+      }
+    }
   }
+  h0 = a0; h1 = a1; h2 = a2;
+}
+
+// One Heun phase over the STENCIL layout: workgroup -> (basis site b, block
+// of wx x wy lines, 64 z), wave -> line, lane -> z.
+template <int PREC, bool DIAG, int PHASE, bool GEN>
+__global__ void __launch_bounds__(kStMaxBlock) k_heun_st(const KParams p) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  const StView &st = p.st;
+  const int64_t lb = xcd_block(blockIdx.x, gridDim.x);
+  const int bz = (int)(lb % st.nbz);
+  int64_t r = lb / st.nbz;
+  const int by = (int)(r % st.nby);
+  r /= st.nby;
+  const int bx = (int)(r % st.nbx);
+  const int b = (int)(r / st.nbx);
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int lane = (int)(threadIdx.x & 63);
+  const int xi = wave / st.wy, yi = wave - (wave / st.wy) * st.wy;
+  const int X0 = bx * st.wx, Y0 = by * st.wy, Z0 = bz * 64;
+  const int x = X0 + xi, y = Y0 + yi;
+  const int64_t C = (int64_t)st.lx * st.ly * st.lz;
+  const int64_t i = (int64_t)b * C + ((int64_t)x * st.ly + y) * st.lz + Z0 + lane;
+
+  st_load_tile<PREC>(p, PHASE == 0 ? p.s : p.sp, p.g32, b, X0, Y0, Z0, lds);
+  __syncthreads();
+
+  const uint32_t c = p.cls ? (uint32_t)p.cls[i] : 0u;
+  const uint64_t step = launch_step(p);
+  const double t0 = launch_time(p, step);
+  double s0, s1, s2;
+  OwnState own;
+  phase_spin<PREC, PHASE>(p, i, s0, s1, s2, own);
+  const double t = PHASE == 0 ? t0 : t0 + p.dt;
+  const int sid = st.wseq ? st.wseq[((int64_t)b * st.lx + x) * st.ly + y] : b;
+  const uint32_t e0 = (uint32_t)((xi * st.ty[b] + yi) * st.tz[b] + lane);
+  const uint32_t E = (uint32_t)(st.nb * st.tx[b] * st.ty[b] * st.tz[b]);
+  double x0, x1, x2;
+  st_field<PREC, DIAG>(st, sid, lds, 8 * E, e0 * (PREC == PR_FP64 ? 8u : 16u), x0, x1, x2);
+  if (st.fixmask) {
+    const uint64_t fm = st.fixmask[lb * (int64_t)(blockDim.x >> 6) + wave];
+    if ((fm >> lane) & 1ull) return;  // this row is the row-list launch's
+  }
+  heun_spin<PREC, PHASE, GEN>(p, i, c, step, t, s0, s1, s2, x0, x1, x2, own);
 }
 
 // One stage of the classical RK4 step of llg-rk4-gpu
@@ -724,9 +1039,10 @@ static size_t table_lds(const KParams &p, int prec, int xv) {
 
 template <int PREC, int XVk>
 static hipError_t launch_heun_t(const KParams &p, int phase, int bs, hipStream_t st) {
-  const int64_t nb = (p.n + bs - 1) / bs;
   const size_t lds = table_lds(p, PREC, XVk);
   const bool gen = p.has_ac || p.ani_power != 2;
+  const int64_t rows = p.rowlist ? p.nrowlist : p.n;
+  const int64_t nb = (rows + bs - 1) / bs;
   if (phase == 0 && !gen)
     hipLaunchKernelGGL((k_heun<PREC, XVk, 0, false>), dim3((unsigned)nb), dim3(bs), lds, st, p);
   else if (phase == 0)
@@ -756,6 +1072,12 @@ static hipError_t launch_heun_p(const KParams &p, int xv, int phase, int bs, hip
     case XV_UFULL:
       if constexpr (PREC == PR_FP32) return launch_heun_t<PREC, XV_UFULL>(p, phase, bs, st);
       break;
+    case XV_QISO:
+      if constexpr (PREC == PR_FP32) return launch_heun_t<PREC, XV_QISO>(p, phase, bs, st);
+      break;
+    case XV_QDMI:
+      if constexpr (PREC == PR_FP32) return launch_heun_t<PREC, XV_QDMI>(p, phase, bs, st);
+      break;
   }
   return hipErrorInvalidValue;
 }
@@ -764,6 +1086,48 @@ hipError_t launch_heun(const KParams &p, int prec, int xv, int phase, int bs, hi
   if (bs < 64 || bs > kMaxBlock || bs % 64 != 0) return hipErrorInvalidValue;
   return prec == PR_FP64 ? launch_heun_p<PR_FP64>(p, xv, phase, bs, st)
                          : launch_heun_p<PR_FP32>(p, xv, phase, bs, st);
+}
+
+// LDS bytes of a STENCIL workgroup of source basis site b
+size_t st_tile_bytes(const StView &v, int prec, int b) {
+  return (size_t)v.nb * v.tx[b] * v.ty[b] * v.tz[b] * (prec == PR_FP64 ? kStElem<PR_FP64>
+                                                                       : kStElem<PR_FP32>);
+}
+
+template <int PREC, bool DIAG, int PHASE, bool GEN>
+static hipError_t launch_heun_st_k(const KParams &p, size_t lds, hipStream_t st) {
+  const StView &v = p.st;
+  const auto fn = k_heun_st<PREC, DIAG, PHASE, GEN>;
+  if (lds > 64 * 1024) {
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  const int64_t nwg = (int64_t)v.nb * v.nbx * v.nby * v.nbz;
+  hipLaunchKernelGGL(fn, dim3((unsigned)nwg), dim3(64 * v.wx * v.wy), lds, st, p);
+  return hipGetLastError();
+}
+
+template <int PREC, bool DIAG>
+static hipError_t launch_heun_st_t(const KParams &p, int phase, size_t lds, hipStream_t st) {
+  const bool gen = p.has_ac || p.ani_power != 2;
+  if (phase == 0)
+    return gen ? launch_heun_st_k<PREC, DIAG, 0, true>(p, lds, st)
+               : launch_heun_st_k<PREC, DIAG, 0, false>(p, lds, st);
+  return gen ? launch_heun_st_k<PREC, DIAG, 1, true>(p, lds, st)
+             : launch_heun_st_k<PREC, DIAG, 1, false>(p, lds, st);
+}
+
+hipError_t launch_heun_st(const KParams &p, int prec, bool diag, int phase, hipStream_t st) {
+  const StView &v = p.st;
+  if (v.nb < 1 || v.nb > kStMaxBasis || v.wx * v.wy * 64 > kStMaxBlock) return hipErrorInvalidValue;
+  size_t lds = 0;
+  for (int b = 0; b < v.nb; ++b) lds = std::max(lds, st_tile_bytes(v, prec, b));
+  if (prec == PR_FP64)
+    return diag ? launch_heun_st_t<PR_FP64, true>(p, phase, lds, st)
+                : launch_heun_st_t<PR_FP64, false>(p, phase, lds, st);
+  return diag ? launch_heun_st_t<PR_FP32, true>(p, phase, lds, st)
+              : launch_heun_st_t<PR_FP32, false>(p, phase, lds, st);
 }
 
 template <int PREC, int XVk>
@@ -794,6 +1158,12 @@ static hipError_t launch_rk4_p(const KParams &p, int xv, int bs, hipStream_t st)
       break;
     case XV_UFULL:
       if constexpr (PREC == PR_FP32) return launch_rk4_t<PREC, XV_UFULL>(p, bs, st);
+      break;
+    case XV_QISO:
+      if constexpr (PREC == PR_FP32) return launch_rk4_t<PREC, XV_QISO>(p, bs, st);
+      break;
+    case XV_QDMI:
+      if constexpr (PREC == PR_FP32) return launch_rk4_t<PREC, XV_QDMI>(p, bs, st);
       break;
   }
   return hipErrorInvalidValue;
@@ -842,6 +1212,12 @@ static hipError_t launch_field_p(const KParams &p, int xv, int term, double *out
       break;
     case XV_UFULL:
       if constexpr (PREC == PR_FP32) return launch_field_t<PREC, XV_UFULL>(p, term, out, st);
+      break;
+    case XV_QISO:
+      if constexpr (PREC == PR_FP32) return launch_field_t<PREC, XV_QISO>(p, term, out, st);
+      break;
+    case XV_QDMI:
+      if constexpr (PREC == PR_FP32) return launch_field_t<PREC, XV_QDMI>(p, term, out, st);
       break;
   }
   return hipErrorInvalidValue;

@@ -19,7 +19,7 @@ PREC_FP64 = 0
 PREC_FP32_FIELD = 1
 INTEGRATOR_HEUN = 0
 INTEGRATOR_RK4 = 1
-XFMT_AUTO, XFMT_TYPED, XFMT_BLOCK, XFMT_PATTERN, XFMT_PATTERN_GLOBAL = 0, 1, 2, 3, 4
+XFMT_AUTO, XFMT_TYPED, XFMT_BLOCK, XFMT_PATTERN, XFMT_PATTERN_GLOBAL, XFMT_STENCIL = 0, 1, 2, 3, 4, 5
 TERM_EXCHANGE, TERM_ZEEMAN, TERM_UNIAXIAL, TERM_DIPOLE = 0, 1, 2, 3
 
 # every entry point declared in include/jams_hip.h
@@ -29,6 +29,7 @@ ABI = {
     "jh_destroy": (C.c_int, [C.c_void_p]),
     "jh_last_error": (C.c_char_p, [C.c_void_p]),
     "jh_set_sites": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "jh_set_lattice": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
     "jh_set_spins": (C.c_int, [C.c_void_p, C.c_void_p]),
     "jh_get_spins": (C.c_int, [C.c_void_p, C.c_void_p]),
     "jh_set_time_step": (C.c_int, [C.c_void_p, C.c_double]),
@@ -150,6 +151,13 @@ class Engine:
         mus, gyro, alpha = _f64(mus), _f64(gyro), _f64(alpha)
         self.n = int(mus.size)
         self._chk(self.lib.jh_set_sites(self.h, self.n, _ptr(mus), _ptr(gyro), _ptr(alpha)))
+
+    def set_lattice(self, size, periodic, nbasis):
+        """Lattice shape: spin ((x Ly + y) Lz + z) nbasis + b (optional; it
+        lets a crystal's exchange run as a stencil over LDS tiles)."""
+        sz = np.ascontiguousarray(size, dtype=np.int32).reshape(3)
+        pe = np.ascontiguousarray([1 if p else 0 for p in periodic], dtype=np.int32)
+        self._chk(self.lib.jh_set_lattice(self.h, _ptr(sz), _ptr(pe), int(nbasis)))
 
     def set_spins(self, s):
         s = _f64(s, (self.n, 3))

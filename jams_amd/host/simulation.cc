@@ -115,6 +115,10 @@ void initialize(jams_sim *s, const std::vector<std::string> &sources, int device
     engine_check(nullptr, jh_create(device, &s->ctx));
     engine_check(s->ctx, jh_set_sites(s->ctx, n, s->lattice.mus.data(), s->lattice.gyro.data(),
                                       s->lattice.alpha.data()));
+    const int32_t lsize[3] = {s->lattice.size(0), s->lattice.size(1), s->lattice.size(2)};
+    const int32_t lper[3] = {s->lattice.periodic(0) ? 1 : 0, s->lattice.periodic(1) ? 1 : 0,
+                             s->lattice.periodic(2) ? 1 : 0};
+    engine_check(s->ctx, jh_set_lattice(s->ctx, lsize, lper, s->lattice.num_basis_sites()));
     engine_check(s->ctx, jh_set_time_step(s->ctx, s->solver->time_step()));
     engine_check(s->ctx, jh_set_spins(s->ctx, s->lattice.spins.data()));
     s->solver->attach(s->ctx);

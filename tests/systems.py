@@ -56,9 +56,11 @@ class Built:
                 o.add_uniaxial(self.ani_K, self.ani_axis, 2)
         return o
 
-    def engine(self, make, prec=0, fmt=0, use_list=False):
+    def engine(self, make, prec=0, fmt=0, use_list=False, lattice=False):
         e = make()
         e.set_sites(self.mus, self.gyro, self.alpha)
+        if lattice:
+            e.set_lattice(self.lat.size, self.lat.periodic, self.lat.nbasis)
         for k in self.order:
             if k == "exchange":
                 if use_list:

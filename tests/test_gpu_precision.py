@@ -28,8 +28,22 @@ def open_bcc_dmi():
                    ["exchange"], dt=S.dt_ps(0.5e-15), spins="random", seed=3)
 
 
+def bcc_aniso_dmi(L=6, with_csr=True):
+    """bcc L^3 x 2, the five-shell template with each J tensor's zz entry
+    scaled by 0.8: not isotropic + DMI, so no pair programs (the plain
+    wave-uniform path where row patterns apply)."""
+    rows = []
+    for ti, tj, r, J in S.bcc_five_shell_dmi_rows():
+        J = np.array(J, np.float64).reshape(3, 3).copy()
+        J[2, 2] *= 0.8
+        rows.append((ti, tj, r, J))
+    return S.build(S.bcc_lattice(L), rows, 2.0, 0.05, ["exchange"], dt=S.dt_ps(0.5e-15),
+                   spins="random", seed=3, with_csr=with_csr)
+
+
 SYSTEMS = {
     "bcc_dmi_periodic": lambda: S.c3(6, with_csr=True),
+    "bcc_aniso_dmi": bcc_aniso_dmi,
     "bcc_two_shell_diag": lambda: S.c2(6),
     "sc_open_film": open_film,
     "bcc_dmi_open_z": open_bcc_dmi,
@@ -76,14 +90,20 @@ def test_fp32_field_trajectory_angle_c3_small(engine_factory):
 def test_fp32_uniform_slices_are_used(engine_factory):
     """Periodic crystals large enough for row patterns (bcc 16^3 x 2 with five
     shells, bcc 8^3 x 2 with two) get the wave-uniform tensor variant in every
-    slice; an open film, whose every slice mixes surface and interior rows,
-    keeps the per-lane variant; FP64 keeps the reference's row order."""
+    slice, with pair programs where the tensors are isotropic (+ DMI) and
+    without where they are not; an open film, whose every slice mixes surface
+    and interior rows, keeps the per-lane variant; FP64 keeps the reference's
+    row order."""
     e = S.c3(16).engine(engine_factory, prec=1, use_list=True)
     d = e.describe()
-    assert "exchange=pattern-uniform-full" in d and "uniform_slices=128/128" in d, d
+    assert "exchange=pattern-uniform-pairs-dmi" in d and "uniform_slices=128/128" in d, d
+    assert "paired_slices=128 " in d, d
     e = S.c2(8).engine(engine_factory, prec=1)
     d = e.describe()
-    assert "exchange=pattern-uniform-diag" in d and "uniform_slices=16/16" in d, d
+    assert "exchange=pattern-uniform-pairs-iso" in d and "uniform_slices=16/16" in d, d
+    e = bcc_aniso_dmi(16, with_csr=False).engine(engine_factory, prec=1, use_list=True)
+    d = e.describe()
+    assert "exchange=pattern-uniform-full" in d and "paired_slices=0 " in d, d
     e = open_film().engine(engine_factory, prec=1)
     d = e.describe()
     assert "exchange=pattern-diag " in d, d
