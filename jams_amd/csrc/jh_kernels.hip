@@ -513,21 +513,27 @@ __device__ __forceinline__ void renorm(double &v0, double &v1, double &v2) {
 }
 
 // What a corrector reads of its own spin besides s*: s (the step's start)
-// and ds, loaded once.
+// and ds.  The FP32-field corrector loads them before its field (it forms s*
+// from them) and keeps them; the FP64 one loads them after its field (fewer
+// live registers across the neighbour loop).
 struct OwnState {
   double s0, s1, s2, d0, d1, d2;
 };
+template <int PREC>
+constexpr bool kEarlyOwn = PREC == PR_FP32;
+__device__ __forceinline__ void load_own(const KParams &p, int64_t i, OwnState &own) {
+  const int64_t n = p.n;
+  load_spin(p.s, n, i, own.s0, own.s1, own.s2);
+  own.d0 = p.ds[i]; own.d1 = p.ds[n + i]; own.d2 = p.ds[2 * n + i];
+}
 
 // The spin a Heun phase evaluates the field at: s (predictor) or s*
-// (corrector); the corrector also loads its s and ds into `own`.
+// (corrector).
 template <int PREC, int PHASE>
 __device__ __forceinline__ void phase_spin(const KParams &p, int64_t i, double &s0, double &s1,
                                            double &s2, OwnState &own) {
   const int64_t n = p.n;
-  if constexpr (PHASE == 1) {
-    load_spin(p.s, n, i, own.s0, own.s1, own.s2);
-    own.d0 = p.ds[i]; own.d1 = p.ds[n + i]; own.d2 = p.ds[2 * n + i];
-  }
+  if constexpr (PHASE == 1 && kEarlyOwn<PREC>) load_own(p, i, own);
   if (PREC == PR_FP32 && PHASE == 1 && p.spfree) {
     // s* = unit(s + dt rhs) as the predictor formed it: 2 ds is its rhs
     // exactly (ds = rhs / 2; only a subnormal rhs would lose its last bit)
@@ -597,10 +603,12 @@ __device__ __forceinline__ void heun_spin(const KParams &p, int64_t i, uint32_t 
     if (!(PREC == PR_FP32 && p.spfree)) store_spin(p.sp, n, i, v0, v1, v2);
     if constexpr (PREC == PR_FP32) store_g32(p.w32, i, v0, v1, v2);
   } else {
-    const double d0 = own.d0 + 0.5 * r0;
-    const double d1 = own.d1 + 0.5 * r1;
-    const double d2 = own.d2 + 0.5 * r2;
-    double v0 = own.s0, v1 = own.s1, v2 = own.s2;
+    OwnState o = own;
+    if constexpr (!kEarlyOwn<PREC>) load_own(p, i, o);
+    const double d0 = o.d0 + 0.5 * r0;
+    const double d1 = o.d1 + 0.5 * r1;
+    const double d2 = o.d2 + 0.5 * r2;
+    double v0 = o.s0, v1 = o.s1, v2 = o.s2;
     v0 = v0 + p.dt * d0; v1 = v1 + p.dt * d1; v2 = v2 + p.dt * d2;
     renorm<PREC>(v0, v1, v2);
     store_spin(p.snew, n, i, v0, v1, v2);
