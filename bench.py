@@ -257,16 +257,19 @@ def measure_c3(L: int, steps: int, warmup: int, reps: int, accuracy_steps: int,
                tail: int = 1000, every: int = 10):
     """The headline system (bcc L^3 x 2, five shells with DMI, T = 100 K).
 
-    Accuracy (untimed, accuracy_steps > 0, run first): from the ordered start (+z) and
-    from the random start, FP64, FP32-field and FP64 from the start rotated by
-    ~1e-12 rad per spin, all on the same noise stream: the per-spin angle and
-    |dm| of each against FP64 after accuracy_steps, and <|m|> and the exchange
-    energy over the last `tail` steps (a statistical comparison, which is the
-    meaningful one where the trajectory itself amplifies a 1e-12 difference).
+    For FP64 and then the FP32-field form:
 
-    Speed: `steps` Heun steps after `warmup`, from the workload's own random
-    start, in FP64 and then in the FP32-field form, plus HIP-event times of
-    each phase kernel; then the median of five 500-step runs of each."""
+    Accuracy (untimed, accuracy_steps > 0): from the ordered start (+z) and
+    from the random start (FP64 also from the start rotated by ~1e-12 rad per
+    spin), all on the same noise stream; afterwards the per-spin angle and
+    |dm| of each against FP64 after accuracy_steps, and <|m|> and the
+    exchange energy over the last `tail` steps (a statistical comparison,
+    which is the meaningful one where the trajectory itself amplifies a
+    1e-12 difference).
+
+    Speed: `steps` Heun steps after `warmup` from the workload's own random
+    start, HIP-event times of each phase kernel, and the median of five
+    500-step runs."""
     import numpy as np
 
     from jams_amd import configs
@@ -282,39 +285,76 @@ def measure_c3(L: int, steps: int, warmup: int, reps: int, accuracy_steps: int,
     mus = sim.sites()[0]
     out = {"n": n, "setup_s": setup_s, "temperature_K": 100.0}
     log(f"  set-up {setup_s:.1f} s")
-    # the untimed accuracy runs first: the timed steps then start on a GPU
-    # that has been busy (clocks settled), as in a long simulation
+    # Per precision: its untimed accuracy trajectories, then its timed steps
+    # (the timed region then starts on a GPU that has been busy with the same
+    # device layout, as in a long simulation, not after the re-layout's idle
+    # second).
+    rng = np.random.default_rng(11)
+    m_of = lambda s: (mus[:, None] * s).sum(0) / mus.sum()
+    starts = {"ordered": np.tile([0.0, 0.0, 1.0], (n, 1)), "random": s_random}
+    perturbed = {}
+    for label, s0 in starts.items():
+        d = rng.normal(size=s0.shape) * 1e-12
+        sp = s0 + d - np.einsum("ij,ij->i", d, s0)[:, None] * s0
+        perturbed[label] = sp / np.linalg.norm(sp, axis=1)[:, None]
+    tail = min(tail, accuracy_steps) if accuracy_steps > 0 else 0
+    finals = {label: {} for label in starts}
+    stats = {label: {} for label in starts}
+    runs_of = {"fp64": (("fp64", False), ("fp64_perturbed", True)),
+               "fp32_field": (("fp32_field", False),)}
+    for name, mode in (("fp64", PREC_FP64), ("fp32_field", PREC_FP32_FIELD)):
+        eng.set_precision(mode)
+        if accuracy_steps > 0:
+            for label, s0 in starts.items():
+                for run, pert in runs_of[name]:
+                    eng.set_spins(perturbed[label] if pert else s0)
+                    eng.set_iteration(0)
+                    mabs, energy = _tail_run(eng, mus, accuracy_steps, tail, every)
+                    finals[label][run] = eng.get_spins()
+                    mm, me = _blocked_mean(mabs)
+                    em, ee = _blocked_mean(energy)
+                    stats[label][run] = {"mean_abs_m": mm, "mean_abs_m_stderr": me,
+                                         "exchange_energy_meV_per_spin": em,
+                                         "exchange_energy_stderr": ee}
+        eng.set_spins(s_random)
+        eng.set_iteration(0)
+        eng.step(warmup)
+        eng.synchronize()
+        w0 = time.perf_counter()
+        ms = eng.time_steps(steps)
+        wall = time.perf_counter() - w0
+        out[name] = {"value": n * steps / (ms * 1e-3), "ms": ms, "ms_per_step": ms / steps,
+                     "wall_s": wall, "phases": _phase_times(eng, reps),
+                     "describe": eng.describe()}
+        log(f"  C3 {name}: {ms / steps:.4f} ms/step, {out[name]['value']:.4e} spin-updates/s; "
+            f"{eng.describe()}")
+        # steady state: the median of five runs of 500 steps each (reported
+        # beside the headline's timing)
+        eng.set_spins(s_random)
+        eng.set_iteration(0)
+        eng.step(50)
+        runs = sorted(eng.time_steps(500) / 500 for _ in range(5))
+        out[name]["median5_ms_per_step"] = runs[2]
+        out[name]["median5_value"] = n / (runs[2] * 1e-3)
+        log(f"  C3 {name} steady state: median of 5 x 500 steps {runs[2]:.4f} ms/step")
+    out["speedup_fp32_vs_fp64"] = out["fp32_field"]["value"] / out["fp64"]["value"]
+    out["median5_speedup_fp32_vs_fp64"] = (out["fp64"]["median5_ms_per_step"]
+                                           / out["fp32_field"]["median5_ms_per_step"])
+    log(f"  C3 speed-up FP32-field / FP64: {out['speedup_fp32_vs_fp64']:.3f} "
+        f"(steady state {out['median5_speedup_fp32_vs_fp64']:.3f})")
     if accuracy_steps > 0:
-        tail = min(tail, accuracy_steps)
-        rng = np.random.default_rng(11)
-        m_of = lambda s: (mus[:, None] * s).sum(0) / mus.sum()
         acc = {"steps": accuracy_steps, "tail_steps": tail, "sample_every": every}
-        for label, s0 in (("ordered", np.tile([0.0, 0.0, 1.0], (n, 1))), ("random", s_random)):
-            d = rng.normal(size=s0.shape) * 1e-12
-            s_pert = s0 + d - np.einsum("ij,ij->i", d, s0)[:, None] * s0
-            s_pert /= np.linalg.norm(s_pert, axis=1)[:, None]
-            finals, stats = {}, {}
-            for name, mode, init in (("fp64", PREC_FP64, s0), ("fp32_field", PREC_FP32_FIELD, s0),
-                                     ("fp64_perturbed", PREC_FP64, s_pert)):
-                eng.set_precision(mode)
-                eng.set_spins(init)
-                eng.set_iteration(0)
-                mabs, energy = _tail_run(eng, mus, accuracy_steps, tail, every)
-                finals[name] = eng.get_spins()
-                mm, me = _blocked_mean(mabs)
-                em, ee = _blocked_mean(energy)
-                stats[name] = {"mean_abs_m": mm, "mean_abs_m_stderr": me,
-                               "exchange_energy_meV_per_spin": em,
-                               "exchange_energy_stderr": ee}
-            m64 = m_of(finals["fp64"])
-            st = {"m_fp64": float(np.linalg.norm(m64)), "tail": stats}
+        for label in starts:
+            fin, stt = finals[label], stats[label]
+            m64 = m_of(fin["fp64"])
+            st = {"m_fp64": float(np.linalg.norm(m64)), "tail": stt}
             for other in ("fp32_field", "fp64_perturbed"):
-                ang = _angles(finals["fp64"], finals[other])
-                a, b = stats[other], stats["fp64"]
+                ang = _angles(fin["fp64"], fin[other])
+                a, b = stt[other], stt["fp64"]
                 st[other + "_vs_fp64"] = {
                     "angle_rad_max": float(ang.max()),
                     "angle_rad_rms": float(np.sqrt((ang ** 2).mean())),
-                    "dm_abs": float(np.linalg.norm(m64 - m_of(finals[other]))),
+                    "dm_abs": float(np.linalg.norm(m64 - m_of(fin[other]))),
                     "tail_mean_abs_m_diff": a["mean_abs_m"] - b["mean_abs_m"],
                     "tail_mean_abs_m_diff_sigma": float(np.hypot(a["mean_abs_m_stderr"],
                                                                  b["mean_abs_m_stderr"])),
@@ -330,35 +370,6 @@ def measure_c3(L: int, steps: int, warmup: int, reps: int, accuracy_steps: int,
                 f"energy diff {f['tail_energy_diff']:.2e} +- {f['tail_energy_diff_sigma']:.1e} meV; "
                 f"FP64 1e-12-perturbed vs FP64 max angle {c['angle_rad_max']:.3e}, |dm| {c['dm_abs']:.3e}")
         out["accuracy"] = acc
-    for name, mode in (("fp64", PREC_FP64), ("fp32_field", PREC_FP32_FIELD)):
-        eng.set_precision(mode)
-        eng.set_spins(s_random)
-        eng.set_iteration(0)
-        eng.step(warmup)
-        eng.synchronize()
-        w0 = time.perf_counter()
-        ms = eng.time_steps(steps)
-        wall = time.perf_counter() - w0
-        out[name] = {"value": n * steps / (ms * 1e-3), "ms": ms, "ms_per_step": ms / steps,
-                     "wall_s": wall, "phases": _phase_times(eng, reps),
-                     "describe": eng.describe()}
-        log(f"  C3 {name}: {ms / steps:.4f} ms/step, {out[name]['value']:.4e} spin-updates/s; "
-            f"{eng.describe()}")
-    out["speedup_fp32_vs_fp64"] = out["fp32_field"]["value"] / out["fp64"]["value"]
-    log(f"  C3 speed-up FP32-field / FP64: {out['speedup_fp32_vs_fp64']:.3f}")
-    # steady state: the median of five runs of 500 steps each (untimed by the
-    # headline contract, reported beside it)
-    for name, mode in (("fp64", PREC_FP64), ("fp32_field", PREC_FP32_FIELD)):
-        eng.set_precision(mode)
-        eng.set_spins(s_random)
-        eng.set_iteration(0)
-        eng.step(50)
-        runs = sorted(eng.time_steps(500) / 500 for _ in range(5))
-        out[name]["median5_ms_per_step"] = runs[2]
-        out[name]["median5_value"] = n / (runs[2] * 1e-3)
-        log(f"  C3 {name} steady state: median of 5 x 500 steps {runs[2]:.4f} ms/step")
-    out["median5_speedup_fp32_vs_fp64"] = (out["fp64"]["median5_ms_per_step"]
-                                           / out["fp32_field"]["median5_ms_per_step"])
     sim.close()
     return out
 
@@ -421,7 +432,7 @@ def measure_dipole(steps: int, reps: int):
     return out
 
 
-def measure_batch(steps: int, nrep: int = 256):
+def measure_batch(steps: int, nrep: int = 256, graph_steps: int = 50):
     """The batching form: nrep independent replicas (sizes drawn from a seed,
     sc L^3 or bcc (L/2)^3 x 2 with L in 8..48, T_r = 1000 r/(nrep-1) K,
     B_r = 2 r/(nrep-1) T along z) in one context, one launch per Heun phase
@@ -444,7 +455,7 @@ def measure_batch(steps: int, nrep: int = 256):
         # launches vs graph replay over the same short run, state restored after
         s0 = e.get_spins()
         ms_plain = e.time_steps(200)
-        e.set_graph_steps(50)
+        e.set_graph_steps(graph_steps)
         e.set_spins(s0)
         e.set_iteration(0)
         e.step(0)
@@ -520,6 +531,8 @@ def main() -> int:
     ap.add_argument("--no-c5", action="store_true")
     ap.add_argument("--c4-steps", type=int, default=5000)
     ap.add_argument("--c5-steps", type=int, default=2000)
+    ap.add_argument("--c4-graph-steps", type=int, default=50,
+                    help="steps per captured hipGraph in the replica batch (0: plain launches)")
     # FP32-field and perturbed-FP64 runs of this many steps against FP64,
     # untimed, on the headline system and on C2 (0: none)
     ap.add_argument("--accuracy-steps", type=int, default=10000)
@@ -571,7 +584,7 @@ def main() -> int:
     c4 = None
     if not (extras or args.no_c4):
         log(f"C4 batching form: 256 replicas, {args.c4_steps} steps FP64 and FP32-field")
-        c4 = measure_batch(args.c4_steps)
+        c4 = measure_batch(args.c4_steps, graph_steps=args.c4_graph_steps)
     c5 = None
     if not (extras or args.no_c5):
         log(f"C5 dipolar film: sc 256x256x8 open, {args.c5_steps} steps FP64 and FP32 transforms")

@@ -1054,12 +1054,15 @@ void prepare_exchange(jh_ctx *c) {
         if (!seqtab.empty())
           HIPCHK(hipMemcpy(c->seqtab.p, seqtab.data(), sizeof(double) * seqtab.size(), hipMemcpyHostToDevice));
         c->xv = diag ? XV_UDIAG : XV_UFULL;
-        // pair programs (isotropic + DMI tensors), when most slices have one
+        // pair programs (isotropic + DMI tensors), when most slices have one;
+        // with isotropic tensors alone (no DMI) the plain wave-uniform
+        // variant is kept: its three products per neighbour leave the pair
+        // sums little to save
         std::vector<int32_t> spair, pgrp;
         std::vector<double> pval;
         bool dmi = false;
         const int64_t npr = kPairPrograms ? build_pairs(x, seq_ids, slice_seq, spair, pgrp, pval, dmi) : 0;
-        if (npr * 2 >= c->nslices && !pgrp.empty()) {
+        if (npr * 2 >= c->nslices && !pgrp.empty() && dmi) {
           c->spair.alloc(spair.size());
           HIPCHK(hipMemcpy(c->spair.p, spair.data(), sizeof(int32_t) * spair.size(), hipMemcpyHostToDevice));
           c->pgrp.alloc(pgrp.size());
@@ -1362,14 +1365,6 @@ KParams make_params(jh_ctx *c) {
   p.x.spair = reinterpret_cast<const int2 *>(c->spair.p);
   p.x.pgrp = reinterpret_cast<const int4 *>(c->pgrp.p);
   p.x.pval = c->pval.p;
-  if (c->st_on) {
-    p.st = c->st;
-    p.st.seq = reinterpret_cast<const int2 *>(c->st_seq.p);
-    p.st.off = c->st_off.p;
-    p.st.val = c->st_val.p;
-    p.st.wseq = c->st_wseq.p;
-    p.st.fixmask = c->st_fixmask.p;
-  }
   p.rowlist = nullptr;
   p.nrowlist = 0;
   p.nterms = (int)c->terms.size();
@@ -1424,7 +1419,13 @@ void launch_phase(jh_ctx *c, const KParams &p, int phase) {
     HIPCHK(launch_heun(p, c->prec, c->xv, phase, c->launch_block(), c->stream));
     return;
   }
-  HIPCHK(launch_heun_st(p, c->prec, xv_diag(c->xv), phase, c->stream));
+  StView v = c->st;
+  v.seq = reinterpret_cast<const int2 *>(c->st_seq.p);
+  v.off = c->st_off.p;
+  v.val = c->st_val.p;
+  v.wseq = c->st_wseq.p;
+  v.fixmask = c->st_fixmask.p;
+  HIPCHK(launch_heun_st(p, v, c->prec, xv_diag(c->xv), phase, c->stream));
   if (c->st_nfix > 0) {
     KParams q = p;
     q.rowlist = c->st_rows.p;

@@ -184,7 +184,6 @@ struct KParams {
   double t;
   int order_nb;         // basis-major period (1: reference order)
   int64_t order_cells;  // N / order_nb
-  StView st;            // STENCIL step kernels
   // row-list launch (the FP64 stencil's wrapped rows): thread j computes
   // device row rowlist[j] (pattern format) instead of row j
   const int32_t *rowlist;

@@ -680,10 +680,10 @@ __device__ __forceinline__ bool st_wrap(int &a, int l, int periodic) {
 // Stage the workgroup's tile: wave w loads tile rows w, w + nw, ... (a row is
 // one (basis site, x, y) line of tz cells along z, lanes along z).
 template <int PREC>
-__device__ __forceinline__ void st_load_tile(const KParams &p, const double *__restrict__ src64,
+__device__ __forceinline__ void st_load_tile(const KParams &p, const StView &st,
+                                             const double *__restrict__ src64,
                                              const float *__restrict__ src32, int b, int X0,
                                              int Y0, int Z0, unsigned char *lds) {
-  const StView &st = p.st;
   const int TX = st.tx[b], TY = st.ty[b], TZ = st.tz[b];
   const int X1 = X0 + st.ox[b], Y1 = Y0 + st.oy[b], Z1 = Z0 + st.oz[b];
   const int rows = st.nb * TX * TY;
@@ -777,9 +777,8 @@ __device__ __forceinline__ void st_field(const StView &st, int sid, const unsign
 // One Heun phase over the STENCIL layout: workgroup -> (basis site b, block
 // of wx x wy lines, 64 z), wave -> line, lane -> z.
 template <int PREC, bool DIAG, int PHASE, bool GEN>
-__global__ void __launch_bounds__(kStMaxBlock) k_heun_st(const KParams p) {
+__global__ void __launch_bounds__(kStMaxBlock) k_heun_st(const KParams p, const StView st) {
   extern __shared__ __align__(16) unsigned char lds[];
-  const StView &st = p.st;
   const int64_t lb = xcd_block(blockIdx.x, gridDim.x);
   const int bz = (int)(lb % st.nbz);
   int64_t r = lb / st.nbz;
@@ -795,7 +794,7 @@ __global__ void __launch_bounds__(kStMaxBlock) k_heun_st(const KParams p) {
   const int64_t C = (int64_t)st.lx * st.ly * st.lz;
   const int64_t i = (int64_t)b * C + ((int64_t)x * st.ly + y) * st.lz + Z0 + lane;
 
-  st_load_tile<PREC>(p, PHASE == 0 ? p.s : p.sp, p.g32, b, X0, Y0, Z0, lds);
+  st_load_tile<PREC>(p, st, PHASE == 0 ? p.s : p.sp, p.g32, b, X0, Y0, Z0, lds);
   __syncthreads();
 
   const uint32_t c = p.cls ? (uint32_t)p.cls[i] : 0u;
@@ -1103,8 +1102,8 @@ size_t st_tile_bytes(const StView &v, int prec, int b) {
 }
 
 template <int PREC, bool DIAG, int PHASE, bool GEN>
-static hipError_t launch_heun_st_k(const KParams &p, size_t lds, hipStream_t st) {
-  const StView &v = p.st;
+static hipError_t launch_heun_st_k(const KParams &p, const StView &v, size_t lds,
+                                   hipStream_t st) {
   const auto fn = k_heun_st<PREC, DIAG, PHASE, GEN>;
   if (lds > 64 * 1024) {
     const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
@@ -1112,30 +1111,31 @@ static hipError_t launch_heun_st_k(const KParams &p, size_t lds, hipStream_t st)
     if (e != hipSuccess) return e;
   }
   const int64_t nwg = (int64_t)v.nb * v.nbx * v.nby * v.nbz;
-  hipLaunchKernelGGL(fn, dim3((unsigned)nwg), dim3(64 * v.wx * v.wy), lds, st, p);
+  hipLaunchKernelGGL(fn, dim3((unsigned)nwg), dim3(64 * v.wx * v.wy), lds, st, p, v);
   return hipGetLastError();
 }
 
 template <int PREC, bool DIAG>
-static hipError_t launch_heun_st_t(const KParams &p, int phase, size_t lds, hipStream_t st) {
+static hipError_t launch_heun_st_t(const KParams &p, const StView &v, int phase, size_t lds,
+                                   hipStream_t st) {
   const bool gen = p.has_ac || p.ani_power != 2;
   if (phase == 0)
-    return gen ? launch_heun_st_k<PREC, DIAG, 0, true>(p, lds, st)
-               : launch_heun_st_k<PREC, DIAG, 0, false>(p, lds, st);
-  return gen ? launch_heun_st_k<PREC, DIAG, 1, true>(p, lds, st)
-             : launch_heun_st_k<PREC, DIAG, 1, false>(p, lds, st);
+    return gen ? launch_heun_st_k<PREC, DIAG, 0, true>(p, v, lds, st)
+               : launch_heun_st_k<PREC, DIAG, 0, false>(p, v, lds, st);
+  return gen ? launch_heun_st_k<PREC, DIAG, 1, true>(p, v, lds, st)
+             : launch_heun_st_k<PREC, DIAG, 1, false>(p, v, lds, st);
 }
 
-hipError_t launch_heun_st(const KParams &p, int prec, bool diag, int phase, hipStream_t st) {
-  const StView &v = p.st;
+hipError_t launch_heun_st(const KParams &p, const StView &v, int prec, bool diag, int phase,
+                          hipStream_t st) {
   if (v.nb < 1 || v.nb > kStMaxBasis || v.wx * v.wy * 64 > kStMaxBlock) return hipErrorInvalidValue;
   size_t lds = 0;
   for (int b = 0; b < v.nb; ++b) lds = std::max(lds, st_tile_bytes(v, prec, b));
   if (prec == PR_FP64)
-    return diag ? launch_heun_st_t<PR_FP64, true>(p, phase, lds, st)
-                : launch_heun_st_t<PR_FP64, false>(p, phase, lds, st);
-  return diag ? launch_heun_st_t<PR_FP32, true>(p, phase, lds, st)
-              : launch_heun_st_t<PR_FP32, false>(p, phase, lds, st);
+    return diag ? launch_heun_st_t<PR_FP64, true>(p, v, phase, lds, st)
+                : launch_heun_st_t<PR_FP64, false>(p, v, phase, lds, st);
+  return diag ? launch_heun_st_t<PR_FP32, true>(p, v, phase, lds, st)
+              : launch_heun_st_t<PR_FP32, false>(p, v, phase, lds, st);
 }
 
 template <int PREC, int XVk>

@@ -9,8 +9,9 @@ namespace jh {
 
 // bs: threads per workgroup of the step kernels (64 .. 512, multiple of 64)
 hipError_t launch_heun(const KParams &p, int prec, int xv, int phase, int bs, hipStream_t st);
-// one Heun phase over the STENCIL layout (p.st), full or diagonal tensors
-hipError_t launch_heun_st(const KParams &p, int prec, bool diag, int phase, hipStream_t st);
+// one Heun phase over the STENCIL layout v, full or diagonal tensors
+hipError_t launch_heun_st(const KParams &p, const StView &v, int prec, bool diag, int phase,
+                          hipStream_t st);
 // LDS bytes of a STENCIL workgroup of source basis site b
 size_t st_tile_bytes(const StView &v, int prec, int b);
 // one RK4 stage (p.stage)

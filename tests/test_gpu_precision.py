@@ -90,8 +90,8 @@ def test_fp32_field_trajectory_angle_c3_small(engine_factory):
 def test_fp32_uniform_slices_are_used(engine_factory):
     """Periodic crystals large enough for row patterns (bcc 16^3 x 2 with five
     shells, bcc 8^3 x 2 with two) get the wave-uniform tensor variant in every
-    slice, with pair programs where the tensors are isotropic (+ DMI) and
-    without where they are not; an open film, whose every slice mixes surface
+    slice, with pair programs where the tensors are isotropic + DMI and
+    without where they are isotropic alone or anisotropic; an open film, whose every slice mixes surface
     and interior rows, keeps the per-lane variant; FP64 keeps the reference's
     row order."""
     e = S.c3(16).engine(engine_factory, prec=1, use_list=True)
@@ -100,7 +100,7 @@ def test_fp32_uniform_slices_are_used(engine_factory):
     assert "paired_slices=128 " in d, d
     e = S.c2(8).engine(engine_factory, prec=1)
     d = e.describe()
-    assert "exchange=pattern-uniform-pairs-iso" in d and "uniform_slices=16/16" in d, d
+    assert "exchange=pattern-uniform-diag" in d and "uniform_slices=16/16" in d, d
     e = bcc_aniso_dmi(16, with_csr=False).engine(engine_factory, prec=1, use_list=True)
     d = e.describe()
     assert "exchange=pattern-uniform-full" in d and "paired_slices=0 " in d, d
