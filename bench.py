@@ -10,7 +10,9 @@ the solver, and runs on one GPU.
 
 A "step" is one full Heun step (predictor + corrector, both field
 evaluations, in-kernel noise).  value = spins x steps / event time of the
-step loop, inputs resident in HBM.
+step loop, inputs resident in HBM.  Before the warm-up, 0.3 s of untimed steps
+let the GPU clock recover from the host-side work that precedes each timing
+(set-up, spin copies).
 
 The precision section runs the same system for --accuracy-steps steps in
 FP64, in the FP32-field form and in FP64 from a state rotated by ~1e-12 rad,
@@ -56,6 +58,7 @@ def measure(sim_text: str, steps: int, warmup: int, reps: int = 200):
     eng = sim.engine()
     setup_s = time.perf_counter() - t0
     log(f"  set-up {setup_s:.1f} s: {eng.describe()}")
+    _settle(eng)
     eng.step(warmup)
     eng.synchronize()
     w0 = time.perf_counter()
@@ -227,6 +230,23 @@ def _phase_times(eng, reps):
     return phases
 
 
+SETTLE_S = 0.3
+
+
+def _settle(eng, seconds: float = SETTLE_S, chunk: int = 20) -> int:
+    """Untimed steps for `seconds` of GPU time before a warm-up: the GPU's
+    clock falls during host-side work (set-up, spin copies) and takes a few
+    hundred milliseconds of load to come back, which a warm-up of a few steps
+    does not cover.  Returns the steps taken."""
+    done = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        eng.step(chunk)
+        eng.synchronize()
+        done += chunk
+    return done
+
+
 def _tail_run(eng, mus, steps, tail, every):
     """steps Heun steps; over the last `tail` of them, every `every` steps,
     |m| and the exchange energy per spin (meV)."""
@@ -267,8 +287,9 @@ def measure_c3(L: int, steps: int, warmup: int, reps: int, accuracy_steps: int,
     which is the meaningful one where the trajectory itself amplifies a
     1e-12 difference).
 
-    Speed: `steps` Heun steps after `warmup` from the workload's own random
-    start, HIP-event times of each phase kernel, and the median of five
+    Speed: from the workload's own random start, 0.3 s of untimed steps (the
+    clock's recovery after the spin copy), `warmup` steps, then `steps` Heun
+    steps timed; HIP-event times of each phase kernel; and the median of five
     500-step runs."""
     import numpy as np
 
@@ -318,14 +339,15 @@ def measure_c3(L: int, steps: int, warmup: int, reps: int, accuracy_steps: int,
                                          "exchange_energy_stderr": ee}
         eng.set_spins(s_random)
         eng.set_iteration(0)
+        settle = _settle(eng)
         eng.step(warmup)
         eng.synchronize()
         w0 = time.perf_counter()
         ms = eng.time_steps(steps)
         wall = time.perf_counter() - w0
         out[name] = {"value": n * steps / (ms * 1e-3), "ms": ms, "ms_per_step": ms / steps,
-                     "wall_s": wall, "phases": _phase_times(eng, reps),
-                     "describe": eng.describe()}
+                     "wall_s": wall, "settle_steps": settle,
+                     "phases": _phase_times(eng, reps), "describe": eng.describe()}
         log(f"  C3 {name}: {ms / steps:.4f} ms/step, {out[name]['value']:.4e} spin-updates/s; "
             f"{eng.describe()}")
         # steady state: the median of five runs of 500 steps each (reported
@@ -395,7 +417,6 @@ def measure_dipole(steps: int, reps: int):
     mus = sim.sites()[0]
     dip = [t for t in range(eng.num_terms()) if eng.term_kind(t) == 3][0]
     p_c = 512 * 512 * (16 // 2 + 1)
-    b_formula = 48.0 * n + 6 * p_c * 16 + 12 * p_c * 16
     out = {"n": n, "steps": steps, "setup_s": setup_s}
     finals = {}
     for name, mode in (("fp64", PREC_FP64), ("fp32_field", PREC_FP32_FIELD)):
@@ -408,12 +429,15 @@ def measure_dipole(steps: int, reps: int):
         eng.set_spins(s0)
         t_dip = eng.time_term(dip, reps)
         b_dip = eng.term_bytes(dip)
+        # the scope table's fixed formula for one evaluation, 48 N + 6 P_c c
+        # (tensor) + 12 P_c c (k-space spin and field, written and read),
+        # P_c = 512 x 512 x (16/2 + 1), c the bytes of one complex element
+        # (16 for the FP64 transforms, 8 for the FP32 ones)
+        cb = 16 if mode == PREC_FP64 else 8
+        b_formula = 48.0 * n + 6 * p_c * cb + 12 * p_c * cb
         out[name] = {"value": n * steps / (ms * 1e-3), "ms_per_step": ms / steps,
                      "dipole_eval_ms": t_dip, "dipole_eval_bytes": b_dip,
                      "dipole_eval_gbs": b_dip / (t_dip * 1e-3) / 1e9,
-                     # the scope table's fixed formula for one evaluation,
-                     # 48 N + 6 P_c 16 (tensor) + 12 P_c 16 (k-space spin and
-                     # field, written and read), P_c = 512 x 512 x (16/2 + 1)
                      "dipole_eval_bytes_formula": b_formula,
                      "dipole_eval_gbs_formula": b_formula / (t_dip * 1e-3) / 1e9,
                      "engine": eng.describe()}
@@ -466,8 +490,10 @@ def measure_batch(steps: int, nrep: int = 256, graph_steps: int = 50):
                      "ms_per_step_plain_launches": ms_plain / 200, "setup_s": setup_s,
                      "autotune": tune.splitlines()[-1], "engine": e.describe(),
                      "m_coldest": float(m[0]), "m_hottest": float(m[-1])}
-        log(f"  C4 {name}: {b.n} spins in {nrep} replicas, {ms / steps:.4f} ms/step with graph "
-            f"({ms_plain / 200:.4f} plain), {out[name]['value']:.4e} spin-updates/s; "
+        how = f"as {graph_steps}-step graph replays" if graph_steps > 0 else "in plain launches"
+        out[name]["graph_steps"] = graph_steps
+        log(f"  C4 {name}: {b.n} spins in {nrep} replicas, {ms / steps:.4f} ms/step {how} "
+            f"({ms_plain / 200:.4f} plain, 200 steps), {out[name]['value']:.4e} spin-updates/s; "
             f"{tune.splitlines()[-1]}; |m| coldest {m[0]:.4f} hottest {m[-1]:.4f}")
         b.close()
     out["speedup_fp32_vs_fp64"] = out["fp32_field"]["value"] / out["fp64"]["value"]
@@ -609,6 +635,7 @@ def main() -> int:
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
+        "settle_s": SETTLE_S,
         "ms_per_step": h["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",

@@ -319,8 +319,8 @@ Functional make_functional(const std::string &name, const std::vector<double> &p
   const int want = expected_parameters(name);
   if (want < 0) throw RuntimeError("unknown exchange functional: " + name);
   if ((int)p.size() != want)
-    throw RuntimeError("exchange functional '" + name + "' expects " + std::to_string(want) +
-                       " parameters, got " + std::to_string(p.size()));
+    throw RuntimeError("the '" + name + "' exchange functional takes " + std::to_string(want) +
+                       " parameters (" + std::to_string(p.size()) + " given)");
   const double tol = kLatticeTolerance;
   const auto nonzero = [&](int k, const char *what) {
     if (approximately_zero(p[(size_t)k], tol))
@@ -339,7 +339,7 @@ Functional make_functional(const std::string &name, const std::vector<double> &p
       const double r = norm(rij);
       const double kr = 2 * kF * (r - r0);
       if (approximately_zero(kr, kLatticeTolerance))
-        throw RuntimeError("exchange functional rkky is singular for k_F*(r-r0) = 0");
+        throw RuntimeError("rkky exchange functional: the form is singular where 2 k_F (r - r0) = 0");
       return -J0 * (kr * std::cos(kr) - std::sin(kr)) / (kr * kr * kr * kr);
     };
   }

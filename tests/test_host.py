@@ -437,7 +437,7 @@ def test_exchange_functional_symmetry_checks_and_errors():
     for a, b in zip(forced.exchange_csr(), both.exchange_csr()):
         assert np.array_equal(a, b)
     for bad, msg in [('("A", "A", "cubic", 0.9, [1.0])', "unknown exchange functional"),
-                     ('("A", "A", "gaussian", 0.9, [1.0, 1.0])', "expects 3 parameters"),
+                     ('("A", "A", "gaussian", 0.9, [1.0, 1.0])', "takes 3 parameters"),
                      ('("A", "A", "gaussian", 0.9, [1.0, 1.0, 0.0])', "non-zero parameter"),
                      ('("A", "A", "step", 5.0, [1.0, 1.0])', "larger than the maximum cutoff"),
                      ('("A", "A", "step", 0.9, [1.0, 1.0]), ("A", "A", "step", 0.9, [1.0, 1.0])',
