@@ -68,8 +68,15 @@ extern "C" {
 #define JH_XFMT_STENCIL 5     /* PATTERN, and the Heun step kernels stage each
                                  workgroup's neighbourhood in LDS: a crystal
                                  given by jh_set_lattice whose rows all
-                                 translate one stencil per basis site (AUTO
-                                 takes it where it applies)                      */
+                                 translate one stencil per basis site (taken
+                                 when asked for or chosen by jh_autotune)        */
+#define JH_XFMT_LINE 6        /* PATTERN, and the Heun step kernels hold one
+                                 whole z line per wavefront, loading each
+                                 neighbour line once and reading neighbours
+                                 along z by lane rotation: a periodic crystal
+                                 given by jh_set_lattice, Lz = 64 or 128, at
+                                 most two basis sites (taken when asked for or
+                                 chosen by jh_autotune)                          */
 
 /* term kinds reported by jh_term_kind */
 #define JH_TERM_EXCHANGE 0
@@ -100,7 +107,8 @@ int jh_set_sites(jh_ctx *ctx, int64_t n, const double *mus, const double *gyro,
  * ((x*size[1] + y)*size[2] + z)*nbasis + b, periodic[a] != 0 on periodic
  * axes (Lattice::generate_supercell, core/lattice.cc:577-749, :622-649;
  * boundaries :987-1007).  Optional: it lets a crystal's exchange run as a
- * stencil over LDS tiles (JH_XFMT_STENCIL). */
+ * stencil over LDS tiles (JH_XFMT_STENCIL) or over whole z lines held in
+ * registers (JH_XFMT_LINE). */
 int jh_set_lattice(jh_ctx *ctx, const int32_t size[3], const int32_t periodic[3], int nbasis);
 int jh_set_spins(jh_ctx *ctx, const double *s);
 int jh_get_spins(jh_ctx *ctx, double *s);

@@ -274,6 +274,14 @@ struct jh_ctx {
   DevBuf<uint64_t> st_fixmask;
   int64_t st_nfix = 0;
   int st_nseq = 0;
+  // LINE step kernels (also over the PATTERN format; the FP64 row list is
+  // st_rows / st_nfix)
+  bool ln_on = false;
+  LnView ln{};
+  DevBuf<int32_t> ln_seq, ln_prog, ln_wseq;
+  DevBuf<double> ln_val;
+  DevBuf<uint64_t> ln_fixmask;
+  int ln_nseq = 0, ln_ngroups = 0;
 
   // replicas: independent systems in consecutive index ranges
   int nrep = 0;
@@ -717,44 +725,45 @@ int64_t build_pairs(const XHost &x, const std::vector<std::vector<uint32_t>> &se
   return np;
 }
 
-// ---- STENCIL layout ------------------------------------------------------
+// ---- crystal stencils (STENCIL and LINE layouts) ---------------------------
 //
-// Available when the context holds one crystal whose shape jh_set_lattice
-// gave, whose device order is basis-major with the lattice's basis sites,
-// with Lz a multiple of 64, and whose exchange rows are all the translates of
-// one stencil per basis site (every neighbour (dx, dy, dz, b', tensor) of the
-// row at the middle of the lattice, wrapped on periodic axes and dropped
-// outside open ones).  Builds the sequences, the tile geometry and, in FP64,
-// the rows the row-list launch computes; returns false (and leaves the
-// context without it) otherwise.
-bool build_stencil(jh_ctx *c, const XHost &x, bool diag) {
+// Both apply to a context holding one crystal whose shape jh_set_lattice
+// gave, in basis-major device order, whose exchange rows are all the
+// translates of one stencil per basis site: every neighbour (dx, dy, dz, b',
+// tensor) of the row at the middle of the lattice, wrapped on periodic axes
+// and dropped outside open ones.
+
+struct StEnt {
+  int d[3];
+  int bb;
+  int t;
+};
+
+// The stencil of each basis site (from its middle row) and its extent;
+// false if the context is no such crystal.
+static bool crystal_stencil(jh_ctx *c, const XHost &x, std::vector<std::vector<StEnt>> &sten,
+                            int lo[kStMaxBasis][3], int hi[kStMaxBasis][3]) {
   if (!c->have_lattice || c->nrep > 0 || c->integrator != JH_INTEGRATOR_HEUN) return false;
   const int L[3] = {c->lsize[0], c->lsize[1], c->lsize[2]};
   const int nb = c->lnb;
   const int64_t C = (int64_t)L[0] * L[1] * L[2];
   if (nb < 1 || nb > kStMaxBasis || C * nb != c->n || L[2] % 64 != 0) return false;
   if (c->order_nb != nb && !(nb == 1 && c->order_nb == 1)) return false;
-  const bool fp64 = c->prec == PR_FP64;
-  const int elem = fp64 ? 24 : 16;
-  struct Ent { int d[3]; int bb; int t; };
-  std::vector<std::vector<Ent>> sten((size_t)nb);
-  int lo[kStMaxBasis][3], hi[kStMaxBasis][3];
-  auto decomp = [&](int64_t j, int cell[3], int &bb) {
-    bb = (int)(j % nb);
-    int64_t q = j / nb;
-    cell[2] = (int)(q % L[2]);
-    q /= L[2];
-    cell[1] = (int)(q % L[1]);
-    cell[0] = (int)(q / L[1]);
-  };
+  sten.assign((size_t)nb, {});
   for (int b = 0; b < nb; ++b) {
     const int mid[3] = {L[0] / 2, L[1] / 2, L[2] / 2};
     const int64_t i = (((int64_t)mid[0] * L[1] + mid[1]) * L[2] + mid[2]) * nb + b;
     for (int a = 0; a < 3; ++a) { lo[b][a] = 0; hi[b][a] = 0; }
     for (int64_t q = x.ptr[i]; q < x.ptr[i + 1]; ++q) {
-      Ent e;
+      StEnt e;
+      const int64_t j = x.col[q];
+      e.bb = (int)(j % nb);
+      int64_t r = j / nb;
       int cell[3];
-      decomp(x.col[q], cell, e.bb);
+      cell[2] = (int)(r % L[2]);
+      r /= L[2];
+      cell[1] = (int)(r % L[1]);
+      cell[0] = (int)(r / L[1]);
       for (int a = 0; a < 3; ++a) {
         int d = cell[a] - mid[a];
         if (c->lper[a]) {
@@ -772,6 +781,124 @@ bool build_stencil(jh_ctx *c, const XHost &x, bool diag) {
       sten[b].push_back(e);
     }
   }
+  return true;
+}
+
+// reference column of entry e of the row at `cell`, or -1 outside an open
+// boundary
+static int64_t stencil_col(const jh_ctx *c, const StEnt &e, const int cell[3]) {
+  const int *L = c->lsize;
+  int cc[3];
+  for (int a = 0; a < 3; ++a) {
+    cc[a] = cell[a] + e.d[a];
+    if (cc[a] < 0 || cc[a] >= L[a]) {
+      if (!c->lper[a]) return -1;
+      cc[a] = (cc[a] + L[a]) % L[a];
+    }
+  }
+  return (((int64_t)cc[0] * L[1] + cc[1]) * L[2] + cc[2]) * c->lnb + e.bb;
+}
+
+// FP64 sequences: per line (b, x, y) the column order of its rows away from
+// the z ends (where no z neighbour wraps), entries outside an open boundary
+// last; wseq[(b lx + x) ly + y] = sequence id, seq_basis[id] = b.
+static bool line_orders(const jh_ctx *c, const std::vector<std::vector<StEnt>> &sten,
+                        std::vector<std::vector<int>> &seqs, std::vector<int32_t> &wseq,
+                        std::vector<int> &seq_basis) {
+  const int *L = c->lsize;
+  const int nb = c->lnb;
+  wseq.assign((size_t)nb * L[0] * L[1], -1);
+  seqs.clear();
+  seq_basis.clear();
+  std::map<std::vector<int>, int> ids;
+  for (int b = 0; b < nb; ++b)
+    for (int xx = 0; xx < L[0]; ++xx)
+      for (int yy = 0; yy < L[1]; ++yy) {
+        const int cell[3] = {xx, yy, L[2] / 2};
+        const auto &sb = sten[b];
+        std::vector<std::pair<int64_t, int>> keyed(sb.size());
+        for (size_t k = 0; k < sb.size(); ++k) {
+          const int64_t j = stencil_col(c, sb[k], cell);
+          keyed[k] = {j < 0 ? INT64_MAX : j, (int)k};
+        }
+        std::sort(keyed.begin(), keyed.end());
+        std::vector<int> o(sb.size() + 1);
+        o[0] = b;  // the key carries the basis site (entries are per basis)
+        for (size_t k = 0; k < sb.size(); ++k) o[k + 1] = keyed[k].second;
+        auto it = ids.find(o);
+        int id;
+        if (it == ids.end()) {
+          id = (int)seqs.size();
+          ids.emplace(o, id);
+          seqs.emplace_back(o.begin() + 1, o.end());
+          seq_basis.push_back(b);
+        } else {
+          id = it->second;
+        }
+        wseq[((size_t)b * L[0] + xx) * L[1] + yy] = id;
+      }
+  return seqs.size() <= 4096;
+}
+
+// Every row must be its stencil's translate (else false); with `fp64`, rows
+// whose column order is not their line's sequence (a z neighbour wraps) are
+// marked in fix.
+static bool check_rows(const jh_ctx *c, const XHost &x, const std::vector<std::vector<StEnt>> &sten,
+                       const std::vector<std::vector<int>> &seqs, const std::vector<int32_t> &wseq,
+                       bool fp64, std::vector<uint8_t> &fix) {
+  const int *L = c->lsize;
+  const int nb = c->lnb;
+  const int64_t C = (int64_t)L[0] * L[1] * L[2];
+  const int64_t n = c->n;
+  fix.assign((size_t)n, 0);
+  std::atomic<int> bad{0};
+  parallel_for(n, [&](int64_t d) {
+    const int b = (int)(d / C);
+    const int64_t cellidx = d % C;
+    int cell[3];
+    cell[2] = (int)(cellidx % L[2]);
+    cell[1] = (int)((cellidx / L[2]) % L[1]);
+    cell[0] = (int)(cellidx / ((int64_t)L[2] * L[1]));
+    const int64_t i = cellidx * nb + b;
+    const auto &sb = sten[b];
+    const std::vector<int> &o =
+        fp64 ? seqs[wseq[((size_t)b * L[0] + cell[0]) * L[1] + cell[1]]] : seqs[b];
+    thread_local std::vector<std::pair<int64_t, int32_t>> got;
+    got.clear();
+    bool ordered = true;
+    int64_t last = -1;
+    for (int k : o) {
+      const int64_t j = stencil_col(c, sb[k], cell);
+      if (j < 0) continue;
+      if (j <= last) ordered = false;
+      last = j;
+      got.push_back({j, sb[k].t});
+    }
+    if (!ordered) std::sort(got.begin(), got.end());
+    const int64_t len = x.ptr[i + 1] - x.ptr[i];
+    bool ok = len == (int64_t)got.size();
+    for (int64_t q = 0; ok && q < len; ++q)
+      ok = got[q].first == x.col[x.ptr[i] + q] && got[q].second == x.typ[x.ptr[i] + q];
+    if (!ok) bad.store(1, std::memory_order_relaxed);
+    if (fp64 && !ordered) fix[d] = 1;
+  });
+  return bad.load() == 0;
+}
+
+// ---- STENCIL layout ------------------------------------------------------
+//
+// Lz a multiple of 64.  Builds the sequences, the tile geometry and, in
+// FP64, the rows the row-list launch computes; returns false (and leaves the
+// context without it) otherwise.
+bool build_stencil(jh_ctx *c, const XHost &x, bool diag) {
+  std::vector<std::vector<StEnt>> sten;
+  int lo[kStMaxBasis][3], hi[kStMaxBasis][3];
+  if (!crystal_stencil(c, x, sten, lo, hi)) return false;
+  const int L[3] = {c->lsize[0], c->lsize[1], c->lsize[2]};
+  const int nb = c->lnb;
+  const int64_t C = (int64_t)L[0] * L[1] * L[2];
+  const bool fp64 = c->prec == PR_FP64;
+  const int elem = fp64 ? 24 : 16;
   // workgroup shape: with jh_set_block_size the shape of that many threads,
   // else the most waves whose tile leaves the LDS for two workgroups per CU
   // (one workgroup's tile load then overlaps the other's sums), else one
@@ -801,105 +928,25 @@ bool build_stencil(jh_ctx *c, const XHost &x, bool diag) {
     v.ty[b] = wy + hi[b][1] - lo[b][1];
     v.tz[b] = 64 + hi[b][2] - lo[b][2];
   }
-  // column of entry e of row (b, x, y, z), or -1 outside an open boundary
-  auto col_of = [&](const Ent &e, const int cell[3]) -> int64_t {
-    int cc[3];
-    for (int a = 0; a < 3; ++a) {
-      cc[a] = cell[a] + e.d[a];
-      if (cc[a] < 0 || cc[a] >= L[a]) {
-        if (!c->lper[a]) return -1;
-        cc[a] = (cc[a] + L[a]) % L[a];
-      }
-    }
-    return (((int64_t)cc[0] * L[1] + cc[1]) * L[2] + cc[2]) * nb + e.bb;
-  };
   // sequences: FP32 form, the stencil order of each basis site (any order
   // serves); FP64, per line (b, x, y) the column order of its rows away from
-  // the z ends (where no z neighbour wraps), entries outside an open
-  // boundary last
+  // the z ends
   std::vector<std::vector<int>> seqs;
   std::vector<int32_t> wseq;
+  std::vector<int> seq_basis;
   if (!fp64) {
     for (int b = 0; b < nb; ++b) {
       std::vector<int> o(sten[b].size());
       for (size_t k = 0; k < o.size(); ++k) o[k] = (int)k;
       seqs.push_back(o);
+      seq_basis.push_back(b);
     }
-  } else {
-    wseq.assign((size_t)nb * L[0] * L[1], -1);
-    std::map<std::vector<int>, int> ids;
-    for (int b = 0; b < nb; ++b)
-      for (int xx = 0; xx < L[0]; ++xx)
-        for (int yy = 0; yy < L[1]; ++yy) {
-          const int cell[3] = {xx, yy, L[2] / 2};
-          const auto &sb = sten[b];
-          std::vector<std::pair<int64_t, int>> keyed(sb.size());
-          for (size_t k = 0; k < sb.size(); ++k) {
-            const int64_t j = col_of(sb[k], cell);
-            keyed[k] = {j < 0 ? INT64_MAX : j, (int)k};
-          }
-          std::sort(keyed.begin(), keyed.end());
-          std::vector<int> o(sb.size() + 1);
-          o[0] = b;  // the key carries the basis site (entries are per basis)
-          for (size_t k = 0; k < sb.size(); ++k) o[k + 1] = keyed[k].second;
-          auto it = ids.find(o);
-          int id;
-          if (it == ids.end()) {
-            id = (int)seqs.size();
-            ids.emplace(o, id);
-            seqs.emplace_back(o.begin() + 1, o.end());
-          } else {
-            id = it->second;
-          }
-          wseq[((size_t)b * L[0] + xx) * L[1] + yy] = id;
-        }
-    if (seqs.size() > 4096) return false;
+  } else if (!line_orders(c, sten, seqs, wseq, seq_basis)) {
+    return false;
   }
-  std::vector<int> seq_basis(seqs.size());
-  if (!fp64) {
-    for (int b = 0; b < nb; ++b) seq_basis[b] = b;
-  } else {
-    for (int b = 0; b < nb; ++b)
-      for (int xx = 0; xx < L[0]; ++xx)
-        for (int yy = 0; yy < L[1]; ++yy)
-          seq_basis[wseq[((size_t)b * L[0] + xx) * L[1] + yy]] = b;
-  }
-  // every row must be its stencil's translate; FP64 rows whose column order
-  // is not their line's sequence (a z neighbour wraps) go to the row list
+  std::vector<uint8_t> fix;
+  if (!check_rows(c, x, sten, seqs, wseq, fp64, fix)) return false;
   const int64_t n = c->n;
-  std::vector<uint8_t> fix((size_t)n, 0);
-  std::atomic<int> bad{0};
-  parallel_for(n, [&](int64_t d) {
-    const int b = (int)(d / C);
-    const int64_t cellidx = d % C;
-    int cell[3];
-    cell[2] = (int)(cellidx % L[2]);
-    cell[1] = (int)((cellidx / L[2]) % L[1]);
-    cell[0] = (int)(cellidx / ((int64_t)L[2] * L[1]));
-    const int64_t i = cellidx * nb + b;
-    const auto &sb = sten[b];
-    const std::vector<int> &o =
-        fp64 ? seqs[wseq[((size_t)b * L[0] + cell[0]) * L[1] + cell[1]]] : seqs[b];
-    thread_local std::vector<std::pair<int64_t, int32_t>> got;
-    got.clear();
-    bool ordered = true;
-    int64_t last = -1;
-    for (int k : o) {
-      const int64_t j = col_of(sb[k], cell);
-      if (j < 0) continue;
-      if (j <= last) ordered = false;
-      last = j;
-      got.push_back({j, sb[k].t});
-    }
-    if (!ordered) std::sort(got.begin(), got.end());
-    const int64_t len = x.ptr[i + 1] - x.ptr[i];
-    bool ok = len == (int64_t)got.size();
-    for (int64_t q = 0; ok && q < len; ++q)
-      ok = got[q].first == x.col[x.ptr[i] + q] && got[q].second == x.typ[x.ptr[i] + q];
-    if (!ok) bad.store(1, std::memory_order_relaxed);
-    if (fp64 && !ordered) fix[d] = 1;
-  });
-  if (bad.load()) return false;
   // tables
   std::vector<int32_t> seqv, offv;
   std::vector<double> valv;
@@ -909,7 +956,7 @@ bool build_stencil(jh_ctx *c, const XHost &x, bool diag) {
     seqv.push_back((int32_t)offv.size());
     seqv.push_back((int32_t)seqs[s].size());
     for (int k : seqs[s]) {
-      const Ent &e = sten[b][k];
+      const StEnt &e = sten[b][k];
       const int64_t el = (((int64_t)e.bb * v.tx[b] + e.d[0] - v.ox[b]) * v.ty[b] + e.d[1] - v.oy[b]) *
                              v.tz[b] + e.d[2] - v.oz[b];
       // FP64: byte offset in the z plane (the (x, y) plane's is twice it);
@@ -963,9 +1010,202 @@ bool build_stencil(jh_ctx *c, const XHost &x, bool diag) {
   return true;
 }
 
+// ---- LINE layout ---------------------------------------------------------
+//
+// A fully periodic crystal with Lz = 64 or 128 and at most two basis sites
+// whose stencils reach at most two cells along z.  Each sequence becomes a
+// program of groups, every group loading exactly two lines (a repeat pads a
+// group that needs one) and then summing the entries that read them, in the
+// sequence's order.  FP64: the per-line column orders and the row list as
+// in STENCIL; FP32 form: one sequence per basis site, entries grouped by
+// column so that each line is loaded once.
+bool build_lines(jh_ctx *c, const XHost &x, bool diag) {
+  std::vector<std::vector<StEnt>> sten;
+  int lo[kStMaxBasis][3], hi[kStMaxBasis][3];
+  if (!crystal_stencil(c, x, sten, lo, hi)) return false;
+  const int L[3] = {c->lsize[0], c->lsize[1], c->lsize[2]};
+  const int nb = c->lnb;
+  if (nb > kLnSlots || (L[2] != 64 && L[2] != 128)) return false;
+  if (!c->lper[0] || !c->lper[1] || !c->lper[2]) return false;
+  const int P = L[2] / 64;
+  for (int b = 0; b < nb; ++b) {
+    if (lo[b][2] < -2 || hi[b][2] > 2) return false;
+    for (int a = 0; a < 2; ++a)
+      if (lo[b][a] < -2000 || hi[b][a] > 2000) return false;
+  }
+  const bool fp64 = c->prec == PR_FP64;
+  std::vector<std::vector<int>> seqs;
+  std::vector<int32_t> wseq;
+  std::vector<int> seq_basis;
+  if (!fp64) {
+    // FP32 form: column by column (dx, dy ascending), so each column's lines
+    // are loaded once
+    for (int b = 0; b < nb; ++b) {
+      std::vector<int> o(sten[b].size());
+      for (size_t k = 0; k < o.size(); ++k) o[k] = (int)k;
+      const auto &sb = sten[b];
+      std::stable_sort(o.begin(), o.end(), [&](int p, int q) {
+        if (sb[p].d[0] != sb[q].d[0]) return sb[p].d[0] < sb[q].d[0];
+        if (sb[p].d[1] != sb[q].d[1]) return sb[p].d[1] < sb[q].d[1];
+        return sb[p].bb < sb[q].bb;
+      });
+      seqs.push_back(o);
+      seq_basis.push_back(b);
+    }
+  } else if (!line_orders(c, sten, seqs, wseq, seq_basis)) {
+    return false;
+  }
+  std::vector<uint8_t> fix;
+  if (!check_rows(c, x, sten, seqs, wseq, fp64, fix)) return false;
+  // programs: per sequence a list of groups of kLnGroupWords words (a
+  // bitmask of the entry codes present, the index of its first entry's
+  // values, two unused words, the two line words).  The code of an entry is
+  // slot | (dz + 2) << 1, and a group's entries are summed in ascending code
+  // order, so each code occurs at most once in a group (it names one
+  // neighbour).  FP64: a group is one (dx, dy) column with the line of basis
+  // site b' in slot b', whose column order (z' then b') is the code order;
+  // FP32 form: columns are packed two lines to a group.
+  const int V = diag ? 3 : 9;
+  std::vector<int32_t> seqv, prog;
+  std::vector<double> valv;
+  int ngroups = 0;
+  auto line_word = [](const StEnt &e) {
+    return (int32_t)(e.bb | ((e.d[0] + 2048) << 4) | ((e.d[1] + 2048) << 16));
+  };
+  for (size_t s = 0; s < seqs.size(); ++s) {
+    const int b = seq_basis[s];
+    const auto &sb = sten[b];
+    seqv.push_back((int32_t)(prog.size() / kLnGroupWords));
+    int groups = 0;
+    const auto &o = seqs[s];
+    std::vector<int32_t> lines;           // the group's lines by slot (-1: free)
+    std::vector<std::pair<int, const StEnt *>> ents;  // (code, entry)
+    bool ok = true;
+    auto close_group = [&]() {
+      if (ents.empty()) return;
+      int32_t fill = -1;
+      for (int32_t lw : lines)
+        if (lw != -1) fill = lw;
+      int32_t w[kLnGroupWords] = {0};
+      uint32_t mask = 0;
+      // FP64 keeps the sequence order, which must be the code order
+      for (size_t k = 1; k < ents.size(); ++k)
+        if (ents[k].first <= ents[k - 1].first) {
+          if (fp64) ok = false;
+        }
+      if (!fp64) std::sort(ents.begin(), ents.end(),
+                           [](const auto &p, const auto &q) { return p.first < q.first; });
+      w[1] = (int32_t)(valv.size() / V);
+      for (const auto &pe : ents) {
+        if (mask & (1u << pe.first)) ok = false;  // one neighbour per code
+        mask |= 1u << pe.first;
+        const StEnt &e = *pe.second;
+        for (int m = 0; m < V; ++m) valv.push_back(diag ? x.tab[9 * e.t + 4 * m] : x.tab[9 * e.t + m]);
+      }
+      w[0] = (int32_t)mask;
+      for (int k = 0; k < kLnSlots; ++k) w[4 + k] = lines[k] != -1 ? lines[k] : fill;
+      prog.insert(prog.end(), w, w + kLnGroupWords);
+      ++groups;
+      ents.clear();
+      lines.assign(kLnSlots, -1);
+    };
+    lines.assign(kLnSlots, -1);
+    size_t k = 0;
+    while (k < o.size()) {
+      // the run of consecutive entries in one (dx, dy) column
+      size_t q = k;
+      while (q < o.size() && sb[o[q]].d[0] == sb[o[k]].d[0] && sb[o[q]].d[1] == sb[o[k]].d[1]) ++q;
+      // the slots its lines take: FP64 slot b'; FP32 the free slots in turn
+      std::vector<int32_t> want;
+      for (size_t u = k; u < q; ++u) {
+        const int32_t lw = line_word(sb[o[u]]);
+        if (std::find(want.begin(), want.end(), lw) == want.end()) want.push_back(lw);
+      }
+      auto place = [&](bool dry) {
+        std::vector<int32_t> L = lines;
+        for (int32_t lw : want) {
+          if (std::find(L.begin(), L.end(), lw) != L.end()) continue;
+          const int bb = lw & 15;
+          int slot = -1;
+          if (fp64) {
+            slot = bb;
+            if (L[slot] != -1) return false;
+          } else {
+            for (int u = 0; u < kLnSlots && slot < 0; ++u)
+              if (L[u] == -1) slot = u;
+            if (slot < 0) return false;
+          }
+          L[slot] = lw;
+        }
+        if (!dry) lines = L;
+        return true;
+      };
+      // FP64: one column per group
+      if (fp64 || !place(true)) close_group();
+      if (!place(false)) return false;
+      for (size_t u = k; u < q; ++u) {
+        const StEnt &e = sb[o[u]];
+        const int slot = (int)(std::find(lines.begin(), lines.end(), line_word(e)) - lines.begin());
+        ents.push_back({slot | ((e.d[2] + 2) << 1), &e});
+      }
+      k = q;
+    }
+    close_group();
+    if (!ok) return false;
+    seqv.push_back(groups);
+    ngroups += groups;
+  }
+  // one spare entry of values: the kernels read the next entry's values
+  // while summing the current one
+  for (int m = 0; m < V; ++m) valv.push_back(0.0);
+  if (valv.size() / V >= ((size_t)1 << 23) || prog.size() >= ((size_t)1 << 30)) return false;
+  const int64_t n = c->n;
+  c->ln_seq.alloc(seqv.size());
+  HIPCHK(hipMemcpy(c->ln_seq.p, seqv.data(), sizeof(int32_t) * seqv.size(), hipMemcpyHostToDevice));
+  c->ln_prog.alloc(std::max<size_t>(prog.size(), 1));
+  if (!prog.empty())
+    HIPCHK(hipMemcpy(c->ln_prog.p, prog.data(), sizeof(int32_t) * prog.size(), hipMemcpyHostToDevice));
+  c->ln_val.alloc(std::max<size_t>(valv.size(), 1));
+  if (!valv.empty())
+    HIPCHK(hipMemcpy(c->ln_val.p, valv.data(), sizeof(double) * valv.size(), hipMemcpyHostToDevice));
+  c->ln_wseq.release();
+  c->ln_fixmask.release();
+  c->st_rows.release();
+  c->st_nfix = 0;
+  if (fp64) {
+    c->ln_wseq.alloc(wseq.size());
+    HIPCHK(hipMemcpy(c->ln_wseq.p, wseq.data(), sizeof(int32_t) * wseq.size(), hipMemcpyHostToDevice));
+    std::vector<int32_t> rows;
+    for (int64_t d = 0; d < n; ++d)
+      if (fix[d]) rows.push_back((int32_t)d);
+    if (!rows.empty()) {
+      // per line, P words: bit l of word h is cell P l + h
+      std::vector<uint64_t> mask((size_t)(n / L[2]) * P, 0);
+      for (int32_t d : rows) {
+        const int64_t line = d / L[2];
+        const int z = (int)(d % L[2]);
+        mask[(size_t)(line * P + z % P)] |= 1ull << (z / P);
+      }
+      c->st_rows.alloc(rows.size());
+      HIPCHK(hipMemcpy(c->st_rows.p, rows.data(), sizeof(int32_t) * rows.size(), hipMemcpyHostToDevice));
+      c->ln_fixmask.alloc(mask.size());
+      HIPCHK(hipMemcpy(c->ln_fixmask.p, mask.data(), sizeof(uint64_t) * mask.size(), hipMemcpyHostToDevice));
+      c->st_nfix = (int64_t)rows.size();
+    }
+  }
+  LnView v{};
+  v.lx = L[0]; v.ly = L[1]; v.lz = L[2]; v.nb = nb;
+  v.per[0] = v.per[1] = 1;
+  c->ln = v;
+  c->ln_nseq = (int)seqs.size();
+  c->ln_ngroups = ngroups;
+  return true;
+}
+
 void prepare_exchange(jh_ctx *c) {
   c->xv = XV_NONE;
   c->st_on = false;
+  c->ln_on = false;
   c->ntab = 0;
   c->nent = 0;
   if (c->xh.empty()) return;
@@ -990,7 +1230,7 @@ void prepare_exchange(jh_ctx *c) {
   const bool by_tensor = c->prec == PR_FP32;
   const bool want_pattern = c->xfmt_req == JH_XFMT_AUTO || c->xfmt_req == JH_XFMT_PATTERN ||
                             c->xfmt_req == JH_XFMT_PATTERN_GLOBAL ||
-                            c->xfmt_req == JH_XFMT_STENCIL;
+                            c->xfmt_req == JH_XFMT_STENCIL || c->xfmt_req == JH_XFMT_LINE;
   bool pattern_ok = typed_ok && want_pattern &&
                     build_patterns(c, x, tbits, by_tensor, pofs, pat, c->npatterns);
   int fmt = c->xfmt_req;
@@ -998,9 +1238,10 @@ void prepare_exchange(jh_ctx *c) {
   // STENCIL launch measured slower (its tile staging costs more VALU than the
   // texture-address time it saves); jh_autotune times it beside the others
   const bool want_stencil = fmt == JH_XFMT_STENCIL;
-  if (fmt == JH_XFMT_STENCIL && !pattern_ok)
-    JH_THROW(JH_ERR_UNSUPPORTED, "STENCIL exchange format needs the row patterns of a crystal");
-  if (fmt == JH_XFMT_AUTO || fmt == JH_XFMT_STENCIL)
+  const bool want_line = fmt == JH_XFMT_LINE;
+  if ((want_stencil || want_line) && !pattern_ok)
+    JH_THROW(JH_ERR_UNSUPPORTED, "STENCIL and LINE exchange formats need the row patterns of a crystal");
+  if (fmt == JH_XFMT_AUTO || want_stencil || want_line)
     fmt = pattern_ok ? JH_XFMT_PATTERN : typed_ok ? JH_XFMT_TYPED : JH_XFMT_BLOCK;
   if (fmt != JH_XFMT_BLOCK && !typed_ok)
     JH_THROW(JH_ERR_UNSUPPORTED, "TYPED exchange format does not fit (too many distinct tensors)");
@@ -1095,6 +1336,13 @@ void prepare_exchange(jh_ctx *c) {
                "STENCIL exchange format does not apply (needs jh_set_lattice, one crystal whose "
                "rows all translate one stencil per basis site, Lz a multiple of 64, Heun)");
     if (c->st_on) c->xfmt = JH_XFMT_STENCIL;
+    c->ln_on = want_line && build_lines(c, x, diag);
+    if (c->xfmt_req == JH_XFMT_LINE && !c->ln_on)
+      JH_THROW(JH_ERR_UNSUPPORTED,
+               "LINE exchange format does not apply (needs jh_set_lattice, one periodic crystal "
+               "of at most two basis sites whose rows all translate one stencil reaching at "
+               "most two cells along z, Lz = 64 or 128, Heun)");
+    if (c->ln_on) c->xfmt = JH_XFMT_LINE;
     return;
   }
   c->pofs.release();
@@ -1412,20 +1660,37 @@ void enqueue_rk4(jh_ctx *c, KParams p) {
 
 // One Heun step's launches (dipole fields, predictor, corrector) for step
 // counter / time in p (or on the device when p.dstep is set).
-// One Heun phase: the STENCIL launch and, in FP64, the row-list launch of
-// its wrapped rows; or one launch of the context's exchange variant.
+// One Heun phase: the LINE or STENCIL launch and, in FP64, the row-list
+// launch of its wrapped rows; or one launch of the context's exchange
+// variant.
 void launch_phase(jh_ctx *c, const KParams &p, int phase) {
-  if (!c->st_on) {
+  if (c->ln_on) {
+    LnView v = c->ln;
+    v.seq = reinterpret_cast<const int2 *>(c->ln_seq.p);
+    v.prog = c->ln_prog.p;
+    v.val = c->ln_val.p;
+    v.wseq = c->ln_wseq.p;
+    v.fixmask = c->ln_fixmask.p;
+    // the row list rides in the same launch (its first workgroups)
+    const int64_t lines = (int64_t)v.nb * v.lx * v.ly;
+    const int bs = lines % (kLnBlock / 64) == 0 ? kLnBlock : 64;
+    v.rows = c->st_nfix > 0 ? c->st_rows.p : nullptr;
+    v.nrows = c->st_nfix;
+    v.rlblocks = (int)((c->st_nfix + bs - 1) / bs);
+    HIPCHK(launch_heun_ln(p, v, c->prec, xv_diag(c->xv), phase, c->stream));
+    return;
+  } else if (c->st_on) {
+    StView v = c->st;
+    v.seq = reinterpret_cast<const int2 *>(c->st_seq.p);
+    v.off = c->st_off.p;
+    v.val = c->st_val.p;
+    v.wseq = c->st_wseq.p;
+    v.fixmask = c->st_fixmask.p;
+    HIPCHK(launch_heun_st(p, v, c->prec, xv_diag(c->xv), phase, c->stream));
+  } else {
     HIPCHK(launch_heun(p, c->prec, c->xv, phase, c->launch_block(), c->stream));
     return;
   }
-  StView v = c->st;
-  v.seq = reinterpret_cast<const int2 *>(c->st_seq.p);
-  v.off = c->st_off.p;
-  v.val = c->st_val.p;
-  v.wseq = c->st_wseq.p;
-  v.fixmask = c->st_fixmask.p;
-  HIPCHK(launch_heun_st(p, v, c->prec, xv_diag(c->xv), phase, c->stream));
   if (c->st_nfix > 0) {
     KParams q = p;
     q.rowlist = c->st_rows.p;
@@ -1828,7 +2093,7 @@ int jh_set_integrator(jh_ctx *c, int integrator) {
 int jh_set_exchange_format(jh_ctx *c, int fmt) {
   return guarded(c, [&] {
     if (fmt != JH_XFMT_AUTO && fmt != JH_XFMT_TYPED && fmt != JH_XFMT_BLOCK && fmt != JH_XFMT_PATTERN &&
-        fmt != JH_XFMT_PATTERN_GLOBAL && fmt != JH_XFMT_STENCIL)
+        fmt != JH_XFMT_PATTERN_GLOBAL && fmt != JH_XFMT_STENCIL && fmt != JH_XFMT_LINE)
       JH_THROW(JH_ERR_INVALID, "unknown exchange format");
     c->xfmt_req = fmt;
     c->dirty = true;
@@ -2149,10 +2414,11 @@ int jh_autotune(jh_ctx *c, int steps, char *report, int64_t len) {
     std::string rep;
     double best = 1e300;
     int best_fmt = fmt0, best_bs = bs0;
-    for (int fmt : {JH_XFMT_PATTERN, JH_XFMT_TYPED, JH_XFMT_BLOCK, JH_XFMT_STENCIL}) {
+    for (int fmt : {JH_XFMT_PATTERN, JH_XFMT_TYPED, JH_XFMT_BLOCK, JH_XFMT_STENCIL, JH_XFMT_LINE}) {
       if (c->xh.empty() && fmt != JH_XFMT_PATTERN) continue;
       const bool stencil = fmt == JH_XFMT_STENCIL;
-      if (stencil && (!c->have_lattice || c->integrator != JH_INTEGRATOR_HEUN)) continue;
+      const bool line = fmt == JH_XFMT_LINE;
+      if ((stencil || line) && (!c->have_lattice || c->integrator != JH_INTEGRATOR_HEUN)) continue;
       c->xfmt_req = fmt;
       c->dirty = true;
       if (!stencil) {
@@ -2164,6 +2430,7 @@ int jh_autotune(jh_ctx *c, int steps, char *report, int64_t len) {
       }
       for (int bs : {128, 256, 512, 1024}) {
         if (bs == 1024 && !stencil) continue;
+        if (line && bs != kLnBlock) continue;  // one shape: a wave per line
         c->block = bs;
         if (stencil) {
           // the block size is the tile shape: lay it out again
@@ -2245,6 +2512,12 @@ int jh_describe(jh_ctx *c, char *buf, int64_t buflen) {
       out += st;
     } else {
       out += " stencil=off";
+    }
+    if (c->ln_on) {
+      char ln[160];
+      std::snprintf(ln, sizeof ln, " line=%dx64 sequences=%d groups=%d wrapped_rows=%lld",
+                    c->ln.lz / 64, c->ln_nseq, c->ln_ngroups, (long long)c->st_nfix);
+      out += ln;
     }
     const int dip = dipole_term(c);
     if (dip >= 0) out += " " + c->dipoles[dip]->describe(c->prec == PR_FP32);

@@ -92,6 +92,45 @@ struct StView {
   const uint64_t *fixmask; // FP64: per wave, the lanes whose row the row-list
                            // launch computes (their z neighbours wrap)
 };
+// LINE (crystal lattices with periodic z, Lz = 64 P, P = 1 or 2, at most two
+// basis sites; Heun step kernels): a wave holds one whole (basis site, x, y)
+// line along z, cell z = P lane + h in lane `lane` (h < P).  A neighbour
+// line (b', x + dx, y + dy) is loaded once, coalesced, into registers, and
+// the neighbour at z + dz is read from the lane that holds it by a
+// wave-wide lane rotation (DPP), so a wave issues one load per neighbour
+// line instead of one gather per neighbour.  The wave's program, wave-
+// uniform, is a list of groups: the lines of one (x + dx, y + dy) column
+// (one per source basis site, at most two) and the entries that read them,
+// in the order the sum takes.  FP64 sums in the reference column order,
+// which lines share except in rows whose z neighbours wrap round the
+// periodic boundary; those rows are left to the row-list launch, as in
+// STENCIL.
+constexpr int kLnSlots = 2;          // lines of one group (basis sites)
+constexpr int kLnBlock = 256;        // threads per workgroup: 4 lines
+constexpr int kLnGroupWords = 8;     // program words per group
+struct LnView {
+  int lx, ly, lz, nb;
+  int per[2];              // periodic x and y (z is periodic)
+  // per sequence: (first group, number of groups).  A group is 8 words: a
+  // bitmask of the entry codes slot | (dz + 2) << 1 present (one neighbour
+  // each), the index of its first entry's values, two unused words and its
+  // two lines b' | (dx + 2048) << 4 | (dy + 2048) << 16; its entries are
+  // summed in ascending code order, the k-th reading values first + k
+  const int2 *seq;
+  const int32_t *prog;
+  const double *val;       // per value index: 3 (diagonal) or 9 tensor values,
+                           // and one spare index at the end
+  const int32_t *wseq;     // FP64: sequence of line (b, x, y) at (b lx + x) ly + y;
+                           // null: sequence b (FP32 form)
+  const uint64_t *fixmask; // FP64: per line, P words; bit l of word h marks
+                           // cell P l + h as the row list's
+  // FP64 row list, computed by the launch's first `rlblocks` workgroups
+  // (gathers over the row patterns, one row per thread) while the others
+  // run the lines
+  const int32_t *rows;
+  int64_t nrows;
+  int rlblocks;
+};
 constexpr bool xv_uniform(int v) { return v == XV_UDIAG || v == XV_UFULL || xv_pair(v); }
 constexpr bool xv_pattern(int v) { return (v >= XV_PDIAG && v <= XV_GFULL) || xv_uniform(v); }
 constexpr bool xv_pattern_lds(int v) {

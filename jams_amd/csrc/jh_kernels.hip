@@ -12,6 +12,7 @@
 // tensors and accumulates in FP64 (one rounding per product and per sum); it
 // may order a row by tensor id so that tensors become wave-uniform.
 #include <algorithm>
+#include <type_traits>
 
 #include "jh_device.h"
 #include "jh_internal.h"
@@ -816,6 +817,289 @@ __global__ void __launch_bounds__(kStMaxBlock) k_heun_st(const KParams p, const 
   heun_spin<PREC, PHASE, GEN>(p, i, c, step, t, s0, s1, s2, x0, x1, x2, own);
 }
 
+// ---------------------------------------------------------------------------
+// LINE step kernels (LnView in jh_internal.h): wave -> one (basis site, x, y)
+// line of Lz = 64 P cells along z, cell z = P lane + h in lane `lane`.
+
+// Wave-wide lane rotation by one (DPP): lane l reads lane l + 1 (wave_rol)
+// or lane l - 1 (wave_ror), mod 64.  Every lane of the wave is active
+// wherever these run, so each reads a live value.
+constexpr int kDppRol = 0x134;
+constexpr int kDppRor = 0x13C;
+template <int CTRL>
+__device__ __forceinline__ float lane_rot(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xf, 0xf, true));
+}
+template <int CTRL>
+__device__ __forceinline__ double lane_rot(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xf, 0xf, true);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+// the value lane l + S holds (|S| <= 2)
+template <int S, typename T>
+__device__ __forceinline__ T lane_shift(T v) {
+  if constexpr (S > 0) return lane_shift<S - 1>(lane_rot<kDppRol>(v));
+  else if constexpr (S < 0) return lane_shift<S + 1>(lane_rot<kDppRor>(v));
+  else return v;
+}
+
+template <int PREC>
+using LnVal = typename std::conditional<PREC == PR_FP64, double, float>::type;
+template <int PREC, int P>
+struct LnLine {
+  LnVal<PREC> v[P][3];  // cell P lane + h: v[h][component]
+};
+
+// Load line (bb, xx, yy) (wrapped coordinates): lane `lane` gets its P cells,
+// coalesced (FP64: the (x, y) pairs as P 16-byte loads and the z values as
+// one P * 8-byte load; FP32: the P * 12 bytes of the gather copy).
+template <int PREC, int P>
+__device__ __forceinline__ void ln_load(const LnView &ln, const double *__restrict__ src64,
+                                        const float *__restrict__ src32, int64_t n, int bb,
+                                        int xx, int yy, int lane, LnLine<PREC, P> &L) {
+  const int64_t base = (((int64_t)bb * ln.lx + xx) * ln.ly + yy) * ln.lz + (int64_t)P * lane;
+  if constexpr (PREC == PR_FP64) {
+    const double2 *xy = reinterpret_cast<const double2 *>(src64) + base;
+#pragma unroll
+    for (int h = 0; h < P; ++h) {
+      const double2 v = xy[h];
+      L.v[h][0] = v.x;
+      L.v[h][1] = v.y;
+    }
+    if constexpr (P == 2) {
+      const double2 z = *reinterpret_cast<const double2 *>(src64 + 2 * n + base);
+      L.v[0][2] = z.x;
+      L.v[1][2] = z.y;
+    } else {
+      L.v[0][2] = src64[2 * n + base];
+    }
+  } else if constexpr (P == 2 && kG32 == 3) {
+    // the lane's 24 bytes (8-byte aligned) as one 16- and one 8-byte load,
+    // taken apart in registers
+    typedef float f4a8 __attribute__((ext_vector_type(4), aligned(8)));
+    typedef float f2a8 __attribute__((ext_vector_type(2), aligned(8)));
+    const float *q = src32 + (size_t)kG32 * base;
+    const f4a8 u = *reinterpret_cast<const f4a8 *>(q);
+    const f2a8 w = *reinterpret_cast<const f2a8 *>(q + 4);
+    L.v[0][0] = u.x; L.v[0][1] = u.y; L.v[0][2] = u.z;
+    L.v[1][0] = u.w; L.v[1][1] = w.x; L.v[1][2] = w.y;
+  } else {
+    const float *q = src32 + (size_t)kG32 * base;
+#pragma unroll
+    for (int h = 0; h < P; ++h) {
+      L.v[h][0] = q[kG32 * h];
+      L.v[h][1] = q[kG32 * h + 1];
+      L.v[h][2] = q[kG32 * h + 2];
+    }
+  }
+}
+
+// one neighbour's product into (a0, a1, a2): FP64 in the reference's order
+// with one rounding per product and per sum; FP32 form as FP64 FMAs
+template <int PREC, bool DIAG>
+__device__ __forceinline__ void ln_acc(const double *__restrict__ T, double sx, double sy,
+                                       double sz, double &a0, double &a1, double &a2) {
+  if constexpr (PREC == PR_FP64) {
+    if constexpr (DIAG) {
+      a0 = a0 + sx * T[0];
+      a1 = a1 + sy * T[1];
+      a2 = a2 + sz * T[2];
+    } else {
+      a0 = a0 + sx * T[0]; a0 = a0 + sy * T[1]; a0 = a0 + sz * T[2];
+      a1 = a1 + sx * T[3]; a1 = a1 + sy * T[4]; a1 = a1 + sz * T[5];
+      a2 = a2 + sx * T[6]; a2 = a2 + sy * T[7]; a2 = a2 + sz * T[8];
+    }
+  } else {
+    if constexpr (DIAG) {
+      a0 = __builtin_fma(sx, T[0], a0);
+      a1 = __builtin_fma(sy, T[1], a1);
+      a2 = __builtin_fma(sz, T[2], a2);
+    } else {
+      a0 = __builtin_fma(sx, T[0], a0); a0 = __builtin_fma(sy, T[1], a0); a0 = __builtin_fma(sz, T[2], a0);
+      a1 = __builtin_fma(sx, T[3], a1); a1 = __builtin_fma(sy, T[4], a1); a1 = __builtin_fma(sz, T[5], a1);
+      a2 = __builtin_fma(sx, T[6], a2); a2 = __builtin_fma(sy, T[7], a2); a2 = __builtin_fma(sz, T[8], a2);
+    }
+  }
+}
+
+// The neighbours at z + DZ of the lane's P cells from line L, as read (FP32
+// or FP64).
+template <int P, int DZ, typename Ln, typename T>
+__device__ __forceinline__ void ln_gather(const Ln &L, T (&v)[P][3]) {
+#pragma unroll
+  for (int h = 0; h < P; ++h) {
+    const int t = h + DZ;
+    const int R = ((t % P) + P) % P;
+    const int Q = (t - R) / P;
+    // Q is -1, 0 or 1 (P = 2) or DZ (P = 1): resolved at compile time
+    if (Q == 1) {
+      v[h][0] = lane_shift<1>(L.v[R][0]); v[h][1] = lane_shift<1>(L.v[R][1]);
+      v[h][2] = lane_shift<1>(L.v[R][2]);
+    } else if (Q == -1) {
+      v[h][0] = lane_shift<-1>(L.v[R][0]); v[h][1] = lane_shift<-1>(L.v[R][1]);
+      v[h][2] = lane_shift<-1>(L.v[R][2]);
+    } else if (Q == 2) {
+      v[h][0] = lane_shift<2>(L.v[R][0]); v[h][1] = lane_shift<2>(L.v[R][1]);
+      v[h][2] = lane_shift<2>(L.v[R][2]);
+    } else if (Q == -2) {
+      v[h][0] = lane_shift<-2>(L.v[R][0]); v[h][1] = lane_shift<-2>(L.v[R][1]);
+      v[h][2] = lane_shift<-2>(L.v[R][2]);
+    } else {
+      v[h][0] = L.v[R][0]; v[h][1] = L.v[R][1]; v[h][2] = L.v[R][2];
+    }
+  }
+}
+
+struct LnGroup {
+  uint32_t mask;  // entry codes present
+  int vbase;      // values of the group's first entry
+};
+
+// Entry code CODE = slot | (dz + 2) << 1 of a group, if present: its
+// neighbours come from slot CODE & 1 by the rotations of dz, its values are
+// the next in the group's run (T advances).
+template <int PREC, bool DIAG, int P, int CODE>
+__device__ __forceinline__ void ln_code(const LnLine<PREC, P> (&S)[kLnSlots], uint32_t mask,
+                                        const double *__restrict__ &T, double (&a)[P][3]) {
+  constexpr int V = DIAG ? 3 : 9;
+  if (mask & (1u << CODE)) {
+    LnVal<PREC> v[P][3];
+    ln_gather<P, (CODE >> 1) - 2>(S[CODE & 1], v);
+#pragma unroll
+    for (int h = 0; h < P; ++h)
+      ln_acc<PREC, DIAG>(T, (double)v[h][0], (double)v[h][1], (double)v[h][2], a[h][0], a[h][1],
+                         a[h][2]);
+    T += V;
+  }
+}
+
+// Exchange field of the wave's line (b, xw, yw) over its program sid.  The
+// two lines of group g + 1 are loaded (buffer B, or A) while group g's
+// entries sum, so a group's loads are in flight during the previous group's
+// arithmetic; the steady-state loop issues its loads unconditionally and the
+// last one or two groups are peeled, which keeps the compiler's memory waits
+// exact.  A group's entries are its present codes in ascending order, each a
+// wave-uniform branch over straight-line code.
+template <int PREC, bool DIAG, int P>
+__device__ __forceinline__ void ln_field(const LnView &ln, int sid, int xw, int yw,
+                                         const double *__restrict__ src64,
+                                         const float *__restrict__ src32, int64_t n, int lane,
+                                         double (&a)[P][3]) {
+  constexpr int V = DIAG ? 3 : 9;
+#pragma unroll
+  for (int h = 0; h < P; ++h) a[h][0] = a[h][1] = a[h][2] = 0.0;
+  const int2 sq = ln.seq[sid];
+  const int g0 = __builtin_amdgcn_readfirstlane(sq.x);
+  const int G = __builtin_amdgcn_readfirstlane(sq.y);
+  const int32_t *__restrict__ prog = ln.prog + (size_t)kLnGroupWords * g0;
+  LnLine<PREC, P> A[kLnSlots], B[kLnSlots];
+  LnGroup hA{}, hB{};
+  auto load_group = [&](LnLine<PREC, P>(&S)[kLnSlots], LnGroup &hg, int g) {
+    const int32_t *w = prog + (size_t)kLnGroupWords * g;
+    hg.mask = (uint32_t)__builtin_amdgcn_readfirstlane(w[0]);
+    hg.vbase = __builtin_amdgcn_readfirstlane(w[1]);
+#pragma unroll
+    for (int k = 0; k < kLnSlots; ++k) {
+      const int lw = __builtin_amdgcn_readfirstlane(w[4 + k]);
+      int xx = xw + ((lw >> 4) & 0xFFF) - 2048;
+      int yy = yw + ((lw >> 16) & 0xFFF) - 2048;
+      xx = xx < 0 ? xx + ln.lx : xx >= ln.lx ? xx - ln.lx : xx;
+      yy = yy < 0 ? yy + ln.ly : yy >= ln.ly ? yy - ln.ly : yy;
+      ln_load<PREC, P>(ln, src64, src32, n, lw & 15, xx, yy, lane, S[k]);
+    }
+  };
+  auto run_group = [&](const LnLine<PREC, P>(&S)[kLnSlots], const LnGroup &hg) {
+    const double *__restrict__ T = ln.val + (size_t)hg.vbase * V;
+    const uint32_t m = hg.mask;
+    ln_code<PREC, DIAG, P, 0>(S, m, T, a);
+    ln_code<PREC, DIAG, P, 1>(S, m, T, a);
+    ln_code<PREC, DIAG, P, 2>(S, m, T, a);
+    ln_code<PREC, DIAG, P, 3>(S, m, T, a);
+    ln_code<PREC, DIAG, P, 4>(S, m, T, a);
+    ln_code<PREC, DIAG, P, 5>(S, m, T, a);
+    ln_code<PREC, DIAG, P, 6>(S, m, T, a);
+    ln_code<PREC, DIAG, P, 7>(S, m, T, a);
+    ln_code<PREC, DIAG, P, 8>(S, m, T, a);
+    ln_code<PREC, DIAG, P, 9>(S, m, T, a);
+  };
+  if (G <= 0) return;
+  load_group(A, hA, 0);
+  int g = 0;
+  for (; g + 2 < G; g += 2) {
+    load_group(B, hB, g + 1);
+    run_group(A, hA);
+    load_group(A, hA, g + 2);
+    run_group(B, hB);
+  }
+  if (g + 1 < G) {
+    load_group(B, hB, g + 1);
+    run_group(A, hA);
+    run_group(B, hB);
+  } else {
+    run_group(A, hA);
+  }
+}
+
+// One Heun phase over the LINE layout: workgroup -> 4 consecutive lines,
+// wave -> line (b, x, y), lane -> cells P lane .. P lane + P - 1.
+template <int PREC, bool DIAG, int PHASE, bool GEN, int P>
+__global__ void __launch_bounds__(kLnBlock) k_heun_ln(const KParams p, const LnView ln) {
+  const int rlb = ln.rlblocks;
+  if ((int)blockIdx.x < rlb) {
+    // FP64 row list: the rows whose z neighbours wrap, each summed in its own
+    // column order from its row pattern (pattern table in global memory,
+    // tensor table in LDS)
+    if constexpr (PREC == PR_FP64) {
+      extern __shared__ __align__(16) unsigned char lds[];
+      constexpr int XG = DIAG ? XV_GDIAG : XV_GFULL;
+      load_table<PREC, XG>(p.x, lds);
+      __syncthreads();
+      const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+      if (j >= ln.nrows) return;
+      const int64_t i = ln.rows[j];
+      const uint32_t c = p.cls ? (uint32_t)p.cls[i] : 0u;
+      const uint64_t step = launch_step(p);
+      const double t0 = launch_time(p, step);
+      double s0, s1, s2;
+      OwnState own;
+      phase_spin<PREC, PHASE>(p, i, s0, s1, s2, own);
+      const double t = PHASE == 0 ? t0 : t0 + p.dt;
+      double x0, x1, x2;
+      exchange_at<PREC, XG>(p, PHASE == 0 ? p.s : p.sp, p.g32, i, lds, x0, x1, x2);
+      heun_spin<PREC, PHASE, GEN>(p, i, c, step, t, s0, s1, s2, x0, x1, x2, own);
+    }
+    return;
+  }
+  const int64_t blk = xcd_block(blockIdx.x - rlb, gridDim.x - rlb);
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int64_t line = blk * (int64_t)(blockDim.x >> 6) + wave;  // (b lx + x) ly + y
+  const int lane = (int)(threadIdx.x & 63);
+  const int yw = (int)(line % ln.ly);
+  const int64_t r = line / ln.ly;
+  const int xw = (int)(r % ln.lx);
+  const int b = (int)(r / ln.lx);
+  const int64_t i0 = line * ln.lz + (int64_t)P * lane;
+  const uint64_t step = launch_step(p);
+  const double t0 = launch_time(p, step);
+  const double t = PHASE == 0 ? t0 : t0 + p.dt;
+  double s[P][3];
+  OwnState own[P];
+#pragma unroll
+  for (int h = 0; h < P; ++h) phase_spin<PREC, PHASE>(p, i0 + h, s[h][0], s[h][1], s[h][2], own[h]);
+  const int sid = ln.wseq ? __builtin_amdgcn_readfirstlane(ln.wseq[line]) : b;
+  double a[P][3];
+  ln_field<PREC, DIAG, P>(ln, sid, xw, yw, PHASE == 0 ? p.s : p.sp, p.g32, p.n, lane, a);
+#pragma unroll
+  for (int h = 0; h < P; ++h) {
+    if (ln.fixmask && ((ln.fixmask[line * P + h] >> lane) & 1ull)) continue;  // the row list's
+    const uint32_t c = p.cls ? (uint32_t)p.cls[i0 + h] : 0u;
+    heun_spin<PREC, PHASE, GEN>(p, i0 + h, c, step, t, s[h][0], s[h][1], s[h][2], a[h][0],
+                                a[h][1], a[h][2], own[h]);
+  }
+}
+
 // One stage of the classical RK4 step of llg-rk4-gpu
 // (solvers/cuda_rk4_base.cu:57-108, cuda_llg_rk4_kernel.cuh:8-58,
 // cuda_rk4_base_kernel.cuh:1-18, cuda/cuda_spin_ops.cu:4-17):
@@ -1136,6 +1420,47 @@ hipError_t launch_heun_st(const KParams &p, const StView &v, int prec, bool diag
                 : launch_heun_st_t<PR_FP64, false>(p, v, phase, lds, st);
   return diag ? launch_heun_st_t<PR_FP32, true>(p, v, phase, lds, st)
               : launch_heun_st_t<PR_FP32, false>(p, v, phase, lds, st);
+}
+
+template <int PREC, bool DIAG, int P>
+static hipError_t launch_heun_ln_t(const KParams &p, const LnView &v, int phase, int bs,
+                                   hipStream_t st) {
+  const int64_t lines = (int64_t)v.nb * v.lx * v.ly;
+  const dim3 grid((unsigned)(lines / (bs / 64) + v.rlblocks)), block((unsigned)bs);
+  const size_t lds = v.rlblocks > 0 ? table_bytes(PREC, DIAG ? XV_GDIAG : XV_GFULL, p.x.ntab) : 0;
+  const bool gen = p.has_ac || p.ani_power != 2;
+  if (phase == 0) {
+    if (gen) hipLaunchKernelGGL((k_heun_ln<PREC, DIAG, 0, true, P>), grid, block, lds, st, p, v);
+    else hipLaunchKernelGGL((k_heun_ln<PREC, DIAG, 0, false, P>), grid, block, lds, st, p, v);
+  } else {
+    if (gen) hipLaunchKernelGGL((k_heun_ln<PREC, DIAG, 1, true, P>), grid, block, lds, st, p, v);
+    else hipLaunchKernelGGL((k_heun_ln<PREC, DIAG, 1, false, P>), grid, block, lds, st, p, v);
+  }
+  return hipGetLastError();
+}
+
+template <int PREC, bool DIAG>
+static hipError_t launch_heun_ln_p(const KParams &p, const LnView &v, int phase, int bs,
+                                   hipStream_t st) {
+  if (v.lz == 64) return launch_heun_ln_t<PREC, DIAG, 1>(p, v, phase, bs, st);
+  return launch_heun_ln_t<PREC, DIAG, 2>(p, v, phase, bs, st);
+}
+
+hipError_t launch_heun_ln(const KParams &p, const LnView &v, int prec, bool diag, int phase,
+                          hipStream_t st) {
+  // the grid covers the lines exactly: 4 lines per workgroup, else one
+  const int64_t lines = (int64_t)v.nb * v.lx * v.ly;
+  if (v.nb < 1 || v.nb > kLnSlots || (v.lz != 64 && v.lz != 128) || lines <= 0 ||
+      lines * v.lz != p.n)
+    return hipErrorInvalidValue;
+  const int bs = lines % (kLnBlock / 64) == 0 ? kLnBlock : 64;
+  if (v.rlblocks != (v.nrows + bs - 1) / bs || (v.nrows > 0 && (!v.rows || prec != PR_FP64)))
+    return hipErrorInvalidValue;
+  if (prec == PR_FP64)
+    return diag ? launch_heun_ln_p<PR_FP64, true>(p, v, phase, bs, st)
+                : launch_heun_ln_p<PR_FP64, false>(p, v, phase, bs, st);
+  return diag ? launch_heun_ln_p<PR_FP32, true>(p, v, phase, bs, st)
+              : launch_heun_ln_p<PR_FP32, false>(p, v, phase, bs, st);
 }
 
 template <int PREC, int XVk>

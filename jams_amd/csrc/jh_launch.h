@@ -14,6 +14,9 @@ hipError_t launch_heun_st(const KParams &p, const StView &v, int prec, bool diag
                           hipStream_t st);
 // LDS bytes of a STENCIL workgroup of source basis site b
 size_t st_tile_bytes(const StView &v, int prec, int b);
+// one Heun phase over the LINE layout v, full or diagonal tensors
+hipError_t launch_heun_ln(const KParams &p, const LnView &v, int prec, bool diag, int phase,
+                          hipStream_t st);
 // one RK4 stage (p.stage)
 hipError_t launch_rk4(const KParams &p, int prec, int xv, int bs, hipStream_t st);
 hipError_t launch_reduce_replicas(const KParams &p, const int64_t *off, int nrep, double *out,

@@ -20,6 +20,7 @@ PREC_FP32_FIELD = 1
 INTEGRATOR_HEUN = 0
 INTEGRATOR_RK4 = 1
 XFMT_AUTO, XFMT_TYPED, XFMT_BLOCK, XFMT_PATTERN, XFMT_PATTERN_GLOBAL, XFMT_STENCIL = 0, 1, 2, 3, 4, 5
+XFMT_LINE = 6
 TERM_EXCHANGE, TERM_ZEEMAN, TERM_UNIAXIAL, TERM_DIPOLE = 0, 1, 2, 3
 
 # every entry point declared in include/jams_hip.h
