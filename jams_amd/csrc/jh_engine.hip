@@ -1175,17 +1175,15 @@ bool build_lines(jh_ctx *c, const XHost &x, bool diag) {
   if (fp64) {
     c->ln_wseq.alloc(wseq.size());
     HIPCHK(hipMemcpy(c->ln_wseq.p, wseq.data(), sizeof(int32_t) * wseq.size(), hipMemcpyHostToDevice));
+    // the row list owns whole lanes: a lane with a wrapped row gives the
+    // list all its P cells (the line kernel stores a lane's cells together)
+    std::vector<uint64_t> mask((size_t)(n / L[2]), 0);
+    for (int64_t d = 0; d < n; ++d)
+      if (fix[d]) mask[(size_t)(d / L[2])] |= 1ull << ((d % L[2]) / P);
     std::vector<int32_t> rows;
     for (int64_t d = 0; d < n; ++d)
-      if (fix[d]) rows.push_back((int32_t)d);
+      if ((mask[(size_t)(d / L[2])] >> ((d % L[2]) / P)) & 1ull) rows.push_back((int32_t)d);
     if (!rows.empty()) {
-      // per line, P words: bit l of word h is cell P l + h
-      std::vector<uint64_t> mask((size_t)(n / L[2]) * P, 0);
-      for (int32_t d : rows) {
-        const int64_t line = d / L[2];
-        const int z = (int)(d % L[2]);
-        mask[(size_t)(line * P + z % P)] |= 1ull << (z / P);
-      }
       c->st_rows.alloc(rows.size());
       HIPCHK(hipMemcpy(c->st_rows.p, rows.data(), sizeof(int32_t) * rows.size(), hipMemcpyHostToDevice));
       c->ln_fixmask.alloc(mask.size());

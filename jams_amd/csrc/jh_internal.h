@@ -122,8 +122,8 @@ struct LnView {
                            // and one spare index at the end
   const int32_t *wseq;     // FP64: sequence of line (b, x, y) at (b lx + x) ly + y;
                            // null: sequence b (FP32 form)
-  const uint64_t *fixmask; // FP64: per line, P words; bit l of word h marks
-                           // cell P l + h as the row list's
+  const uint64_t *fixmask; // FP64: per line, bit l marks lane l's cells as
+                           // the row list's
   // FP64 row list, computed by the launch's first `rlblocks` workgroups
   // (gathers over the row patterns, one row per thread) while the others
   // run the lines

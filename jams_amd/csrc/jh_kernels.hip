@@ -550,18 +550,25 @@ __device__ __forceinline__ void phase_spin(const KParams &p, int64_t i, double &
 // Everything of a Heun phase after the exchange field (x0, x1, x2) of spin i
 // at (s0, s1, s2): the ordered field sum, the thermal field, h / mus, the LLG
 // right-hand side and the phase's update and stores.
+// The arithmetic of a Heun phase for spin i after its exchange field
+// (x0, x1, x2) at (s0, s1, s2): the ordered field sum, the thermal field,
+// h / mus, the LLG right-hand side and the update.  w is the step's thermal
+// field: read by the caller in the corrector (or injected), drawn here in the
+// predictor from the Philox stream.  Out: the predictor's ds = rhs / 2 in
+// (d0, d1, d2) and s* in (v0, v1, v2); the corrector's new s in (v0, v1, v2)
+// from its own s and ds in `own`.
 template <int PREC, int PHASE, bool GEN>
-__device__ __forceinline__ void heun_spin(const KParams &p, int64_t i, uint32_t c, uint64_t step,
+__device__ __forceinline__ void heun_math(const KParams &p, int64_t i, uint32_t c, uint64_t step,
                                           double t, double s0, double s1, double s2, double x0,
-                                          double x1, double x2, const OwnState &own) {
-  const int64_t n = p.n;
+                                          double x1, double x2, const OwnState &own, double &w0,
+                                          double &w1, double &w2, double &d0, double &d1,
+                                          double &d2, double &v0, double &v1, double &v2) {
   double h0, h1, h2;
   sum_terms<GEN>(p, c, i, s0, s1, s2, t, x0, x1, x2, h0, h1, h2);
 
   // thermal field w of this step (solvers/cpu_llg_heun.cc:53-62): one draw
   // per step serves both phases, so the predictor generates it and leaves it
   // for the corrector
-  double w0 = 0.0, w1 = 0.0, w2 = 0.0;
   if (p.noise == NM_PHILOX && PHASE == 0) {
     double z0, z1, z2, sT;
     if (p.cls) {
@@ -574,11 +581,6 @@ __device__ __forceinline__ void heun_spin(const KParams &p, int64_t i, uint32_t 
     }
     const double sg = cparam(p, c, CF_SIGMA);
     w0 = z0 * sg * sT; w1 = z1 * sg * sT; w2 = z2 * sg * sT;
-    p.wbuf[i] = w0; p.wbuf[n + i] = w1; p.wbuf[2 * n + i] = w2;
-  } else if (p.noise == NM_PHILOX) {
-    w0 = p.wbuf[i]; w1 = p.wbuf[n + i]; w2 = p.wbuf[2 * n + i];
-  } else if (p.noise == NM_INJECTED) {
-    w0 = p.winj[i]; w1 = p.winj[n + i]; w2 = p.winj[2 * n + i];
   }
 
   // h = w + h / mus  (solvers/cpu_llg_heun.cc:68-81); the FP32-field form
@@ -598,23 +600,55 @@ __device__ __forceinline__ void heun_spin(const KParams &p, int64_t i, uint32_t 
   llg_rhs(s0, s1, s2, h0, h1, h2, cparam(p, c, CF_GYRO), cparam(p, c, CF_ALPHA), r0, r1, r2);
 
   if constexpr (PHASE == 0) {
-    p.ds[i] = 0.5 * r0; p.ds[n + i] = 0.5 * r1; p.ds[2 * n + i] = 0.5 * r2;
-    double v0 = s0 + p.dt * r0, v1 = s1 + p.dt * r1, v2 = s2 + p.dt * r2;
+    d0 = 0.5 * r0; d1 = 0.5 * r1; d2 = 0.5 * r2;
+    v0 = s0 + p.dt * r0; v1 = s1 + p.dt * r1; v2 = s2 + p.dt * r2;
     renorm<PREC>(v0, v1, v2);
-    if (!(PREC == PR_FP32 && p.spfree)) store_spin(p.sp, n, i, v0, v1, v2);
-    if constexpr (PREC == PR_FP32) store_g32(p.w32, i, v0, v1, v2);
   } else {
-    OwnState o = own;
-    if constexpr (!kEarlyOwn<PREC>) load_own(p, i, o);
-    const double d0 = o.d0 + 0.5 * r0;
-    const double d1 = o.d1 + 0.5 * r1;
-    const double d2 = o.d2 + 0.5 * r2;
-    double v0 = o.s0, v1 = o.s1, v2 = o.s2;
+    d0 = own.d0 + 0.5 * r0;
+    d1 = own.d1 + 0.5 * r1;
+    d2 = own.d2 + 0.5 * r2;
+    v0 = own.s0; v1 = own.s1; v2 = own.s2;
     v0 = v0 + p.dt * d0; v1 = v1 + p.dt * d1; v2 = v2 + p.dt * d2;
     renorm<PREC>(v0, v1, v2);
-    store_spin(p.snew, n, i, v0, v1, v2);
-    if constexpr (PREC == PR_FP32) store_g32(p.w32, i, v0, v1, v2);
   }
+}
+
+// The thermal field a phase reads: the predictor's draw (corrector) or the
+// injected stream; zero otherwise (the predictor draws its own)
+template <int PHASE>
+__device__ __forceinline__ void read_noise(const KParams &p, int64_t i, double &w0, double &w1,
+                                           double &w2) {
+  const int64_t n = p.n;
+  w0 = w1 = w2 = 0.0;
+  if (p.noise == NM_PHILOX && PHASE == 1) {
+    w0 = p.wbuf[i]; w1 = p.wbuf[n + i]; w2 = p.wbuf[2 * n + i];
+  } else if (p.noise == NM_INJECTED) {
+    w0 = p.winj[i]; w1 = p.winj[n + i]; w2 = p.winj[2 * n + i];
+  }
+}
+
+// Everything of a Heun phase after the exchange field (x0, x1, x2) of spin i
+// at (s0, s1, s2): heun_math and the phase's loads and stores.
+template <int PREC, int PHASE, bool GEN>
+__device__ __forceinline__ void heun_spin(const KParams &p, int64_t i, uint32_t c, uint64_t step,
+                                          double t, double s0, double s1, double s2, double x0,
+                                          double x1, double x2, const OwnState &own) {
+  const int64_t n = p.n;
+  double w0, w1, w2;
+  read_noise<PHASE>(p, i, w0, w1, w2);
+  OwnState o = own;
+  if constexpr (PHASE == 1 && !kEarlyOwn<PREC>) load_own(p, i, o);
+  double d0, d1, d2, v0, v1, v2;
+  heun_math<PREC, PHASE, GEN>(p, i, c, step, t, s0, s1, s2, x0, x1, x2, o, w0, w1, w2, d0, d1,
+                              d2, v0, v1, v2);
+  if constexpr (PHASE == 0) {
+    if (p.noise == NM_PHILOX) { p.wbuf[i] = w0; p.wbuf[n + i] = w1; p.wbuf[2 * n + i] = w2; }
+    p.ds[i] = d0; p.ds[n + i] = d1; p.ds[2 * n + i] = d2;
+    if (!(PREC == PR_FP32 && p.spfree)) store_spin(p.sp, n, i, v0, v1, v2);
+  } else {
+    store_spin(p.snew, n, i, v0, v1, v2);
+  }
+  if constexpr (PREC == PR_FP32) store_g32(p.w32, i, v0, v1, v2);
 }
 
 // Minimum waves per SIMD the compiler keeps room for (VGPR budget): the
@@ -843,6 +877,48 @@ __device__ __forceinline__ T lane_shift(T v) {
   if constexpr (S > 0) return lane_shift<S - 1>(lane_rot<kDppRol>(v));
   else if constexpr (S < 0) return lane_shift<S + 1>(lane_rot<kDppRor>(v));
   else return v;
+}
+
+// Two adjacent spins i0, i0 + 1 (i0 even, so every access is 16-byte
+// aligned) of an FP64 spin array ((x, y) pairs, then z) or SoA vector, and
+// of the FP32 gather copy (24 bytes, 8-byte aligned).
+__device__ __forceinline__ void ld_spin2(const double *__restrict__ a, int64_t n, int64_t i0,
+                                         double (&v)[2][3]) {
+  const double2 p0 = reinterpret_cast<const double2 *>(a)[i0];
+  const double2 p1 = reinterpret_cast<const double2 *>(a)[i0 + 1];
+  const double2 z = *reinterpret_cast<const double2 *>(a + 2 * n + i0);
+  v[0][0] = p0.x; v[0][1] = p0.y; v[0][2] = z.x;
+  v[1][0] = p1.x; v[1][1] = p1.y; v[1][2] = z.y;
+}
+__device__ __forceinline__ void st_spin2(double *a, int64_t n, int64_t i0, const double (&v)[2][3]) {
+  reinterpret_cast<double2 *>(a)[i0] = make_double2(v[0][0], v[0][1]);
+  reinterpret_cast<double2 *>(a)[i0 + 1] = make_double2(v[1][0], v[1][1]);
+  *reinterpret_cast<double2 *>(a + 2 * n + i0) = make_double2(v[0][2], v[1][2]);
+}
+__device__ __forceinline__ void ld_soa2(const double *__restrict__ a, int64_t n, int64_t i0,
+                                        double (&v)[2][3]) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double2 q = *reinterpret_cast<const double2 *>(a + c * n + i0);
+    v[0][c] = q.x;
+    v[1][c] = q.y;
+  }
+}
+__device__ __forceinline__ void st_soa2(double *a, int64_t n, int64_t i0, const double (&v)[2][3]) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) *reinterpret_cast<double2 *>(a + c * n + i0) = make_double2(v[0][c], v[1][c]);
+}
+__device__ __forceinline__ void st_g32_2(float *dst, int64_t i0, const double (&v)[2][3]) {
+  static_assert(kG32 == 3, "the two-spin FP32 store assumes 12-byte spins");
+  typedef float f4a8 __attribute__((ext_vector_type(4), aligned(8)));
+  typedef float f2a8 __attribute__((ext_vector_type(2), aligned(8)));
+  float *q = dst + (size_t)3 * i0;
+  f4a8 u;
+  u.x = (float)v[0][0]; u.y = (float)v[0][1]; u.z = (float)v[0][2]; u.w = (float)v[1][0];
+  f2a8 w;
+  w.x = (float)v[1][1]; w.y = (float)v[1][2];
+  *reinterpret_cast<f4a8 *>(q) = u;
+  *reinterpret_cast<f2a8 *>(q + 4) = w;
 }
 
 template <int PREC>
@@ -1081,22 +1157,80 @@ __global__ void __launch_bounds__(kLnBlock) k_heun_ln(const KParams p, const LnV
   const int xw = (int)(r % ln.lx);
   const int b = (int)(r / ln.lx);
   const int64_t i0 = line * ln.lz + (int64_t)P * lane;
+  const int64_t n = p.n;
   const uint64_t step = launch_step(p);
   const double t0 = launch_time(p, step);
   const double t = PHASE == 0 ? t0 : t0 + p.dt;
+  const int sid = ln.wseq ? __builtin_amdgcn_readfirstlane(ln.wseq[line]) : b;
+  // a lane whose cells the row list owns (FP64) stores nothing
+  const bool fixed = ln.fixmask && ((ln.fixmask[line] >> lane) & 1ull);
   double s[P][3];
   OwnState own[P];
-#pragma unroll
-  for (int h = 0; h < P; ++h) phase_spin<PREC, PHASE>(p, i0 + h, s[h][0], s[h][1], s[h][2], own[h]);
-  const int sid = ln.wseq ? __builtin_amdgcn_readfirstlane(ln.wseq[line]) : b;
   double a[P][3];
-  ln_field<PREC, DIAG, P>(ln, sid, xw, yw, PHASE == 0 ? p.s : p.sp, p.g32, p.n, lane, a);
+  if constexpr (P == 2) {
+    // the lane's two cells are adjacent: each array is one 16-byte access
+    // per (x, y) pair and per SoA component
+    if constexpr (PHASE == 0) {
+      ld_spin2(p.s, n, i0, s);
+    } else if constexpr (kEarlyOwn<PREC>) {
+      double os[2][3], od[2][3];
+      ld_spin2(p.s, n, i0, os);
+      ld_soa2(p.ds, n, i0, od);
 #pragma unroll
-  for (int h = 0; h < P; ++h) {
-    if (ln.fixmask && ((ln.fixmask[line * P + h] >> lane) & 1ull)) continue;  // the row list's
-    const uint32_t c = p.cls ? (uint32_t)p.cls[i0 + h] : 0u;
-    heun_spin<PREC, PHASE, GEN>(p, i0 + h, c, step, t, s[h][0], s[h][1], s[h][2], a[h][0],
-                                a[h][1], a[h][2], own[h]);
+      for (int h = 0; h < 2; ++h) {
+        own[h] = OwnState{os[h][0], os[h][1], os[h][2], od[h][0], od[h][1], od[h][2]};
+        if (p.spfree) {
+          // s* = unit(s + dt rhs) as the predictor formed it (2 ds is its rhs)
+          s[h][0] = os[h][0] + p.dt * (2.0 * od[h][0]);
+          s[h][1] = os[h][1] + p.dt * (2.0 * od[h][1]);
+          s[h][2] = os[h][2] + p.dt * (2.0 * od[h][2]);
+          renorm<PREC>(s[h][0], s[h][1], s[h][2]);
+        }
+      }
+      if (!p.spfree) ld_spin2(p.sp, n, i0, s);
+    } else {
+      ld_spin2(p.sp, n, i0, s);
+    }
+    ln_field<PREC, DIAG, P>(ln, sid, xw, yw, PHASE == 0 ? p.s : p.sp, p.g32, n, lane, a);
+    if (fixed) return;
+    double w[2][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+    if (p.noise == NM_PHILOX && PHASE == 1) ld_soa2(p.wbuf, n, i0, w);
+    else if (p.noise == NM_INJECTED) ld_soa2(p.winj, n, i0, w);
+    if constexpr (PHASE == 1 && !kEarlyOwn<PREC>) {
+      double os[2][3], od[2][3];
+      ld_spin2(p.s, n, i0, os);
+      ld_soa2(p.ds, n, i0, od);
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+        own[h] = OwnState{os[h][0], os[h][1], os[h][2], od[h][0], od[h][1], od[h][2]};
+    }
+    double d[2][3], v[2][3];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const uint32_t c = p.cls ? (uint32_t)p.cls[i0 + h] : 0u;
+      heun_math<PREC, PHASE, GEN>(p, i0 + h, c, step, t, s[h][0], s[h][1], s[h][2], a[h][0],
+                                  a[h][1], a[h][2], own[h], w[h][0], w[h][1], w[h][2], d[h][0],
+                                  d[h][1], d[h][2], v[h][0], v[h][1], v[h][2]);
+    }
+    if constexpr (PHASE == 0) {
+      if (p.noise == NM_PHILOX) st_soa2(p.wbuf, n, i0, w);
+      st_soa2(p.ds, n, i0, d);
+      if (!(PREC == PR_FP32 && p.spfree)) st_spin2(p.sp, n, i0, v);
+    } else {
+      st_spin2(p.snew, n, i0, v);
+    }
+    if constexpr (PREC == PR_FP32) st_g32_2(p.w32, i0, v);
+  } else {
+#pragma unroll
+    for (int h = 0; h < P; ++h) phase_spin<PREC, PHASE>(p, i0 + h, s[h][0], s[h][1], s[h][2], own[h]);
+    ln_field<PREC, DIAG, P>(ln, sid, xw, yw, PHASE == 0 ? p.s : p.sp, p.g32, n, lane, a);
+    if (fixed) return;
+#pragma unroll
+    for (int h = 0; h < P; ++h) {
+      const uint32_t c = p.cls ? (uint32_t)p.cls[i0 + h] : 0u;
+      heun_spin<PREC, PHASE, GEN>(p, i0 + h, c, step, t, s[h][0], s[h][1], s[h][2], a[h][0],
+                                  a[h][1], a[h][2], own[h]);
+    }
   }
 }
 
