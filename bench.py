@@ -623,10 +623,14 @@ def main() -> int:
     h = c3["fp64"]
     dom = max(h["phases"], key=lambda p: p["ms"])
     # counter-measured HBM bytes of the dominant phase kernel of this workload
-    # (k_heun<FP64, pattern-full, predictor or corrector>); null otherwise
+    # (the LINE kernel k_heun_ln<FP64>, or the gathering k_heun<FP64,
+    # pattern-full>, predictor or corrector); null otherwise
     traffic = None
+    is_line = " line=" in h["describe"]
+    kernel = "k_heun_ln<FP64, line>" if is_line else "k_heun<FP64, pattern-full>"
     if "exchange=pattern-full " in h["describe"] and args.L3 == 128:
-        traffic = read_traffic("traffic_c3_fp64.json", dom["phase"])
+        traffic = read_traffic("traffic_c3_fp64_line.json" if is_line else "traffic_c3_fp64.json",
+                               dom["phase"])
     f32 = c3["fp32_field"]
     line = {
         "metric": METRIC,
@@ -647,7 +651,7 @@ def main() -> int:
                                f"alpha=0.05, dt=0.5 fs, T=100 K (Philox noise), random start, "
                                f"FP64 Heun",
                    "spins": c3["n"], "engine": h["describe"]},
-        "roofline": {"bound": "hbm", "kernel": f"k_heun<FP64, pattern-full> {dom['phase']}",
+        "roofline": {"bound": "hbm", "kernel": f"{kernel} {dom['phase']}",
                      "achieved": dom["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": dom["gbs"] / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": dom["bytes"],
