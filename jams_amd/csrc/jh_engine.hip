@@ -29,6 +29,10 @@
 #include "jh_internal.h"
 #include "jh_launch.h"
 
+#ifndef JH_LN_BAND
+#define JH_LN_BAND 64
+#endif
+
 using namespace jh;
 
 // ---------------------------------------------------------------------------
@@ -75,6 +79,7 @@ constexpr double kBoltzmannIU = 0.0861733326;  // meV/K, helpers/consts.h:35
 constexpr size_t kMaxTableLds = 48 * 1024;     // bytes of LDS for the tensor table
 constexpr size_t kStMaxTileLds = 160 * 1024;   // bytes of LDS for a STENCIL tile (a CU's LDS)
 constexpr int64_t kLnAutoMinRow = 24;           // neighbours per row from which AUTO takes LINE
+constexpr int kLnBand = JH_LN_BAND;             // y rows per band of the LINE wave order
 // Tensor tables of the FP32 form (LDS table, wave-uniform sequences) keep
 // the exact FP64 values: they are per tensor, not per spin, so FP32 would
 // save no bandwidth, and rounding them perturbs the Hamiltonian itself (a
@@ -1195,6 +1200,7 @@ bool build_lines(jh_ctx *c, const XHost &x, bool diag) {
   LnView v{};
   v.lx = L[0]; v.ly = L[1]; v.lz = L[2]; v.nb = nb;
   v.per[0] = v.per[1] = 1;
+  v.ty = L[1] % kLnBand == 0 ? kLnBand : L[1];
   c->ln = v;
   c->ln_nseq = (int)seqs.size();
   c->ln_ngroups = ngroups;

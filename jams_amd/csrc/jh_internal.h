@@ -111,6 +111,7 @@ constexpr int kLnGroupWords = 8;     // program words per group
 struct LnView {
   int lx, ly, lz, nb;
   int per[2];              // periodic x and y (z is periodic)
+  int ty;                  // rows of y per band of the wave order (divides ly)
   // per sequence: (first group, number of groups).  A group is 8 words: a
   // bitmask of the entry codes slot | (dz + 2) << 1 present (one neighbour
   // each), the index of its first entry's values, two unused words and its

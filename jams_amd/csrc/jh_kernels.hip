@@ -1118,9 +1118,6 @@ __device__ __forceinline__ void ln_field(const LnView &ln, int sid, int xw, int 
   }
 }
 
-#ifndef JH_LN_BASIS_FASTEST
-#define JH_LN_BASIS_FASTEST 1
-#endif
 // One Heun phase over the LINE layout: workgroup -> 4 consecutive lines,
 // wave -> line (b, x, y), lane -> cells P lane .. P lane + P - 1.
 template <int PREC, bool DIAG, int PHASE, bool GEN, int P>
@@ -1153,21 +1150,19 @@ __global__ void __launch_bounds__(kLnBlock) k_heun_ln(const KParams p, const LnV
   }
   const int64_t blk = xcd_block(blockIdx.x - rlb, gridDim.x - rlb);
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  // waves take the lines basis site fastest, then y, then x, so the lines of
-  // one (x, y) column, which read each other, run side by side on one XCD
+  // waves take the lines basis site fastest, then y within a band of ty
+  // rows, then x, then band: the lines of one (x, y) column, which read each
+  // other, run side by side on one XCD, and the lines in flight on an XCD
+  // stay within a few x of each other in one band, so their neighbour lines
+  // stay in its L2
   const int64_t q = blk * (int64_t)(blockDim.x >> 6) + wave;
   const int lane = (int)(threadIdx.x & 63);
-#if JH_LN_BASIS_FASTEST
   const int b = (int)(q % ln.nb);
-  const int64_t r = q / ln.nb;
-  const int yw = (int)(r % ln.ly);
-  const int xw = (int)(r / ln.ly);
-#else
-  const int yw = (int)(q % ln.ly);
-  const int64_t r = q / ln.ly;
+  int64_t r = q / ln.nb;
+  const int yin = (int)(r % ln.ty);
+  r /= ln.ty;
   const int xw = (int)(r % ln.lx);
-  const int b = (int)(r / ln.lx);
-#endif
+  const int yw = (int)(r / ln.lx) * ln.ty + yin;
   const int64_t line = ((int64_t)b * ln.lx + xw) * ln.ly + yw;
   const int64_t i0 = line * ln.lz + (int64_t)P * lane;
   const int64_t n = p.n;
