@@ -80,6 +80,10 @@ constexpr size_t kMaxTableLds = 48 * 1024;     // bytes of LDS for the tensor ta
 constexpr size_t kStMaxTileLds = 160 * 1024;   // bytes of LDS for a STENCIL tile (a CU's LDS)
 constexpr int64_t kLnAutoMinRow = 24;           // neighbours per row from which AUTO takes LINE
 constexpr int kLnBand = JH_LN_BAND;             // y rows per band of the LINE wave order
+#ifndef JH_LN_PAIRS
+#define JH_LN_PAIRS 1
+#endif
+constexpr bool kLnPairs = JH_LN_PAIRS;          // FP32-form LINE pair programs
 // Tensor tables of the FP32 form (LDS table, wave-uniform sequences) keep
 // the exact FP64 values: they are per tensor, not per spin, so FP32 would
 // save no bandwidth, and rounding them perturbs the Hamiltonian itself (a
@@ -285,8 +289,9 @@ struct jh_ctx {
   bool ln_on = false;
   LnView ln{};
   DevBuf<int32_t> ln_seq, ln_prog, ln_wseq;
-  DevBuf<double> ln_val;
+  DevBuf<double> ln_val, ln_pval;
   DevBuf<uint64_t> ln_fixmask;
+  bool ln_pairs = false;
   int ln_nseq = 0, ln_ngroups = 0;
 
   // replicas: independent systems in consecutive index ranges
@@ -1017,6 +1022,105 @@ bool build_stencil(jh_ctx *c, const XHost &x, bool diag) {
 }
 
 // ---- LINE layout ---------------------------------------------------------
+
+// FP32-form pair program of one sequence (entries o of stencil sb): every
+// tensor J I + [D]x, and every line's entries matched with those of one
+// line (itself or another) whose tensors are their transposes, J I - [D]x,
+// at z offsets dz' = -dz + delta with |delta| <= 1.  A pair (e, e') adds
+// J (s_e + s_e') + D x (s_e - s_e'), which is T_e s_e + T_e' s_e'.  One
+// group per pair of lines (slot 0 the line of e, slot 1 of e'); its word 0
+// is the bitmask of the pair codes (dz + 2) * 3 + delta + 1 present, with
+// bit 31 marking a pair group; word 1 the index of its first pair's values
+// (J, D) in pval, 4 doubles a pair.  False if the sequence does not pair.
+static bool line_pair_program(const std::vector<StEnt> &sb, const std::vector<int> &o,
+                              const XHost &x, std::vector<int32_t> &seqv,
+                              std::vector<int32_t> &prog, std::vector<double> &pval,
+                              int &ngroups) {
+  struct PE {
+    int32_t lw;
+    int dz;
+    double J, D[3];
+    bool used;
+  };
+  std::vector<PE> es;
+  std::vector<int32_t> lines;  // in sequence order of first appearance
+  for (int k : o) {
+    const StEnt &e = sb[k];
+    const double *T = &x.tab[9 * (size_t)e.t];
+    if (!(T[0] == T[4] && T[0] == T[8] && T[1] == -T[3] && T[2] == -T[6] && T[5] == -T[7]))
+      return false;
+    const int32_t lw = (int32_t)(e.bb | ((e.d[0] + 2048) << 4) | ((e.d[1] + 2048) << 16));
+    es.push_back(PE{lw, e.d[2], T[0], {T[7], T[2], T[3]}, false});
+    if (std::find(lines.begin(), lines.end(), lw) == lines.end()) lines.push_back(lw);
+  }
+  auto partners = [](const PE &a, const PE &b) {
+    const int delta = a.dz + b.dz;
+    return a.J == b.J && a.D[0] == -b.D[0] && a.D[1] == -b.D[1] && a.D[2] == -b.D[2] &&
+           delta >= -1 && delta <= 1;
+  };
+  seqv.push_back((int32_t)(prog.size() / kLnGroupWords));
+  int groups = 0;
+  for (int32_t L : lines) {
+    std::vector<int> uL;
+    for (int k = 0; k < (int)es.size(); ++k)
+      if (es[k].lw == L && !es[k].used) uL.push_back(k);
+    if (uL.empty()) continue;
+    bool found = false;
+    std::vector<std::pair<int, int>> pairs;
+    int32_t Mfound = 0;
+    // the partner line: L itself first, then the others in order
+    std::vector<int32_t> cand{L};
+    for (int32_t M : lines)
+      if (M != L) cand.push_back(M);
+    for (int32_t M : cand) {
+      std::vector<int> uM;
+      for (int k = 0; k < (int)es.size(); ++k)
+        if (es[k].lw == M && !es[k].used) uM.push_back(k);
+      pairs.clear();
+      std::vector<char> taken(es.size(), 0);
+      bool ok = M == L ? uL.size() % 2 == 0 : uM.size() == uL.size();
+      for (size_t u = 0; ok && u < uL.size(); ++u) {
+        const int a = uL[u];
+        if (taken[a]) continue;
+        taken[a] = 1;
+        int pick = -1;
+        for (int bidx : uM)
+          if (!taken[bidx] && partners(es[a], es[bidx])) { pick = bidx; break; }
+        if (pick < 0) { ok = false; break; }
+        taken[pick] = 1;
+        pairs.push_back({a, pick});
+      }
+      if (ok) { found = true; Mfound = M; break; }
+    }
+    if (!found) return false;
+    int32_t w[kLnGroupWords] = {0};
+    uint32_t mask = 0;
+    std::vector<std::pair<int, std::pair<int, int>>> coded;
+    for (const auto &pr : pairs) {
+      const int code = (es[pr.first].dz + 2) * 3 + (es[pr.first].dz + es[pr.second].dz + 1);
+      if (mask & (1u << code)) return false;
+      mask |= 1u << code;
+      coded.push_back({code, pr});
+      es[pr.first].used = es[pr.second].used = true;
+    }
+    std::sort(coded.begin(), coded.end());
+    w[0] = (int32_t)(mask | 0x80000000u);
+    w[1] = (int32_t)(pval.size() / 4);
+    for (const auto &cp : coded) {
+      const PE &a = es[cp.second.first];
+      pval.push_back(a.J);
+      for (int m = 0; m < 3; ++m) pval.push_back(a.D[m]);
+    }
+    w[4] = L;
+    w[5] = Mfound;
+    prog.insert(prog.end(), w, w + kLnGroupWords);
+    ++groups;
+  }
+  seqv.push_back(groups);
+  ngroups += groups;
+  return true;
+}
+
 //
 // A fully periodic crystal with Lz = 64 or 128 and at most two basis sites
 // whose stencils reach at most two cells along z.  Each sequence becomes a
@@ -1063,6 +1167,14 @@ bool build_lines(jh_ctx *c, const XHost &x, bool diag) {
   }
   std::vector<uint8_t> fix;
   if (!check_rows(c, x, sten, seqs, wseq, fp64, fix)) return false;
+  // FP32 form with isotropic (+ DMI) tensors whose entries pair up as
+  // (T, T^T): pair programs (one group per pair of lines)
+  std::vector<int32_t> pseqv, pprog;
+  std::vector<double> ppval;
+  int pgroups = 0;
+  bool pairs_ok = !fp64 && kLnPairs;
+  for (size_t s = 0; s < seqs.size() && pairs_ok; ++s)
+    pairs_ok = line_pair_program(sten[seq_basis[s]], seqs[s], x, pseqv, pprog, ppval, pgroups);
   // programs: per sequence a list of groups of kLnGroupWords words (a
   // bitmask of the entry codes present, the index of its first entry's
   // values, two unused words, the two line words).  The code of an entry is
@@ -1161,6 +1273,11 @@ bool build_lines(jh_ctx *c, const XHost &x, bool diag) {
     seqv.push_back(groups);
     ngroups += groups;
   }
+  if (pairs_ok) {
+    seqv = pseqv;
+    prog = pprog;
+    ngroups = pgroups;
+  }
   // one spare entry of values: the kernels read the next entry's values
   // while summing the current one
   for (int m = 0; m < V; ++m) valv.push_back(0.0);
@@ -1174,6 +1291,13 @@ bool build_lines(jh_ctx *c, const XHost &x, bool diag) {
   c->ln_val.alloc(std::max<size_t>(valv.size(), 1));
   if (!valv.empty())
     HIPCHK(hipMemcpy(c->ln_val.p, valv.data(), sizeof(double) * valv.size(), hipMemcpyHostToDevice));
+  c->ln_pval.release();
+  c->ln_pairs = pairs_ok;
+  if (pairs_ok) {
+    c->ln_pval.alloc(std::max<size_t>(ppval.size(), 1));
+    if (!ppval.empty())
+      HIPCHK(hipMemcpy(c->ln_pval.p, ppval.data(), sizeof(double) * ppval.size(), hipMemcpyHostToDevice));
+  }
   c->ln_wseq.release();
   c->ln_fixmask.release();
   c->st_rows.release();
@@ -1683,6 +1807,7 @@ void launch_phase(jh_ctx *c, const KParams &p, int phase) {
     v.val = c->ln_val.p;
     v.wseq = c->ln_wseq.p;
     v.fixmask = c->ln_fixmask.p;
+    v.pval = c->ln_pairs ? c->ln_pval.p : nullptr;
     // the row list rides in the same launch (its first workgroups)
     const int64_t lines = (int64_t)v.nb * v.lx * v.ly;
     const int bs = lines % (kLnBlock / 64) == 0 ? kLnBlock : 64;
@@ -2531,8 +2656,9 @@ int jh_describe(jh_ctx *c, char *buf, int64_t buflen) {
     }
     if (c->ln_on) {
       char ln[160];
-      std::snprintf(ln, sizeof ln, " line=%dx64 sequences=%d groups=%d wrapped_rows=%lld",
-                    c->ln.lz / 64, c->ln_nseq, c->ln_ngroups, (long long)c->st_nfix);
+      std::snprintf(ln, sizeof ln, " line=%dx64 sequences=%d groups=%d pairs=%d wrapped_rows=%lld",
+                    c->ln.lz / 64, c->ln_nseq, c->ln_ngroups, c->ln_pairs ? 1 : 0,
+                    (long long)c->st_nfix);
       out += ln;
     }
     const int dip = dipole_term(c);

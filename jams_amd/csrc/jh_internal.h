@@ -121,6 +121,10 @@ struct LnView {
   const int32_t *prog;
   const double *val;       // per value index: 3 (diagonal) or 9 tensor values,
                            // and one spare index at the end
+  // FP32 form, pair groups (word 0 bit 31): bit (dz + 2) * 3 + delta + 1 of
+  // word 0 marks the pair of slot 0's neighbour at dz and slot 1's at
+  // -dz + delta; pval holds (J, D) per pair, 4 doubles, from index word 1
+  const double *pval;
   const int32_t *wseq;     // FP64: sequence of line (b, x, y) at (b lx + x) ly + y;
                            // null: sequence b (FP32 form)
   const uint64_t *fixmask; // FP64: per line, bit l marks lane l's cells as

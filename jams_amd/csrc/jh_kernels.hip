@@ -1051,6 +1051,34 @@ __device__ __forceinline__ void ln_code(const LnLine<PREC, P> (&S)[kLnSlots], ui
   }
 }
 
+// Pair code PC = (dz + 2) * 3 + delta + 1 of an FP32-form pair group, if
+// present: slot 0's neighbours at dz and slot 1's at -dz + delta, whose
+// tensors are J I + [D]x and its transpose, add J (s + s') + D x (s - s')
+// (spin sums and differences in FP32, products and sums in FP64, as the
+// gathering pair programs do); (J, D) are the next 4 values (Tp advances).
+template <bool DIAG, int P, int PC>
+__device__ __forceinline__ void ln_pair(const LnLine<PR_FP32, P> (&S)[kLnSlots], uint32_t mask,
+                                        const double *__restrict__ &Tp, double (&a)[P][3]) {
+  if (mask & (1u << PC)) {
+    constexpr int dz = PC / 3 - 2;
+    constexpr int dz2 = -dz + (PC % 3 - 1);
+    float u[P][3], w[P][3];
+    ln_gather<P, dz>(S[0], u);
+    ln_gather<P, dz2>(S[1], w);
+    const double J = Tp[0];
+#pragma unroll
+    for (int h = 0; h < P; ++h) {
+      a[h][0] = __builtin_fma(J, (double)(u[h][0] + w[h][0]), a[h][0]);
+      a[h][1] = __builtin_fma(J, (double)(u[h][1] + w[h][1]), a[h][1]);
+      a[h][2] = __builtin_fma(J, (double)(u[h][2] + w[h][2]), a[h][2]);
+      if constexpr (!DIAG)
+        dmi_add(Tp + 1, u[h][0] - w[h][0], u[h][1] - w[h][1], u[h][2] - w[h][2], a[h][0],
+                a[h][1], a[h][2]);
+    }
+    Tp += 4;
+  }
+}
+
 // Exchange field of the wave's line (b, xw, yw) over its program sid.  The
 // two lines of group g + 1 are loaded (buffer B, or A) while group g's
 // entries sum, so a group's loads are in flight during the previous group's
@@ -1087,8 +1115,29 @@ __device__ __forceinline__ void ln_field(const LnView &ln, int sid, int xw, int 
     }
   };
   auto run_group = [&](const LnLine<PREC, P>(&S)[kLnSlots], const LnGroup &hg) {
-    const double *__restrict__ T = ln.val + (size_t)hg.vbase * V;
     const uint32_t m = hg.mask;
+    if constexpr (PREC == PR_FP32) {
+      if (m & 0x80000000u) {
+        const double *__restrict__ Tp = ln.pval + (size_t)hg.vbase * 4;
+        ln_pair<DIAG, P, 0>(S, m, Tp, a);
+        ln_pair<DIAG, P, 1>(S, m, Tp, a);
+        ln_pair<DIAG, P, 2>(S, m, Tp, a);
+        ln_pair<DIAG, P, 3>(S, m, Tp, a);
+        ln_pair<DIAG, P, 4>(S, m, Tp, a);
+        ln_pair<DIAG, P, 5>(S, m, Tp, a);
+        ln_pair<DIAG, P, 6>(S, m, Tp, a);
+        ln_pair<DIAG, P, 7>(S, m, Tp, a);
+        ln_pair<DIAG, P, 8>(S, m, Tp, a);
+        ln_pair<DIAG, P, 9>(S, m, Tp, a);
+        ln_pair<DIAG, P, 10>(S, m, Tp, a);
+        ln_pair<DIAG, P, 11>(S, m, Tp, a);
+        ln_pair<DIAG, P, 12>(S, m, Tp, a);
+        ln_pair<DIAG, P, 13>(S, m, Tp, a);
+        ln_pair<DIAG, P, 14>(S, m, Tp, a);
+        return;
+      }
+    }
+    const double *__restrict__ T = ln.val + (size_t)hg.vbase * V;
     ln_code<PREC, DIAG, P, 0>(S, m, T, a);
     ln_code<PREC, DIAG, P, 1>(S, m, T, a);
     ln_code<PREC, DIAG, P, 2>(S, m, T, a);

@@ -140,6 +140,16 @@ def test_line_fp32_field_trajectory(engine_factory, name):
     assert ang.max() < 1e-5
 
 
+def test_line_fp32_pair_programs_are_used(engine_factory):
+    """The five-shell DMI template pairs every bond (T, T^T) in the FP32-field
+    LINE programs; the two-shell isotropic one too (diagonal tensors)."""
+    for name in ("dmi_lz128", "two_shell_lz64"):
+        e = line_engine(engine_factory, SYSTEMS[name](), prec=E.PREC_FP32_FIELD)
+        assert " pairs=1 " in e.describe(), e.describe()
+    e = line_engine(engine_factory, SYSTEMS["dmi_lz128"]())
+    assert " pairs=0 " in e.describe(), e.describe()
+
+
 def test_line_fp32_field_one_step(engine_factory):
     """One FP32-field step over the LINE layout from random spins against the
     oracle's FP64 step: FP32 spins in an FP64 sum move s by far less than
