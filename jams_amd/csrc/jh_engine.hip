@@ -1136,6 +1136,8 @@ bool build_lines(jh_ctx *c, const XHost &x, bool diag) {
   const int L[3] = {c->lsize[0], c->lsize[1], c->lsize[2]};
   const int nb = c->lnb;
   if (nb > kLnSlots || (L[2] != 64 && L[2] != 128)) return false;
+  // the kernels address cells with 32-bit offsets (3 N doubles per array)
+  if (c->n >= ((int64_t)1 << 31) / 3) return false;
   if (!c->lper[0] || !c->lper[1] || !c->lper[2]) return false;
   const int P = L[2] / 64;
   for (int b = 0; b < nb; ++b) {

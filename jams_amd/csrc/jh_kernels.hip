@@ -935,34 +935,39 @@ template <int PREC, int P>
 __device__ __forceinline__ void ln_load(const LnView &ln, const double *__restrict__ src64,
                                         const float *__restrict__ src32, int64_t n, int bb,
                                         int xx, int yy, int lane, LnLine<PREC, P> &L) {
-  const int64_t base = (((int64_t)bb * ln.lx + xx) * ln.ly + yy) * ln.lz + (int64_t)P * lane;
+  // the line's first cell (wave-uniform, 32-bit: the host keeps N below
+  // 2^31 / 3) and the lane's cell within it
+  const uint32_t cell0 = (((uint32_t)bb * (uint32_t)ln.lx + (uint32_t)xx) * (uint32_t)ln.ly +
+                          (uint32_t)yy) * (uint32_t)ln.lz;
+  const uint32_t lc = (uint32_t)(P * lane);
   if constexpr (PREC == PR_FP64) {
-    const double2 *xy = reinterpret_cast<const double2 *>(src64) + base;
+    const double2 *xy = reinterpret_cast<const double2 *>(src64) + cell0;
 #pragma unroll
     for (int h = 0; h < P; ++h) {
-      const double2 v = xy[h];
+      const double2 v = xy[lc + h];
       L.v[h][0] = v.x;
       L.v[h][1] = v.y;
     }
+    const double *zl = src64 + 2 * n + cell0;
     if constexpr (P == 2) {
-      const double2 z = *reinterpret_cast<const double2 *>(src64 + 2 * n + base);
+      const double2 z = *reinterpret_cast<const double2 *>(zl + lc);
       L.v[0][2] = z.x;
       L.v[1][2] = z.y;
     } else {
-      L.v[0][2] = src64[2 * n + base];
+      L.v[0][2] = zl[lc];
     }
   } else if constexpr (P == 2 && kG32 == 3) {
     // the lane's 24 bytes (8-byte aligned) as one 16- and one 8-byte load,
     // taken apart in registers
     typedef float f4a8 __attribute__((ext_vector_type(4), aligned(8)));
     typedef float f2a8 __attribute__((ext_vector_type(2), aligned(8)));
-    const float *q = src32 + (size_t)kG32 * base;
+    const float *q = src32 + (size_t)kG32 * cell0 + kG32 * lc;
     const f4a8 u = *reinterpret_cast<const f4a8 *>(q);
     const f2a8 w = *reinterpret_cast<const f2a8 *>(q + 4);
     L.v[0][0] = u.x; L.v[0][1] = u.y; L.v[0][2] = u.z;
     L.v[1][0] = u.w; L.v[1][1] = w.x; L.v[1][2] = w.y;
   } else {
-    const float *q = src32 + (size_t)kG32 * base;
+    const float *q = src32 + (size_t)kG32 * cell0 + kG32 * lc;
 #pragma unroll
     for (int h = 0; h < P; ++h) {
       L.v[h][0] = q[kG32 * h];
@@ -1054,8 +1059,8 @@ __device__ __forceinline__ void ln_code(const LnLine<PREC, P> (&S)[kLnSlots], ui
 // Pair code PC = (dz + 2) * 3 + delta + 1 of an FP32-form pair group, if
 // present: slot 0's neighbours at dz and slot 1's at -dz + delta, whose
 // tensors are J I + [D]x and its transpose, add J (s + s') + D x (s - s')
-// (spin sums and differences in FP32, products and sums in FP64, as the
-// gathering pair programs do); (J, D) are the next 4 values (Tp advances).
+// (the FP32 spins' sums and differences, exact in FP64, and FP64 products
+// and sums); (J, D) are the next 4 values (Tp advances).
 template <bool DIAG, int P, int PC>
 __device__ __forceinline__ void ln_pair(const LnLine<PR_FP32, P> (&S)[kLnSlots], uint32_t mask,
                                         const double *__restrict__ &Tp, double (&a)[P][3]) {
@@ -1068,12 +1073,19 @@ __device__ __forceinline__ void ln_pair(const LnLine<PR_FP32, P> (&S)[kLnSlots],
     const double J = Tp[0];
 #pragma unroll
     for (int h = 0; h < P; ++h) {
-      a[h][0] = __builtin_fma(J, (double)(u[h][0] + w[h][0]), a[h][0]);
-      a[h][1] = __builtin_fma(J, (double)(u[h][1] + w[h][1]), a[h][1]);
-      a[h][2] = __builtin_fma(J, (double)(u[h][2] + w[h][2]), a[h][2]);
-      if constexpr (!DIAG)
-        dmi_add(Tp + 1, u[h][0] - w[h][0], u[h][1] - w[h][1], u[h][2] - w[h][2], a[h][0],
-                a[h][1], a[h][2]);
+      // the sum and difference of two FP32 values are exact in FP64
+      const double u0 = u[h][0], u1 = u[h][1], u2 = u[h][2];
+      const double w0 = w[h][0], w1 = w[h][1], w2 = w[h][2];
+      a[h][0] = __builtin_fma(J, u0 + w0, a[h][0]);
+      a[h][1] = __builtin_fma(J, u1 + w1, a[h][1]);
+      a[h][2] = __builtin_fma(J, u2 + w2, a[h][2]);
+      if constexpr (!DIAG) {
+        const double *D = Tp + 1;
+        const double X = u0 - w0, Y = u1 - w1, Z = u2 - w2;
+        a[h][0] = __builtin_fma(D[1], Z, a[h][0]); a[h][0] = __builtin_fma(-D[2], Y, a[h][0]);
+        a[h][1] = __builtin_fma(D[2], X, a[h][1]); a[h][1] = __builtin_fma(-D[0], Z, a[h][1]);
+        a[h][2] = __builtin_fma(D[0], Y, a[h][2]); a[h][2] = __builtin_fma(-D[1], X, a[h][2]);
+      }
     }
     Tp += 4;
   }
