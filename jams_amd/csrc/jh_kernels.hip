@@ -1181,8 +1181,16 @@ __device__ __forceinline__ void ln_field(const LnView &ln, int sid, int xw, int 
 
 // One Heun phase over the LINE layout: workgroup -> 4 consecutive lines,
 // wave -> line (b, x, y), lane -> cells P lane .. P lane + P - 1.
+// Waves per SIMD the compiler keeps VGPR room for: the predictors fit 5
+// (96 VGPRs or fewer) without spilling, the correctors would spill.
+#ifndef JH_LN_PRED_WAVES
+#define JH_LN_PRED_WAVES 5
+#endif
+template <int PHASE>
+constexpr int kLnWaves = PHASE == 0 ? JH_LN_PRED_WAVES : 1;
 template <int PREC, bool DIAG, int PHASE, bool GEN, int P>
-__global__ void __launch_bounds__(kLnBlock) k_heun_ln(const KParams p, const LnView ln) {
+__global__ void __launch_bounds__(kLnBlock)
+__attribute__((amdgpu_waves_per_eu(kLnWaves<PHASE>))) k_heun_ln(const KParams p, const LnView ln) {
   const int rlb = ln.rlblocks;
   if ((int)blockIdx.x < rlb) {
     // FP64 row list: the rows whose z neighbours wrap, each summed in its own
