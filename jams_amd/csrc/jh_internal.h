@@ -227,6 +227,7 @@ struct KParams {
   double dt;
   double t;
   int order_nb;         // basis-major period (1: reference order)
+  int basis_ilv;        // gathering step kernels: workgroups alternate basis sites
   int64_t order_cells;  // N / order_nb
   // row-list launch (the FP64 stencil's wrapped rows): thread j computes
   // device row rowlist[j] (pattern format) instead of row j

@@ -669,7 +669,14 @@ __attribute__((amdgpu_waves_per_eu(kMinWaves<PREC, XVk>))) k_heun(const KParams 
   extern __shared__ __align__(16) unsigned char lds[];
   load_table<PREC, XVk>(p.x, lds);
   if constexpr (xv_typed(XVk)) __syncthreads();
-  const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
+  int64_t blk = xcd_block(blockIdx.x, gridDim.x);
+  if (p.basis_ilv && !p.rowlist && p.order_nb > 1 && p.order_cells % blockDim.x == 0) {
+    // workgroups alternate between the basis sites' index ranges, so the
+    // workgroups of one XCD hold the same cells of every basis site and the
+    // other sites' spins their rows gather stay in that XCD's L2
+    const int64_t per = p.order_cells / blockDim.x;
+    blk = (blk % p.order_nb) * per + blk / p.order_nb;
+  }
   int64_t i = blk * blockDim.x + threadIdx.x;
   if (p.rowlist) {
     if (i >= p.nrowlist) return;
