@@ -1195,6 +1195,13 @@ __device__ __forceinline__ void ln_field(const LnView &ln, int sid, int xw, int 
 #endif
 template <int PHASE>
 constexpr int kLnWaves = PHASE == 0 ? JH_LN_PRED_WAVES : 1;
+// The corrector's own s, ds and noise: read before the field (FP32-field
+// form: it forms s* from them) or after it (FP64, fewer live registers)
+#ifndef JH_LN_EARLY64
+#define JH_LN_EARLY64 0
+#endif
+template <int PREC>
+constexpr bool kLnEarlyOwn = kEarlyOwn<PREC> || JH_LN_EARLY64;
 template <int PREC, bool DIAG, int PHASE, bool GEN, int P>
 __global__ void __launch_bounds__(kLnBlock)
 __attribute__((amdgpu_waves_per_eu(kLnWaves<PHASE>))) k_heun_ln(const KParams p, const LnView ln) {
@@ -1254,9 +1261,11 @@ __attribute__((amdgpu_waves_per_eu(kLnWaves<PHASE>))) k_heun_ln(const KParams p,
   if constexpr (P == 2) {
     // the lane's two cells are adjacent: each array is one 16-byte access
     // per (x, y) pair and per SoA component
+    double w[2][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
     if constexpr (PHASE == 0) {
       ld_spin2(p.s, n, i0, s);
-    } else if constexpr (kEarlyOwn<PREC>) {
+    } else if constexpr (kLnEarlyOwn<PREC>) {
+      if (p.noise == NM_PHILOX) ld_soa2(p.wbuf, n, i0, w);
       double os[2][3], od[2][3];
       ld_spin2(p.s, n, i0, os);
       ld_soa2(p.ds, n, i0, od);
@@ -1277,10 +1286,9 @@ __attribute__((amdgpu_waves_per_eu(kLnWaves<PHASE>))) k_heun_ln(const KParams p,
     }
     ln_field<PREC, DIAG, P>(ln, sid, xw, yw, PHASE == 0 ? p.s : p.sp, p.g32, n, lane, a);
     if (fixed) return;
-    double w[2][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
-    if (p.noise == NM_PHILOX && PHASE == 1) ld_soa2(p.wbuf, n, i0, w);
+    if (p.noise == NM_PHILOX && PHASE == 1 && !kLnEarlyOwn<PREC>) ld_soa2(p.wbuf, n, i0, w);
     else if (p.noise == NM_INJECTED) ld_soa2(p.winj, n, i0, w);
-    if constexpr (PHASE == 1 && !kEarlyOwn<PREC>) {
+    if constexpr (PHASE == 1 && !kLnEarlyOwn<PREC>) {
       double os[2][3], od[2][3];
       ld_spin2(p.s, n, i0, os);
       ld_soa2(p.ds, n, i0, od);
