@@ -1478,8 +1478,8 @@ void prepare_exchange(jh_ctx *c) {
     if (c->xfmt_req == JH_XFMT_LINE && !c->ln_on)
       JH_THROW(JH_ERR_UNSUPPORTED,
                "LINE exchange format does not apply (needs jh_set_lattice, one periodic crystal "
-               "of at most two basis sites whose rows all translate one stencil reaching at "
-               "most two cells along z, Lz = 64 or 128, Heun)");
+               "of at most two basis sites, one with a dipole term, whose rows all translate "
+               "one stencil reaching at most two cells along z, Lz = 64 or 128, Heun)");
     if (c->ln_on) c->xfmt = JH_XFMT_LINE;
     return;
   }

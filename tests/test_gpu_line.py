@@ -189,3 +189,65 @@ def test_autotune_with_line_candidate_keeps_the_bits(engine_factory):
     e.step(40)
     got = e.get_spins()
     assert np.array_equal(got, oracle_run(b, 23, 40))
+
+
+def _general_engine(factory, b, fmt, prec, dipole):
+    """The system's exchange plus the other terms the Heun kernels fuse:
+    uniaxial anisotropy of power 4 about a tilted axis and an ac Zeeman field
+    (both take the general path, GEN), optionally the dipole field (FFT),
+    then the thermostat."""
+    n = b.n
+    e = factory()
+    e.set_sites(b.mus, b.gyro, b.alpha)
+    e.set_lattice(b.lat.size, b.lat.periodic, b.lat.nbasis)
+    e.add_exchange_csr(*b.csr)
+    axis = np.tile(np.array([0.36, 0.48, 0.8]), (n, 1))
+    e.add_uniaxial(np.full(n, 0.05), axis, 4)
+    dc = b.mus[:, None] * np.array([0.0, 0.2, 1.0])[None, :]
+    ac = b.mus[:, None] * np.array([0.5, 0.0, 0.0])[None, :]
+    e.add_zeeman(dc, ac, np.full(n, 2.0e3))
+    if dipole:
+        e.add_dipole_fft(b.lat.size, b.lat.periodic, b.lat.nbasis, b.lat.cell,
+                         b.lat.basis_frac, 0.3e-9, 2.0)
+    e.set_time_step(b.dt)
+    e.set_precision(prec)
+    e.set_exchange_format(fmt)
+    e.set_spins(b.s0)
+    e.set_thermostat(100.0, 61)
+    return e
+
+
+@pytest.mark.parametrize("name,dipole", [("dmi_lz64", False), ("sc_nn_zeeman_lz128", True)])
+def test_line_general_terms_match_pattern(engine_factory, name, dipole):
+    """With the general local terms (power-4 anisotropy, ac Zeeman: the GEN
+    kernels), and on the one-basis lattice a dipole field (which keeps the
+    reference spin order, so LINE with it needs one basis site), the LINE
+    kernels give the gathering PATTERN kernels' FP64 trajectory bit for bit
+    (both sum the same ordered rows and share the Heun arithmetic), and the
+    FP32-field forms agree to 1e-7 per component after 30 steps (their FP32
+    roundings differ: the LINE pair sums are exact, the gathering ones are
+    FP32)."""
+    b = SYSTEMS[name]()
+    got = {}
+    for prec in (E.PREC_FP64, E.PREC_FP32_FIELD):
+        for fmt in (E.XFMT_LINE, E.XFMT_PATTERN):
+            e = _general_engine(engine_factory, b, fmt, prec, dipole)
+            d = e.describe()
+            assert (" line=" in d) == (fmt == E.XFMT_LINE), d
+            e.step(30)
+            got[(prec, fmt)] = e.get_spins()
+    assert np.array_equal(got[(E.PREC_FP64, E.XFMT_LINE)], got[(E.PREC_FP64, E.XFMT_PATTERN)])
+    a, p = got[(E.PREC_FP32_FIELD, E.XFMT_LINE)], got[(E.PREC_FP32_FIELD, E.XFMT_PATTERN)]
+    # |a - p| rather than an arccos, which cannot resolve angles below ~2e-8
+    diff = np.abs(a - p).max()
+    print(f"{name}: FP32-field LINE vs PATTERN, general terms, 30 steps: max |ds| {diff:.3e}")
+    assert diff < 1e-7
+
+
+def test_line_refuses_two_basis_sites_with_a_dipole(engine_factory):
+    """The dipole term keeps the reference spin order, which LINE cannot use
+    with two basis sites: asking for it is an error."""
+    b = SYSTEMS["dmi_lz64"]()
+    e = _general_engine(engine_factory, b, E.XFMT_LINE, E.PREC_FP64, True)
+    with pytest.raises(Exception, match="LINE"):
+        e.step(1)
