@@ -1106,6 +1106,7 @@ static bool line_pair_program(const std::vector<StEnt> &sb, const std::vector<in
     std::sort(coded.begin(), coded.end());
     w[0] = (int32_t)(mask | 0x80000000u);
     w[1] = (int32_t)(pval.size() / 4);
+    w[2] = 3;  // both slots
     for (const auto &cp : coded) {
       const PE &a = es[cp.second.first];
       pval.push_back(a.J);
@@ -1223,6 +1224,8 @@ bool build_lines(jh_ctx *c, const XHost &x, bool diag) {
         for (int m = 0; m < V; ++m) valv.push_back(diag ? x.tab[9 * e.t + 4 * m] : x.tab[9 * e.t + m]);
       }
       w[0] = (int32_t)mask;
+      // the slots the entries read: a slot no entry reads is not loaded
+      for (const auto &pe : ents) w[2] |= 1 << (pe.first & 1);
       for (int k = 0; k < kLnSlots; ++k) w[4 + k] = lines[k] != -1 ? lines[k] : fill;
       prog.insert(prog.end(), w, w + kLnGroupWords);
       ++groups;

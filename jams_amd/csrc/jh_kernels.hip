@@ -1123,8 +1123,12 @@ __device__ __forceinline__ void ln_field(const LnView &ln, int sid, int xw, int 
     const int32_t *w = prog + (size_t)kLnGroupWords * g;
     hg.mask = (uint32_t)__builtin_amdgcn_readfirstlane(w[0]);
     hg.vbase = __builtin_amdgcn_readfirstlane(w[1]);
+    const int used = __builtin_amdgcn_readfirstlane(w[2]);
 #pragma unroll
     for (int k = 0; k < kLnSlots; ++k) {
+#if JH_LN_SLOT_MASK
+      if (!((used >> k) & 1)) continue;  // no entry reads this slot
+#endif
       const int lw = __builtin_amdgcn_readfirstlane(w[4 + k]);
       int xx = xw + ((lw >> 4) & 0xFFF) - 2048;
       int yy = yw + ((lw >> 16) & 0xFFF) - 2048;
@@ -1190,6 +1194,9 @@ __device__ __forceinline__ void ln_field(const LnView &ln, int sid, int xw, int 
 // wave -> line (b, x, y), lane -> cells P lane .. P lane + P - 1.
 // Waves per SIMD the compiler keeps VGPR room for: the predictors fit 5
 // (96 VGPRs or fewer) without spilling, the correctors would spill.
+#ifndef JH_LN_SLOT_MASK
+#define JH_LN_SLOT_MASK 1
+#endif
 #ifndef JH_LN_PRED_WAVES
 #define JH_LN_PRED_WAVES 5
 #endif
