@@ -364,6 +364,22 @@ def measure_c3(L: int, steps: int, warmup: int, reps: int, accuracy_steps: int,
                                            / out["fp32_field"]["median5_ms_per_step"])
     log(f"  C3 speed-up FP32-field / FP64: {out['speedup_fp32_vs_fp64']:.3f} "
         f"(steady state {out['median5_speedup_fp32_vs_fp64']:.3f})")
+    # beside them, FP64 with the gathering row-pattern kernels (the plain
+    # path before the whole-line layout), steady state only
+    if " line=" in out["fp64"]["describe"]:
+        from jams_amd.engine import XFMT_AUTO, XFMT_PATTERN
+        eng.set_precision(PREC_FP64)
+        eng.set_exchange_format(XFMT_PATTERN)
+        eng.set_spins(s_random)
+        eng.set_iteration(0)
+        _settle(eng)
+        runs = sorted(eng.time_steps(500) / 500 for _ in range(5))
+        out["fp64_gathering"] = {"median5_ms_per_step": runs[2], "describe": eng.describe()}
+        out["median5_speedup_fp32_vs_fp64_gathering"] = (runs[2] /
+                                                         out["fp32_field"]["median5_ms_per_step"])
+        log(f"  C3 fp64 gathering kernels, steady state: {runs[2]:.4f} ms/step; FP32-field "
+            f"(whole lines) / FP64 gathering: {out['median5_speedup_fp32_vs_fp64_gathering']:.3f}")
+        eng.set_exchange_format(XFMT_AUTO)
     if accuracy_steps > 0:
         acc = {"steps": accuracy_steps, "tail_steps": tail, "sample_every": every}
         for label in starts:
@@ -662,7 +678,10 @@ def main() -> int:
                        "speedup_vs_fp64": c3["speedup_fp32_vs_fp64"], "phases": f32["phases"],
                        "engine": f32["describe"],
                        "median5_ms_per_step": f32["median5_ms_per_step"],
-                       "median5_speedup_vs_fp64": c3["median5_speedup_fp32_vs_fp64"]},
+                       "median5_speedup_vs_fp64": c3["median5_speedup_fp32_vs_fp64"],
+                       "median5_speedup_vs_fp64_gathering":
+                           c3.get("median5_speedup_fp32_vs_fp64_gathering")},
+        "fp64_gathering_median5_ms_per_step": c3.get("fp64_gathering", {}).get("median5_ms_per_step"),
         "fp64_median5_ms_per_step": h["median5_ms_per_step"],
         "cpu_baseline": cpu,
     }
