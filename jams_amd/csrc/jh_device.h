@@ -18,16 +18,37 @@ namespace jh {
 // 0xCD9E8D57; Weyl key increments 0x9E3779B9, 0xBB67AE85; 10 rounds.
 struct u32x4 { uint32_t x, y, z, w; };
 
+// a ^ b ^ k as one v_bitop3_b32 (truth table 0x96, odd parity, which is
+// symmetric in its three inputs); the compiler issues two v_xor_b32.  UK: k
+// is wave-uniform and is read from a scalar register; otherwise from a
+// vector register (a divergent value must never take the scalar form).
+template <bool UK>
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t k) {
+  uint32_t r;
+  if constexpr (UK)
+    asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0x96" : "=v"(r) : "v"(a), "v"(b), "s"(k));
+  else
+    asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0x96" : "=v"(r) : "v"(a), "v"(b), "v"(k));
+  return r;
+}
+
 // Each round's two 32 x 32 -> 64-bit products are formed whole (one
-// v_mad_u64_u32 each, not a low and a high multiply).
+// v_mad_u64_u32 each, not a low and a high multiply).  UK: the key (k0, k1)
+// is the same in every lane.
+template <bool UK>
 __device__ __forceinline__ u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1) {
+  if constexpr (UK) {
+    // each call forms its own ten round keys as it goes (scalar adds):
+    // shared between calls, the twenty would stay live and spill
+    asm volatile("" : "+s"(k0), "+s"(k1));
+  }
 #pragma unroll
   for (int r = 0; r < 10; ++r) {
     const uint64_t p0 = (uint64_t)0xD2511F53u * c.x;
     const uint64_t p1 = (uint64_t)0xCD9E8D57u * c.z;
     const uint32_t lo0 = (uint32_t)p0, hi0 = (uint32_t)(p0 >> 32);
     const uint32_t lo1 = (uint32_t)p1, hi1 = (uint32_t)(p1 >> 32);
-    c = u32x4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+    c = u32x4{xor3<UK>(hi1, c.y, k0), lo1, xor3<UK>(hi0, c.w, k1), lo0};
     k0 += 0x9E3779B9u;
     k1 += 0xBB67AE85u;
   }
@@ -103,13 +124,14 @@ __device__ __forceinline__ void noise_sincos2pi(double u, double &c, double &s) 
 // Three standard normal deviates of the thermostat stream for (seed, step,
 // spin): two Philox blocks with counter (spin lo, call | spin hi << 1,
 // step lo, step hi) and key (seed lo, seed hi), Box-Muller on each block.
+template <bool UK>
 __device__ __forceinline__ void normals3(uint64_t seed, uint64_t step, uint64_t spin,
                                          double &z0, double &z1, double &z2) {
   const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
   const uint32_t shi = (uint32_t)(spin >> 32) << 1;
   double c, s;
   {
-    const u32x4 a = philox4x32_10(
+    const u32x4 a = philox4x32_10<UK>(
         u32x4{(uint32_t)spin, 0u | shi, (uint32_t)step, (uint32_t)(step >> 32)}, k0, k1);
     const double r0 = sqrt(-2.0 * noise_log(u01(a.x, a.y)));
     noise_sincos2pi(u01(a.z, a.w), c, s);
@@ -117,7 +139,7 @@ __device__ __forceinline__ void normals3(uint64_t seed, uint64_t step, uint64_t 
     z1 = r0 * s;
   }
   {
-    const u32x4 b = philox4x32_10(
+    const u32x4 b = philox4x32_10<UK>(
         u32x4{(uint32_t)spin, 1u | shi, (uint32_t)step, (uint32_t)(step >> 32)}, k0, k1);
     const double r1 = sqrt(-2.0 * noise_log(u01(b.x, b.y)));
     noise_sincos2pi(u01(b.z, b.w), c, s);

@@ -1775,6 +1775,7 @@ KParams make_params(jh_ctx *c) {
   p.dt = c->dt;
   p.order_nb = c->order_nb;
   p.order_cells = c->order_cells;
+  p.order_rcells = c->order_cells > 0 ? 1.0 / (double)c->order_cells : 0.0;
   // short rows (the two-shell bcc workload): alternating the basis sites'
   // workgroups keeps each XCD's gathers of the other site in its L2 (C2: HBM
   // reads 40 % lower in the predictor, 19 % in the corrector, same speed);

@@ -229,6 +229,7 @@ struct KParams {
   int order_nb;         // basis-major period (1: reference order)
   int basis_ilv;        // gathering step kernels: workgroups alternate basis sites
   int64_t order_cells;  // N / order_nb
+  double order_rcells;  // 1 / order_cells, rounded
   // row-list launch (the FP64 stencil's wrapped rows): thread j computes
   // device row rowlist[j] (pattern format) instead of row j
   const int32_t *rowlist;
@@ -236,9 +237,24 @@ struct KParams {
 };
 
 // reference spin index of device index d (the noise counter uses it, so the
-// stream does not depend on the device layout)
+// stream does not depend on the device layout).  In the basis-major order
+// d = b cells + cell and the reference index is cell nb + b.  b = d / cells
+// comes from the double quotient d (1 / cells), truncated and corrected once:
+// for d < 2^53 that quotient is within 2^-21 of d / cells (< nb < 2^31), so
+// the truncation is the floor or one off it.  On the GPU this is a few vector
+// operations against several tens for a 64-bit integer division.
 __host__ __device__ inline int64_t ref_index(const KParams &p, int64_t d) {
-  return p.order_nb == 1 ? d : (d % p.order_cells) * p.order_nb + d / p.order_cells;
+  if (p.order_nb == 1) return d;
+  int b = (int)((double)d * p.order_rcells);
+  int64_t r = d - (int64_t)b * p.order_cells;
+  if (r < 0) {
+    b -= 1;
+    r += p.order_cells;
+  } else if (r >= p.order_cells) {
+    b += 1;
+    r -= p.order_cells;
+  }
+  return r * p.order_nb + b;
 }
 
 }  // namespace jh

@@ -417,8 +417,11 @@ __device__ __forceinline__ double cparam(const KParams &p, uint32_t c, int f) {
 __device__ __forceinline__ uint64_t noise_seed(const KParams &p, uint32_t c) {
   return (uint64_t)__double_as_longlong(cparam(p, c, CF_SEED));
 }
+__device__ __forceinline__ uint64_t noise_index_of_ref(const KParams &p, uint32_t c, int64_t ri) {
+  return (uint64_t)ri - (uint64_t)__double_as_longlong(cparam(p, c, CF_ROFF));
+}
 __device__ __forceinline__ uint64_t noise_index(const KParams &p, uint32_t c, int64_t i) {
-  return (uint64_t)ref_index(p, i) - (uint64_t)__double_as_longlong(cparam(p, c, CF_ROFF));
+  return noise_index_of_ref(p, c, ref_index(p, i));
 }
 
 // Local (non-exchange) term q of spin i with spin vector (s0, s1, s2).
@@ -550,15 +553,17 @@ __device__ __forceinline__ void phase_spin(const KParams &p, int64_t i, double &
 // Everything of a Heun phase after the exchange field (x0, x1, x2) of spin i
 // at (s0, s1, s2): the ordered field sum, the thermal field, h / mus, the LLG
 // right-hand side and the phase's update and stores.
-// The arithmetic of a Heun phase for spin i after its exchange field
-// (x0, x1, x2) at (s0, s1, s2): the ordered field sum, the thermal field,
-// h / mus, the LLG right-hand side and the update.  w is the step's thermal
+// The arithmetic of a Heun phase for spin i (reference index ri) after its
+// exchange field (x0, x1, x2) at (s0, s1, s2): the ordered field sum, the
+// thermal field, h / mus, the LLG right-hand side and the update.  w is the
+// step's thermal
 // field: read by the caller in the corrector (or injected), drawn here in the
 // predictor from the Philox stream.  Out: the predictor's ds = rhs / 2 in
 // (d0, d1, d2) and s* in (v0, v1, v2); the corrector's new s in (v0, v1, v2)
 // from its own s and ds in `own`.
 template <int PREC, int PHASE, bool GEN>
-__device__ __forceinline__ void heun_math(const KParams &p, int64_t i, uint32_t c, uint64_t step,
+__device__ __forceinline__ void heun_math(const KParams &p, int64_t i, int64_t ri, uint32_t c,
+                                          uint64_t step,
                                           double t, double s0, double s1, double s2, double x0,
                                           double x1, double x2, const OwnState &own, double &w0,
                                           double &w1, double &w2, double &d0, double &d1,
@@ -572,11 +577,11 @@ __device__ __forceinline__ void heun_math(const KParams &p, int64_t i, uint32_t 
   if (p.noise == NM_PHILOX && PHASE == 0) {
     double z0, z1, z2, sT;
     if (p.cls) {
-      normals3(noise_seed(p, c), step, noise_index(p, c, i), z0, z1, z2);
+      normals3<false>(noise_seed(p, c), step, noise_index_of_ref(p, c, ri), z0, z1, z2);
       sT = cparam(p, c, CF_SQRTT);
     } else {
       // one site class: the key schedule stays in scalar registers
-      normals3(p.seed, step, (uint64_t)ref_index(p, i) - p.roff, z0, z1, z2);
+      normals3<true>(p.seed, step, (uint64_t)ri - p.roff, z0, z1, z2);
       sT = p.sqrtT;
     }
     const double sg = cparam(p, c, CF_SIGMA);
@@ -639,7 +644,8 @@ __device__ __forceinline__ void heun_spin(const KParams &p, int64_t i, uint32_t 
   OwnState o = own;
   if constexpr (PHASE == 1 && !kEarlyOwn<PREC>) load_own(p, i, o);
   double d0, d1, d2, v0, v1, v2;
-  heun_math<PREC, PHASE, GEN>(p, i, c, step, t, s0, s1, s2, x0, x1, x2, o, w0, w1, w2, d0, d1,
+  heun_math<PREC, PHASE, GEN>(p, i, ref_index(p, i), c, step, t, s0, s1, s2, x0, x1, x2, o, w0,
+                              w1, w2, d0, d1,
                               d2, v0, v1, v2);
   if constexpr (PHASE == 0) {
     if (p.noise == NM_PHILOX) { p.wbuf[i] = w0; p.wbuf[n + i] = w1; p.wbuf[2 * n + i] = w2; }
@@ -1304,10 +1310,13 @@ __attribute__((amdgpu_waves_per_eu(kLnWaves<PHASE>))) k_heun_ln(const KParams p,
         own[h] = OwnState{os[h][0], os[h][1], os[h][2], od[h][0], od[h][1], od[h][2]};
     }
     double d[2][3], v[2][3];
+    // reference index of cell 0: (cell of the line's (x, y) column + z) nb + b
+    const int64_t r0 =
+        (((int64_t)xw * ln.ly + yw) * ln.lz + (int64_t)P * lane) * ln.nb + b;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const uint32_t c = p.cls ? (uint32_t)p.cls[i0 + h] : 0u;
-      heun_math<PREC, PHASE, GEN>(p, i0 + h, c, step, t, s[h][0], s[h][1], s[h][2], a[h][0],
+      heun_math<PREC, PHASE, GEN>(p, i0 + h, r0 + h * ln.nb, c, step, t, s[h][0], s[h][1], s[h][2], a[h][0],
                                   a[h][1], a[h][2], own[h], w[h][0], w[h][1], w[h][2], d[h][0],
                                   d[h][1], d[h][2], v[h][0], v[h][1], v[h][2]);
     }
@@ -1372,10 +1381,10 @@ __attribute__((amdgpu_waves_per_eu(kMinWaves<PREC, XVk>))) k_rk4(const KParams p
   if (p.noise == NM_PHILOX && stage == 0) {
     double z0, z1, z2, sT;
     if (p.cls) {
-      normals3(noise_seed(p, c), step, noise_index(p, c, i), z0, z1, z2);
+      normals3<false>(noise_seed(p, c), step, noise_index(p, c, i), z0, z1, z2);
       sT = cparam(p, c, CF_SQRTT);
     } else {
-      normals3(p.seed, step, (uint64_t)ref_index(p, i) - p.roff, z0, z1, z2);
+      normals3<true>(p.seed, step, (uint64_t)ref_index(p, i) - p.roff, z0, z1, z2);
       sT = p.sqrtT;
     }
     const double sg = cparam(p, c, CF_SIGMA);
@@ -1450,7 +1459,7 @@ __global__ void __launch_bounds__(kBlock) k_noise(const KParams p, double *out) 
   if (i >= p.n) return;
   const uint32_t c = p.cls ? (uint32_t)p.cls[i] : 0u;
   double z0, z1, z2;
-  normals3(noise_seed(p, c), p.step, noise_index(p, c, i), z0, z1, z2);
+  normals3<false>(noise_seed(p, c), p.step, noise_index(p, c, i), z0, z1, z2);
   const double sg = cparam(p, c, CF_SIGMA), sT = cparam(p, c, CF_SQRTT);
   out[i] = z0 * sg * sT;
   out[p.n + i] = z1 * sg * sT;
