@@ -502,14 +502,25 @@ def measure_batch(steps: int, nrep: int = 256, graph_steps: int = 50):
         ms = e.time_steps(steps)
         m = b.magnetisation()
         finals[name] = (e.get_spins(), m)
+        # the same format and block size as one persistent launch per step
+        # (predictor, grid barrier, corrector), graph-replayed, from the
+        # final state (timing only)
+        from jams_amd.engine import STEP_LAUNCH_PERSISTENT, STEP_LAUNCH_TWO
+        tuned_launch = "launch=persistent" in tune.splitlines()[-1]
+        e.set_step_launch(STEP_LAUNCH_PERSISTENT)
+        e.step(graph_steps)
+        ms_pers = e.time_steps(1000) / 1000
+        e.set_step_launch(STEP_LAUNCH_PERSISTENT if tuned_launch else STEP_LAUNCH_TWO)
         out[name] = {"spins": b.n, "value": b.n * steps / (ms * 1e-3), "ms_per_step": ms / steps,
-                     "ms_per_step_plain_launches": ms_plain / 200, "setup_s": setup_s,
+                     "ms_per_step_plain_launches": ms_plain / 200,
+                     "ms_per_step_one_persistent_launch": ms_pers, "setup_s": setup_s,
                      "autotune": tune.splitlines()[-1], "engine": e.describe(),
                      "m_coldest": float(m[0]), "m_hottest": float(m[-1])}
         how = f"as {graph_steps}-step graph replays" if graph_steps > 0 else "in plain launches"
         out[name]["graph_steps"] = graph_steps
         log(f"  C4 {name}: {b.n} spins in {nrep} replicas, {ms / steps:.4f} ms/step {how} "
-            f"({ms_plain / 200:.4f} plain, 200 steps), {out[name]['value']:.4e} spin-updates/s; "
+            f"({ms_plain / 200:.4f} plain, 200 steps; {ms_pers:.4f} as one persistent launch "
+            f"per step), {out[name]['value']:.4e} spin-updates/s; "
             f"{tune.splitlines()[-1]}; |m| coldest {m[0]:.4f} hottest {m[-1]:.4f}")
         b.close()
     out["speedup_fp32_vs_fp64"] = out["fp32_field"]["value"] / out["fp64"]["value"]

@@ -233,10 +233,29 @@ int jh_set_graph_steps(jh_ctx *ctx, int steps);
 /* Threads per workgroup of the step kernels: 64 .. 512, a multiple of 64;
  * 0 (the default) picks 256 or 512 by exchange format. */
 int jh_set_block_size(jh_ctx *ctx, int threads);
+/* How jh_step launches a Heun step.  JH_STEP_LAUNCH_TWO (the default): a
+ * predictor and a corrector launch, as the reference's step launches
+ * kernelA and kernelB (solvers/cuda_llg_heun.cu:84-115).
+ * JH_STEP_LAUNCH_PERSISTENT: one launch per step whose workgroups, as many as
+ * are resident at once, take the predictor over every spin (every replica),
+ * meet at a grid barrier and take the corrector; the same results bit for
+ * bit.  It needs the Heun integrator, no dipole term and a gathering exchange
+ * format (PATTERN, TYPED or BLOCK; AUTO then does not take LINE); the next
+ * jh_step reports a context that does not fit.  A barrier that does not
+ * complete within seconds is given up and reported by the next
+ * jh_synchronize or jh_get_spins; two contexts' persistent launches running
+ * on one GPU at the same time can starve each other that way.  No
+ * counterpart in the reference: this is the batching form's one launch per
+ * step. */
+#define JH_STEP_LAUNCH_TWO 0
+#define JH_STEP_LAUNCH_PERSISTENT 1
+int jh_set_step_launch(jh_ctx *ctx, int mode);
 /* Time `steps` steps of every exchange format that fits (PATTERN, TYPED,
- * BLOCK) at 128, 256 and 512 threads per workgroup from the current state,
- * keep the fastest, restore the state; report (may be NULL) gets one line
- * per candidate. */
+ * BLOCK, and on a lattice STENCIL and LINE) at 128, 256 and 512 threads per
+ * workgroup (STENCIL also 1024, LINE its one shape), each gathering format
+ * also as one persistent launch per step, from the current state; keep the
+ * fastest, restore the state; report (may be NULL) gets one line per
+ * candidate and a last "chosen ..." line. */
 int jh_autotune(jh_ctx *ctx, int steps, char *report, int64_t len);
 
 /* ---- measurement -------------------------------------------------------- */

@@ -79,6 +79,11 @@ constexpr double kBoltzmannIU = 0.0861733326;  // meV/K, helpers/consts.h:35
 constexpr size_t kMaxTableLds = 48 * 1024;     // bytes of LDS for the tensor table
 constexpr size_t kStMaxTileLds = 160 * 1024;   // bytes of LDS for a STENCIL tile (a CU's LDS)
 constexpr int64_t kLnAutoMinRow = 24;           // neighbours per row from which AUTO takes LINE
+// polls of the persistent step's grid barrier before a workgroup gives up:
+// each is a 256-cycle sleep plus an uncached load, so at least 0.1 s and
+// typically about a second, against a phase of under a millisecond for a few
+// million spins
+constexpr unsigned kStepPollLimit = 1u << 20;
 constexpr int kLnBand = JH_LN_BAND;             // y rows per band of the LINE wave order
 #ifndef JH_LN_PAIRS
 #define JH_LN_PAIRS 1
@@ -312,6 +317,12 @@ struct jh_ctx {
     if (block) return std::min(block, kMaxBlock);  // 1024 applies to the STENCIL launch
     return (xv_diag(xv) && !xv_uniform(xv)) ? 256 : 512;
   }
+  // JH_STEP_LAUNCH_PERSISTENT: one launch per Heun step with a grid barrier
+  // between the phases; step_sync holds the barrier's words, cus the
+  // device's compute units (the grid is sized to what is resident)
+  int step_launch = JH_STEP_LAUNCH_TWO;
+  DevBuf<unsigned> step_sync;
+  int cus = 0;
   int graph_steps = 0;              // steps per captured graph (0: plain launches)
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -1378,7 +1389,8 @@ void prepare_exchange(jh_ctx *c) {
   // bcc 64^3 x 2 with 14 it runs 13 % and 5 % slower
   const bool want_line = fmt == JH_XFMT_LINE ||
                          (fmt == JH_XFMT_AUTO && c->have_lattice && c->nrep == 0 &&
-                          c->integrator == JH_INTEGRATOR_HEUN && n > 0 &&
+                          c->integrator == JH_INTEGRATOR_HEUN &&
+                          c->step_launch != JH_STEP_LAUNCH_PERSISTENT && n > 0 &&
                           x.ptr[n / 2 + 1] - x.ptr[n / 2] >= kLnAutoMinRow);
   if ((want_stencil || want_line) && !pattern_ok)
     JH_THROW(JH_ERR_UNSUPPORTED, "STENCIL and LINE exchange formats need the row patterns of a crystal");
@@ -1660,6 +1672,40 @@ int dipole_term(const jh_ctx *c) {
   return -1;
 }
 
+// Why the context cannot run a Heun step as one persistent launch (null: it
+// can).  The dipole transforms run between the phases, and the STENCIL and
+// LINE layouts have launches of their own.
+const char *persistent_unfit(const jh_ctx *c) {
+  if (c->integrator != JH_INTEGRATOR_HEUN) return "the Heun integrator";
+  if (dipole_term(c) >= 0) return "no dipole term (its transforms run between the phases)";
+  if (c->st_on || c->ln_on) return "a gathering exchange format (PATTERN, TYPED or BLOCK)";
+  return nullptr;
+}
+
+// The grid barrier's words, zero before the first persistent launch (set up
+// outside any captured graph: prepare runs before a capture)
+void ensure_step_sync(jh_ctx *c) {
+  if (c->step_sync.p) return;
+  c->step_sync.alloc(4);
+  HIPCHK(hipMemsetAsync(c->step_sync.p, 0, 4 * sizeof(unsigned), c->stream));
+}
+
+// A persistent step whose barrier timed out left its spins invalid: report
+// it (once) when the host next reads the state or synchronizes.
+void check_step_sync(jh_ctx *c) {
+  if (!c->step_sync.p) return;
+  unsigned w[4] = {0, 0, 0, 0};
+  HIPCHK(hipMemcpyAsync(w, c->step_sync.p, sizeof w, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (w[2]) {
+    HIPCHK(hipMemsetAsync(c->step_sync.p + 2, 0, sizeof(unsigned), c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    JH_THROW(JH_ERR_STATE,
+             "a persistent step's grid barrier timed out (its workgroups were not all resident "
+             "at once); the spins are invalid");
+  }
+}
+
 void prepare(jh_ctx *c) {
   require_sites(c);
   if (!c->spins_set) JH_THROW(JH_ERR_STATE, "spins have not been set (jh_set_spins)");
@@ -1710,6 +1756,11 @@ void prepare(jh_ctx *c) {
   }
   prepare_exchange(c);
   prepare_classes(c);
+  if (c->step_launch == JH_STEP_LAUNCH_PERSISTENT) {
+    const char *why = persistent_unfit(c);
+    if (why) JH_THROW(JH_ERR_UNSUPPORTED, std::string("one persistent launch per step needs ") + why);
+    ensure_step_sync(c);
+  }
   if (c->noise_mode == NM_PHILOX) c->wbuf.alloc((size_t)(3 * c->n));
   if (c->integrator == JH_INTEGRATOR_RK4) {
     c->sq.alloc((size_t)(3 * c->n));
@@ -1858,10 +1909,15 @@ void enqueue_step(jh_ctx *c, KParams p) {
     enqueue_rk4(c, p);
     return;
   }
-  const int dip = dipole_term(c);
-  if (dip >= 0) HIPCHK(c->dipoles[dip]->field(c->s.p, c->hext.p, c->prec == PR_FP32, c->stream));
   p.g32 = c->s32.p;
   p.w32 = c->sp32.p;
+  if (c->step_launch == JH_STEP_LAUNCH_PERSISTENT) {
+    HIPCHK(launch_heun_step(p, c->prec, c->xv, c->launch_block(), c->cus, c->step_sync.p,
+                            kStepPollLimit, c->stream));
+    return;
+  }
+  const int dip = dipole_term(c);
+  if (dip >= 0) HIPCHK(c->dipoles[dip]->field(c->s.p, c->hext.p, c->prec == PR_FP32, c->stream));
   launch_phase(c, p, 0);
   // corrector at t + dt
   if (dip >= 0) HIPCHK(c->dipoles[dip]->field(c->sp.p, c->hext.p, c->prec == PR_FP32, c->stream));
@@ -1979,6 +2035,12 @@ int jh_create(int device, jh_ctx **out) {
   jh_ctx *c = new (std::nothrow) jh_ctx();
   if (!c) return JH_ERR_NOMEM;
   c->device = device;
+  if ((e = hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, device)) !=
+      hipSuccess) {
+    g_create_error = std::string("compute unit count: ") + hipGetErrorString(e);
+    delete c;
+    return JH_ERR_HIP;
+  }
   if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess ||
       (e = hipEventCreate(&c->ev0)) != hipSuccess || (e = hipEventCreate(&c->ev1)) != hipSuccess) {
     g_create_error = std::string("stream/event creation: ") + hipGetErrorString(e);
@@ -2078,6 +2140,7 @@ int jh_get_spins(jh_ctx *c, double *s) {
   return guarded(c, [&] {
     require_sites(c);
     if (!s) JH_THROW(JH_ERR_INVALID, "jh_get_spins: null");
+    check_step_sync(c);
     download_vec(c, c->s.p, s, true);
   });
 }
@@ -2567,11 +2630,12 @@ int jh_autotune(jh_ctx *c, int steps, char *report, int64_t len) {
     HIPCHK(hipMemcpyAsync(save.p, c->s.p, sizeof(double) * 3 * c->n, hipMemcpyDeviceToDevice, c->stream));
     const int64_t it0 = c->iteration;
     const double t0 = c->time;
-    const int fmt0 = c->xfmt_req, bs0 = c->block, gs0 = c->graph_steps;
+    const int fmt0 = c->xfmt_req, bs0 = c->block, gs0 = c->graph_steps, sl0 = c->step_launch;
     c->graph_steps = 0;
+    c->step_launch = JH_STEP_LAUNCH_TWO;
     std::string rep;
     double best = 1e300;
-    int best_fmt = fmt0, best_bs = bs0;
+    int best_fmt = fmt0, best_bs = bs0, best_sl = sl0;
     for (int fmt : {JH_XFMT_PATTERN, JH_XFMT_TYPED, JH_XFMT_BLOCK, JH_XFMT_STENCIL, JH_XFMT_LINE}) {
       if (c->xh.empty() && fmt != JH_XFMT_PATTERN) continue;
       const bool stencil = fmt == JH_XFMT_STENCIL;
@@ -2599,30 +2663,42 @@ int jh_autotune(jh_ctx *c, int steps, char *report, int64_t len) {
             continue;  // no stencil of this many threads fits
           }
         }
-        HIPCHK(hipMemcpyAsync(c->s.p, save.p, sizeof(double) * 3 * c->n, hipMemcpyDeviceToDevice, c->stream));
-        sync_f32(c);
-        c->iteration = it0;
-        c->time = t0;
-        do_steps(c, 2);
-        HIPCHK(hipEventRecord(c->ev0, c->stream));
-        do_steps(c, steps);
-        HIPCHK(hipEventRecord(c->ev1, c->stream));
-        HIPCHK(hipEventSynchronize(c->ev1));
-        float f = 0.f;
-        HIPCHK(hipEventElapsedTime(&f, c->ev0, c->ev1));
-        const double ms = (double)f / steps;
-        char line[128];
-        std::snprintf(line, sizeof line, "format=%d block=%d %.4f ms/step\n", fmt, bs, ms);
-        rep += line;
-        if (ms < best) {
-          best = ms;
-          best_fmt = fmt;
-          best_bs = bs;
+        // the gathering formats also as one persistent launch per step
+        const bool persist = persistent_unfit(c) == nullptr;
+        for (int sl : {JH_STEP_LAUNCH_TWO, JH_STEP_LAUNCH_PERSISTENT}) {
+          if (sl == JH_STEP_LAUNCH_PERSISTENT && !persist) continue;
+          c->step_launch = sl;
+          if (sl == JH_STEP_LAUNCH_PERSISTENT) ensure_step_sync(c);
+          HIPCHK(hipMemcpyAsync(c->s.p, save.p, sizeof(double) * 3 * c->n, hipMemcpyDeviceToDevice, c->stream));
+          sync_f32(c);
+          c->iteration = it0;
+          c->time = t0;
+          do_steps(c, 2);
+          HIPCHK(hipEventRecord(c->ev0, c->stream));
+          do_steps(c, steps);
+          HIPCHK(hipEventRecord(c->ev1, c->stream));
+          HIPCHK(hipEventSynchronize(c->ev1));
+          check_step_sync(c);
+          float f = 0.f;
+          HIPCHK(hipEventElapsedTime(&f, c->ev0, c->ev1));
+          const double ms = (double)f / steps;
+          char line[128];
+          std::snprintf(line, sizeof line, "format=%d block=%d launch=%s %.4f ms/step\n", fmt, bs,
+                        sl == JH_STEP_LAUNCH_PERSISTENT ? "persistent" : "two", ms);
+          rep += line;
+          if (ms < best) {
+            best = ms;
+            best_fmt = fmt;
+            best_bs = bs;
+            best_sl = sl;
+          }
         }
+        c->step_launch = JH_STEP_LAUNCH_TWO;
       }
     }
     c->xfmt_req = best_fmt;
     c->block = best_bs;
+    c->step_launch = best_sl;
     c->graph_steps = gs0;
     c->dirty = true;
     prepare(c);
@@ -2632,7 +2708,8 @@ int jh_autotune(jh_ctx *c, int steps, char *report, int64_t len) {
     c->iteration = it0;
     c->time = t0;
     char line[128];
-    std::snprintf(line, sizeof line, "chosen format=%d block=%d", best_fmt, best_bs);
+    std::snprintf(line, sizeof line, "chosen format=%d block=%d launch=%s", best_fmt, best_bs,
+                  best_sl == JH_STEP_LAUNCH_PERSISTENT ? "persistent" : "two");
     rep += line;
     if (report && len > 0) {
       std::strncpy(report, rep.c_str(), (size_t)len - 1);
@@ -2680,6 +2757,7 @@ int jh_describe(jh_ctx *c, char *buf, int64_t buflen) {
     }
     const int dip = dipole_term(c);
     if (dip >= 0) out += " " + c->dipoles[dip]->describe(c->prec == PR_FP32);
+    out += c->step_launch == JH_STEP_LAUNCH_PERSISTENT ? " launch=persistent" : " launch=two";
     if (buf && buflen > 0) {
       std::strncpy(buf, out.c_str(), (size_t)buflen - 1);
       buf[buflen - 1] = 0;
@@ -2688,7 +2766,22 @@ int jh_describe(jh_ctx *c, char *buf, int64_t buflen) {
 }
 
 int jh_synchronize(jh_ctx *c) {
-  return guarded(c, [&] { HIPCHK(hipStreamSynchronize(c->stream)); });
+  return guarded(c, [&] {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    check_step_sync(c);
+  });
+}
+
+int jh_set_step_launch(jh_ctx *c, int mode) {
+  return guarded(c, [&] {
+    if (mode != JH_STEP_LAUNCH_TWO && mode != JH_STEP_LAUNCH_PERSISTENT)
+      JH_THROW(JH_ERR_INVALID, "step launch mode must be JH_STEP_LAUNCH_TWO or _PERSISTENT");
+    if (mode != c->step_launch) {
+      c->step_launch = mode;
+      c->dirty = true;
+      c->drop_graph();
+    }
+  });
 }
 
 }  // extern "C"

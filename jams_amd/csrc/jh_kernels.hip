@@ -12,6 +12,9 @@
 // tensors and accumulates in FP64 (one rounding per product and per sum); it
 // may order a row by tensor id so that tensors become wave-uniform.
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <type_traits>
 
 #include "jh_device.h"
@@ -669,27 +672,24 @@ template <int PREC, int XVk>
 constexpr int kMinWaves = (PREC == PR_FP32 && xv_uniform(XVk)) ? JH_UNIFORM_WAVES
                           : xv_diag(XVk) ? JH_DIAG_WAVES : JH_WAVES_PER_EU;
 
-template <int PREC, int XVk, int PHASE, bool GEN>
-__global__ void __launch_bounds__(kMaxBlock)
-__attribute__((amdgpu_waves_per_eu(kMinWaves<PREC, XVk>))) k_heun(const KParams p) {
-  extern __shared__ __align__(16) unsigned char lds[];
-  load_table<PREC, XVk>(p.x, lds);
-  if constexpr (xv_typed(XVk)) __syncthreads();
-  int64_t blk = xcd_block(blockIdx.x, gridDim.x);
+// The spin block a gathering step kernel's workgroup takes for its XCD-
+// ordered block number blk.  With basis_ilv the workgroups alternate between
+// the basis sites' index ranges, so the workgroups of one XCD hold the same
+// cells of every basis site and the other sites' spins their rows gather stay
+// in that XCD's L2.
+__device__ __forceinline__ int64_t step_block(const KParams &p, int64_t blk) {
   if (p.basis_ilv && !p.rowlist && p.order_nb > 1 && p.order_cells % blockDim.x == 0) {
-    // workgroups alternate between the basis sites' index ranges, so the
-    // workgroups of one XCD hold the same cells of every basis site and the
-    // other sites' spins their rows gather stay in that XCD's L2
     const int64_t per = p.order_cells / blockDim.x;
     blk = (blk % p.order_nb) * per + blk / p.order_nb;
   }
-  int64_t i = blk * blockDim.x + threadIdx.x;
-  if (p.rowlist) {
-    if (i >= p.nrowlist) return;
-    i = p.rowlist[i];
-  } else if (i >= p.n) {
-    return;
-  }
+  return blk;
+}
+
+// One Heun phase of spin i with the gathering exchange formats: the field at
+// the phase's spins and the phase's update and stores.
+template <int PREC, int XVk, int PHASE, bool GEN>
+__device__ __forceinline__ void heun_gather_spin(const KParams &p, int64_t i,
+                                                 const unsigned char *lds) {
   const uint32_t c = p.cls ? (uint32_t)p.cls[i] : 0u;
   const uint64_t step = launch_step(p);
   const double t0 = launch_time(p, step);
@@ -700,6 +700,92 @@ __attribute__((amdgpu_waves_per_eu(kMinWaves<PREC, XVk>))) k_heun(const KParams 
   double x0, x1, x2;
   exchange_at<PREC, XVk>(p, PHASE == 0 ? p.s : p.sp, p.g32, i, lds, x0, x1, x2);
   heun_spin<PREC, PHASE, GEN>(p, i, c, step, t, s0, s1, s2, x0, x1, x2, own);
+}
+
+template <int PREC, int XVk, int PHASE, bool GEN>
+__global__ void __launch_bounds__(kMaxBlock)
+__attribute__((amdgpu_waves_per_eu(kMinWaves<PREC, XVk>))) k_heun(const KParams p) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  load_table<PREC, XVk>(p.x, lds);
+  if constexpr (xv_typed(XVk)) __syncthreads();
+  int64_t i = step_block(p, xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x;
+  if (p.rowlist) {
+    if (i >= p.nrowlist) return;
+    i = p.rowlist[i];
+  } else if (i >= p.n) {
+    return;
+  }
+  heun_gather_spin<PREC, XVk, PHASE, GEN>(p, i, lds);
+}
+
+// Grid barrier of the persistent step kernel over sync = {arrived count,
+// generation, failed flag} in device memory (zero before the first launch;
+// each barrier leaves the count at zero again).  Every work-item publishes
+// its stores at agent scope first (the release fence writes back the XCD's
+// L2, which other XCDs do not share), then one work-item per workgroup counts
+// the workgroup in: the last to arrive resets the count and advances the
+// generation, the others poll the generation.  The polling is bounded: after
+// `limit` polls a workgroup sets the failed flag and leaves, so every wave
+// ends even if the grid were not resident at once (the host launches at most
+// as many workgroups as fit, and reports the flag).  The acquire fence after
+// the barrier drops stale lines before the next phase's loads.
+__device__ __forceinline__ bool grid_barrier(unsigned *sync, unsigned limit) {
+  __shared__ int passed;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int ok = 1;
+    const unsigned gen = __hip_atomic_load(sync + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned before =
+        __hip_atomic_fetch_add(sync, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (before == gridDim.x - 1) {
+      __hip_atomic_store(sync, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(sync + 1, gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      unsigned polls = 0;
+      while (__hip_atomic_load(sync + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+        if (++polls > limit) {
+          __hip_atomic_store(sync + 2, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          ok = 0;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(4);
+      }
+    }
+    passed = ok;
+  }
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  return passed != 0;
+}
+
+// A whole Heun step in one persistent launch (the batching form: every
+// replica's predictor, one grid barrier, every replica's corrector).  The
+// grid holds at most as many workgroups as are resident at once; workgroup w
+// takes the spin blocks w, w + G, w + 2G, ... of the XCD-ordered numbering
+// (with G a multiple of 8 each XCD keeps a contiguous range, as in k_heun).
+// The tables are staged in LDS once for both phases.  The corrector reads the
+// FP32 copy the predictor wrote and writes the other (g32 and w32 swap).
+template <int PREC, int XVk, bool GEN>
+__global__ void __launch_bounds__(kMaxBlock)
+__attribute__((amdgpu_waves_per_eu(kMinWaves<PREC, XVk>)))
+k_heun_step(const KParams p, unsigned *sync, unsigned limit) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  load_table<PREC, XVk>(p.x, lds);
+  if constexpr (xv_typed(XVk)) __syncthreads();
+  const int64_t nv = (p.n + blockDim.x - 1) / blockDim.x;
+  for (int64_t v = blockIdx.x; v < nv; v += gridDim.x) {
+    const int64_t i = step_block(p, xcd_block(v, nv)) * blockDim.x + threadIdx.x;
+    if (i < p.n) heun_gather_spin<PREC, XVk, 0, GEN>(p, i, lds);
+  }
+  if (!grid_barrier(sync, limit)) return;
+  KParams q = p;
+  q.g32 = p.w32;
+  q.w32 = const_cast<float *>(p.g32);
+  for (int64_t v = blockIdx.x; v < nv; v += gridDim.x) {
+    const int64_t i = step_block(q, xcd_block(v, nv)) * blockDim.x + threadIdx.x;
+    if (i < q.n) heun_gather_spin<PREC, XVk, 1, GEN>(q, i, lds);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1613,6 +1699,82 @@ static hipError_t launch_heun_p(const KParams &p, int xv, int phase, int bs, hip
       break;
   }
   return hipErrorInvalidValue;
+}
+
+// Workgroups of one persistent step launch: as many as are resident at once
+// (the occupancy of this instantiation at bs threads and its LDS, times the
+// CUs), at most one per spin block, a multiple of 8 when there are 8 or more.
+// The occupancy query is cached per (kernel, bs, LDS bytes).
+template <int PREC, int XVk, bool GEN>
+static hipError_t launch_heun_step_k(const KParams &p, int bs, int cus, unsigned *sync,
+                                     unsigned limit, hipStream_t st) {
+  const auto fn = k_heun_step<PREC, XVk, GEN>;
+  const size_t lds = table_lds(p, PREC, XVk);
+  static std::mutex mu;
+  static std::map<std::tuple<const void *, int, size_t>, int> occ;
+  int per = 0;
+  {
+    std::lock_guard<std::mutex> lock(mu);
+    const auto key = std::make_tuple(reinterpret_cast<const void *>(fn), bs, lds);
+    const auto it = occ.find(key);
+    if (it != occ.end()) {
+      per = it->second;
+    } else {
+      const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, bs, lds);
+      if (e != hipSuccess) return e;
+      occ[key] = per;
+    }
+  }
+  if (per < 1 || cus < 1) return hipErrorInvalidConfiguration;
+  const int64_t nv = (p.n + bs - 1) / bs;
+  int64_t g = std::min<int64_t>(nv, (int64_t)per * cus);
+  if (g >= 8) g -= g % 8;
+  hipLaunchKernelGGL(fn, dim3((unsigned)g), dim3(bs), lds, st, p, sync, limit);
+  return hipGetLastError();
+}
+
+template <int PREC, int XVk>
+static hipError_t launch_heun_step_t(const KParams &p, int bs, int cus, unsigned *sync,
+                                     unsigned limit, hipStream_t st) {
+  const bool gen = p.has_ac || p.ani_power != 2;
+  return gen ? launch_heun_step_k<PREC, XVk, true>(p, bs, cus, sync, limit, st)
+             : launch_heun_step_k<PREC, XVk, false>(p, bs, cus, sync, limit, st);
+}
+
+template <int PREC>
+static hipError_t launch_heun_step_p(const KParams &p, int xv, int bs, int cus, unsigned *sync,
+                                     unsigned limit, hipStream_t st) {
+  switch (xv) {
+    case XV_NONE: return launch_heun_step_t<PREC, XV_NONE>(p, bs, cus, sync, limit, st);
+    case XV_TDIAG: return launch_heun_step_t<PREC, XV_TDIAG>(p, bs, cus, sync, limit, st);
+    case XV_TFULL: return launch_heun_step_t<PREC, XV_TFULL>(p, bs, cus, sync, limit, st);
+    case XV_BDIAG: return launch_heun_step_t<PREC, XV_BDIAG>(p, bs, cus, sync, limit, st);
+    case XV_BFULL: return launch_heun_step_t<PREC, XV_BFULL>(p, bs, cus, sync, limit, st);
+    case XV_PDIAG: return launch_heun_step_t<PREC, XV_PDIAG>(p, bs, cus, sync, limit, st);
+    case XV_PFULL: return launch_heun_step_t<PREC, XV_PFULL>(p, bs, cus, sync, limit, st);
+    case XV_GDIAG: return launch_heun_step_t<PREC, XV_GDIAG>(p, bs, cus, sync, limit, st);
+    case XV_GFULL: return launch_heun_step_t<PREC, XV_GFULL>(p, bs, cus, sync, limit, st);
+    case XV_UDIAG:
+      if constexpr (PREC == PR_FP32) return launch_heun_step_t<PREC, XV_UDIAG>(p, bs, cus, sync, limit, st);
+      break;
+    case XV_UFULL:
+      if constexpr (PREC == PR_FP32) return launch_heun_step_t<PREC, XV_UFULL>(p, bs, cus, sync, limit, st);
+      break;
+    case XV_QISO:
+      if constexpr (PREC == PR_FP32) return launch_heun_step_t<PREC, XV_QISO>(p, bs, cus, sync, limit, st);
+      break;
+    case XV_QDMI:
+      if constexpr (PREC == PR_FP32) return launch_heun_step_t<PREC, XV_QDMI>(p, bs, cus, sync, limit, st);
+      break;
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_heun_step(const KParams &p, int prec, int xv, int bs, int cus, unsigned *sync,
+                            unsigned limit, hipStream_t st) {
+  if (bs < 64 || bs > kMaxBlock || bs % 64 != 0 || !sync || p.rowlist) return hipErrorInvalidValue;
+  return prec == PR_FP64 ? launch_heun_step_p<PR_FP64>(p, xv, bs, cus, sync, limit, st)
+                         : launch_heun_step_p<PR_FP32>(p, xv, bs, cus, sync, limit, st);
 }
 
 hipError_t launch_heun(const KParams &p, int prec, int xv, int phase, int bs, hipStream_t st) {

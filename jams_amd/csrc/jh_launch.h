@@ -9,6 +9,12 @@ namespace jh {
 
 // bs: threads per workgroup of the step kernels (64 .. 512, multiple of 64)
 hipError_t launch_heun(const KParams &p, int prec, int xv, int phase, int bs, hipStream_t st);
+// one whole Heun step (predictor, grid barrier, corrector) in one persistent
+// launch of the gathering kernels; cus: compute units of the device; sync:
+// the barrier's 3 words (zero before the first launch); limit: polls before
+// a workgroup gives the barrier up and sets sync[2]
+hipError_t launch_heun_step(const KParams &p, int prec, int xv, int bs, int cus, unsigned *sync,
+                            unsigned limit, hipStream_t st);
 // one Heun phase over the STENCIL layout v, full or diagonal tensors
 hipError_t launch_heun_st(const KParams &p, const StView &v, int prec, bool diag, int phase,
                           hipStream_t st);

@@ -21,6 +21,7 @@ INTEGRATOR_HEUN = 0
 INTEGRATOR_RK4 = 1
 XFMT_AUTO, XFMT_TYPED, XFMT_BLOCK, XFMT_PATTERN, XFMT_PATTERN_GLOBAL, XFMT_STENCIL = 0, 1, 2, 3, 4, 5
 XFMT_LINE = 6
+STEP_LAUNCH_TWO, STEP_LAUNCH_PERSISTENT = 0, 1
 TERM_EXCHANGE, TERM_ZEEMAN, TERM_UNIAXIAL, TERM_DIPOLE = 0, 1, 2, 3
 
 # every entry point declared in include/jams_hip.h
@@ -68,6 +69,7 @@ ABI = {
     "jh_reduce_magnetisation_replicas": (C.c_int, [C.c_void_p, C.c_void_p]),
     "jh_set_graph_steps": (C.c_int, [C.c_void_p, C.c_int]),
     "jh_set_block_size": (C.c_int, [C.c_void_p, C.c_int]),
+    "jh_set_step_launch": (C.c_int, [C.c_void_p, C.c_int]),
     "jh_autotune": (C.c_int, [C.c_void_p, C.c_int, C.c_char_p, C.c_int64]),
     "jh_time_term": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     "jh_term_bytes": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
@@ -309,6 +311,11 @@ class Engine:
 
     def set_block_size(self, threads: int):
         self._chk(self.lib.jh_set_block_size(self.h, int(threads)))
+
+    def set_step_launch(self, mode: int):
+        """STEP_LAUNCH_TWO (predictor and corrector launches) or
+        STEP_LAUNCH_PERSISTENT (one launch per step with a grid barrier)."""
+        self._chk(self.lib.jh_set_step_launch(self.h, int(mode)))
 
     def autotune(self, steps: int = 20) -> str:
         buf = C.create_string_buffer(4096)
