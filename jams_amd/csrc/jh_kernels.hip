@@ -1396,13 +1396,13 @@ __attribute__((amdgpu_waves_per_eu(kLnWaves<PHASE>))) k_heun_ln(const KParams p,
         own[h] = OwnState{os[h][0], os[h][1], os[h][2], od[h][0], od[h][1], od[h][2]};
     }
     double d[2][3], v[2][3];
-    // reference index of cell 0: (cell of the line's (x, y) column + z) nb + b
-    const int64_t r0 =
-        (((int64_t)xw * ln.ly + yw) * ln.lz + (int64_t)P * lane) * ln.nb + b;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const uint32_t c = p.cls ? (uint32_t)p.cls[i0 + h] : 0u;
-      heun_math<PREC, PHASE, GEN>(p, i0 + h, r0 + h * ln.nb, c, step, t, s[h][0], s[h][1], s[h][2], a[h][0],
+      // (formed here from i0: a reference index computed from the line's
+      // coordinates up front stays live across the field and spills)
+      heun_math<PREC, PHASE, GEN>(p, i0 + h, ref_index(p, i0 + h), c, step, t, s[h][0], s[h][1],
+                                  s[h][2], a[h][0],
                                   a[h][1], a[h][2], own[h], w[h][0], w[h][1], w[h][2], d[h][0],
                                   d[h][1], d[h][2], v[h][0], v[h][1], v[h][2]);
     }
