@@ -262,7 +262,8 @@ int jh_autotune(jh_ctx *ctx, int steps, char *report, int64_t len);
 /* Run nsteps and return the HIP-event time on the context stream (ms). */
 int jh_time_steps(jh_ctx *ctx, int64_t nsteps, double *ms);
 /* Average duration of one launch of the step's dominant kernel (phase 0 =
- * predictor, 1 = corrector), timed with HIP events over `reps` launches. */
+ * predictor, 1 = corrector), timed with HIP events over `reps` launches
+ * (the two-launch form's phase kernels, whatever jh_set_step_launch says). */
 int jh_time_phase(jh_ctx *ctx, int phase, int reps, double *ms_per_launch);
 /* Algorithmic bytes per launch of the phase kernel in the format actually
  * used, and the same quantity for the reference's FP64 scalar CSR. */

@@ -129,3 +129,31 @@ def test_persistent_refuses_dipole_and_line(engine_factory):
     e.step(2)
     d = e.describe()
     assert " line=" not in d and "launch=persistent" in d, d
+
+
+def test_persistent_with_general_local_terms(engine_factory):
+    """The general local terms (uniaxial anisotropy of power 4 about a tilted
+    axis, an ac Zeeman field: the GEN instantiations) on the C2 template at
+    L = 8, FP64 and FP32-field, T = 300 K: one persistent launch per step
+    gives the two-launch bits."""
+    b = S.c2(8)
+    n = b.n
+    axis = np.tile(np.array([0.36, 0.48, 0.8]), (n, 1))
+    dc = b.mus[:, None] * np.array([0.0, 0.2, 1.0])[None, :]
+    ac = b.mus[:, None] * np.array([0.5, 0.0, 0.0])[None, :]
+    for prec in (E.PREC_FP64, E.PREC_FP32_FIELD):
+        got = []
+        for launch in (E.STEP_LAUNCH_TWO, E.STEP_LAUNCH_PERSISTENT):
+            e = engine_factory()
+            e.set_sites(b.mus, b.gyro, b.alpha)
+            e.add_exchange_csr(*b.csr)
+            e.add_uniaxial(np.full(n, 0.05), axis, 4)
+            e.add_zeeman(dc, ac, np.full(n, 2.0e3))
+            e.set_time_step(b.dt)
+            e.set_precision(prec)
+            e.set_spins(b.s0)
+            e.set_thermostat(300.0, 17)
+            e.set_step_launch(launch)
+            e.step(30)
+            got.append(e.get_spins())
+        assert np.array_equal(got[0], got[1]), prec
