@@ -78,7 +78,7 @@ def build(verbose: bool = False) -> dict:
         tmp = host.with_suffix(".so.tmp")
         _run([CXX, "-shared", "-fopenmp", "-o", str(tmp),
               *[str(OBJ / (s.stem + ".o")) for s in host_srcs],
-              "-L", str(LIB), "-ljamship", "-Wl,-rpath,$ORIGIN"])
+              "-L", str(LIB), "-ljamship", "-ldl", "-Wl,-rpath,$ORIGIN"])
         tmp.replace(host)
         out["host"] = str(host)
         if main_src.exists():

@@ -1,6 +1,8 @@
 // lattice.cc -- see lattice.h.
 #include "lattice.h"
 
+#include "h5.h"
+
 #include <algorithm>
 #include <cctype>
 #include <cstring>
@@ -85,7 +87,15 @@ void random_sphere_spins(uint64_t seed, int64_t n, double *out) {
 
 std::vector<double> load_spins_tsv(const std::string &path, int64_t n) {
   const std::string ext = path.size() > 3 ? path.substr(path.size() - 3) : "";
-  if (ext == ".h5") throw RuntimeError("HDF5 spin files are not supported by this build: " + path);
+  if (ext == ".h5") {
+    // helpers/load.h:13-19, :53-57: the dataset /spins of an HDF5 file
+    std::vector<uint64_t> dims;
+    std::vector<double> s = h5::read_doubles(path, "/spins", &dims);
+    if ((int64_t)s.size() != 3 * n)
+      throw RuntimeError("loading array from file: '" + path + "' expected size: " +
+                         std::to_string(3 * n) + " actual size: " + std::to_string(s.size()));
+    return s;
+  }
   std::ifstream f(path);
   if (!f.is_open()) throw RuntimeError("failed to open file: " + path);
   std::vector<double> s;

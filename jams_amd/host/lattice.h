@@ -80,7 +80,8 @@ class Lattice {
   void find_symmetry();
 };
 
-// Spins from a TSV file of N x 3 doubles (helpers/load.h:22-48).
+// Spins from a TSV file of N x 3 doubles (helpers/load.h:22-48), or from
+// the dataset /spins of an HDF5 file named *.h5 (helpers/load.h:13-19, :53-57).
 std::vector<double> load_spins_tsv(const std::string &path, int64_t n);
 
 // Marsaglia point on the unit sphere (helpers/random.h:12-26) from a seeded

@@ -1,7 +1,11 @@
 // plugins.cc -- see plugins.h.
 #include "plugins.h"
 
+#include "h5.h"
+
 #include <algorithm>
+#include <cstring>
+#include <climits>
 #include <cstdio>
 #include <iomanip>
 #include <iostream>
@@ -416,6 +420,9 @@ void Solver::compute_fields(std::vector<double> &h) {
 
 // ---- monitors ------------------------------------------------------------------
 
+// the closing tags of the XDMF time series, rewritten after each snapshot
+static const char kXdmfTail[] = "    </Grid>\n  </Domain>\n</Xdmf>";
+
 std::unique_ptr<Monitor> Monitor::create(const Setting &s, const std::string &prefix, Solver &solver,
                                          const Lattice &lat) {
   const std::string m = lowercase(s["module"].as_string());
@@ -423,8 +430,15 @@ std::unique_ptr<Monitor> Monitor::create(const Setting &s, const std::string &pr
   if (m == "energy") return std::make_unique<EnergyMonitor>(s, prefix + "eng.tsv", solver);
   if (m == "checkpoint")
     return std::make_unique<CheckpointMonitor>(s, prefix + "checkpoint.tsv", lat.num_sites());
+  if (m == "hdf5") {
+    // the simulation name: the file name part of prefix, less its "_"
+    std::string name = prefix.substr(0, prefix.empty() ? 0 : prefix.size() - 1);
+    const size_t slash = name.find_last_of('/');
+    if (slash != std::string::npos) name = name.substr(slash + 1);
+    return std::make_unique<Hdf5Monitor>(s, prefix, name, lat);
+  }
   throw RuntimeError("unknown monitor " + s["module"].as_string() +
-                     " (this build provides magnetisation, energy, checkpoint)");
+                     " (this build provides magnetisation, energy, checkpoint, hdf5)");
   return nullptr;
 }
 
@@ -499,6 +513,129 @@ void CheckpointMonitor::finish(Solver &solver) {
   if (written_ == solver.iteration()) return;
   write_checkpoint(solver.engine(), n_, path_, solver.iteration(), solver.time(), solver.seed());
   written_ = solver.iteration();
+}
+
+Hdf5Monitor::Hdf5Monitor(const Setting &s, const std::string &prefix, const std::string &sim_name,
+                         const Lattice &lat)
+    : Monitor(s), prefix_(prefix), sim_name_(sim_name), n_(lat.num_sites()) {
+  if (output_steps_ == 0) output_steps_ = INT_MAX;  // only the final snapshot
+  if (output_steps_ < 0) throw ConfigError("hdf5: output_steps must not be negative");
+  compressed_ = s.get<bool>("compressed", false);
+  fields_ = s.get<bool>("fields", false);
+  for (const char *k : {"slice", "ds_dt", "energies"})
+    if (s.exists(k) && (std::string(k) == "slice" || s[k].as_bool()))
+      throw RuntimeError(std::string("hdf5: the ") + k + " setting is not supported by this build");
+  // <sim>_lattice.h5 (monitors/hdf5.cc:151-193): material ids, moments and
+  // positions in nm
+  std::vector<int32_t> types((size_t)n_);
+  std::vector<double> pos((size_t)(3 * n_));
+  for (int64_t i = 0; i < n_; ++i) {
+    types[(size_t)i] = lat.site_material(i);
+    const Vec3 r = lat.site_position(i);
+    for (int q = 0; q < 3; ++q) pos[(size_t)(3 * i + q)] = lat.parameter() * r[q] / 1e-9;
+  }
+  std::vector<h5::Data> sets(3);
+  sets[0].path = "/types";
+  sets[0].integer = true;
+  sets[0].dims = {(uint64_t)n_};
+  sets[0].data = types.data();
+  sets[1].path = "/moments";
+  sets[1].dims = {(uint64_t)n_};
+  sets[1].data = lat.mus.data();
+  sets[2].path = "/positions";
+  sets[2].dims = {(uint64_t)n_, 3};
+  sets[2].data = pos.data();
+  sets[2].units = "nm";
+  h5::write_file(prefix_ + "lattice.h5", sets);
+  // <sim>.xdmf (monitors/hdf5.cc:197-214): an empty temporal collection,
+  // each snapshot inserted before the closing tags
+  const std::string xdmf = prefix_.substr(0, prefix_.size() - 1) + ".xdmf";
+  xdmf_ = std::fopen(xdmf.c_str(), "w");
+  if (!xdmf_) throw RuntimeError("failed to open XDMF output file: " + xdmf);
+  std::fputs("<?xml version=\"1.0\"?>\n", xdmf_);
+  std::fputs("<!DOCTYPE Xdmf SYSTEM \"https://gitlab.kitware.com/xdmf/xdmf/raw/master/Xdmf.dtd\"[]>\n",
+             xdmf_);
+  std::fputs("<Xdmf Version=\"3.0\">\n", xdmf_);
+  std::fputs("  <Domain Name=\"JAMS\">\n", xdmf_);
+  std::fputs("    <Information Name=\"Commit\" Value=\"jams_amd\" />\n", xdmf_);
+  std::fprintf(xdmf_, "    <Information Name=\"Configuration\" Value=\"%s\" />\n", sim_name_.c_str());
+  std::fputs("    <Grid Name=\"TimeSeries\" GridType=\"Collection\" CollectionType=\"Temporal\">\n",
+             xdmf_);
+  std::fputs(kXdmfTail, xdmf_);
+  std::fflush(xdmf_);
+}
+
+Hdf5Monitor::~Hdf5Monitor() {
+  if (xdmf_) std::fclose(xdmf_);
+}
+
+void Hdf5Monitor::write_spins(Solver &solver, const std::string &file) {
+  std::vector<double> s((size_t)(3 * n_)), h;
+  engine_check(solver.engine(), jh_get_spins(solver.engine(), s.data()));
+  std::vector<h5::Data> sets(1);
+  sets[0].path = "/spins";
+  sets[0].dims = {(uint64_t)n_, 3};
+  sets[0].data = s.data();
+  if (fields_) {
+    solver.compute_fields(h);
+    h5::Data f;
+    f.path = "/fields";
+    f.dims = {(uint64_t)n_, 3};
+    f.data = h.data();
+    sets.push_back(f);
+  }
+  // chunks of up to 4095 rows, deflate level 6 (monitors/hdf5.cc:22-25)
+  h5::write_file(file, sets, compressed_ ? 4095 : 0, 6);
+  append_xdmf(file, solver.time());
+}
+
+void Hdf5Monitor::append_xdmf(const std::string &file, double time) {
+  const size_t slash = file.find_last_of('/');
+  const std::string base = slash == std::string::npos ? file : file.substr(slash + 1);
+  const unsigned nn = (unsigned)n_;
+  if (std::fseek(xdmf_, -(long)std::strlen(kXdmfTail), SEEK_CUR) != 0)
+    throw RuntimeError("failed to seek XDMF file while appending a new timestep");
+  std::fprintf(xdmf_, "      <Grid Name=\"Lattice\" GridType=\"Uniform\">\n");
+  std::fprintf(xdmf_, "        <Time Value=\"%f\" />\n", time);
+  std::fprintf(xdmf_, "        <Topology TopologyType=\"Polyvertex\" Dimensions=\"%u\" />\n", nn);
+  std::fputs("       <Geometry GeometryType=\"XYZ\">\n", xdmf_);
+  std::fprintf(xdmf_, "         <DataItem Dimensions=\"%u 3\" NumberType=\"Float\" Precision=\"8\" Format=\"HDF\">\n", nn);
+  std::fprintf(xdmf_, "           %s_lattice.h5:/positions\n", sim_name_.c_str());
+  std::fputs("         </DataItem>\n       </Geometry>\n", xdmf_);
+  std::fputs("       <Attribute Name=\"type\" AttributeType=\"Scalar\" Center=\"Node\">\n", xdmf_);
+  std::fprintf(xdmf_, "         <DataItem Dimensions=\"%u\" NumberType=\"Int\" Precision=\"4\" Format=\"HDF\">\n", nn);
+  std::fprintf(xdmf_, "           %s_lattice.h5:/types\n", sim_name_.c_str());
+  std::fputs("         </DataItem>\n       </Attribute>\n", xdmf_);
+  std::fputs("       <Attribute Name=\"moment\" AttributeType=\"Scalar\" Center=\"Node\">\n", xdmf_);
+  std::fprintf(xdmf_, "         <DataItem Dimensions=\"%u\" NumberType=\"Float\" Precision=\"8\" Format=\"HDF\">\n", nn);
+  std::fprintf(xdmf_, "           %s_lattice.h5:/moments\n", sim_name_.c_str());
+  std::fputs("         </DataItem>\n       </Attribute>\n", xdmf_);
+  std::fputs("       <Attribute Name=\"spin\" AttributeType=\"Vector\" Center=\"Node\">\n", xdmf_);
+  std::fprintf(xdmf_, "         <DataItem Dimensions=\"%u 3\" NumberType=\"Float\" Precision=\"8\" Format=\"HDF\">\n", nn);
+  std::fprintf(xdmf_, "           %s:/spins\n", base.c_str());
+  std::fputs("         </DataItem>\n       </Attribute>\n", xdmf_);
+  if (fields_) {
+    std::fputs("       <Attribute Name=\"fields\" AttributeType=\"Vector\" Center=\"Node\">\n", xdmf_);
+    std::fprintf(xdmf_, "         <DataItem Dimensions=\"%u 3\" NumberType=\"Float\" Precision=\"8\" Format=\"HDF\">\n", nn);
+    std::fprintf(xdmf_, "           %s:/fields\n", base.c_str());
+    std::fputs("         </DataItem>\n       </Attribute>\n", xdmf_);
+  }
+  std::fputs("      </Grid>\n", xdmf_);
+  std::fputs(kXdmfTail, xdmf_);
+  std::fflush(xdmf_);
+}
+
+void Hdf5Monitor::update(Solver &solver) {
+  if (!is_updating(solver.iteration())) return;
+  char num[32];
+  std::snprintf(num, sizeof num, "%07lld", (long long)(solver.iteration() / output_steps_));
+  write_spins(solver, prefix_ + num + ".h5");
+}
+
+void Hdf5Monitor::finish(Solver &solver) {
+  if (final_written_) return;
+  write_spins(solver, prefix_ + "final.h5");
+  final_written_ = true;
 }
 
 EnergyMonitor::EnergyMonitor(const Setting &s, const std::string &path, Solver &solver)

@@ -10,6 +10,7 @@
 
 #include <cstdint>
 #include <fstream>
+#include <cstdio>
 #include <memory>
 #include <string>
 #include <vector>
@@ -197,7 +198,8 @@ class Monitor {
 
 // Checkpoint of the spin state (an addition: the reference restarts from
 // lattice.spins = <file> with the time and the noise stream reset, and its
-// HDF5 writer needs a library this build does not have).  The file is the
+// HDF5 snapshots, written here by the hdf5 monitor, hold no step counter).
+// The file is the
 // reference's TSV spin format (helpers/load.h:22-51: N rows of sx sy sz,
 // '#' lines skipped), written lossless (%.17g), headed by comment lines with
 // the iteration, time and seed; sim.resume = <file> continues from it.
@@ -224,6 +226,31 @@ struct CheckpointHeader {
   uint64_t seed = 0;
 };
 CheckpointHeader read_checkpoint_header(const std::string &path);
+
+// monitors/hdf5.cc:27-52, :64-148, :151-289 (the "hdf5" module): at
+// construction <sim>_lattice.h5 (types, moments, positions in nm) and the
+// XDMF time series <sim>.xdmf; every output_steps (0: only the final one)
+// <sim>_<count, 7 digits>.h5 with /spins (and /fields with fields = true),
+// each appended to the XDMF; after the run <sim>_final.h5.  compressed =
+// true stores the datasets chunked, shuffled and deflated.  The ds_dt,
+// energies and slice settings are refused.  The HDF5 C library is loaded at
+// run time (h5.h).
+class Hdf5Monitor : public Monitor {
+ public:
+  Hdf5Monitor(const Setting &s, const std::string &prefix, const std::string &sim_name,
+              const Lattice &lat);
+  ~Hdf5Monitor() override;
+  void update(Solver &solver) override;
+  void finish(Solver &solver) override;
+
+ private:
+  void write_spins(Solver &solver, const std::string &file);
+  void append_xdmf(const std::string &file, double time);
+  std::string prefix_, sim_name_;
+  std::FILE *xdmf_ = nullptr;
+  int64_t n_ = 0;
+  bool compressed_ = false, fields_ = false, final_written_ = false;
+};
 
 // monitors/magnetisation.cc:77-104 (one group: all spins)
 class MagnetisationMonitor : public Monitor {
