@@ -84,6 +84,8 @@ constexpr int64_t kLnAutoMinRow = 24;           // neighbours per row from which
 // typically about a second, against a phase of under a millisecond for a few
 // million spins
 constexpr unsigned kStepPollLimit = 1u << 20;
+// spins from which the step kernels store their outputs non-temporally
+constexpr int64_t kNtStoreMin = (int64_t)1 << 21;
 constexpr int kLnBand = JH_LN_BAND;             // y rows per band of the LINE wave order
 #ifndef JH_LN_PAIRS
 #define JH_LN_PAIRS 1
@@ -1827,6 +1829,11 @@ KParams make_params(jh_ctx *c) {
   p.order_nb = c->order_nb;
   p.order_cells = c->order_cells;
   p.order_rcells = c->order_cells > 0 ? 1.0 / (double)c->order_cells : 0.0;
+  // non-temporal step-kernel stores from kNtStoreMin spins: the 256-replica
+  // workload (6.4 M spins) FP64 step 561 -> 529 µs, the C3 LINE step 3 %
+  // faster; the C2 system (0.5 M spins, whose outputs the next launch still
+  // finds in the caches) 3-7 % slower with them
+  p.ntstore = c->n >= kNtStoreMin ? 1 : 0;
   // short rows (the two-shell bcc workload): alternating the basis sites'
   // workgroups keeps each XCD's gathers of the other site in its L2 (C2: HBM
   // reads 40 % lower in the predictor, 19 % in the corrector, same speed);

@@ -228,6 +228,7 @@ struct KParams {
   double t;
   int order_nb;         // basis-major period (1: reference order)
   int basis_ilv;        // gathering step kernels: workgroups alternate basis sites
+  int ntstore;          // step kernels store their outputs non-temporally
   int64_t order_cells;  // N / order_nb
   double order_rcells;  // 1 / order_cells, rounded
   // row-list launch (the FP64 stencil's wrapped rows): thread j computes

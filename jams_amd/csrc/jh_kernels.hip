@@ -650,7 +650,21 @@ __device__ __forceinline__ void heun_spin(const KParams &p, int64_t i, uint32_t 
   heun_math<PREC, PHASE, GEN>(p, i, ref_index(p, i), c, step, t, s0, s1, s2, x0, x1, x2, o, w0,
                               w1, w2, d0, d1,
                               d2, v0, v1, v2);
-  if constexpr (PHASE == 0) {
+  if (p.ntstore) {
+    if constexpr (PHASE == 0) {
+      if (p.noise == NM_PHILOX) {
+        __builtin_nontemporal_store(w0, p.wbuf + i);
+        __builtin_nontemporal_store(w1, p.wbuf + n + i);
+        __builtin_nontemporal_store(w2, p.wbuf + 2 * n + i);
+      }
+      __builtin_nontemporal_store(d0, p.ds + i);
+      __builtin_nontemporal_store(d1, p.ds + n + i);
+      __builtin_nontemporal_store(d2, p.ds + 2 * n + i);
+      if (!(PREC == PR_FP32 && p.spfree)) store_spin_nt(p.sp, n, i, v0, v1, v2);
+    } else {
+      store_spin_nt(p.snew, n, i, v0, v1, v2);
+    }
+  } else if constexpr (PHASE == 0) {
     if (p.noise == NM_PHILOX) { p.wbuf[i] = w0; p.wbuf[n + i] = w1; p.wbuf[2 * n + i] = w2; }
     p.ds[i] = d0; p.ds[n + i] = d1; p.ds[2 * n + i] = d2;
     if (!(PREC == PR_FP32 && p.spfree)) store_spin(p.sp, n, i, v0, v1, v2);
@@ -989,10 +1003,23 @@ __device__ __forceinline__ void ld_spin2(const double *__restrict__ a, int64_t n
   v[0][0] = p0.x; v[0][1] = p0.y; v[0][2] = z.x;
   v[1][0] = p1.x; v[1][1] = p1.y; v[1][2] = z.y;
 }
-__device__ __forceinline__ void st_spin2(double *a, int64_t n, int64_t i0, const double (&v)[2][3]) {
-  reinterpret_cast<double2 *>(a)[i0] = make_double2(v[0][0], v[0][1]);
-  reinterpret_cast<double2 *>(a)[i0 + 1] = make_double2(v[1][0], v[1][1]);
-  *reinterpret_cast<double2 *>(a + 2 * n + i0) = make_double2(v[0][2], v[1][2]);
+// Two adjacent doubles at p (16-byte aligned), non-temporal (nt) or not.
+// The step kernels store non-temporally when p.ntstore says so: a phase's
+// outputs are read only by the next launch, and at millions of spins they
+// outgrow the caches anyway, so streaming them past the L2 leaves it to the
+// neighbour spins that the waves in flight re-read.
+typedef double nt_double2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void st_d2(double *p, double x, double y, bool nt) {
+  if (nt)
+    __builtin_nontemporal_store((nt_double2){x, y}, reinterpret_cast<nt_double2 *>(p));
+  else
+    *reinterpret_cast<double2 *>(p) = make_double2(x, y);
+}
+__device__ __forceinline__ void st_spin2(double *a, int64_t n, int64_t i0, const double (&v)[2][3],
+                                         bool nt) {
+  st_d2(a + 2 * i0, v[0][0], v[0][1], nt);
+  st_d2(a + 2 * (i0 + 1), v[1][0], v[1][1], nt);
+  st_d2(a + 2 * n + i0, v[0][2], v[1][2], nt);
 }
 __device__ __forceinline__ void ld_soa2(const double *__restrict__ a, int64_t n, int64_t i0,
                                         double (&v)[2][3]) {
@@ -1003,9 +1030,10 @@ __device__ __forceinline__ void ld_soa2(const double *__restrict__ a, int64_t n,
     v[1][c] = q.y;
   }
 }
-__device__ __forceinline__ void st_soa2(double *a, int64_t n, int64_t i0, const double (&v)[2][3]) {
+__device__ __forceinline__ void st_soa2(double *a, int64_t n, int64_t i0, const double (&v)[2][3],
+                                        bool nt) {
 #pragma unroll
-  for (int c = 0; c < 3; ++c) *reinterpret_cast<double2 *>(a + c * n + i0) = make_double2(v[0][c], v[1][c]);
+  for (int c = 0; c < 3; ++c) st_d2(a + c * n + i0, v[0][c], v[1][c], nt);
 }
 __device__ __forceinline__ void st_g32_2(float *dst, int64_t i0, const double (&v)[2][3]) {
   static_assert(kG32 == 3, "the two-spin FP32 store assumes 12-byte spins");
@@ -1407,11 +1435,11 @@ __attribute__((amdgpu_waves_per_eu(kLnWaves<PHASE>))) k_heun_ln(const KParams p,
                                   d[h][1], d[h][2], v[h][0], v[h][1], v[h][2]);
     }
     if constexpr (PHASE == 0) {
-      if (p.noise == NM_PHILOX) st_soa2(p.wbuf, n, i0, w);
-      st_soa2(p.ds, n, i0, d);
-      if (!(PREC == PR_FP32 && p.spfree)) st_spin2(p.sp, n, i0, v);
+      if (p.noise == NM_PHILOX) st_soa2(p.wbuf, n, i0, w, p.ntstore);
+      st_soa2(p.ds, n, i0, d, p.ntstore);
+      if (!(PREC == PR_FP32 && p.spfree)) st_spin2(p.sp, n, i0, v, p.ntstore);
     } else {
-      st_spin2(p.snew, n, i0, v);
+      st_spin2(p.snew, n, i0, v, p.ntstore);
     }
     if constexpr (PREC == PR_FP32) st_g32_2(p.w32, i0, v);
   } else {

@@ -27,5 +27,12 @@ __device__ __forceinline__ void store_spin(double *s, int64_t n, int64_t i, doub
   reinterpret_cast<double2 *>(s)[i] = make_double2(x, y);
   s[2 * n + i] = z;
 }
+// the same as non-temporal stores (streamed past the caches)
+__device__ __forceinline__ void store_spin_nt(double *s, int64_t n, int64_t i, double x, double y,
+                                              double z) {
+  typedef double d2 __attribute__((ext_vector_type(2)));
+  __builtin_nontemporal_store((d2){x, y}, reinterpret_cast<d2 *>(s) + i);
+  __builtin_nontemporal_store(z, s + 2 * n + i);
+}
 
 }  // namespace jh
