@@ -233,6 +233,23 @@ def _phase_times(eng, reps):
 SETTLE_S = 0.3
 
 
+def _boundary_times(eng, s, reps: int = 3):
+    """The host-buffer crossings of the C ABI, wall clock with the device idle
+    before each (median of `reps`): jh_set_spins (host AoS -> device layout)
+    and jh_get_spins (device layout -> a new host AoS array), in ms."""
+    set_t, get_t = [], []
+    for _ in range(reps):
+        eng.synchronize()
+        t0 = time.perf_counter()
+        eng.set_spins(s)
+        eng.synchronize()
+        set_t.append((time.perf_counter() - t0) * 1e3)
+        t0 = time.perf_counter()
+        eng.get_spins()
+        get_t.append((time.perf_counter() - t0) * 1e3)
+    return sorted(set_t)[reps // 2], sorted(get_t)[reps // 2]
+
+
 def _settle(eng, seconds: float = SETTLE_S, chunk: int = 20) -> int:
     """Untimed steps for `seconds` of GPU time before a warm-up: the GPU's
     clock falls during host-side work (set-up, spin copies) and takes a few
@@ -350,6 +367,17 @@ def measure_c3(L: int, steps: int, warmup: int, reps: int, accuracy_steps: int,
                      "phases": _phase_times(eng, reps), "describe": eng.describe()}
         log(f"  C3 {name}: {ms / steps:.4f} ms/step, {out[name]['value']:.4e} spin-updates/s; "
             f"{eng.describe()}")
+        # the same run with the spins handed over in host memory at both ends
+        # (never the headline value: the timed region starts with them in HBM)
+        set_ms, get_ms = _boundary_times(eng, s_random)
+        tot = set_ms + ms + get_ms
+        out[name]["host_boundary"] = {
+            "set_spins_ms": set_ms, "get_spins_ms": get_ms, "bytes_each_way": n * 24,
+            "steps": steps, "ms_per_step_with_transfers": tot / steps,
+            "value_with_transfers": n * steps / (tot * 1e-3)}
+        log(f"  C3 {name} with host buffers at both ends: set_spins {set_ms:.1f} ms, get_spins "
+            f"{get_ms:.1f} ms ({n * 24 / 1e6:.1f} MB each way); {steps} steps "
+            f"{tot / steps:.4f} ms/step, {n * steps / (tot * 1e-3):.4e} spin-updates/s")
         # steady state: the median of five runs of 500 steps each (reported
         # beside the headline's timing)
         eng.set_spins(s_random)
@@ -694,6 +722,7 @@ def main() -> int:
                            c3.get("median5_speedup_fp32_vs_fp64_gathering")},
         "fp64_gathering_median5_ms_per_step": c3.get("fp64_gathering", {}).get("median5_ms_per_step"),
         "fp64_median5_ms_per_step": h["median5_ms_per_step"],
+        "host_boundary": h["host_boundary"],
         "cpu_baseline": cpu,
     }
     if "accuracy" in c3:
