@@ -700,8 +700,15 @@ __device__ __forceinline__ int64_t step_block(const KParams &p, int64_t blk) {
 }
 
 // One Heun phase of spin i with the gathering exchange formats: the field at
-// the phase's spins and the phase's update and stores.
-template <int PREC, int XVk, int PHASE, bool GEN>
+// the phase's spins and the phase's update and stores.  UC (for contexts with
+// a per-spin class table, such as replica batches): when every lane of the
+// wave has the same site class, which holds for all but the waves that
+// straddle a replica boundary, the class index is taken from the first lane,
+// so the compiler knows it is wave-uniform and reads the class table through
+// the scalar unit instead of one vector load per parameter.  (Without a class
+// table every spin has class 0; there the second copy of the update would only
+// cost registers.)
+template <int PREC, int XVk, int PHASE, bool GEN, bool UC = false>
 __device__ __forceinline__ void heun_gather_spin(const KParams &p, int64_t i,
                                                  const unsigned char *lds) {
   const uint32_t c = p.cls ? (uint32_t)p.cls[i] : 0u;
@@ -713,10 +720,18 @@ __device__ __forceinline__ void heun_gather_spin(const KParams &p, int64_t i,
   const double t = PHASE == 0 ? t0 : t0 + p.dt;
   double x0, x1, x2;
   exchange_at<PREC, XVk>(p, PHASE == 0 ? p.s : p.sp, p.g32, i, lds, x0, x1, x2);
-  heun_spin<PREC, PHASE, GEN>(p, i, c, step, t, s0, s1, s2, x0, x1, x2, own);
+  if constexpr (UC) {
+    const uint32_t c0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)c);
+    if (__builtin_amdgcn_ballot_w64(c != c0) == 0)
+      heun_spin<PREC, PHASE, GEN>(p, i, c0, step, t, s0, s1, s2, x0, x1, x2, own);
+    else
+      heun_spin<PREC, PHASE, GEN>(p, i, c, step, t, s0, s1, s2, x0, x1, x2, own);
+  } else {
+    heun_spin<PREC, PHASE, GEN>(p, i, c, step, t, s0, s1, s2, x0, x1, x2, own);
+  }
 }
 
-template <int PREC, int XVk, int PHASE, bool GEN>
+template <int PREC, int XVk, int PHASE, bool GEN, bool UC>
 __global__ void __launch_bounds__(kMaxBlock)
 __attribute__((amdgpu_waves_per_eu(kMinWaves<PREC, XVk>))) k_heun(const KParams p) {
   extern __shared__ __align__(16) unsigned char lds[];
@@ -729,7 +744,7 @@ __attribute__((amdgpu_waves_per_eu(kMinWaves<PREC, XVk>))) k_heun(const KParams 
   } else if (i >= p.n) {
     return;
   }
-  heun_gather_spin<PREC, XVk, PHASE, GEN>(p, i, lds);
+  heun_gather_spin<PREC, XVk, PHASE, GEN, UC>(p, i, lds);
 }
 
 // Grid barrier of the persistent step kernel over sync = {arrived count,
@@ -1690,14 +1705,21 @@ static hipError_t launch_heun_t(const KParams &p, int phase, int bs, hipStream_t
   const bool gen = p.has_ac || p.ani_power != 2;
   const int64_t rows = p.rowlist ? p.nrowlist : p.n;
   const int64_t nb = (rows + bs - 1) / bs;
-  if (phase == 0 && !gen)
-    hipLaunchKernelGGL((k_heun<PREC, XVk, 0, false>), dim3((unsigned)nb), dim3(bs), lds, st, p);
+  // the wave-uniform class split for contexts with a class table (the common
+  // forms only; the GEN forms keep one copy of the update)
+  const bool uc = p.cls != nullptr;
+  if (phase == 0 && !gen && uc)
+    hipLaunchKernelGGL((k_heun<PREC, XVk, 0, false, true>), dim3((unsigned)nb), dim3(bs), lds, st, p);
+  else if (phase == 0 && !gen)
+    hipLaunchKernelGGL((k_heun<PREC, XVk, 0, false, false>), dim3((unsigned)nb), dim3(bs), lds, st, p);
   else if (phase == 0)
-    hipLaunchKernelGGL((k_heun<PREC, XVk, 0, true>), dim3((unsigned)nb), dim3(bs), lds, st, p);
+    hipLaunchKernelGGL((k_heun<PREC, XVk, 0, true, false>), dim3((unsigned)nb), dim3(bs), lds, st, p);
+  else if (!gen && uc)
+    hipLaunchKernelGGL((k_heun<PREC, XVk, 1, false, true>), dim3((unsigned)nb), dim3(bs), lds, st, p);
   else if (!gen)
-    hipLaunchKernelGGL((k_heun<PREC, XVk, 1, false>), dim3((unsigned)nb), dim3(bs), lds, st, p);
+    hipLaunchKernelGGL((k_heun<PREC, XVk, 1, false, false>), dim3((unsigned)nb), dim3(bs), lds, st, p);
   else
-    hipLaunchKernelGGL((k_heun<PREC, XVk, 1, true>), dim3((unsigned)nb), dim3(bs), lds, st, p);
+    hipLaunchKernelGGL((k_heun<PREC, XVk, 1, true, false>), dim3((unsigned)nb), dim3(bs), lds, st, p);
   return hipGetLastError();
 }
 
