@@ -699,19 +699,26 @@ __device__ __forceinline__ int64_t step_block(const KParams &p, int64_t blk) {
   return blk;
 }
 
+// How the gathering step kernels find a spin's site class (its row of the
+// class table: mus, gyro, alpha, sigma, the noise seed and offset, the local
+// terms).  A per-lane class index makes every parameter one vector load per
+// spin; a wave-uniform one lets the compiler read the table through the
+// scalar unit.
+//   CM_LANE:  per lane (the GEN forms).
+//   CM_SPLIT: contexts with a class table (replica batches, several
+//             materials): when every lane of the wave has the same class,
+//             which holds for all but the waves that straddle a replica
+//             boundary, the update runs with the first lane's class; the
+//             other waves run it per lane.
+//   CM_ONE:   no class table: every spin has class 0, known at compile time.
+enum { CM_LANE = 0, CM_SPLIT = 1, CM_ONE = 2 };
+
 // One Heun phase of spin i with the gathering exchange formats: the field at
-// the phase's spins and the phase's update and stores.  UC (for contexts with
-// a per-spin class table, such as replica batches): when every lane of the
-// wave has the same site class, which holds for all but the waves that
-// straddle a replica boundary, the class index is taken from the first lane,
-// so the compiler knows it is wave-uniform and reads the class table through
-// the scalar unit instead of one vector load per parameter.  (Without a class
-// table every spin has class 0; there the second copy of the update would only
-// cost registers.)
-template <int PREC, int XVk, int PHASE, bool GEN, bool UC = false>
+// the phase's spins and the phase's update and stores.
+template <int PREC, int XVk, int PHASE, bool GEN, int CM = CM_LANE>
 __device__ __forceinline__ void heun_gather_spin(const KParams &p, int64_t i,
                                                  const unsigned char *lds) {
-  const uint32_t c = p.cls ? (uint32_t)p.cls[i] : 0u;
+  const uint32_t c = CM == CM_ONE ? 0u : p.cls ? (uint32_t)p.cls[i] : 0u;
   const uint64_t step = launch_step(p);
   const double t0 = launch_time(p, step);
   double s0, s1, s2;
@@ -720,7 +727,7 @@ __device__ __forceinline__ void heun_gather_spin(const KParams &p, int64_t i,
   const double t = PHASE == 0 ? t0 : t0 + p.dt;
   double x0, x1, x2;
   exchange_at<PREC, XVk>(p, PHASE == 0 ? p.s : p.sp, p.g32, i, lds, x0, x1, x2);
-  if constexpr (UC) {
+  if constexpr (CM == CM_SPLIT) {
     const uint32_t c0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)c);
     if (__builtin_amdgcn_ballot_w64(c != c0) == 0)
       heun_spin<PREC, PHASE, GEN>(p, i, c0, step, t, s0, s1, s2, x0, x1, x2, own);
@@ -731,7 +738,7 @@ __device__ __forceinline__ void heun_gather_spin(const KParams &p, int64_t i,
   }
 }
 
-template <int PREC, int XVk, int PHASE, bool GEN, bool UC>
+template <int PREC, int XVk, int PHASE, bool GEN, int CM>
 __global__ void __launch_bounds__(kMaxBlock)
 __attribute__((amdgpu_waves_per_eu(kMinWaves<PREC, XVk>))) k_heun(const KParams p) {
   extern __shared__ __align__(16) unsigned char lds[];
@@ -744,7 +751,7 @@ __attribute__((amdgpu_waves_per_eu(kMinWaves<PREC, XVk>))) k_heun(const KParams 
   } else if (i >= p.n) {
     return;
   }
-  heun_gather_spin<PREC, XVk, PHASE, GEN, UC>(p, i, lds);
+  heun_gather_spin<PREC, XVk, PHASE, GEN, CM>(p, i, lds);
 }
 
 // Grid barrier of the persistent step kernel over sync = {arrived count,
@@ -1705,21 +1712,21 @@ static hipError_t launch_heun_t(const KParams &p, int phase, int bs, hipStream_t
   const bool gen = p.has_ac || p.ani_power != 2;
   const int64_t rows = p.rowlist ? p.nrowlist : p.n;
   const int64_t nb = (rows + bs - 1) / bs;
-  // the wave-uniform class split for contexts with a class table (the common
-  // forms only; the GEN forms keep one copy of the update)
+  // the common forms find the site class wave-uniformly (CM_SPLIT with a
+  // class table, CM_ONE without); the GEN forms per lane
   const bool uc = p.cls != nullptr;
   if (phase == 0 && !gen && uc)
-    hipLaunchKernelGGL((k_heun<PREC, XVk, 0, false, true>), dim3((unsigned)nb), dim3(bs), lds, st, p);
+    hipLaunchKernelGGL((k_heun<PREC, XVk, 0, false, CM_SPLIT>), dim3((unsigned)nb), dim3(bs), lds, st, p);
   else if (phase == 0 && !gen)
-    hipLaunchKernelGGL((k_heun<PREC, XVk, 0, false, false>), dim3((unsigned)nb), dim3(bs), lds, st, p);
+    hipLaunchKernelGGL((k_heun<PREC, XVk, 0, false, CM_ONE>), dim3((unsigned)nb), dim3(bs), lds, st, p);
   else if (phase == 0)
-    hipLaunchKernelGGL((k_heun<PREC, XVk, 0, true, false>), dim3((unsigned)nb), dim3(bs), lds, st, p);
+    hipLaunchKernelGGL((k_heun<PREC, XVk, 0, true, CM_LANE>), dim3((unsigned)nb), dim3(bs), lds, st, p);
   else if (!gen && uc)
-    hipLaunchKernelGGL((k_heun<PREC, XVk, 1, false, true>), dim3((unsigned)nb), dim3(bs), lds, st, p);
+    hipLaunchKernelGGL((k_heun<PREC, XVk, 1, false, CM_SPLIT>), dim3((unsigned)nb), dim3(bs), lds, st, p);
   else if (!gen)
-    hipLaunchKernelGGL((k_heun<PREC, XVk, 1, false, false>), dim3((unsigned)nb), dim3(bs), lds, st, p);
+    hipLaunchKernelGGL((k_heun<PREC, XVk, 1, false, CM_ONE>), dim3((unsigned)nb), dim3(bs), lds, st, p);
   else
-    hipLaunchKernelGGL((k_heun<PREC, XVk, 1, true, false>), dim3((unsigned)nb), dim3(bs), lds, st, p);
+    hipLaunchKernelGGL((k_heun<PREC, XVk, 1, true, CM_LANE>), dim3((unsigned)nb), dim3(bs), lds, st, p);
   return hipGetLastError();
 }
 
