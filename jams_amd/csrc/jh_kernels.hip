@@ -1351,7 +1351,9 @@ constexpr int kLnWaves = PHASE == 0 ? JH_LN_PRED_WAVES : 1;
 #endif
 template <int PREC>
 constexpr bool kLnEarlyOwn = kEarlyOwn<PREC> || JH_LN_EARLY64;
-template <int PREC, bool DIAG, int PHASE, bool GEN, int P>
+// ONE: no class table (one site class), so every spin's class is the
+// constant 0 and its parameters are read through the scalar unit.
+template <int PREC, bool DIAG, int PHASE, bool GEN, int P, bool ONE>
 __global__ void __launch_bounds__(kLnBlock)
 __attribute__((amdgpu_waves_per_eu(kLnWaves<PHASE>))) k_heun_ln(const KParams p, const LnView ln) {
   const int rlb = ln.rlblocks;
@@ -1367,7 +1369,7 @@ __attribute__((amdgpu_waves_per_eu(kLnWaves<PHASE>))) k_heun_ln(const KParams p,
       const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
       if (j >= ln.nrows) return;
       const int64_t i = ln.rows[j];
-      const uint32_t c = p.cls ? (uint32_t)p.cls[i] : 0u;
+      const uint32_t c = ONE ? 0u : p.cls ? (uint32_t)p.cls[i] : 0u;
       const uint64_t step = launch_step(p);
       const double t0 = launch_time(p, step);
       double s0, s1, s2;
@@ -1448,7 +1450,7 @@ __attribute__((amdgpu_waves_per_eu(kLnWaves<PHASE>))) k_heun_ln(const KParams p,
     double d[2][3], v[2][3];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-      const uint32_t c = p.cls ? (uint32_t)p.cls[i0 + h] : 0u;
+      const uint32_t c = ONE ? 0u : p.cls ? (uint32_t)p.cls[i0 + h] : 0u;
       // (formed here from i0: a reference index computed from the line's
       // coordinates up front stays live across the field and spills)
       heun_math<PREC, PHASE, GEN>(p, i0 + h, ref_index(p, i0 + h), c, step, t, s[h][0], s[h][1],
@@ -1471,7 +1473,7 @@ __attribute__((amdgpu_waves_per_eu(kLnWaves<PHASE>))) k_heun_ln(const KParams p,
     if (fixed) return;
 #pragma unroll
     for (int h = 0; h < P; ++h) {
-      const uint32_t c = p.cls ? (uint32_t)p.cls[i0 + h] : 0u;
+      const uint32_t c = ONE ? 0u : p.cls ? (uint32_t)p.cls[i0 + h] : 0u;
       heun_spin<PREC, PHASE, GEN>(p, i0 + h, c, step, t, s[h][0], s[h][1], s[h][2], a[h][0],
                                   a[h][1], a[h][2], own[h]);
     }
@@ -1890,12 +1892,15 @@ static hipError_t launch_heun_ln_t(const KParams &p, const LnView &v, int phase,
   const dim3 grid((unsigned)(lines / (bs / 64) + v.rlblocks)), block((unsigned)bs);
   const size_t lds = v.rlblocks > 0 ? table_bytes(PREC, DIAG ? XV_GDIAG : XV_GFULL, p.x.ntab) : 0;
   const bool gen = p.has_ac || p.ani_power != 2;
+  const bool one = p.cls == nullptr;
   if (phase == 0) {
-    if (gen) hipLaunchKernelGGL((k_heun_ln<PREC, DIAG, 0, true, P>), grid, block, lds, st, p, v);
-    else hipLaunchKernelGGL((k_heun_ln<PREC, DIAG, 0, false, P>), grid, block, lds, st, p, v);
+    if (gen) hipLaunchKernelGGL((k_heun_ln<PREC, DIAG, 0, true, P, false>), grid, block, lds, st, p, v);
+    else if (one) hipLaunchKernelGGL((k_heun_ln<PREC, DIAG, 0, false, P, true>), grid, block, lds, st, p, v);
+    else hipLaunchKernelGGL((k_heun_ln<PREC, DIAG, 0, false, P, false>), grid, block, lds, st, p, v);
   } else {
-    if (gen) hipLaunchKernelGGL((k_heun_ln<PREC, DIAG, 1, true, P>), grid, block, lds, st, p, v);
-    else hipLaunchKernelGGL((k_heun_ln<PREC, DIAG, 1, false, P>), grid, block, lds, st, p, v);
+    if (gen) hipLaunchKernelGGL((k_heun_ln<PREC, DIAG, 1, true, P, false>), grid, block, lds, st, p, v);
+    else if (one) hipLaunchKernelGGL((k_heun_ln<PREC, DIAG, 1, false, P, true>), grid, block, lds, st, p, v);
+    else hipLaunchKernelGGL((k_heun_ln<PREC, DIAG, 1, false, P, false>), grid, block, lds, st, p, v);
   }
   return hipGetLastError();
 }
