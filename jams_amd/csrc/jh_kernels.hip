@@ -1492,7 +1492,9 @@ __attribute__((amdgpu_waves_per_eu(kLnWaves<PHASE>))) k_heun_ln(const KParams p,
 //   as the reference's combination is written;
 //   stage 3 writes s = v / sqrt(v.v), v = s + (dt sum) / 6 (the reference
 //   multiplies by CUDA's rsqrt; here 1 / sqrt, each correctly rounded).
-template <int PREC, int XVk, bool GEN>
+// ONE: no class table, the class is the constant 0 (its parameters then
+// come through the scalar unit).
+template <int PREC, int XVk, bool GEN, bool ONE>
 __global__ void __launch_bounds__(kMaxBlock)
 __attribute__((amdgpu_waves_per_eu(kMinWaves<PREC, XVk>))) k_rk4(const KParams p) {
   extern __shared__ __align__(16) unsigned char lds[];
@@ -1502,7 +1504,7 @@ __attribute__((amdgpu_waves_per_eu(kMinWaves<PREC, XVk>))) k_rk4(const KParams p
   const int64_t i = blk * blockDim.x + threadIdx.x;
   if (i >= p.n) return;
   const int64_t n = p.n;
-  const uint32_t c = p.cls ? (uint32_t)p.cls[i] : 0u;
+  const uint32_t c = ONE ? 0u : p.cls ? (uint32_t)p.cls[i] : 0u;
   const int stage = p.stage;
   const uint64_t step = p.dstep ? *p.dstep + p.step : p.step;
   const double t0 = p.dstep ? (double)step * p.dt : p.t;
@@ -1934,9 +1936,11 @@ static hipError_t launch_rk4_t(const KParams &p, int bs, hipStream_t st) {
   const int64_t nb = (p.n + bs - 1) / bs;
   const size_t lds = table_lds(p, PREC, XVk);
   if (p.has_ac || p.ani_power != 2)
-    hipLaunchKernelGGL((k_rk4<PREC, XVk, true>), dim3((unsigned)nb), dim3(bs), lds, st, p);
+    hipLaunchKernelGGL((k_rk4<PREC, XVk, true, false>), dim3((unsigned)nb), dim3(bs), lds, st, p);
+  else if (p.cls)
+    hipLaunchKernelGGL((k_rk4<PREC, XVk, false, false>), dim3((unsigned)nb), dim3(bs), lds, st, p);
   else
-    hipLaunchKernelGGL((k_rk4<PREC, XVk, false>), dim3((unsigned)nb), dim3(bs), lds, st, p);
+    hipLaunchKernelGGL((k_rk4<PREC, XVk, false, true>), dim3((unsigned)nb), dim3(bs), lds, st, p);
   return hipGetLastError();
 }
 
